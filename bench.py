@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: particle-updates/s of the resident HIP MLS-MPM step.
+
+Workload: the single large elastic scene, 1,048,576 particles in four
+jittered cubes of edge 0.2 with N(0, 0.5) m/s velocities, corotated elastic
+(E=1e4, nu=0.3, rho=1e3), 4x4x4-cell blocks over a 256^3-cell domain
+(h=1/256), dt=5e-5.  A step is one substep of the whole scene (binning,
+P2G, grid update, G2P).  The timed region covers K device-resident substeps
+with the particle state already in HBM: it starts after a device wait and
+ends with a device wait; readback and validation follow outside it.
+
+Printed JSON fields beyond the common contract:
+  roofline      dominant kernel's algorithmic bytes per launch over its
+                average duration (HIP events on the session stream), against
+                MI355X's 8 TB/s HBM peak; plus the whole-substep figure
+  cpu_baseline  the FP64 C++ restatement of the reference CPU substep
+                (oracle/, one thread) timed on a bounded sample of the same
+                scene on this host
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (FP32 state): P2G reads x, v, C, F, m, V0 (104 B per
+# particle) and writes one 16-B (m, p) accumulator per live node; G2P writes
+# x, v, C, F (96 B per particle) and reads one 16-B velocity per live node.
+# Together: 200 B per particle-update + 32 B per live node per substep.
+P2G_BYTES_PARTICLE, P2G_BYTES_NODE = 104, 16
+G2P_BYTES_PARTICLE, G2P_BYTES_NODE = 96, 16
+
+
+def submit(session, steps: int) -> None:
+    left = steps
+    while left > 0:
+        k = min(64, left)
+        session.submit_steps(k)
+        left -= k
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--mode", choices=["fp32", "fixed64"], default="fp32")
+    ap.add_argument("--timing-steps", type=int, default=128, help="substeps of the per-kernel event pass")
+    ap.add_argument("--cpu-substeps", type=int, default=2, help="substeps of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra-modes", action="store_true")
+    args = ap.parse_args()
+
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py: this build measures one GPU; every GPU box here has one GPU and the path shards by "
+              "scene or by spatial slab (see DESIGN.md), so --gpus must be 1.", file=sys.stderr)
+        return 2
+
+    from tessera_amd.mpm import HipMpmResidentSession, HipMpmTransfers
+    from tessera_amd.scenes import four_cubes, record_hash
+
+    scene = four_cubes()
+    n = len(scene.world.particles)
+    transfers = HipMpmTransfers(0)
+    session = HipMpmResidentSession(transfers, scene.world, scene.dt, args.mode)
+    print(f"bench: {scene.name}, {n} particles, mode {args.mode}, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+
+    submit(session, args.warmup)
+    session.wait()
+    t0 = time.perf_counter()
+    submit(session, args.steps)
+    session.wait()
+    t1 = time.perf_counter()
+    session.synchronize()  # validates device flags and reads the state back (untimed)
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    print(f"bench: timed {args.steps} substeps in {elapsed:.4f}s -> {value:.4e} particle-updates/s",
+          file=sys.stderr, flush=True)
+
+    # Per-kernel durations from HIP events on the session stream.
+    session.set_timing(True)
+    submit(session, args.timing_steps)
+    session.synchronize()
+    times = session.kernel_times()
+    session.set_timing(False)
+    stats = session.topology_stats()
+    live = stats["mass_nodes"]
+
+    substep_ms = sum(ms for ms, _ in times.values())
+    kernels = {k: {"avg_ms": ms, "calls": c} for k, (ms, c) in sorted(times.items(), key=lambda kv: -kv[1][0])}
+    p2g_name = "k_p2g_fixed64" if args.mode == "fixed64" else "k_p2g"
+    alg = {
+        p2g_name: P2G_BYTES_PARTICLE * n + P2G_BYTES_NODE * live,
+        "k_g2p": G2P_BYTES_PARTICLE * n + G2P_BYTES_NODE * live,
+    }
+    dominant = max(alg, key=lambda k: times.get(k, (0.0, 0))[0])
+    dom_ms = times[dominant][0]
+    achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
+    substep_bytes = (P2G_BYTES_PARTICLE + G2P_BYTES_PARTICLE) * n + (P2G_BYTES_NODE + G2P_BYTES_NODE) * live
+    wall_substep_gbs = substep_bytes / (elapsed / args.steps) / 1e9
+
+    extra_modes = {}
+    if not args.no_extra_modes:
+        other = "fixed64" if args.mode == "fp32" else "fp32"
+        s2 = HipMpmResidentSession(transfers, scene.world, scene.dt, other)
+        submit(s2, args.warmup)
+        s2.wait()
+        k2 = min(args.steps, 256)
+        ta = time.perf_counter()
+        submit(s2, k2)
+        s2.wait()
+        tb = time.perf_counter()
+        s2.synchronize()
+        extra_modes[other] = {"value": n * k2 / (tb - ta), "steps": k2}
+        s2.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        ow = O.OracleWorld(scene.world)
+        ca = time.perf_counter()
+        ow.substeps_fixed(scene.dt, args.cpu_substeps)
+        cb = time.perf_counter()
+        cpu = {
+            "value": n * args.cpu_substeps / (cb - ca),
+            "unit": "particle-updates/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_substeps} substeps of the full {n}-particle scene, FP64 C++ restatement of "
+                      f"MpmWorld::substep (single thread, {cb - ca:.1f}s)",
+        }
+
+    out = {
+        "metric": "particle-updates/s",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "four_cubes_elastic_1M",
+            "particles": n,
+            "cell_width": scene.world.params.cell_width,
+            "dt": scene.dt,
+            "block_cells": 4,
+            "p2g_mode": args.mode,
+            "scene_hash": record_hash(scene.world.particles)[:16],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": alg[dominant],
+            "avg_ms": dom_ms,
+            "live_nodes": live,
+            "substep_algorithmic_bytes": substep_bytes,
+            "substep_wall_gbs": wall_substep_gbs,
+            "substep_wall_frac": wall_substep_gbs / HBM_PEAK_GBS,
+        },
+        "kernels": kernels,
+        "event_substep_ms": substep_ms,
+        "other_modes": extra_modes,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,277 @@
+/*
+ * tessera_mpm.h — C ABI of the MI355X (gfx950) HIP backend for tessera-mpm's
+ * sparse-grid MLS-MPM substep.
+ *
+ * The backend is a drop-in for the reference's resident WebGPU session
+ * (crates/tessera-mpm/src/gpu.rs:206-662, `GpuMpmResidentSession`) and its
+ * world-level entry points (gpu.rs:2177-2293).  Every function below names the
+ * reference interface it replaces.  Plain pointers and sizes only; no
+ * exceptions cross the boundary; every call returns a tmpm_status whose
+ * values map one-to-one onto the reference's error enums
+ * (`GpuMpmError`, gpu.rs:135-154, and `MpmError`, world.rs:165-181).
+ *
+ * Threading: a session is externally serialized (the reference takes
+ * `&mut self` for every mutating call).  Each session owns one HIP stream on
+ * its context's device.  Different sessions may be used from different
+ * threads concurrently.
+ *
+ * Units and layouts follow the reference world exactly: f64 SI values,
+ * vectors as 3 consecutive doubles, 3x3 matrices as 9 doubles in nalgebra's
+ * column-major order (element (row r, column c) at index c*3 + r), particles
+ * in insertion order.
+ */
+#ifndef TESSERA_MPM_H
+#define TESSERA_MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  gpu.rs:135-154 (GpuMpmError) and world.rs:165-181 (MpmError). */
+typedef enum tmpm_status {
+  TMPM_OK = 0,
+  TMPM_INVALID_INPUT = 1,           /* GpuMpmError::InvalidInput / MpmError::InvalidInput */
+  TMPM_CAPACITY = 2,                /* GpuMpmError::Capacity (grid, hash or fixed-point range exhausted) */
+  TMPM_PENDING_STEPS = 3,           /* GpuMpmError::PendingSteps */
+  TMPM_READBACK = 4,                /* GpuMpmError::Readback (HIP runtime failure; message in tmpm_last_error) */
+  TMPM_RESIDENT_UNAVAILABLE = 5,    /* GpuMpmError::ResidentUnavailable */
+  TMPM_EXCESSIVE_SUBSTEPS = 6,      /* MpmError::ExcessiveSubsteps */
+  TMPM_NON_FINITE_STATE = 7,        /* MpmError::NonFiniteState */
+  TMPM_UNKNOWN_CHUNK = 8,           /* MpmError::UnknownChunk */
+  TMPM_PARTICLE_CAPACITY = 9,       /* MpmError::ParticleCapacity */
+  TMPM_NO_DEVICE = 10               /* no usable gfx950 device for the requested ordinal */
+} tmpm_status;
+
+/* Material kinds in `MaterialModel` variant order (material.rs:28-91).
+ * Parameters p[0..4] in the variant's field order:
+ *   LINEAR_ELASTIC:   young_modulus, poisson_ratio              (corotated)
+ *   NEO_HOOKEAN:      young_modulus, poisson_ratio
+ *   SAND:             young_modulus, poisson_ratio, friction_angle, cohesion
+ *   FLUID:            bulk_modulus, gamma, viscosity, tensile_stiffness
+ *   SNOW:             young_modulus, poisson_ratio, critical_compression, critical_stretch, hardening
+ *   SAND_NEO_HOOKEAN: young_modulus, poisson_ratio, friction_angle, cohesion */
+enum {
+  TMPM_MATERIAL_LINEAR_ELASTIC = 0,
+  TMPM_MATERIAL_NEO_HOOKEAN = 1,
+  TMPM_MATERIAL_SAND = 2,
+  TMPM_MATERIAL_FLUID = 3,
+  TMPM_MATERIAL_SNOW = 4,
+  TMPM_MATERIAL_SAND_NEO_HOOKEAN = 5
+};
+
+typedef struct tmpm_material {
+  int32_t kind;
+  int32_t reserved;
+  double p[5];
+} tmpm_material;
+
+/* `MpmParams` (world.rs:27-49) with `use_cpic` (world.rs:217-219). */
+typedef struct tmpm_params {
+  double cell_width;
+  double gravity[3];
+  double max_substep;
+  int32_t has_bounds;
+  int32_t use_cpic;
+  double bound_min[3];
+  double bound_max[3];
+} tmpm_params;
+
+/* Obstacle shapes in `ObstacleShape` order (obstacle.rs:10-61) and boundary
+ * responses in `ObstacleBoundary` order (obstacle.rs:65-77). */
+enum {
+  TMPM_SHAPE_SPHERE = 0,
+  TMPM_SHAPE_BOX = 1,
+  TMPM_SHAPE_CAPSULE = 2,
+  TMPM_SHAPE_CYLINDER = 3,
+  TMPM_SHAPE_CONE = 4,
+  TMPM_SHAPE_GROUND = 5,
+  TMPM_SHAPE_TRIANGLE_PRISM = 6,
+  TMPM_SHAPE_CONVEX = 7
+};
+enum {
+  TMPM_BOUNDARY_SLIP = 0,
+  TMPM_BOUNDARY_STICK = 1,
+  TMPM_BOUNDARY_SEPARATE = 2,
+  TMPM_BOUNDARY_NON_REFLECTING = 3,
+  TMPM_BOUNDARY_FREE_SLIP = 4
+};
+
+/* `RigidObstacle` (obstacle.rs:79-98).  dims: sphere {radius}; box {half
+ * extents xyz}; capsule/cylinder/cone {half_height, radius}; ground {half
+ * extents xy}; triangle prism {half_thickness}; convex {bound_radius}.
+ * orientation is the unit quaternion (i, j, k, w). */
+typedef struct tmpm_obstacle {
+  int32_t shape;
+  int32_t boundary;
+  int32_t cpic_group; /* -1: none */
+  uint32_t plane_count;
+  double center[3];
+  double orientation[4];
+  double linear_velocity[3];
+  double angular_velocity[3];
+  double friction;
+  double dims[4];
+  double vertices[9];
+  const double* planes; /* convex: plane_count x {nx, ny, nz, offset} */
+} tmpm_obstacle;
+
+/* Structure-of-arrays view of `MpmParticle` records (world.rs:67-102), all
+ * arrays of `count` entries in insertion order.  material_index selects an
+ * entry of the accompanying material table.  plastic holds
+ * {plastic_det, hardening, log_volume_gain} (material.rs:5-14). */
+typedef struct tmpm_particles {
+  uint64_t count;
+  double* position;      /* 3 per particle */
+  double* velocity;      /* 3 */
+  double* deformation;   /* 9, column-major */
+  double* affine;        /* 9, column-major */
+  double* force;         /* 3 */
+  double* mass;          /* 1 */
+  double* rest_volume;   /* 1 */
+  double* radius;        /* 1 */
+  double* damping;       /* 1 */
+  double* plastic;       /* 3 */
+  uint32_t* material_index;
+  uint32_t* group_id;
+  uint16_t* transfer_color;
+  uint8_t* enabled;
+  uint8_t* fixed;
+  uint64_t* chunk_id;    /* read-out only; ignored on input */
+} tmpm_particles;
+
+/* A whole `MpmWorld` (world.rs:206-221) handed over at session creation. */
+typedef struct tmpm_world_desc {
+  tmpm_params params;
+  tmpm_particles particles;
+  const tmpm_material* materials;
+  uint32_t material_count;
+  uint32_t obstacle_count;
+  const tmpm_obstacle* obstacles;
+  uint64_t substeps;
+} tmpm_world_desc;
+
+/* P2G accumulation precision.  FP32 uses hardware float atomics (order
+ * dependent in the last bits, like the reference's CAS loops,
+ * gpu.wgsl:739-766).  FIXED64 accumulates 64-bit fixed-point integers, so
+ * every substep is bit-reproducible run to run. */
+typedef enum tmpm_p2g_mode { TMPM_P2G_FP32 = 0, TMPM_P2G_FIXED64 = 1 } tmpm_p2g_mode;
+
+typedef struct tmpm_context tmpm_context;
+typedef struct tmpm_session tmpm_session;
+
+/* Message of the last failing call on this thread ("" when none). */
+const char* tmpm_last_error(void);
+
+/* Library version string. */
+const char* tmpm_version(void);
+
+/* Device context; replaces `GpuMpmTransfers::new(&Device)` (gpu.rs:927-992),
+ * which compiles the pipelines once per device.  Fails with TMPM_NO_DEVICE
+ * when the ordinal does not name a gfx950 device. */
+tmpm_status tmpm_context_create(int32_t device_ordinal, tmpm_context** out);
+void tmpm_context_destroy(tmpm_context* ctx);
+
+/* `GpuMpmResidentSession::new(gpu, device, queue, world, substep_dt)`
+ * (gpu.rs:253-280, validation gpu.rs:664-677 and world.rs:225-241).  The
+ * world is copied; the caller's arrays are not retained. */
+tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* world, double substep_dt,
+                                tmpm_p2g_mode mode, tmpm_session** out);
+void tmpm_session_destroy(tmpm_session* s);
+
+/* `submit_steps(steps)` (gpu.rs:303-322): 1..=64 substeps, asynchronous. */
+tmpm_status tmpm_session_submit_steps(tmpm_session* s, uint32_t steps);
+/* `synchronize()` (gpu.rs:382-411): wait, validate device flags, update the
+ * host world.  Any failure invalidates the session and keeps the last world. */
+tmpm_status tmpm_session_synchronize(tmpm_session* s);
+/* Wait for the submitted substeps to finish on the device without reading
+ * the particle state back or validating it (no reference counterpart; the
+ * throughput benchmark uses it to time device-resident stepping alone). */
+tmpm_status tmpm_session_wait(tmpm_session* s);
+/* `step(steps)` (gpu.rs:286-292) = submit_steps + synchronize. */
+tmpm_status tmpm_session_step(tmpm_session* s, uint32_t steps);
+/* `MpmWorld::step_with_gpu_transfers(dt)` (gpu.rs:2208-2240): CFL-limited
+ * substeps of min(remaining, stable_timestep), synchronized after each. */
+tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt);
+/* `has_pending_steps()` / `pending_substeps()` (gpu.rs:367-375). */
+uint64_t tmpm_session_pending_substeps(const tmpm_session* s);
+/* Whether the session is still usable (no failure since creation). */
+int32_t tmpm_session_valid(const tmpm_session* s);
+/* `world().substeps` (world.rs:215). */
+uint64_t tmpm_session_substeps(const tmpm_session* s);
+/* `world().particles.len()`. */
+uint64_t tmpm_session_particle_count(const tmpm_session* s);
+
+/* Copy the last synchronized world's particle state out, insertion order.
+ * Any array pointer may be NULL to skip that field.  `world()` (gpu.rs:653-656). */
+tmpm_status tmpm_session_read_particles(const tmpm_session* s, tmpm_particles* out);
+
+/* `set_particle_force(index, force)` (gpu.rs:457-485): one-shot force
+ * consumed by the next substep. */
+tmpm_status tmpm_session_set_particle_force(tmpm_session* s, uint64_t index, const double force[3]);
+/* `set_obstacles(obstacles)` (gpu.rs:491-529). */
+tmpm_status tmpm_session_set_obstacles(tmpm_session* s, const tmpm_obstacle* obstacles, uint32_t count);
+/* `add_particles(particles) -> ParticleChunkId` (gpu.rs:535-549). */
+tmpm_status tmpm_session_add_particles(tmpm_session* s, const tmpm_particles* particles,
+                                       const tmpm_material* materials, uint32_t material_count,
+                                       uint64_t* chunk_out);
+/* `remove_chunk(chunk)` (gpu.rs:553-564). */
+tmpm_status tmpm_session_remove_chunk(tmpm_session* s, uint64_t chunk, uint64_t* removed_out);
+
+/* `encode_particle_snapshot(encoder, output)` (gpu.rs:595-651): writes one
+ * 32-byte record {x, y, z, mass; vx, vy, vz, rest_volume} per particle in
+ * insertion order (mass 0 for disabled particles) into caller-owned device
+ * memory, enqueued on `stream` (NULL: the session's stream).  Returns the
+ * record count through count_out (0 and nothing written when no particle is
+ * enabled). */
+tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, uint64_t output_bytes,
+                                         void* stream, uint32_t* count_out);
+
+/* `MpmWorld::stable_timestep()` (world.rs:289-303) on the session's last
+ * synchronized world. */
+double tmpm_session_stable_timestep(const tmpm_session* s);
+
+/* Topology of the most recent substep, for inspection (the reference's tests
+ * read the indirect node count, gpu.rs:3227-3297).  live_nodes counts the
+ * exact union of particle stencils (computed on demand); grid_blocks and
+ * particle_blocks count 4x4x4-cell blocks. */
+typedef struct tmpm_topology_stats {
+  uint64_t live_nodes;
+  uint64_t grid_blocks;
+  uint64_t particle_blocks;
+  uint64_t binned_particles;
+  uint64_t mass_nodes;  /* nodes with positive mass in the last grid update */
+} tmpm_topology_stats;
+tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* out);
+
+/* Export the live node set of the most recent substep as sorted (x, y, z,
+ * region) int32 quadruples (region = transfer colour).  Writes up to `cap`
+ * keys and the total through count_out. */
+tmpm_status tmpm_session_export_nodes(tmpm_session* s, int32_t* keys, uint64_t cap, uint64_t* count_out);
+/* Export the particle-block partition of the most recent binning: sorted
+ * unique block keys (bx, by, bz, region), and for each particle (insertion
+ * order) the index of its block in that list or -1 when not binned. */
+tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uint64_t cap, int64_t* particle_block,
+                                       uint64_t* count_out);
+
+/* Per-kernel average durations (milliseconds) recorded with HIP events on the
+ * session stream over the substeps since the last reset; names are the
+ * kernel names.  Timing is off unless enabled (it adds one event pair per
+ * kernel launch). */
+tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled);
+tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
+                                      uint64_t* calls, uint32_t cap, uint32_t* count_out);
+
+/* Host-side validation helpers shared with the binding layer. */
+double tmpm_timestep_bound(const tmpm_material* material, double density, const double velocity[3],
+                           double cell_width, double deformation_det);
+int32_t tmpm_material_valid(const tmpm_material* material);
+int32_t tmpm_obstacle_valid(const tmpm_obstacle* obstacle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TESSERA_MPM_H */

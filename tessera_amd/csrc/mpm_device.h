@@ -1,0 +1,129 @@
+// Device data layout shared by the HIP kernels (mpm_kernels.hip) and the
+// host session (mpm_session.hip).
+//
+// HBM layout (structure of arrays, particle capacity `stride`):
+//   pos_mass[i]  float4  x, y, z, mass (mass 0 marks a disabled particle)
+//   vel_vol[i]   float4  vx, vy, vz, rest volume
+//   C[k*stride+i] float  APIC affine matrix, k = column*3 + row
+//   F[k*stride+i] float  deformation gradient, same indexing
+//   cls[i]       u32     particle class (material, radius, damping, flags, region)
+//   ins_id[i]    u32     insertion index of the particle stored in slot i
+// Grid: 4x4x4-cell blocks; node (slot s, local l) at s*64 + l with
+// l = (x&3)*16 + (y&3)*4 + (z&3).
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace tmpm {
+
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr int kBlockCells = 4;        // cells per block edge
+constexpr int kBlockNodes = 64;       // nodes per block
+constexpr int kTileEdge = 6;          // P2G/G2P tile: block + 2-cell halo
+constexpr int kTileNodes = 216;
+constexpr int kScanChunk = 1024;      // buckets per scan workgroup
+constexpr int kMaxObstacles = 256;
+
+// counters[] slots
+enum Counter : uint32_t {
+  CNT_GRID_BLOCKS = 0,
+  CNT_PARTICLE_BLOCKS = 1,
+  CNT_BINNED = 2,
+  CNT_ERROR = 3,
+  CNT_MASS_NODES = 4,
+  CNT_LIVE_NODES = 5,
+  CNT_COUNT = 8
+};
+// error bits in counters[CNT_ERROR]
+enum ErrorBit : uint32_t {
+  ERR_RESIDENT = 1u,        // CFL, out of grid, non-finite: ResidentUnavailable
+  ERR_TOPOLOGY = 2u,        // hash or grid capacity exhausted: Capacity
+  ERR_FIXED_RANGE = 4u,     // fixed-point contribution out of range: Capacity
+  ERR_INTERNAL = 8u         // inconsistent binning (must never happen)
+};
+
+// Device material codes.
+enum DevMaterial : int32_t { DM_COROTATED = 0, DM_NEO_HOOKEAN = 1, DM_FLUID = 2 };
+
+struct DevClass {
+  int32_t kind;
+  uint32_t region;   // (scene << 16) | transfer colour
+  uint32_t fixed;
+  float radius;
+  float damping;
+  float lambda, mu;                         // elastic Lame parameters
+  float bulk, gamma, viscosity, tensile;    // fluid
+  float wave_numerator;                     // lambda + 2 mu, or bulk * gamma
+};
+
+struct DevScene {
+  float h, inv_h, inv_d, dt;
+  float gravity[3];
+  int32_t has_bounds;
+  float bmin[3];
+  float bmax[3];
+  int32_t first_base[3];
+  int32_t last_base[3];
+  int32_t obstacle_begin;
+  int32_t obstacle_count;
+  int32_t apply_force;     // read the force array in this substep's G2P
+  int32_t mass_shift;      // fixed-point scale exponents
+  int32_t mom_shift;
+  int32_t pad;
+};
+
+struct DevObstacle {
+  int32_t shape;
+  int32_t boundary;
+  float friction;
+  float pad0;
+  float center[3];
+  float pad1;
+  float q[4];          // i, j, k, w
+  float lin[3];
+  float pad2;
+  float ang[3];
+  float pad3;
+  float dims[4];
+};
+
+struct Parts {
+  float4* pos_mass;
+  float4* vel_vol;
+  float* C;
+  float* F;
+  uint32_t* cls;
+  float4* force;
+  uint32_t* ins_id;
+  uint32_t n;        // particles in storage
+  uint32_t stride;   // SoA capacity
+};
+
+struct Topo {
+  uint32_t* owner;          // bucket -> owner id (0 empty, 1..n particle, > n neighbour insert)
+  uint32_t* bucket_count;   // particles binned into the bucket's block
+  uint32_t* nbr_bucket;     // 8 per bucket: buckets of block + {0,1}^3
+  uint32_t* bucket_slot;    // bucket -> grid slot
+  uint32_t* bucket_pblock;  // bucket -> particle-block id
+  uint32_t bucket_mask;     // bucket count - 1 (power of two)
+  uint32_t n_chunks;        // scan chunks over buckets
+  uint32_t* chunk_sums;     // 3 per chunk (grid, particle blocks, particles), then 3 totals
+  int4* grid_key;           // slot -> (bx, by, bz, region)
+  uint32_t grid_cap;
+  uint32_t pblock_cap;
+  uint32_t* pblock_bucket;
+  uint32_t* pblock_slot8;   // 8 grid slots per particle block
+  uint32_t* pblock_start;
+  uint32_t* pblock_len;
+  uint32_t* pslot;          // particle -> bucket
+  uint32_t* prank;          // particle -> rank within its block
+  uint32_t* sorted;         // binned order -> particle storage index
+  uint32_t* counters;
+  void* grid_acc;           // FP32: float4 per node; FIXED64: 4 x int64 per node
+  float4* grid_vel;
+  uint32_t* node_mask;      // 2 words per grid slot (diagnostic live-node set)
+};
+
+}  // namespace tmpm

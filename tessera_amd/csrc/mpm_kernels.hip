@@ -1,0 +1,1186 @@
+// Hand-written gfx950 kernels for the sparse-grid MLS-MPM substep.
+//
+// Substep pipeline (all counts stay on the device; every kernel is a
+// grid-stride loop over a device-side count, so the launch sequence never
+// depends on host readback):
+//   1 k_topo_clear        reset hash buckets and counters
+//   2 k_bin_insert        particle -> 4^3-cell block key, lock-free hash insert
+//                         (wave-aggregated), per-block counting-sort rank
+//   3 k_bin_neighbors     insert the 7 upper neighbours of every particle block
+//                         (grid blocks a block's P2G tile touches)
+//   4 k_scan_*            compact buckets -> grid slots, particle-block ids,
+//                         particle offsets (three-phase scan)
+//   5 k_pblock_finalize   per particle block: grid slots of its 8 tile blocks
+//   6 k_bin_scatter       particle -> position in block-sorted order (optionally
+//                         moving the particle state itself: periodic reorder)
+//   7 k_grid_clear        zero the P2G accumulator of the live grid slots
+//   8 k_p2g               per particle block: stress, 27-node B-spline scatter
+//                         into a 6^3 LDS tile, flush with global atomics
+//   9 k_grid_update       per node: v = p/m + g dt, walls, obstacles
+//  10 k_g2p               per particle block: 6^3 velocity tile in LDS, gather,
+//                         v/C/x/F update, material projection, validity flags
+// The reference computes the same substep with an open-addressed hash of
+// all 27N stencil nodes (gpu_topology.wgsl:18-103) and per-particle CAS-loop
+// float atomics (gpu.wgsl:711-1019); here the hash holds blocks, not nodes,
+// and P2G/G2P work on LDS tiles.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "mpm_device.h"
+#include "mpm_kernels.h"
+
+namespace tmpm {
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+__device__ __forceinline__ void flag_error(uint32_t* counters, bool cond, uint32_t bit) {
+  uint64_t m = __ballot(cond);
+  if (m != 0 && lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1)) atomicOr(&counters[CNT_ERROR], bit);
+}
+
+// Cell key of one coordinate: floor(x * inv_h - 0.5) with both operations
+// rounded to f32 and no fused multiply-add (gpu.rs:1145, the host packing of
+// the reference).  Valid when finite and within [i32::MIN, i32::MAX - 2].
+__device__ __forceinline__ bool cell_of(float x, float inv_h, int& cell) {
+#pragma clang fp contract(off)
+  float gp = x * inv_h;
+  float shifted = gp - 0.5f;
+  float f = floorf(shifted);
+  bool ok = (double)f >= -2147483648.0 && (double)f <= 2147483645.0;
+  cell = ok ? (int)f : 0;
+  return ok;
+}
+
+__device__ __forceinline__ bool cell3(float4 pm, float inv_h, int3& c) {
+  bool ok = cell_of(pm.x, inv_h, c.x);
+  ok = cell_of(pm.y, inv_h, c.y) && ok;
+  ok = cell_of(pm.z, inv_h, c.z) && ok;
+  return ok;
+}
+
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ uint32_t hash_key(int4 k) {
+  uint32_t h = (uint32_t)k.x * 0x9E3779B1u;
+  h ^= (uint32_t)k.y * 0x85EBCA77u;
+  h ^= (uint32_t)k.z * 0xC2B2AE3Du;
+  h ^= (uint32_t)k.w * 0x27D4EB2Fu;
+  return fmix32(h);
+}
+__device__ __forceinline__ bool key_eq(int4 a, int4 b) { return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w; }
+
+// Block key of a particle that binning accepted (cell valid).
+__device__ __forceinline__ int4 particle_key(const Parts& p, const DevClass* classes, const DevScene* scenes,
+                                             uint32_t i) {
+  float4 pm = p.pos_mass[i];
+  const DevClass& c = classes[p.cls[i]];
+  float inv_h = scenes[c.region >> 16].inv_h;
+  int3 cell;
+  cell3(pm, inv_h, cell);
+  return make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
+}
+
+__device__ __forceinline__ int4 owner_key(const Parts& p, const Topo& t, const DevClass* classes,
+                                          const DevScene* scenes, uint32_t owner) {
+  if (owner <= p.n) return particle_key(p, classes, scenes, owner - 1);
+  uint32_t r = owner - p.n - 1;
+  uint32_t bucket = r >> 3, d = r & 7;
+  int4 k = particle_key(p, classes, scenes, t.owner[bucket] - 1);
+  k.x += (int)(d >> 2);
+  k.y += (int)((d >> 1) & 1);
+  k.z += (int)(d & 1);
+  return k;
+}
+
+// Lock-free linear-probing insert.  Owners refer to keys that are immutable
+// during the kernel (particle positions, or a particle block's key plus a
+// fixed offset), so a losing thread compares against the winner's key
+// without waiting for anything to be published.  Bounded by the table size.
+__device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
+                                int4 key, uint32_t me) {
+  uint32_t b = hash_key(key) & t.bucket_mask;
+  for (uint32_t probe = 0; probe <= t.bucket_mask; ++probe) {
+    uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
+    if (prev == 0u) return b;
+    if (key_eq(owner_key(p, t, classes, scenes, prev), key)) return b;
+    b = (b + 1) & t.bucket_mask;
+  }
+  return kInvalid;
+}
+
+// ---------------------------------------------------------------------------
+// 1. reset
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
+  uint32_t nb = t.bucket_mask + 1;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    t.owner[b] = 0;
+    t.bucket_count[b] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < CNT_COUNT && threadIdx.x != CNT_ERROR) t.counters[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// 2. particle -> block insert + counting-sort rank
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
+                                                    const DevScene* scenes) {
+  const uint32_t lane = lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t base = blockIdx.x * blockDim.x; base < p.n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool active = false;
+    int4 key = make_int4(0, 0, 0, 0);
+    if (i < p.n) {
+      float4 pm = p.pos_mass[i];
+      if (pm.w > 0.0f) {
+        const DevClass& c = classes[p.cls[i]];
+        int3 cell;
+        bool ok = cell3(pm, scenes[c.region >> 16].inv_h, cell);
+        if (ok) {
+          active = true;
+          key = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
+        } else {
+          atomicOr(&t.counters[CNT_ERROR], ERR_RESIDENT);
+        }
+      }
+      t.pslot[i] = kInvalid;
+    }
+    uint64_t pending = __ballot(active);
+    while (pending) {
+      int leader = __ffsll((unsigned long long)pending) - 1;
+      int4 lk = make_int4(readlane_i(key.x, leader), readlane_i(key.y, leader), readlane_i(key.z, leader),
+                          readlane_i(key.w, leader));
+      bool mine = ((pending >> lane) & 1ull) && key_eq(key, lk);
+      uint64_t group = __ballot(mine);
+      uint32_t bucket = 0, first = 0;
+      if ((int)lane == leader) {
+        bucket = hash_insert(p, t, classes, scenes, key, i + 1);
+        if (bucket != kInvalid) first = atomicAdd(&t.bucket_count[bucket], (uint32_t)__popcll(group));
+      }
+      bucket = readlane_u(bucket, leader);
+      first = readlane_u(first, leader);
+      if (mine) {
+        if (bucket != kInvalid) {
+          t.pslot[i] = bucket;
+          t.prank[i] = first + (uint32_t)__popcll(group & below);
+        }
+      }
+      if (bucket == kInvalid && (int)lane == leader) atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
+      pending &= ~group;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. neighbour blocks of every particle block
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bin_neighbors(Parts p, Topo t, const DevClass* classes,
+                                                       const DevScene* scenes) {
+  uint32_t nb = t.bucket_mask + 1;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    uint32_t o = t.owner[b];
+    if (o == 0 || o > p.n) continue;  // empty, or claimed by a neighbour insert of this pass
+    int4 k = particle_key(p, classes, scenes, o - 1);
+    t.nbr_bucket[8 * b] = b;
+    for (uint32_t d = 1; d < 8; ++d) {
+      int4 kk = make_int4(k.x + (int)(d >> 2), k.y + (int)((d >> 1) & 1), k.z + (int)(d & 1), k.w);
+      uint32_t got = hash_insert(p, t, classes, scenes, kk, p.n + 1 + 8 * b + d);
+      if (got == kInvalid) atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
+      t.nbr_bucket[8 * b + d] = got;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. three-phase scan over buckets
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bucket_flags(const Parts& p, const Topo& t, uint32_t b, uint32_t& g, uint32_t& pb,
+                                             uint32_t& cnt) {
+  uint32_t o = t.owner[b];
+  g = o != 0 ? 1u : 0u;
+  pb = (o != 0 && o <= p.n) ? 1u : 0u;
+  cnt = pb ? t.bucket_count[b] : 0u;
+}
+
+// Workgroup-wide inclusive scan of 3 values per thread (256 threads).
+__device__ __forceinline__ void block_scan3(uint32_t v[3], uint32_t total[3]) {
+  __shared__ uint32_t s[3][256];
+  for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    uint32_t add[3] = {0, 0, 0};
+    if (threadIdx.x >= (unsigned)off)
+      for (int k = 0; k < 3; ++k) add[k] = s[k][threadIdx.x - off];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += add[k];
+    __syncthreads();
+  }
+  for (int k = 0; k < 3; ++k) {
+    v[k] = s[k][threadIdx.x];
+    total[k] = s[k][255];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(Parts p, Topo t) {
+  for (uint32_t c = blockIdx.x; c < t.n_chunks; c += gridDim.x) {
+    uint32_t acc[3] = {0, 0, 0};
+    for (int q = 0; q < kScanChunk / 256; ++q) {
+      uint32_t b = c * kScanChunk + threadIdx.x * (kScanChunk / 256) + q;
+      uint32_t g, pb, cnt;
+      bucket_flags(p, t, b, g, pb, cnt);
+      acc[0] += g;
+      acc[1] += pb;
+      acc[2] += cnt;
+    }
+    uint32_t tot[3];
+    block_scan3(acc, tot);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) t.chunk_sums[3 * c + k] = tot[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(Topo t) {
+  // single workgroup: exclusive scan of chunk sums in place; totals appended
+  uint32_t carry[3] = {0, 0, 0};
+  for (uint32_t base = 0; base < t.n_chunks; base += 256) {
+    uint32_t c = base + threadIdx.x;
+    uint32_t v[3] = {0, 0, 0};
+    if (c < t.n_chunks)
+      for (int k = 0; k < 3; ++k) v[k] = t.chunk_sums[3 * c + k];
+    uint32_t incl[3] = {v[0], v[1], v[2]}, tot[3];
+    block_scan3(incl, tot);
+    if (c < t.n_chunks)
+      for (int k = 0; k < 3; ++k) t.chunk_sums[3 * c + k] = carry[k] + incl[k] - v[k];
+    for (int k = 0; k < 3; ++k) carry[k] += tot[k];
+  }
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 3; ++k) t.chunk_sums[3 * t.n_chunks + k] = carry[k];
+    uint32_t err = 0;
+    if (carry[0] > t.grid_cap || carry[1] > t.pblock_cap) err |= ERR_TOPOLOGY;
+    t.counters[CNT_GRID_BLOCKS] = carry[0] < t.grid_cap ? carry[0] : t.grid_cap;
+    t.counters[CNT_PARTICLE_BLOCKS] = carry[1] < t.pblock_cap ? carry[1] : t.pblock_cap;
+    t.counters[CNT_BINNED] = carry[2];
+    if (err) atomicOr(&t.counters[CNT_ERROR], err);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(Parts p, Topo t, const DevClass* classes,
+                                                    const DevScene* scenes) {
+  for (uint32_t c = blockIdx.x; c < t.n_chunks; c += gridDim.x) {
+    constexpr int per = kScanChunk / 256;
+    uint32_t fl[per][3];
+    uint32_t acc[3] = {0, 0, 0};
+    for (int q = 0; q < per; ++q) {
+      uint32_t b = c * kScanChunk + threadIdx.x * per + q;
+      bucket_flags(p, t, b, fl[q][0], fl[q][1], fl[q][2]);
+      for (int k = 0; k < 3; ++k) acc[k] += fl[q][k];
+    }
+    uint32_t incl[3] = {acc[0], acc[1], acc[2]}, tot[3];
+    block_scan3(incl, tot);
+    uint32_t run[3];
+    for (int k = 0; k < 3; ++k) run[k] = t.chunk_sums[3 * c + k] + incl[k] - acc[k];
+    for (int q = 0; q < per; ++q) {
+      uint32_t b = c * kScanChunk + threadIdx.x * per + q;
+      uint32_t slot = kInvalid, pbid = kInvalid;
+      if (fl[q][0]) {
+        if (run[0] < t.grid_cap) {
+          slot = run[0];
+          t.grid_key[slot] = owner_key(p, t, classes, scenes, t.owner[b]);
+        }
+      }
+      if (fl[q][1]) {
+        if (run[1] < t.pblock_cap) {
+          pbid = run[1];
+          t.pblock_bucket[pbid] = b;
+          t.pblock_start[pbid] = run[2];
+          t.pblock_len[pbid] = fl[q][2];
+        }
+      }
+      t.bucket_slot[b] = slot;
+      t.bucket_pblock[b] = pbid;
+      for (int k = 0; k < 3; ++k) run[k] += fl[q][k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. per particle block: grid slots of the 8 tile blocks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pblock_finalize(Topo t) {
+  uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npb * 8; q += gridDim.x * blockDim.x) {
+    uint32_t k = q >> 3, d = q & 7;
+    uint32_t b = t.pblock_bucket[k];
+    uint32_t nb = t.nbr_bucket[8 * b + d];
+    t.pblock_slot8[q] = nb == kInvalid ? kInvalid : t.bucket_slot[nb];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. scatter particle indices into block-sorted order (optionally the state)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    uint32_t b = t.pslot[i];
+    if (b == kInvalid) continue;
+    uint32_t k = t.bucket_pblock[b];
+    if (k == kInvalid) continue;
+    t.sorted[t.pblock_start[k] + t.prank[i]] = i;
+  }
+}
+
+// Reorder variant: particle i's whole state moves to its sorted position in
+// the destination arrays `q`; sorted[] becomes the identity.  Particles that
+// were not binned (disabled) keep their slots in the tail [binned, n).
+__global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo t, uint32_t tail_begin) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    uint32_t dst = kInvalid;
+    uint32_t b = t.pslot[i];
+    if (b != kInvalid) {
+      uint32_t k = t.bucket_pblock[b];
+      if (k != kInvalid) dst = t.pblock_start[k] + t.prank[i];
+    }
+    if (dst == kInvalid) {
+      if (i < tail_begin) continue;  // failed enabled particle: state is discarded with the batch
+      dst = i;
+    } else {
+      t.sorted[dst] = dst;
+    }
+    q.pos_mass[dst] = p.pos_mass[i];
+    q.vel_vol[dst] = p.vel_vol[i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      q.C[k * q.stride + dst] = p.C[k * p.stride + i];
+      q.F[k * q.stride + dst] = p.F[k * p.stride + i];
+    }
+    q.cls[dst] = p.cls[i];
+    q.ins_id[dst] = p.ins_id[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 7. zero the accumulator of live grid slots
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grid_clear(Topo t, int fixed64) {
+  uint32_t ng = t.counters[CNT_GRID_BLOCKS];
+  uint32_t words = ng * kBlockNodes * (fixed64 ? 2u : 1u);  // float4 or 2 x (2 x int64)
+  float4* acc = reinterpret_cast<float4*>(t.grid_acc);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < words; q += gridDim.x * blockDim.x)
+    acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_mask_clear(Topo t) {
+  uint32_t ng = t.counters[CNT_GRID_BLOCKS];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * ng; q += gridDim.x * blockDim.x)
+    t.node_mask[q] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) t.counters[CNT_LIVE_NODES] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Material and geometry helpers (f32)
+// ---------------------------------------------------------------------------
+struct M3 {
+  float a[9];  // column-major: a[c*3 + r]
+  __device__ float& at(int r, int c) { return a[c * 3 + r]; }
+  __device__ float at(int r, int c) const { return a[c * 3 + r]; }
+};
+__device__ __forceinline__ float3 col(const M3& m, int c) { return make_float3(m.a[3 * c], m.a[3 * c + 1], m.a[3 * c + 2]); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 operator/(float3 a, float s) { return make_float3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len3(float3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ float comp(float3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ void set_comp(float3& v, int i, float s) {
+  if (i == 0) v.x = s; else if (i == 1) v.y = s; else v.z = s;
+}
+
+__device__ __forceinline__ float det3(const M3& m) {
+  return dot3(col(m, 0), cross3(col(m, 1), col(m, 2)));
+}
+
+// gpu.wgsl:451-465: the corotated stress uses the f32 Newton polar iteration
+// only where it converges in 10 steps.
+__device__ __forceinline__ bool regular_corotated(const M3& F) {
+  float3 a = col(F, 0), b = col(F, 1), c = col(F, 2);
+  float det = dot3(a, cross3(b, c));
+  float nrm = sqrtf(dot3(a, a) + dot3(b, b) + dot3(c, c));
+  if (fabsf(det) < 1e-5f || nrm < 1e-5f || nrm > 20.0f) return false;
+  float3 i0 = cross3(b, c) / det, i1 = cross3(c, a) / det, i2 = cross3(a, b) / det;
+  float inrm = sqrtf(dot3(i0, i0) + dot3(i1, i1) + dot3(i2, i2));
+  return nrm * inrm <= 100.0f;
+}
+
+// Newton iteration R <- (R + R^-T) / 2 (gpu.wgsl:467-482).
+__device__ void polar_newton(const M3& F, M3& R) {
+  R = F;
+  for (int it = 0; it < 10; ++it) {
+    float3 a = col(R, 0), b = col(R, 1), c = col(R, 2);
+    float det = dot3(a, cross3(b, c));
+    float inv = 1.0f / det;
+    float3 t0 = cross3(b, c) * inv, t1 = cross3(c, a) * inv, t2 = cross3(a, b) * inv;
+    R.a[0] = 0.5f * (a.x + t0.x); R.a[1] = 0.5f * (a.y + t0.y); R.a[2] = 0.5f * (a.z + t0.z);
+    R.a[3] = 0.5f * (b.x + t1.x); R.a[4] = 0.5f * (b.y + t1.y); R.a[5] = 0.5f * (b.z + t1.z);
+    R.a[6] = 0.5f * (c.x + t2.x); R.a[7] = 0.5f * (c.y + t2.y); R.a[8] = 0.5f * (c.z + t2.z);
+  }
+}
+
+// FP64 fallback for near-degenerate F: one-sided Jacobi SVD, R = U V^T (the
+// rotation the reference CPU takes from nalgebra's SVD, material.rs:447-450).
+__device__ __noinline__ void polar_fp64(const M3& Ff, M3& R) {
+  double A[3][3], V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) A[r][c] = (double)Ff.at(r, c);
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (int i = 0; i < 3; ++i) {
+          alpha += A[i][p] * A[i][p];
+          beta += A[i][q] * A[i][q];
+          gamma += A[i][p] * A[i][q];
+        }
+        if (fabs(gamma) <= 1e-300) continue;
+        double zeta = (beta - alpha) / (2.0 * gamma);
+        double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
+        for (int i = 0; i < 3; ++i) {
+          double ap = A[i][p], aq = A[i][q];
+          A[i][p] = cs * ap - sn * aq;
+          A[i][q] = sn * ap + cs * aq;
+          double vp = V[i][p], vq = V[i][q];
+          V[i][p] = cs * vp - sn * vq;
+          V[i][q] = sn * vp + cs * vq;
+        }
+      }
+  }
+  double s[3], U[3][3], smax = 0;
+  for (int j = 0; j < 3; ++j) {
+    s[j] = sqrt(A[0][j] * A[0][j] + A[1][j] * A[1][j] + A[2][j] * A[2][j]);
+    smax = fmax(smax, s[j]);
+  }
+  int good[3], ng = 0, bad = -1, nbad = 0;
+  for (int j = 0; j < 3; ++j) {
+    if (s[j] > smax * 1e-12 && s[j] > 0) {
+      for (int i = 0; i < 3; ++i) U[i][j] = A[i][j] / s[j];
+      good[ng++] = j;
+    } else {
+      bad = j;
+      ++nbad;
+    }
+  }
+  if (nbad == 1) {
+    int a = good[0], b = good[1];
+    U[0][bad] = U[1][a] * U[2][b] - U[2][a] * U[1][b];
+    U[1][bad] = U[2][a] * U[0][b] - U[0][a] * U[2][b];
+    U[2][bad] = U[0][a] * U[1][b] - U[1][a] * U[0][b];
+  } else if (nbad > 1) {
+    // Rank <= 1: no unique rotation; use the nearest rotation to V's frame.
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) U[r][c] = V[r][c];
+  }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) R.at(r, c) = (float)(U[r][0] * V[c][0] + U[r][1] * V[c][1] + U[r][2] * V[c][2]);
+}
+
+// Kirchhoff stress (material.rs:234-284; f32 op order of gpu.wgsl:638-699).
+__device__ void kirchhoff(const DevClass& c, const M3& F, const M3& C, M3& S) {
+  float j = det3(F);
+  if (c.kind == DM_NEO_HOOKEAN) {
+    float iso = c.lambda * logf(fmaxf(j, 1e-10f)) - c.mu;
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) {
+        float s = F.at(r, 0) * F.at(q, 0) + F.at(r, 1) * F.at(q, 1) + F.at(r, 2) * F.at(q, 2);
+        S.at(r, q) = s * c.mu + (r == q ? iso : 0.0f);
+      }
+  } else if (c.kind == DM_FLUID) {
+    float sj = fmaxf(j, 1e-6f);
+    float pressure = sj <= 1.0f ? c.bulk * (powf(sj, -c.gamma) - 1.0f) : -c.bulk * c.tensile * (sj - 1.0f);
+    float tr3 = (C.at(0, 0) + C.at(1, 1) + C.at(2, 2)) / 3.0f;
+    float diagonal = -2.0f * c.viscosity * sj * tr3 - pressure * sj;
+    float k = 2.0f * c.viscosity * sj;
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) {
+        float sr = (C.at(r, q) + C.at(q, r)) * 0.5f;
+        S.at(r, q) = sr * k + (r == q ? diagonal : 0.0f);
+      }
+  } else {
+    M3 R;
+    if (regular_corotated(F)) polar_newton(F, R);
+    else polar_fp64(F, R);
+    float iso = c.lambda * (j - 1.0f) * j;
+    float k = 2.0f * c.mu;
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) {
+        float s = (F.at(r, 0) - R.at(r, 0)) * F.at(q, 0) + (F.at(r, 1) - R.at(r, 1)) * F.at(q, 1) +
+                  (F.at(r, 2) - R.at(r, 2)) * F.at(q, 2);
+        S.at(r, q) = s * k + (r == q ? iso : 0.0f);
+      }
+  }
+}
+
+__device__ __forceinline__ void quadratic_weights(float fx, float w[3]) {
+  float t0 = 1.5f - fx, t1 = fx - 1.0f, t2 = fx - 0.5f;
+  w[0] = 0.5f * t0 * t0;
+  w[1] = 0.75f - t1 * t1;
+  w[2] = 0.5f * t2 * t2;
+}
+
+// Obstacles (obstacle.rs:373-539), f32.
+__device__ __forceinline__ float3 qrot(const float q[4], float3 v) {
+  float3 qv = make_float3(q[0], q[1], q[2]);
+  float3 t = cross3(qv, v) * 2.0f;
+  return v + t * q[3] + cross3(qv, t);
+}
+__device__ __forceinline__ float3 qrot_inv(const float q[4], float3 v) {
+  float qc[4] = {-q[0], -q[1], -q[2], q[3]};
+  return qrot(qc, v);
+}
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+__device__ void obstacle_surface(const DevObstacle& o, float3 point, float& dist, float3& normal) {
+  float3 rel = point - make_float3(o.center[0], o.center[1], o.center[2]);
+  const float* d = o.dims;
+  switch (o.shape) {
+    case 0: {  // sphere
+      float n = len3(rel);
+      normal = n > 1e-12f ? rel / n : make_float3(0, 0, 1);
+      dist = n - d[0];
+      return;
+    }
+    case 1: {  // box
+      float3 l = qrot_inv(o.q, rel);
+      float3 cl = make_float3(clampf(l.x, -d[0], d[0]), clampf(l.y, -d[1], d[1]), clampf(l.z, -d[2], d[2]));
+      float3 out = l - cl;
+      float n = len3(out);
+      if (n > 1e-12f) {
+        dist = n;
+        normal = qrot(o.q, out / n);
+      } else {
+        float3 dd = make_float3(d[0] - fabsf(l.x), d[1] - fabsf(l.y), d[2] - fabsf(l.z));
+        int axis = (dd.x <= dd.y && dd.x <= dd.z) ? 0 : (dd.y <= dd.z ? 1 : 2);
+        float3 nn = make_float3(0, 0, 0);
+        set_comp(nn, axis, comp(l, axis) < 0.0f ? -1.0f : 1.0f);
+        dist = -comp(dd, axis);
+        normal = qrot(o.q, nn);
+      }
+      return;
+    }
+    case 2: {  // capsule
+      float3 l = qrot_inv(o.q, rel);
+      float cz = clampf(l.z, -d[0], d[0]);
+      float3 delta = l - make_float3(0, 0, cz);
+      float n = len3(delta);
+      float eps = fmaxf(d[0], d[1]) * 1e-6f;
+      float3 nn = n > eps ? delta / n : make_float3(1, 0, 0);
+      dist = n - d[1];
+      normal = qrot(o.q, nn);
+      return;
+    }
+    case 3: {  // cylinder
+      float3 l = qrot_inv(o.q, rel);
+      float hh = d[0], r = d[1];
+      float radial = sqrtf(l.x * l.x + l.y * l.y);
+      float3 rn = radial > 1e-12f ? make_float3(l.x / radial, l.y / radial, 0.0f) : make_float3(1, 0, 0);
+      float rd = radial - r;
+      float cd = fabsf(l.z) - hh;
+      float eps = fmaxf(hh, r) * 1e-6f;
+      float3 nn;
+      if (fabsf(rd) <= eps && fabsf(cd) <= eps) {
+        float3 capn = make_float3(0, 0, l.z < 0.0f ? -1.0f : 1.0f);
+        float3 s = rn + capn;
+        dist = fmaxf(rd, cd);
+        nn = s / len3(s);
+      } else {
+        float ro = fmaxf(rd, 0.0f), co = fmaxf(cd, 0.0f);
+        float3 out = rn * ro + make_float3(0, 0, l.z < 0.0f ? -co : co);
+        float ol = len3(out);
+        if (ol > 1e-12f) {
+          dist = ol;
+          nn = out / ol;
+        } else if (rd >= cd) {
+          dist = rd;
+          nn = rn;
+        } else {
+          dist = cd;
+          nn = make_float3(0, 0, l.z < 0.0f ? -1.0f : 1.0f);
+        }
+      }
+      normal = qrot(o.q, nn);
+      return;
+    }
+    case 4: {  // cone
+      float3 l = qrot_inv(o.q, rel);
+      float hh = d[0], r = d[1];
+      float radial = sqrtf(l.x * l.x + l.y * l.y);
+      float3 rn = radial > 1e-12f ? make_float3(l.x / radial, l.y / radial, 0.0f) : make_float3(1, 0, 0);
+      float px = radial, pz = l.z;
+      float bx = clampf(radial, 0.0f, r), bz = -hh;
+      float sx = r, sz = -hh, dx = -r, dz = 2.0f * hh;
+      float tt = clampf(((px - sx) * dx + (pz - sz) * dz) / (dx * dx + dz * dz), 0.0f, 1.0f);
+      float cx = sx + dx * tt, cz = sz + dz * tt;
+      float eps = fmaxf(hh, r) * 1e-6f;
+      float bd2 = (px - bx) * (px - bx) + (pz - bz) * (pz - bz);
+      float sd2 = (px - cx) * (px - cx) + (pz - cz) * (pz - cz);
+      bool use_base = bd2 <= sd2;
+      float clx = use_base ? bx : cx, clz = use_base ? bz : cz;
+      float ddx = px - clx, ddz = pz - clz;
+      float nr = sqrtf(ddx * ddx + ddz * ddz);
+      bool inside = l.z >= -hh - eps && l.z <= hh + eps && radial <= r * (hh - l.z) / (2.0f * hh) + eps;
+      float snl = sqrtf(4.0f * hh * hh + r * r);
+      float snx = 2.0f * hh / snl, snz = r / snl;
+      bool corner = bd2 <= eps * eps && sd2 <= eps * eps;
+      float fnx, fnz;
+      if (corner) {
+        float ax = snx, az = -1.0f + snz;
+        float al = sqrtf(ax * ax + az * az);
+        fnx = ax / al;
+        fnz = az / al;
+      } else if (use_base) {
+        fnx = 0.0f;
+        fnz = -1.0f;
+      } else {
+        fnx = snx;
+        fnz = snz;
+      }
+      float n2x, n2z;
+      if (inside || nr <= eps) {
+        n2x = fnx;
+        n2z = fnz;
+      } else {
+        n2x = ddx / nr;
+        n2z = ddz / nr;
+      }
+      normal = qrot(o.q, rn * n2x + make_float3(0, 0, n2z));
+      dist = inside ? -nr : nr;
+      return;
+    }
+    default: {  // 5: ground
+      float3 l = qrot_inv(o.q, rel);
+      float ox = l.x - clampf(l.x, -d[0], d[0]);
+      float oy = l.y - clampf(l.y, -d[1], d[1]);
+      float horizontal = sqrtf(ox * ox + oy * oy);
+      if (horizontal <= 1e-12f) {
+        dist = l.z;
+        normal = qrot(o.q, make_float3(0, 0, 1));
+      } else {
+        float3 out = make_float3(ox, oy, l.z);
+        float dd = len3(out);
+        dist = dd;
+        normal = qrot(o.q, out / dd);
+      }
+      return;
+    }
+  }
+}
+
+__device__ __forceinline__ bool obstacle_may_contact(const DevObstacle& o, float3 point, float margin) {
+  if (o.shape == 2) {
+    float3 rel = point - make_float3(o.center[0], o.center[1], o.center[2]);
+    float b = o.dims[0] + o.dims[1] + margin;
+    return dot3(rel, rel) <= b * b;
+  }
+  return true;
+}
+
+__device__ __forceinline__ float3 contact_velocity(const DevObstacle& o, float3 point, float3 v, float3 n) {
+  float3 c = make_float3(o.center[0], o.center[1], o.center[2]);
+  float3 sv = make_float3(o.lin[0], o.lin[1], o.lin[2]) + cross3(make_float3(o.ang[0], o.ang[1], o.ang[2]), point - c);
+  if (o.boundary == 3) return v;
+  if (o.boundary == 1) return sv;
+  float3 rel = v - sv;
+  float ns = dot3(rel, n);
+  if (o.boundary == 4) return v - n * ns;
+  if (ns >= 0.0f) return v;
+  float3 tangent = rel - n * ns;
+  if (o.boundary == 2) return sv + tangent;
+  float ts = len3(tangent);
+  float fs = ts > 1e-12f ? fmaxf(1.0f - o.friction * -ns / ts, 0.0f) : 0.0f;
+  return sv + tangent * fs;
+}
+
+__device__ __forceinline__ bool finite_abs(float v) { return fabsf(v) < 1e30f; }
+
+// ---------------------------------------------------------------------------
+// 8. P2G
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int tile_index(int x, int y, int z) { return (x * kTileEdge + y) * kTileEdge + z; }
+__device__ __forceinline__ void tile_node(int t, int& d, int& local) {
+  int x = t / 36, y = (t / 6) % 6, z = t % 6;
+  d = ((x >> 2) << 2) | ((y >> 2) << 1) | (z >> 2);
+  local = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+}
+
+template <int FIXED>
+struct TileAcc;
+template <>
+struct TileAcc<0> {
+  using T = float;
+};
+template <>
+struct TileAcc<1> {
+  using T = unsigned long long;
+};
+
+template <int FIXED>
+__global__ __launch_bounds__(256) void k_p2g(Parts p, Topo t, const DevClass* classes, const DevScene* scenes) {
+  using T = typename TileAcc<FIXED>::T;
+  __shared__ T tile[4 * kTileNodes];  // [component][node]
+  __shared__ uint32_t s_slot[8];
+  const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
+  for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
+    for (int q = threadIdx.x; q < 4 * kTileNodes; q += blockDim.x) tile[q] = (T)0;
+    if (threadIdx.x < 8) s_slot[threadIdx.x] = t.pblock_slot8[8 * k + threadIdx.x];
+    __syncthreads();
+    const uint32_t own = s_slot[0];
+    if (own != kInvalid) {
+      const int4 key = t.grid_key[own];
+      const DevScene& sc = scenes[(uint32_t)key.w >> 16];
+      const float h = sc.h, inv_h = sc.inv_h;
+      const float stress_scale = -sc.dt * sc.inv_d;
+      const float mscale = FIXED ? exp2f((float)sc.mass_shift) : 1.0f;
+      const float pscale = FIXED ? exp2f((float)sc.mom_shift) : 1.0f;
+      const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
+      const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
+      bool bad = false, range = false;
+      for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
+        const uint32_t i = t.sorted[start + j];
+        const float4 pm = p.pos_mass[i];
+        const float4 vv = p.vel_vol[i];
+        const DevClass& cl = classes[p.cls[i]];
+        M3 F, C;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          F.a[q] = p.F[q * p.stride + i];
+          C.a[q] = p.C[q * p.stride + i];
+        }
+        int3 cell;
+        cell3(pm, inv_h, cell);
+        const int lx = cell.x - ox, ly = cell.y - oy, lz = cell.z - oz;
+        if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
+          bad = true;
+          continue;
+        }
+        const float fx = pm.x * inv_h - (float)cell.x;
+        const float fy = pm.y * inv_h - (float)cell.y;
+        const float fz = pm.z * inv_h - (float)cell.z;
+        float wx[3], wy[3], wz[3];
+        quadratic_weights(fx, wx);
+        quadratic_weights(fy, wy);
+        quadratic_weights(fz, wz);
+        M3 S;
+        kirchhoff(cl, F, C, S);
+        const float m = pm.w;
+        const float sscale = stress_scale * vv.w;
+        // A = m C + stress * (-dt V0 inv_d), columns
+        float3 a0 = col(C, 0) * m + col(S, 0) * sscale;
+        float3 a1 = col(C, 1) * m + col(S, 1) * sscale;
+        float3 a2 = col(C, 2) * m + col(S, 2) * sscale;
+        const float3 mv = make_float3(vv.x, vv.y, vv.z) * m;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+          for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int z = 0; z < 3; ++z) {
+              const float w = wx[x] * wy[y] * wz[z];
+              const float dx = ((float)x - fx) * h, dy = ((float)y - fy) * h, dz = ((float)z - fz) * h;
+              float3 am = a0 * dx + a1 * dy + a2 * dz;
+              float3 mom = (mv + am) * w;
+              const float wm = w * m;
+              const int n = tile_index(lx + x, ly + y, lz + z);
+              if (FIXED) {
+                float sm = wm * mscale, sx = mom.x * pscale, sy = mom.y * pscale, sz = mom.z * pscale;
+                const float lim = 35184372088832.0f;  // 2^45
+                range |= !(fabsf(sm) < lim && fabsf(sx) < lim && fabsf(sy) < lim && fabsf(sz) < lim);
+                atomicAdd(&tile[n], (T)__float2ll_rn(sm));
+                atomicAdd(&tile[kTileNodes + n], (T)__float2ll_rn(sx));
+                atomicAdd(&tile[2 * kTileNodes + n], (T)__float2ll_rn(sy));
+                atomicAdd(&tile[3 * kTileNodes + n], (T)__float2ll_rn(sz));
+              } else {
+                atomicAdd((float*)&tile[n], wm);
+                atomicAdd((float*)&tile[kTileNodes + n], mom.x);
+                atomicAdd((float*)&tile[2 * kTileNodes + n], mom.y);
+                atomicAdd((float*)&tile[3 * kTileNodes + n], mom.z);
+              }
+            }
+      }
+      flag_error(t.counters, bad, ERR_INTERNAL);
+      if (FIXED) flag_error(t.counters, range, ERR_FIXED_RANGE);
+    }
+    __syncthreads();
+    if (own != kInvalid) {
+      for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
+        T m = tile[n];
+        if (m == (T)0 && tile[kTileNodes + n] == (T)0 && tile[2 * kTileNodes + n] == (T)0 &&
+            tile[3 * kTileNodes + n] == (T)0)
+          continue;
+        int d, local;
+        tile_node(n, d, local);
+        uint32_t slot = s_slot[d];
+        if (slot == kInvalid) continue;
+        size_t node = (size_t)slot * kBlockNodes + local;
+        if (FIXED) {
+          unsigned long long* acc = reinterpret_cast<unsigned long long*>(t.grid_acc) + node * 4;
+          atomicAdd(acc, (unsigned long long)m);
+          atomicAdd(acc + 1, (unsigned long long)tile[kTileNodes + n]);
+          atomicAdd(acc + 2, (unsigned long long)tile[2 * kTileNodes + n]);
+          atomicAdd(acc + 3, (unsigned long long)tile[3 * kTileNodes + n]);
+        } else {
+          float* acc = reinterpret_cast<float*>(t.grid_acc) + node * 4;
+          unsafeAtomicAdd(acc, (float)m);
+          unsafeAtomicAdd(acc + 1, (float)tile[kTileNodes + n]);
+          unsafeAtomicAdd(acc + 2, (float)tile[2 * kTileNodes + n]);
+          unsafeAtomicAdd(acc + 3, (float)tile[3 * kTileNodes + n]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 9. grid update (world.rs:401-417; gpu.wgsl:772-819)
+// ---------------------------------------------------------------------------
+template <int FIXED>
+__global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* scenes, const DevObstacle* obstacles) {
+  const uint32_t ng = t.counters[CNT_GRID_BLOCKS];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = blockDim.x >> 6;
+  for (uint32_t s = blockIdx.x * waves + (threadIdx.x >> 6); s < ng; s += gridDim.x * waves) {
+    const int4 key = t.grid_key[s];
+    const DevScene& sc = scenes[(uint32_t)key.w >> 16];
+    const size_t node = (size_t)s * kBlockNodes + lane;
+    float m;
+    float3 mom;
+    if (FIXED) {
+      const long long* acc = reinterpret_cast<const long long*>(t.grid_acc) + node * 4;
+      const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
+      m = __ll2float_rn(acc[0]) * im;
+      mom = make_float3(__ll2float_rn(acc[1]) * ip, __ll2float_rn(acc[2]) * ip, __ll2float_rn(acc[3]) * ip);
+    } else {
+      float4 a = reinterpret_cast<const float4*>(t.grid_acc)[node];
+      m = a.x;
+      mom = make_float3(a.y, a.z, a.w);
+    }
+    float3 v = make_float3(0, 0, 0);
+    const bool live = m > 0.0f;
+    if (live) {
+      v = mom / m + make_float3(sc.gravity[0], sc.gravity[1], sc.gravity[2]) * sc.dt;
+      const int cx = key.x * kBlockCells + (int)(lane >> 4);
+      const int cy = key.y * kBlockCells + (int)((lane >> 2) & 3);
+      const int cz = key.z * kBlockCells + (int)(lane & 3);
+      const float3 pos = make_float3((float)cx * sc.h, (float)cy * sc.h, (float)cz * sc.h);
+      if (sc.has_bounds) {
+        for (int a = 0; a < 3; ++a) {
+          float pa = comp(pos, a), va = comp(v, a);
+          if ((pa <= sc.bmin[a] + sc.h && va < 0.0f) || (pa >= sc.bmax[a] - sc.h && va > 0.0f)) set_comp(v, a, 0.0f);
+        }
+      }
+      for (int oi = 0; oi < sc.obstacle_count; ++oi) {
+        const DevObstacle& o = obstacles[sc.obstacle_begin + oi];
+        if (!obstacle_may_contact(o, pos, sc.h)) continue;
+        float dist;
+        float3 nrm;
+        obstacle_surface(o, pos, dist, nrm);
+        if (dist <= sc.h * (1.0f + 1e-5f)) v = contact_velocity(o, pos, v, nrm);
+      }
+    }
+    t.grid_vel[node] = make_float4(v.x, v.y, v.z, 0.0f);
+    uint64_t lm = __ballot(live);
+    if (lane == 0 && lm) atomicAdd(&t.counters[CNT_MASS_NODES], (uint32_t)__popcll(lm));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 10. G2P (world.rs:419-464; gpu.wgsl:821-1019)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* classes, const DevScene* scenes,
+                                             const DevObstacle* obstacles) {
+  __shared__ float vt[3 * kTileNodes];
+  __shared__ uint32_t s_slot[8];
+  const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
+  for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
+    if (threadIdx.x < 8) s_slot[threadIdx.x] = t.pblock_slot8[8 * k + threadIdx.x];
+    __syncthreads();
+    const uint32_t own = s_slot[0];
+    for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
+      int d, local;
+      tile_node(n, d, local);
+      uint32_t slot = s_slot[d];
+      float4 v = make_float4(0, 0, 0, 0);
+      if (own != kInvalid && slot != kInvalid) v = t.grid_vel[(size_t)slot * kBlockNodes + local];
+      vt[n] = v.x;
+      vt[kTileNodes + n] = v.y;
+      vt[2 * kTileNodes + n] = v.z;
+    }
+    __syncthreads();
+    if (own != kInvalid) {
+      const int4 key = t.grid_key[own];
+      const DevScene& sc = scenes[(uint32_t)key.w >> 16];
+      const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
+      const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
+      const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
+      bool failed = false, bad = false;
+      for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
+        const uint32_t i = t.sorted[start + j];
+        const float4 pm = p.pos_mass[i];
+        const DevClass& cl = classes[p.cls[i]];
+        if (cl.fixed) {
+          const float4 vv = p.vel_vol[i];
+          p.vel_vol[i] = make_float4(0.f, 0.f, 0.f, vv.w);
+          if (sc.apply_force) p.force[p.ins_id[i]] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < 9; ++q) p.C[q * p.stride + i] = 0.0f;
+          continue;
+        }
+        int3 cell;
+        cell3(pm, inv_h, cell);
+        const int lx = cell.x - ox, ly = cell.y - oy, lz = cell.z - oz;
+        if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
+          bad = true;
+          continue;
+        }
+        const float fx = pm.x * inv_h - (float)cell.x;
+        const float fy = pm.y * inv_h - (float)cell.y;
+        const float fz = pm.z * inv_h - (float)cell.z;
+        float wx[3], wy[3], wz[3];
+        quadratic_weights(fx, wx);
+        quadratic_weights(fy, wy);
+        quadratic_weights(fz, wz);
+        float3 v = make_float3(0, 0, 0), c0 = v, c1 = v, c2 = v;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+          for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int z = 0; z < 3; ++z) {
+              const float w = wx[x] * wy[y] * wz[z];
+              const int n = tile_index(lx + x, ly + y, lz + z);
+              const float3 nv = make_float3(vt[n], vt[kTileNodes + n], vt[2 * kTileNodes + n]);
+              const float dx = ((float)x - fx) * h, dy = ((float)y - fy) * h, dz = ((float)z - fz) * h;
+              v = v + nv * w;
+              c0 = c0 + nv * (dx * w * inv_d);
+              c1 = c1 + nv * (dy * w * inv_d);
+              c2 = c2 + nv * (dz * w * inv_d);
+            }
+        if (sc.apply_force) {
+          const uint32_t ii = p.ins_id[i];
+          const float4 f = p.force[ii];
+          v = v + make_float3(f.x, f.y, f.z) / pm.w * dt;
+          p.force[ii] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        v = v * expf(-cl.damping * dt);
+        float3 x = make_float3(pm.x, pm.y, pm.z) + v * dt;
+        const float radius = cl.radius;
+        if (sc.has_bounds) {
+          for (int a = 0; a < 3; ++a) {
+            float lower = sc.bmin[a] + radius, upper = sc.bmax[a] - radius;
+            float xa = comp(x, a), va = comp(v, a);
+            if (xa < lower) {
+              set_comp(x, a, lower);
+              set_comp(v, a, fmaxf(va, 0.0f));
+            } else if (xa > upper) {
+              set_comp(x, a, upper);
+              set_comp(v, a, fminf(va, 0.0f));
+            }
+          }
+        }
+        for (int oi = 0; oi < sc.obstacle_count; ++oi) {
+          const DevObstacle& o = obstacles[sc.obstacle_begin + oi];
+          if (!obstacle_may_contact(o, x, radius)) continue;
+          float dist;
+          float3 nrm;
+          obstacle_surface(o, x, dist, nrm);
+          if (dist < radius) {
+            x = x + nrm * (radius - dist);
+            v = contact_velocity(o, x, v, nrm);
+          }
+        }
+        // F <- (I + C dt) F
+        M3 Fo, Fn;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Fo.a[q] = p.F[q * p.stride + i];
+        const float3 cc[3] = {c0, c1, c2};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            float s = Fo.at(r, q);
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) s += comp(cc[kk], r) * dt * Fo.at(kk, q);
+            Fn.at(r, q) = s;
+          }
+        if (cl.kind == DM_FLUID) {
+          const float sc3 = cbrtf(fmaxf(det3(Fn), 1e-9f));
+#pragma unroll
+          for (int q = 0; q < 9; ++q) Fn.a[q] = (q % 4 == 0) ? sc3 : 0.0f;
+        }
+        // Resident validity (gpu.wgsl:946-993).
+        int3 nb;
+        bool ok = cell3(make_float4(x.x, x.y, x.z, 0.f), inv_h, nb);
+        bool fin = finite_abs(x.x) && finite_abs(x.y) && finite_abs(x.z) && finite_abs(v.x) && finite_abs(v.y) &&
+                   finite_abs(v.z);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) fin = fin && finite_abs(Fn.a[q]);
+        fin = fin && finite_abs(c0.x) && finite_abs(c0.y) && finite_abs(c0.z) && finite_abs(c1.x) &&
+              finite_abs(c1.y) && finite_abs(c1.z) && finite_abs(c2.x) && finite_abs(c2.y) && finite_abs(c2.z);
+        bool out = !ok || nb.x < sc.first_base[0] || nb.y < sc.first_base[1] || nb.z < sc.first_base[2] ||
+                   nb.x > sc.last_base[0] || nb.y > sc.last_base[1] || nb.z > sc.last_base[2];
+        const float density = fmaxf(pm.w / p.vel_vol[i].w, 1e-12f);
+        const float wave = sqrtf(cl.wave_numerator / density);
+        const bool cfl = dt * (wave + len3(v)) > 0.4f * h * (1.0f + 1e-4f);
+        failed |= out || !fin || cfl;
+        p.pos_mass[i] = make_float4(x.x, x.y, x.z, pm.w);
+        const float vol = p.vel_vol[i].w;
+        p.vel_vol[i] = make_float4(v.x, v.y, v.z, vol);
+        p.C[0 * p.stride + i] = c0.x; p.C[1 * p.stride + i] = c0.y; p.C[2 * p.stride + i] = c0.z;
+        p.C[3 * p.stride + i] = c1.x; p.C[4 * p.stride + i] = c1.y; p.C[5 * p.stride + i] = c1.z;
+        p.C[6 * p.stride + i] = c2.x; p.C[7 * p.stride + i] = c2.y; p.C[8 * p.stride + i] = c2.z;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) p.F[q * p.stride + i] = Fn.a[q];
+      }
+      flag_error(t.counters, failed, ERR_RESIDENT);
+      flag_error(t.counters, bad, ERR_INTERNAL);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Snapshot export (gpu_snapshot.wgsl:6-11): 32-byte records, insertion order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_snapshot(Parts p, float4* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    uint32_t dst = p.ins_id[i];
+    out[2 * dst] = p.pos_mass[i];
+    out[2 * dst + 1] = p.vel_vol[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostic: exact live-node set (union of 27-node stencils) as per-block
+// 64-bit masks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_live_mask(Parts p, Topo t, const DevClass* classes, const DevScene* scenes) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    uint32_t b = t.pslot[i];
+    if (b == kInvalid) continue;
+    uint32_t k = t.bucket_pblock[b];
+    if (k == kInvalid) continue;
+    const float4 pm = p.pos_mass[i];
+    const DevClass& cl = classes[p.cls[i]];
+    const DevScene& sc = scenes[cl.region >> 16];
+    int3 cell;
+    cell3(pm, sc.inv_h, cell);
+    const int4 key = t.grid_key[t.pblock_slot8[8 * k]];
+    const int lx = cell.x - key.x * 4, ly = cell.y - key.y * 4, lz = cell.z - key.z * 4;
+    for (int x = 0; x < 3; ++x)
+      for (int y = 0; y < 3; ++y)
+        for (int z = 0; z < 3; ++z) {
+          int d, local;
+          tile_node(tile_index(lx + x, ly + y, lz + z), d, local);
+          uint32_t slot = t.pblock_slot8[8 * k + d];
+          if (slot == kInvalid) continue;
+          atomicOr(&t.node_mask[2 * slot + (local >> 5)], 1u << (local & 31));
+        }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_live_count(Topo t) {
+  uint32_t ng = t.counters[CNT_GRID_BLOCKS];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * ng; q += gridDim.x * blockDim.x) {
+    uint32_t c = __popc(t.node_mask[q]);
+    if (c) atomicAdd(&t.counters[CNT_LIVE_NODES], c);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------
+static inline uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t max_blocks) {
+  uint64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (uint32_t)b;
+}
+
+void launch_topo_clear(const Topo& t, hipStream_t s) {
+  hipLaunchKernelGGL(k_topo_clear, dim3(grid_for(t.bucket_mask + 1ull, 256, 4096)), dim3(256), 0, s, t);
+}
+void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_insert, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t, c, sc);
+}
+void launch_bin_neighbors(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_neighbors, dim3(grid_for(t.bucket_mask + 1ull, 256, 8192)), dim3(256), 0, s, p, t, c, sc);
+}
+void launch_scan(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
+  uint32_t g = grid_for(t.n_chunks, 1, 4096);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(256), 0, s, p, t);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, t);
+  hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(256), 0, s, p, t, c, sc);
+}
+void launch_pblock_finalize(const Topo& t, hipStream_t s) {
+  hipLaunchKernelGGL(k_pblock_finalize, dim3(grid_for(8ull * t.pblock_cap, 256, 4096)), dim3(256), 0, s, t);
+}
+void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t);
+}
+void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
+}
+void launch_grid_clear(const Topo& t, int fixed64, hipStream_t s) {
+  uint64_t words = (uint64_t)t.grid_cap * kBlockNodes * (fixed64 ? 2 : 1);
+  hipLaunchKernelGGL(k_grid_clear, dim3(grid_for(words, 256, 8192)), dim3(256), 0, s, t, fixed64);
+}
+void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
+                uint32_t threads, hipStream_t s) {
+  if (fixed64)
+    hipLaunchKernelGGL(k_p2g<1>, dim3(blocks), dim3(threads), 0, s, p, t, c, sc);
+  else
+    hipLaunchKernelGGL(k_p2g<0>, dim3(blocks), dim3(threads), 0, s, p, t, c, sc);
+}
+void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, hipStream_t s) {
+  uint32_t blocks = grid_for(t.grid_cap, 4, 16384);
+  if (fixed64)
+    hipLaunchKernelGGL(k_grid_update<1>, dim3(blocks), dim3(256), 0, s, t, sc, o);
+  else
+    hipLaunchKernelGGL(k_grid_update<0>, dim3(blocks), dim3(256), 0, s, t, sc, o);
+}
+void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
+                uint32_t blocks, uint32_t threads, hipStream_t s) {
+  hipLaunchKernelGGL(k_g2p, dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+}
+void launch_snapshot(const Parts& p, float4* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_snapshot, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, out);
+}
+void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_mask_clear, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);
+  hipLaunchKernelGGL(k_live_mask, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t, c, sc);
+  hipLaunchKernelGGL(k_live_count, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);
+}
+
+}  // namespace tmpm

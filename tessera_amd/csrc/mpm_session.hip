@@ -1,0 +1,1449 @@
+// Host runtime of the HIP MLS-MPM backend: the C ABI of include/tessera_mpm.h.
+//
+// Mirrors the reference's resident session (crates/tessera-mpm/src/gpu.rs:
+// 206-662): the session owns the last synchronized f64 world (insertion
+// order, the authoritative state), a device-resident f32 copy of the
+// particles, and the per-substep topology.  submit_steps() enqueues up to 64
+// substeps on the session's HIP stream without host round trips;
+// synchronize() validates the device flags, converts the device state back
+// to the f64 world and, on any failure, invalidates the session while keeping
+// the last synchronized world (gpu.rs:377-411).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/tessera_mpm.h"
+#include "mpm_device.h"
+#include "mpm_kernels.h"
+
+using namespace tmpm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+tmpm_status fail(tmpm_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(TMPM_READBACK, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Host-side restatement of the reference's validation rules.
+// ---------------------------------------------------------------------------
+inline bool finite3(const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+// material.rs:429-431
+bool valid_elasticity(double young, double poisson) {
+  return std::isfinite(young) && young > 0.0 && std::isfinite(poisson) && poisson >= -1.0 && poisson < 0.5;
+}
+// material.rs:172-231
+bool material_valid(const tmpm_material& m) {
+  const double* p = m.p;
+  switch (m.kind) {
+    case TMPM_MATERIAL_LINEAR_ELASTIC:
+    case TMPM_MATERIAL_NEO_HOOKEAN: return valid_elasticity(p[0], p[1]);
+    case TMPM_MATERIAL_SAND:
+    case TMPM_MATERIAL_SAND_NEO_HOOKEAN:
+      return valid_elasticity(p[0], p[1]) && std::isfinite(p[2]) && p[2] >= 0.0 && p[2] < M_PI / 2 &&
+             std::isfinite(p[3]) && p[3] >= 0.0;
+    case TMPM_MATERIAL_FLUID:
+      return std::isfinite(p[0]) && p[0] > 0 && std::isfinite(p[1]) && p[1] > 0 && std::isfinite(p[2]) && p[2] >= 0 &&
+             std::isfinite(p[3]) && p[3] >= 0;
+    case TMPM_MATERIAL_SNOW:
+      return valid_elasticity(p[0], p[1]) && std::isfinite(p[2]) && p[2] >= 0 && p[2] < 1.0 && std::isfinite(p[3]) &&
+             p[3] >= 0 && std::isfinite(p[4]) && p[4] >= 0;
+  }
+  return false;
+}
+// material.rs:433-438
+void lame(double young, double poisson, double& lambda, double& mu) {
+  lambda = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson));
+  mu = young / (2.0 * (1.0 + poisson));
+}
+double det_cm(const double* F) {  // nalgebra 3x3 determinant on column-major storage
+  double m11 = F[0], m21 = F[1], m31 = F[2], m12 = F[3], m22 = F[4], m32 = F[5], m13 = F[6], m23 = F[7], m33 = F[8];
+  double a = m22 * m33 - m32 * m23, b = m21 * m33 - m31 * m23, c = m21 * m32 - m31 * m22;
+  return m11 * a - m12 * b + m13 * c;
+}
+// material.rs:380-426
+double timestep_bound(const tmpm_material& m, double density, const double* v, double h, double det) {
+  const double* p = m.p;
+  double speed = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  double lambda, mu;
+  if (m.kind == TMPM_MATERIAL_SAND_NEO_HOOKEAN) {
+    lame(p[0], p[1], lambda, mu);
+    double d = density / std::max(det, 1e-6);
+    double ws = std::sqrt((lambda + 2.0 * mu) / d);
+    return 0.5 * h / std::max(ws, speed);
+  }
+  double wave;
+  if (m.kind == TMPM_MATERIAL_FLUID) {
+    wave = std::sqrt(p[0] * p[1] / std::max(density, 1e-12));
+  } else {
+    lame(p[0], p[1], lambda, mu);
+    wave = std::sqrt((lambda + 2.0 * mu) / std::max(density, 1e-12));
+  }
+  return 0.4 * h / std::max(wave + speed, 1e-12);
+}
+// obstacle.rs:265-341
+bool obstacle_valid(const tmpm_obstacle& o) {
+  const double* d = o.dims;
+  bool sv = false;
+  switch (o.shape) {
+    case TMPM_SHAPE_SPHERE: sv = std::isfinite(d[0]) && d[0] > 0; break;
+    case TMPM_SHAPE_BOX:
+      sv = std::isfinite(d[0]) && d[0] > 0 && std::isfinite(d[1]) && d[1] > 0 && std::isfinite(d[2]) && d[2] > 0;
+      break;
+    case TMPM_SHAPE_CAPSULE:
+      sv = std::isfinite(d[0]) && d[0] >= 0 && std::isfinite(d[1]) && d[1] >= 0 && (d[0] > 0 || d[1] > 0);
+      break;
+    case TMPM_SHAPE_CYLINDER:
+    case TMPM_SHAPE_CONE:
+    case TMPM_SHAPE_GROUND: sv = std::isfinite(d[0]) && d[0] > 0 && std::isfinite(d[1]) && d[1] > 0; break;
+    case TMPM_SHAPE_TRIANGLE_PRISM: {
+      const double* v = o.vertices;
+      double e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+      double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      double a2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+      sv = std::isfinite(d[0]) && d[0] > 0 && finite3(v) && finite3(v + 3) && finite3(v + 6) && std::isfinite(a2) &&
+           a2 > 1e-24;
+      break;
+    }
+    case TMPM_SHAPE_CONVEX: {
+      sv = o.plane_count >= 4 && o.planes != nullptr && std::isfinite(d[0]) && d[0] > 0;
+      for (uint32_t i = 0; sv && i < o.plane_count; ++i) {
+        const double* pl = o.planes + 4 * i;
+        for (int k = 0; k < 4; ++k) sv = sv && std::isfinite(pl[k]);
+        double n2 = std::fma(pl[0], pl[0], std::fma(pl[1], pl[1], pl[2] * pl[2]));
+        sv = sv && std::fabs(n2 - 1.0) <= 1e-4;
+      }
+      break;
+    }
+    default: return false;
+  }
+  bool cpic_ok = o.cpic_group < 0 || (o.cpic_group < 32 && o.shape == TMPM_SHAPE_TRIANGLE_PRISM);
+  return sv && cpic_ok && finite3(o.center) && std::isfinite(o.orientation[0]) && std::isfinite(o.orientation[1]) &&
+         std::isfinite(o.orientation[2]) && std::isfinite(o.orientation[3]) && finite3(o.linear_velocity) &&
+         finite3(o.angular_velocity) && std::isfinite(o.friction) && o.friction >= 0;
+}
+// world.rs:53-64
+bool params_valid(const tmpm_params& p) {
+  if (!std::isfinite(p.cell_width) || p.cell_width <= 0.0 || !finite3(p.gravity) || !std::isfinite(p.max_substep) ||
+      p.max_substep <= 0.0)
+    return false;
+  if (p.has_bounds) {
+    if (!finite3(p.bound_min) || !finite3(p.bound_max)) return false;
+    for (int a = 0; a < 3; ++a)
+      if (!(p.bound_max[a] - p.bound_min[a] > 0.0)) return false;
+  }
+  return true;
+}
+
+inline bool finite_f32(double v, float& out) {
+  out = (float)v;
+  return std::isfinite(out);
+}
+
+// f32 cell key with no fused multiply-add (gpu.rs:1142-1154).
+inline bool cell_f32(float x, float inv_h, int32_t& c) {
+  volatile float gp = x * inv_h;
+  volatile float sh = gp - 0.5f;
+  float f = std::floor((float)sh);
+  if (!std::isfinite(f) || (double)f < -2147483648.0 || (double)f > 2147483645.0) return false;
+  c = (int32_t)f;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Host world (f64, insertion order) — `MpmWorld` (world.rs:206-221).
+// ---------------------------------------------------------------------------
+struct OwnedObstacle {
+  tmpm_obstacle o;
+  std::vector<double> planes;
+};
+
+struct HostWorld {
+  tmpm_params params{};
+  std::vector<double> pos, vel, F, C, force, mass, rest_volume, radius, damping, plastic;
+  std::vector<tmpm_material> material;
+  std::vector<uint32_t> group_id;
+  std::vector<uint16_t> color;
+  std::vector<uint8_t> enabled, fixed;
+  std::vector<uint64_t> chunk;
+  std::vector<OwnedObstacle> obstacles;
+  uint64_t substeps = 0;
+  uint64_t next_chunk_id = 1;
+
+  size_t size() const { return mass.size(); }
+
+  void set_obstacles(const tmpm_obstacle* obs, uint32_t n) {
+    obstacles.clear();
+    obstacles.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      obstacles[i].o = obs[i];
+      if (obs[i].shape == TMPM_SHAPE_CONVEX && obs[i].planes)
+        obstacles[i].planes.assign(obs[i].planes, obs[i].planes + 4 * (size_t)obs[i].plane_count);
+      obstacles[i].o.planes = obstacles[i].planes.empty() ? nullptr : obstacles[i].planes.data();
+    }
+  }
+  HostWorld() = default;
+  HostWorld(const HostWorld& w) { *this = w; }
+  HostWorld& operator=(const HostWorld& w) {
+    if (this == &w) return *this;
+    params = w.params;
+    pos = w.pos; vel = w.vel; F = w.F; C = w.C; force = w.force; mass = w.mass; rest_volume = w.rest_volume;
+    radius = w.radius; damping = w.damping; plastic = w.plastic; material = w.material; group_id = w.group_id;
+    color = w.color; enabled = w.enabled; fixed = w.fixed; chunk = w.chunk;
+    substeps = w.substeps; next_chunk_id = w.next_chunk_id;
+    std::vector<tmpm_obstacle> raw;
+    for (auto& o : w.obstacles) raw.push_back(o.o);
+    set_obstacles(raw.data(), (uint32_t)raw.size());
+    return *this;
+  }
+
+  // world.rs:492-515 (valid_particle, finite_particle_state) for particle i.
+  bool particle_valid(size_t i) const {
+    auto fin = [](const double* p, int n) {
+      for (int k = 0; k < n; ++k)
+        if (!std::isfinite(p[k])) return false;
+      return true;
+    };
+    bool finite = fin(&pos[3 * i], 3) && fin(&vel[3 * i], 3) && fin(&F[9 * i], 9) && fin(&C[9 * i], 9) &&
+                  fin(&force[3 * i], 3) && std::isfinite(mass[i]) && std::isfinite(rest_volume[i]) &&
+                  std::isfinite(radius[i]) && std::isfinite(damping[i]);
+    const double* pl = &plastic[3 * i];
+    return finite && mass[i] > 0.0 && rest_volume[i] > 0.0 && radius[i] > 0.0 && material_valid(material[i]) &&
+           damping[i] >= 0.0 && std::isfinite(pl[0]) && pl[0] > 0.0 && std::isfinite(pl[1]) && std::isfinite(pl[2]);
+  }
+
+  // world.rs:289-303
+  double stable_timestep() const {
+    double s = params.max_substep;
+    for (size_t i = 0; i < size(); ++i) {
+      if (!enabled[i] || fixed[i]) continue;
+      double b = timestep_bound(material[i], mass[i] / rest_volume[i], &vel[3 * i], params.cell_width, det_cm(&F[9 * i]));
+      s = std::fmin(s, b);
+    }
+    return std::fmin(s, params.max_substep);
+  }
+
+  // Appends a particle batch (world.rs:244-265); validation by the caller.
+  void append(const tmpm_particles& p, const tmpm_material* mats, uint64_t chunk_id) {
+    size_t n = p.count;
+    auto app = [n](std::vector<double>& dst, const double* src, int k, double fill) {
+      if (src)
+        dst.insert(dst.end(), src, src + n * k);
+      else
+        dst.insert(dst.end(), n * k, fill);
+    };
+    app(pos, p.position, 3, 0.0);
+    app(vel, p.velocity, 3, 0.0);
+    if (p.deformation) {
+      F.insert(F.end(), p.deformation, p.deformation + 9 * n);
+    } else {
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 9; ++k) F.push_back(k % 4 == 0 ? 1.0 : 0.0);
+    }
+    app(C, p.affine, 9, 0.0);
+    app(force, p.force, 3, 0.0);
+    app(mass, p.mass, 1, 0.0);
+    app(rest_volume, p.rest_volume, 1, 0.0);
+    app(radius, p.radius, 1, 0.0);
+    app(damping, p.damping, 1, 0.0);
+    if (p.plastic) {
+      plastic.insert(plastic.end(), p.plastic, p.plastic + 3 * n);
+    } else {
+      for (size_t i = 0; i < n; ++i) {
+        plastic.push_back(1.0);
+        plastic.push_back(0.0);
+        plastic.push_back(0.0);
+      }
+    }
+    for (size_t i = 0; i < n; ++i) {
+      material.push_back(mats[p.material_index ? p.material_index[i] : 0]);
+      group_id.push_back(p.group_id ? p.group_id[i] : 0);
+      color.push_back(p.transfer_color ? p.transfer_color[i] : 0);
+      enabled.push_back(p.enabled ? (p.enabled[i] != 0) : 1);
+      fixed.push_back(p.fixed ? (p.fixed[i] != 0) : 0);
+      chunk.push_back(chunk_id);
+    }
+  }
+
+  // world.rs:268-277: keep particles whose chunk differs.
+  size_t remove_chunk(uint64_t id) {
+    std::vector<size_t> keep;
+    for (size_t i = 0; i < size(); ++i)
+      if (chunk[i] != id) keep.push_back(i);
+    size_t removed = size() - keep.size();
+    if (removed == 0) return 0;
+    auto sel = [&keep](auto& v, int k) {
+      using V = typename std::remove_reference<decltype(v)>::type;
+      V out;
+      out.reserve(keep.size() * k);
+      for (size_t i : keep)
+        for (int q = 0; q < k; ++q) out.push_back(v[i * k + q]);
+      v.swap(out);
+    };
+    sel(pos, 3); sel(vel, 3); sel(F, 9); sel(C, 9); sel(force, 3); sel(mass, 1); sel(rest_volume, 1);
+    sel(radius, 1); sel(damping, 1); sel(plastic, 3); sel(material, 1); sel(group_id, 1); sel(color, 1);
+    sel(enabled, 1); sel(fixed, 1); sel(chunk, 1);
+    return removed;
+  }
+};
+
+bool validate_particles(const tmpm_particles& p, const tmpm_material* mats, uint32_t mat_count) {
+  if (p.count > 0 && (!p.position || !p.mass || !p.rest_volume || !p.radius)) return false;
+  for (uint64_t i = 0; i < p.count; ++i)
+    if (p.material_index && p.material_index[i] >= mat_count) return false;
+  if (p.count > 0 && mat_count == 0) return false;
+  HostWorld tmp;
+  tmp.append(p, mats, 0);
+  for (size_t i = 0; i < tmp.size(); ++i)
+    if (!tmp.particle_valid(i)) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Device memory helpers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t b) {
+    release();
+    bytes = b ? b : 16;
+    return hipMalloc(&ptr, bytes);
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct ParticleBuffers {
+  DevBuf pos_mass, vel_vol, C, F, cls, ins_id;
+};
+
+struct KernelTimer {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+// Events are created once and reused: creating them per launch would add
+// host latency to the measured stream.
+
+}  // namespace
+
+struct tmpm_context {
+  int device = 0;
+  hipDeviceProp_t props{};
+};
+
+struct tmpm_session {
+  tmpm_context* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  HostWorld world;
+  double substep_dt = 0;
+  int fixed64 = 0;
+  bool valid = true;
+  uint64_t pending = 0;
+  bool prepared = false;
+
+  // device state
+  uint32_t n = 0;          // particles in device storage
+  uint32_t n_enabled = 0;  // enabled particles occupy storage [0, n_enabled)
+  ParticleBuffers buf[2];
+  int cur = 0;
+  DevBuf force;            // float4 per particle, insertion order
+  DevBuf classes, scene_plain, scene_force, obstacles;
+  uint32_t obstacle_cap = 0;
+  std::vector<DevClass> host_classes;
+  DevScene host_scene{};
+  bool force_pending = false;
+  double max_mass = 0.0;
+  uint32_t substeps_since_reorder = 1u << 30;
+  uint32_t reorder_interval = 8;
+
+  // topology
+  DevBuf owner, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, chunk_sums, grid_key, pblock_bucket,
+      pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, grid_acc, grid_vel, node_mask;
+  Topo topo{};
+  uint32_t p2g_blocks = 4096, threads_pblock = 128;
+  bool topo_valid = false;  // topology arrays describe the current device state
+
+  // timing
+  bool timing = false;
+  std::vector<KernelTimer> pending_timers;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, uint64_t>> kernel_ms;
+
+  Parts parts(int which) const {
+    Parts p{};
+    const ParticleBuffers& b = buf[which];
+    p.pos_mass = b.pos_mass.as<float4>();
+    p.vel_vol = b.vel_vol.as<float4>();
+    p.C = b.C.as<float>();
+    p.F = b.F.as<float>();
+    p.cls = b.cls.as<uint32_t>();
+    p.force = force.as<float4>();
+    p.ins_id = b.ins_id.as<uint32_t>();
+    p.n = n;
+    p.stride = n;
+    return p;
+  }
+};
+
+namespace {
+
+// Kernel launch with optional event timing.
+template <class F>
+hipError_t timed(tmpm_session* s, const char* name, F&& launch) {
+  if (!s->timing) {
+    launch();
+    return hipGetLastError();
+  }
+  KernelTimer t;
+  t.name = name;
+  for (hipEvent_t* e : {&t.start, &t.stop}) {
+    if (!s->event_pool.empty()) {
+      *e = s->event_pool.back();
+      s->event_pool.pop_back();
+    } else {
+      (void)hipEventCreate(e);
+    }
+  }
+  (void)hipEventRecord(t.start, s->stream);
+  launch();
+  (void)hipEventRecord(t.stop, s->stream);
+  s->pending_timers.push_back(t);
+  return hipGetLastError();
+}
+
+void collect_timers(tmpm_session* s) {
+  for (auto& t : s->pending_timers) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+      auto& e = s->kernel_ms[t.name];
+      e.first += ms;
+      e.second += 1;
+    }
+    s->event_pool.push_back(t.start);
+    s->event_pool.push_back(t.stop);
+  }
+  s->pending_timers.clear();
+}
+
+uint32_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return (uint32_t)p;
+}
+
+// gpu.rs:664-677
+tmpm_status validate_resident_world(const HostWorld& w, double substep_dt) {
+  if (!params_valid(w.params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
+  if (!std::isfinite(substep_dt) || substep_dt <= 0.0 || substep_dt > w.stable_timestep())
+    return fail(TMPM_INVALID_INPUT, "substep dt is not positive or exceeds the stable timestep");
+  for (auto& o : w.obstacles)
+    if (!obstacle_valid(o.o)) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  return TMPM_OK;
+}
+
+bool pack_obstacle(const tmpm_obstacle& o, DevObstacle& d) {
+  std::memset(&d, 0, sizeof(d));
+  d.shape = o.shape;
+  d.boundary = o.boundary;
+  bool ok = finite_f32(o.friction, d.friction);
+  for (int k = 0; k < 3; ++k) {
+    ok = finite_f32(o.center[k], d.center[k]) && ok;
+    ok = finite_f32(o.linear_velocity[k], d.lin[k]) && ok;
+    ok = finite_f32(o.angular_velocity[k], d.ang[k]) && ok;
+  }
+  for (int k = 0; k < 4; ++k) {
+    ok = finite_f32(o.orientation[k], d.q[k]) && ok;
+    ok = finite_f32(o.dims[k], d.dims[k]) && ok;
+  }
+  return ok;
+}
+
+tmpm_status upload_obstacles(tmpm_session* s) {
+  std::vector<DevObstacle> packed;
+  for (auto& ob : s->world.obstacles) {
+    const tmpm_obstacle& o = ob.o;
+    if (o.shape == TMPM_SHAPE_TRIANGLE_PRISM || o.shape == TMPM_SHAPE_CONVEX)
+      return fail(TMPM_RESIDENT_UNAVAILABLE, "triangle-prism and convex obstacles are not supported by the HIP backend");
+    if (o.cpic_group >= 0 && s->world.params.use_cpic)
+      return fail(TMPM_RESIDENT_UNAVAILABLE, "CPIC obstacle groups are not supported by the HIP backend");
+    DevObstacle d;
+    if (!pack_obstacle(o, d)) return fail(TMPM_INVALID_INPUT, "obstacle is not representable in f32");
+    packed.push_back(d);
+  }
+  if (packed.size() > (size_t)kMaxObstacles) return fail(TMPM_CAPACITY, "too many obstacles");
+  if (packed.size() > s->obstacle_cap || !s->obstacles.ptr) {
+    s->obstacle_cap = std::max<uint32_t>(16, (uint32_t)packed.size());
+    HIP_TRY(s->obstacles.alloc(sizeof(DevObstacle) * s->obstacle_cap));
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (!packed.empty())
+    HIP_TRY(hipMemcpy(s->obstacles.ptr, packed.data(), packed.size() * sizeof(DevObstacle), hipMemcpyHostToDevice));
+  s->host_scene.obstacle_begin = 0;
+  s->host_scene.obstacle_count = (int32_t)packed.size();
+  return TMPM_OK;
+}
+
+tmpm_status upload_scene(tmpm_session* s) {
+  DevScene plain = s->host_scene, forced = s->host_scene;
+  plain.apply_force = 0;
+  forced.apply_force = 1;
+  HIP_TRY(hipMemcpy(s->scene_plain.ptr, &plain, sizeof(DevScene), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->scene_force.ptr, &forced, sizeof(DevScene), hipMemcpyHostToDevice));
+  return TMPM_OK;
+}
+
+// Scene constants; f32 packing as in gpu.rs:1118-1127, 1407-1432.
+tmpm_status build_scene(tmpm_session* s) {
+  const tmpm_params& p = s->world.params;
+  DevScene& sc = s->host_scene;
+  float h, step, inv_d;
+  if (!finite_f32(p.cell_width, h) || !finite_f32(s->substep_dt, step) ||
+      !finite_f32(4.0 / (p.cell_width * p.cell_width), inv_d) || h <= 0.0f || step <= 0.0f)
+    return fail(TMPM_INVALID_INPUT, "cell width or dt not representable in f32");
+  float inv_h = 1.0f / h;
+  if (!std::isfinite(inv_h)) return fail(TMPM_INVALID_INPUT, "cell width too small");
+  sc.h = h;
+  sc.inv_h = inv_h;
+  sc.inv_d = inv_d;
+  sc.dt = step;
+  for (int k = 0; k < 3; ++k)
+    if (!finite_f32(p.gravity[k], sc.gravity[k])) return fail(TMPM_INVALID_INPUT, "gravity not representable");
+  sc.has_bounds = p.has_bounds ? 1 : 0;
+  // Fixed-point exponents: one contribution stays below 2^45 units (checked
+  // per contribution), so up to 2^18 contributions per node cannot wrap the
+  // int64 accumulator.  Momentum is bounded by mass x 64 cells per substep.
+  if (s->max_mass > 0.0) {
+    double vbound = 64.0 * p.cell_width / s->substep_dt;
+    int ms = (int)std::floor(std::log2(std::ldexp(1.0, 44) / s->max_mass));
+    int ps = (int)std::floor(std::log2(std::ldexp(1.0, 44) / (s->max_mass * vbound)));
+    sc.mass_shift = std::max(-120, std::min(120, ms));
+    sc.mom_shift = std::max(-120, std::min(120, ps));
+  }
+  for (int a = 0; a < 3; ++a) {
+    sc.first_base[a] = std::numeric_limits<int32_t>::min();
+    sc.last_base[a] = 2147483645;
+    sc.bmin[a] = sc.bmax[a] = 0.0f;
+    if (p.has_bounds) {
+      if (!finite_f32(p.bound_min[a], sc.bmin[a]) || !finite_f32(p.bound_max[a], sc.bmax[a]) ||
+          sc.bmin[a] >= sc.bmax[a])
+        return fail(TMPM_INVALID_INPUT, "bounds not representable in f32");
+      // gpu.wgsl:962-964: resident particles must stay within two cells of the bounds.
+      volatile float lo = sc.bmin[a] * inv_h;
+      volatile float hi = sc.bmax[a] * inv_h;
+      volatile float lo2 = lo - 0.5f, hi2 = hi - 0.5f;
+      sc.first_base[a] = (int32_t)std::floor((float)lo2) - 2;
+      sc.last_base[a] = (int32_t)std::floor((float)hi2) + 2;
+    }
+  }
+  return TMPM_OK;
+}
+
+// Allocates the topology for `pblocks` particle blocks and `gblocks` grid
+// blocks of capacity.
+tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_cap) {
+  uint32_t buckets = std::max<uint32_t>(next_pow2(2ull * grid_cap), (uint32_t)kScanChunk);
+  if ((uint64_t)s->n + 8ull * buckets + 1 >= 0xffffffffull) return fail(TMPM_CAPACITY, "topology id space exhausted");
+  Topo& t = s->topo;
+  HIP_TRY(s->owner.alloc(4ull * buckets));
+  HIP_TRY(s->bucket_count.alloc(4ull * buckets));
+  HIP_TRY(s->nbr_bucket.alloc(32ull * buckets));
+  HIP_TRY(s->bucket_slot.alloc(4ull * buckets));
+  HIP_TRY(s->bucket_pblock.alloc(4ull * buckets));
+  uint32_t chunks = buckets / kScanChunk;
+  HIP_TRY(s->chunk_sums.alloc(12ull * (chunks + 1)));
+  HIP_TRY(s->grid_key.alloc(16ull * grid_cap));
+  HIP_TRY(s->pblock_bucket.alloc(4ull * pblock_cap));
+  HIP_TRY(s->pblock_slot8.alloc(32ull * pblock_cap));
+  HIP_TRY(s->pblock_start.alloc(4ull * pblock_cap));
+  HIP_TRY(s->pblock_len.alloc(4ull * pblock_cap));
+  HIP_TRY(s->pslot.alloc(4ull * std::max<uint32_t>(s->n, 1)));
+  HIP_TRY(s->prank.alloc(4ull * std::max<uint32_t>(s->n, 1)));
+  HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->n, 1)));
+  HIP_TRY(s->counters.alloc(4ull * CNT_COUNT));
+  HIP_TRY(hipMemset(s->counters.ptr, 0, 4ull * CNT_COUNT));
+  HIP_TRY(s->grid_acc.alloc((size_t)grid_cap * kBlockNodes * 32));
+  HIP_TRY(s->grid_vel.alloc((size_t)grid_cap * kBlockNodes * 16));
+  HIP_TRY(s->node_mask.alloc(8ull * grid_cap));
+  t.owner = s->owner.as<uint32_t>();
+  t.bucket_count = s->bucket_count.as<uint32_t>();
+  t.nbr_bucket = s->nbr_bucket.as<uint32_t>();
+  t.bucket_slot = s->bucket_slot.as<uint32_t>();
+  t.bucket_pblock = s->bucket_pblock.as<uint32_t>();
+  t.bucket_mask = buckets - 1;
+  t.n_chunks = chunks;
+  t.chunk_sums = s->chunk_sums.as<uint32_t>();
+  t.grid_key = s->grid_key.as<int4>();
+  t.grid_cap = grid_cap;
+  t.pblock_cap = pblock_cap;
+  t.pblock_bucket = s->pblock_bucket.as<uint32_t>();
+  t.pblock_slot8 = s->pblock_slot8.as<uint32_t>();
+  t.pblock_start = s->pblock_start.as<uint32_t>();
+  t.pblock_len = s->pblock_len.as<uint32_t>();
+  t.pslot = s->pslot.as<uint32_t>();
+  t.prank = s->prank.as<uint32_t>();
+  t.sorted = s->sorted.as<uint32_t>();
+  t.counters = s->counters.as<uint32_t>();
+  t.grid_acc = s->grid_acc.ptr;
+  t.grid_vel = s->grid_vel.as<float4>();
+  t.node_mask = s->node_mask.as<uint32_t>();
+  s->p2g_blocks = std::min<uint32_t>(pblock_cap, 65535);
+  s->topo_valid = false;
+  return TMPM_OK;
+}
+
+// Counts particle blocks and grid blocks (block + its 7 upper neighbours) of
+// the enabled particles on the host, f32 cell rule.
+void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& region, float inv_h, uint64_t& pblocks,
+                  uint64_t& gblocks) {
+  struct K {
+    int32_t x, y, z;
+    uint32_t r;
+    bool operator==(const K& o) const { return x == o.x && y == o.y && z == o.z && r == o.r; }
+  };
+  struct H {
+    size_t operator()(const K& k) const {
+      return ((size_t)(uint32_t)k.x * 73856093u) ^ ((size_t)(uint32_t)k.y * 19349663u) ^
+             ((size_t)(uint32_t)k.z * 83492791u) ^ ((size_t)k.r * 2654435761u);
+    }
+  };
+  std::unordered_map<K, char, H> pb, gb;
+  size_t n = region.size();
+  for (size_t i = 0; i < n; ++i) {
+    int32_t c[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h, c[a]) && ok;
+    if (!ok) continue;
+    K k{c[0] >> 2, c[1] >> 2, c[2] >> 2, region[i]};
+    if (pb.emplace(k, 1).second)
+      for (int d = 0; d < 8; ++d) gb.emplace(K{k.x + (d >> 2), k.y + ((d >> 1) & 1), k.z + (d & 1), k.r}, 1);
+  }
+  pblocks = pb.size();
+  gblocks = gb.size();
+}
+
+// Upload the world to the device (gpu.rs:1091-1600 `prepare`, resident).
+tmpm_status prepare(tmpm_session* s) {
+  HostWorld& w = s->world;
+  s->prepared = false;
+  s->topo_valid = false;
+  tmpm_status st = build_scene(s);
+  if (st != TMPM_OK) return st;
+  const float inv_h = s->host_scene.inv_h;
+  const float inv_d = s->host_scene.inv_d;
+  (void)inv_d;
+  size_t total = w.size();
+  if (total >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
+  // storage order: enabled particles first (insertion order), then disabled
+  std::vector<uint32_t> order;
+  order.reserve(total);
+  for (size_t i = 0; i < total; ++i)
+    if (w.enabled[i]) order.push_back((uint32_t)i);
+  uint32_t n_enabled = (uint32_t)order.size();
+  for (size_t i = 0; i < total; ++i)
+    if (!w.enabled[i]) order.push_back((uint32_t)i);
+
+  std::vector<float4> pm(total), vv(total);
+  std::vector<float> Cf(9 * total), Ff(9 * total);
+  std::vector<uint32_t> cls(total), ins(total);
+  std::vector<float> pos32(3 * (size_t)n_enabled);
+  std::vector<uint32_t> region(n_enabled);
+  std::map<std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t>, uint32_t>
+      class_ids;
+  s->host_classes.clear();
+  double max_mass = 0;
+  for (size_t slot = 0; slot < total; ++slot) {
+    size_t i = order[slot];
+    bool en = w.enabled[i] != 0;
+    float f[16];
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) ok = finite_f32(w.pos[3 * i + k], f[k]) && ok;
+    ok = finite_f32(en ? w.mass[i] : 0.0, f[3]) && ok;
+    for (int k = 0; k < 3; ++k) ok = finite_f32(w.vel[3 * i + k], f[4 + k]) && ok;
+    ok = finite_f32(w.rest_volume[i], f[7]) && ok;
+    float rad, damp;
+    ok = finite_f32(w.radius[i], rad) && finite_f32(w.damping[i], damp) && ok;
+    if (!ok) return fail(TMPM_INVALID_INPUT, "particle state not representable in f32");
+    if (en && (f[3] <= 0.0f || f[7] <= 0.0f)) return fail(TMPM_INVALID_INPUT, "enabled particle mass or volume underflows f32");
+    if (rad <= 0.0f || w.damping[i] < 0.0) return fail(TMPM_INVALID_INPUT, "radius or damping invalid");
+    pm[slot] = make_float4(f[0], f[1], f[2], f[3]);
+    vv[slot] = make_float4(f[4], f[5], f[6], f[7]);
+    for (int k = 0; k < 9; ++k) {
+      if (!finite_f32(w.C[9 * i + k], Cf[9 * slot + k]) || !finite_f32(w.F[9 * i + k], Ff[9 * slot + k]))
+        return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
+    }
+    ins[slot] = (uint32_t)i;
+    // class
+    const tmpm_material& m = w.material[i];
+    DevClass c{};
+    c.region = w.color[i];
+    c.fixed = w.fixed[i] ? 1u : 0u;
+    c.radius = rad;
+    c.damping = damp;
+    if (en) {
+      switch (m.kind) {
+        case TMPM_MATERIAL_LINEAR_ELASTIC:
+        case TMPM_MATERIAL_NEO_HOOKEAN: {
+          double lambda, mu;
+          lame(m.p[0], m.p[1], lambda, mu);
+          if (!finite_f32(lambda, c.lambda) || !finite_f32(mu, c.mu))
+            return fail(TMPM_INVALID_INPUT, "Lame parameters not representable");
+          c.kind = m.kind == TMPM_MATERIAL_LINEAR_ELASTIC ? DM_COROTATED : DM_NEO_HOOKEAN;
+          c.wave_numerator = c.lambda + 2.0f * c.mu;
+          break;
+        }
+        case TMPM_MATERIAL_FLUID:
+          c.kind = DM_FLUID;
+          if (!finite_f32(m.p[0], c.bulk) || !finite_f32(m.p[1], c.gamma) || !finite_f32(m.p[2], c.viscosity) ||
+              !finite_f32(m.p[3], c.tensile))
+            return fail(TMPM_INVALID_INPUT, "fluid parameters not representable");
+          c.wave_numerator = c.bulk * c.gamma;
+          break;
+        default:
+          return fail(TMPM_RESIDENT_UNAVAILABLE, "plastic materials (sand, snow) are not supported by the HIP backend");
+      }
+      max_mass = std::max(max_mass, w.mass[i]);
+      // base cell must be representable (gpu.rs:1143-1153)
+      for (int a = 0; a < 3; ++a) {
+        int32_t cc;
+        if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
+        pos32[3 * slot + a] = f[a];
+      }
+      region[slot] = c.region;
+    }
+    auto key = std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
+                               c.region, c.fixed);
+    auto it = class_ids.find(key);
+    uint32_t id;
+    if (it == class_ids.end()) {
+      id = (uint32_t)s->host_classes.size();
+      class_ids.emplace(key, id);
+      s->host_classes.push_back(c);
+    } else {
+      id = it->second;
+    }
+    cls[slot] = id;
+  }
+  s->n = (uint32_t)total;
+  s->n_enabled = n_enabled;
+  if (n_enabled == 0) {
+    // gpu.rs:1358-1360: no live node, nothing to prepare; steps only count.
+    return TMPM_OK;
+  }
+  s->max_mass = max_mass;
+  st = build_scene(s);
+  if (st != TMPM_OK) return st;
+
+  // device particle buffers
+  for (int b = 0; b < 2; ++b) {
+    ParticleBuffers& pb = s->buf[b];
+    HIP_TRY(pb.pos_mass.alloc(16ull * total));
+    HIP_TRY(pb.vel_vol.alloc(16ull * total));
+    HIP_TRY(pb.C.alloc(36ull * total));
+    HIP_TRY(pb.F.alloc(36ull * total));
+    HIP_TRY(pb.cls.alloc(4ull * total));
+    HIP_TRY(pb.ins_id.alloc(4ull * total));
+  }
+  HIP_TRY(s->force.alloc(16ull * total));
+  s->cur = 0;
+  ParticleBuffers& pb = s->buf[0];
+  HIP_TRY(hipMemcpy(pb.pos_mass.ptr, pm.data(), 16ull * total, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pb.vel_vol.ptr, vv.data(), 16ull * total, hipMemcpyHostToDevice));
+  // SoA matrices: component k of particle slot at [k * total + slot]
+  std::vector<float> soa(9 * total);
+  for (size_t slot = 0; slot < total; ++slot)
+    for (int k = 0; k < 9; ++k) soa[k * total + slot] = Cf[9 * slot + k];
+  HIP_TRY(hipMemcpy(pb.C.ptr, soa.data(), 36ull * total, hipMemcpyHostToDevice));
+  for (size_t slot = 0; slot < total; ++slot)
+    for (int k = 0; k < 9; ++k) soa[k * total + slot] = Ff[9 * slot + k];
+  HIP_TRY(hipMemcpy(pb.F.ptr, soa.data(), 36ull * total, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pb.cls.ptr, cls.data(), 4ull * total, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pb.ins_id.ptr, ins.data(), 4ull * total, hipMemcpyHostToDevice));
+  // forces (insertion order)
+  std::vector<float4> fz(total);
+  bool any_force = false;
+  for (size_t i = 0; i < total; ++i) {
+    float fx, fy, fzv;
+    if (!finite_f32(w.force[3 * i], fx) || !finite_f32(w.force[3 * i + 1], fy) || !finite_f32(w.force[3 * i + 2], fzv))
+      return fail(TMPM_INVALID_INPUT, "force not representable in f32");
+    fz[i] = make_float4(fx, fy, fzv, 0.f);
+    any_force = any_force || fx != 0.f || fy != 0.f || fzv != 0.f;
+  }
+  HIP_TRY(hipMemcpy(s->force.ptr, fz.data(), 16ull * total, hipMemcpyHostToDevice));
+  s->force_pending = any_force;
+  HIP_TRY(s->classes.alloc(sizeof(DevClass) * std::max<size_t>(1, s->host_classes.size())));
+  HIP_TRY(hipMemcpy(s->classes.ptr, s->host_classes.data(), sizeof(DevClass) * s->host_classes.size(),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(s->scene_plain.alloc(sizeof(DevScene)));
+  HIP_TRY(s->scene_force.alloc(sizeof(DevScene)));
+  st = upload_obstacles(s);
+  if (st != TMPM_OK) return st;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  st = upload_scene(s);
+  if (st != TMPM_OK) return st;
+  // topology capacity from the initial block counts, with headroom
+  uint64_t pblocks = 0, gblocks = 0;
+  count_blocks(pos32, region, inv_h, pblocks, gblocks);
+  uint64_t pcap = std::max<uint64_t>(1024, 4 * pblocks);
+  uint64_t gcap = std::max<uint64_t>(4096, 4 * gblocks);
+  pcap = std::min<uint64_t>(pcap, n_enabled);
+  pcap = std::max<uint64_t>(pcap, 1);
+  if (gcap > (1ull << 26)) return fail(TMPM_CAPACITY, "grid exceeds the block capacity");
+  st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap);
+  if (st != TMPM_OK) return st;
+  s->substeps_since_reorder = 1u << 30;
+  s->prepared = true;
+  return TMPM_OK;
+}
+
+// Binning stages of one substep (k_topo_clear .. k_pblock_finalize).
+hipError_t enqueue_binning(tmpm_session* s, const Parts& p) {
+  const DevClass* c = s->classes.as<DevClass>();
+  const DevScene* sc = s->scene_plain.as<DevScene>();
+  Topo& t = s->topo;
+  hipError_t e;
+  if ((e = timed(s, "k_topo_clear", [&] { launch_topo_clear(t, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p, t, c, sc, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_bin_neighbors", [&] { launch_bin_neighbors(p, t, c, sc, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_scan", [&] { launch_scan(p, t, c, sc, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_pblock_finalize", [&] { launch_pblock_finalize(t, s->stream); })) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
+  Parts p = s->parts(s->cur);
+  Topo& t = s->topo;
+  const DevClass* c = s->classes.as<DevClass>();
+  const DevScene* sc = (apply_force ? s->scene_force : s->scene_plain).as<DevScene>();
+  const DevObstacle* ob = s->obstacles.as<DevObstacle>();
+  hipError_t e = enqueue_binning(s, p);
+  if (e != hipSuccess) return e;
+  if (s->substeps_since_reorder >= s->reorder_interval) {
+    Parts q = s->parts(1 - s->cur);
+    if ((e = timed(s, "k_bin_scatter_move", [&] { launch_bin_scatter_move(p, q, t, s->n_enabled, s->stream); })) !=
+        hipSuccess)
+      return e;
+    s->cur = 1 - s->cur;
+    p = q;
+    s->substeps_since_reorder = 0;
+  } else {
+    if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p, t, s->stream); })) != hipSuccess) return e;
+  }
+  s->substeps_since_reorder++;
+  if ((e = timed(s, "k_grid_clear", [&] { launch_grid_clear(t, s->fixed64, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
+                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->threads_pblock, s->stream); })) != hipSuccess)
+    return e;
+  if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, s->stream); })) != hipSuccess)
+    return e;
+  // The P2G kernel template takes 256-thread launch bounds; G2P uses the same.
+  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, s->stream); })) != hipSuccess) return e;
+  s->topo_valid = false;
+  return hipSuccess;
+}
+
+tmpm_status status_from_error_bits(uint32_t bits) {
+  if (bits & (ERR_TOPOLOGY | ERR_FIXED_RANGE))
+    return fail(TMPM_CAPACITY, (bits & ERR_TOPOLOGY) ? "sparse grid capacity exhausted"
+                                                    : "fixed-point P2G contribution out of range");
+  if (bits & ERR_RESIDENT)
+    return fail(TMPM_RESIDENT_UNAVAILABLE,
+                "a particle exceeded the CFL bound, left the grid or became non-finite; world state is unchanged");
+  if (bits & ERR_INTERNAL) return fail(TMPM_READBACK, "internal binning inconsistency");
+  return TMPM_OK;
+}
+
+// Copies device state into a new world (gpu.rs:2295-2329 apply_gpu_transfers).
+tmpm_status read_back(tmpm_session* s, HostWorld& out) {
+  out = s->world;
+  if (!s->prepared) {
+    for (size_t i = 0; i < out.size(); ++i) {
+      out.force[3 * i] = out.force[3 * i + 1] = out.force[3 * i + 2] = 0.0;
+      if (out.fixed[i] && out.enabled[i]) {
+        for (int k = 0; k < 3; ++k) out.vel[3 * i + k] = 0.0;
+        for (int k = 0; k < 9; ++k) out.C[9 * i + k] = 0.0;
+      }
+    }
+    return TMPM_OK;
+  }
+  size_t n = s->n;
+  const ParticleBuffers& b = s->buf[s->cur];
+  std::vector<float4> pm(n), vv(n);
+  std::vector<float> C(9 * n), F(9 * n);
+  std::vector<uint32_t> ins(n);
+  HIP_TRY(hipMemcpy(pm.data(), b.pos_mass.ptr, 16 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(vv.data(), b.vel_vol.ptr, 16 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(C.data(), b.C.ptr, 36 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(F.data(), b.F.ptr, 36 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ins.data(), b.ins_id.ptr, 4 * n, hipMemcpyDeviceToHost));
+  for (size_t slot = 0; slot < n; ++slot) {
+    size_t i = ins[slot];
+    if (i >= out.size()) return fail(TMPM_READBACK, "corrupt particle order");
+    out.force[3 * i] = out.force[3 * i + 1] = out.force[3 * i + 2] = 0.0;
+    if (!out.enabled[i]) continue;
+    if (out.fixed[i]) {
+      for (int k = 0; k < 3; ++k) out.vel[3 * i + k] = 0.0;
+      for (int k = 0; k < 9; ++k) out.C[9 * i + k] = 0.0;
+      continue;
+    }
+    const float4 p = pm[slot], v = vv[slot];
+    double xs[3] = {p.x, p.y, p.z}, vs[3] = {v.x, v.y, v.z};
+    bool finite = true;
+    for (int k = 0; k < 3; ++k) {
+      out.pos[3 * i + k] = xs[k];
+      out.vel[3 * i + k] = vs[k];
+      finite = finite && std::isfinite(xs[k]) && std::isfinite(vs[k]);
+    }
+    for (int k = 0; k < 9; ++k) {
+      out.C[9 * i + k] = C[k * n + slot];
+      out.F[9 * i + k] = F[k * n + slot];
+      finite = finite && std::isfinite(out.C[9 * i + k]) && std::isfinite(out.F[9 * i + k]);
+    }
+    if (!finite) return fail(TMPM_NON_FINITE_STATE, "MPM integration produced a non-finite state");
+  }
+  return TMPM_OK;
+}
+
+// Grows the topology when the last substep used more than half of it.
+tmpm_status maybe_grow_topology(tmpm_session* s, const uint32_t* counters) {
+  uint32_t g = counters[CNT_GRID_BLOCKS], pbk = counters[CNT_PARTICLE_BLOCKS];
+  if (2ull * g <= s->topo.grid_cap && 2ull * pbk <= s->topo.pblock_cap) return TMPM_OK;
+  uint64_t gcap = std::max<uint64_t>(s->topo.grid_cap, 4ull * g);
+  uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(s->topo.pblock_cap, 4ull * pbk), s->n_enabled);
+  return alloc_topology(s, (uint32_t)pcap, (uint32_t)std::min<uint64_t>(gcap, 1ull << 26));
+}
+
+tmpm_status copy_particles_out(const HostWorld& w, tmpm_particles* out) {
+  size_t n = std::min<size_t>(out->count, w.size());
+  auto put = [n](double* dst, const std::vector<double>& src, int k) {
+    if (dst) std::memcpy(dst, src.data(), sizeof(double) * k * n);
+  };
+  put(out->position, w.pos, 3);
+  put(out->velocity, w.vel, 3);
+  put(out->deformation, w.F, 9);
+  put(out->affine, w.C, 9);
+  put(out->force, w.force, 3);
+  put(out->mass, w.mass, 1);
+  put(out->rest_volume, w.rest_volume, 1);
+  put(out->radius, w.radius, 1);
+  put(out->damping, w.damping, 1);
+  put(out->plastic, w.plastic, 3);
+  for (size_t i = 0; i < n; ++i) {
+    if (out->group_id) out->group_id[i] = w.group_id[i];
+    if (out->transfer_color) out->transfer_color[i] = w.color[i];
+    if (out->enabled) out->enabled[i] = w.enabled[i];
+    if (out->fixed) out->fixed[i] = w.fixed[i];
+    if (out->chunk_id) out->chunk_id[i] = w.chunk[i];
+    if (out->material_index) out->material_index[i] = 0;
+  }
+  return TMPM_OK;
+}
+
+tmpm_status run_diagnostic_binning(tmpm_session* s, std::vector<uint32_t>& counters) {
+  counters.assign(CNT_COUNT, 0);
+  if (!s->prepared) return TMPM_OK;
+  Parts p = s->parts(s->cur);
+  bool was = s->timing;
+  s->timing = false;
+  HIP_TRY(enqueue_binning(s, p));
+  launch_live_nodes(p, s->topo, s->classes.as<DevClass>(), s->scene_plain.as<DevScene>(), s->stream);
+  s->timing = was;
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(counters.data(), s->counters.ptr, 4 * CNT_COUNT, hipMemcpyDeviceToHost));
+  s->topo_valid = true;
+  return TMPM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* tmpm_last_error(void) { return g_last_error.c_str(); }
+const char* tmpm_version(void) { return "tessera-mpm-hip 0.1.0 (gfx950)"; }
+
+tmpm_status tmpm_context_create(int32_t device_ordinal, tmpm_context** out) {
+  if (!out) return fail(TMPM_INVALID_INPUT, "null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device_ordinal < 0 || device_ordinal >= count)
+    return fail(TMPM_NO_DEVICE, "no HIP device with this ordinal");
+  tmpm_context* c = new tmpm_context();
+  c->device = device_ordinal;
+  if (hipGetDeviceProperties(&c->props, device_ordinal) != hipSuccess) {
+    delete c;
+    return fail(TMPM_NO_DEVICE, "cannot query device");
+  }
+  if (std::strncmp(c->props.gcnArchName, "gfx950", 6) != 0) {
+    std::string arch = c->props.gcnArchName;
+    delete c;
+    return fail(TMPM_NO_DEVICE, "device is " + arch + ", this library is built for gfx950");
+  }
+  *out = c;
+  return TMPM_OK;
+}
+
+void tmpm_context_destroy(tmpm_context* ctx) { delete ctx; }
+
+tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* wd, double substep_dt, tmpm_p2g_mode mode,
+                                tmpm_session** out) {
+  if (!ctx || !wd || !out) return fail(TMPM_INVALID_INPUT, "null argument");
+  *out = nullptr;
+  if (mode != TMPM_P2G_FP32 && mode != TMPM_P2G_FIXED64) return fail(TMPM_INVALID_INPUT, "unknown P2G mode");
+  if (!params_valid(wd->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
+  if (!validate_particles(wd->particles, wd->materials, wd->material_count))
+    return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  for (uint32_t i = 0; i < wd->obstacle_count; ++i)
+    if (!obstacle_valid(wd->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
+  tmpm_session* s = new tmpm_session();
+  s->ctx = ctx;
+  s->substep_dt = substep_dt;
+  s->fixed64 = mode == TMPM_P2G_FIXED64;
+  s->world.params = wd->params;
+  s->world.append(wd->particles, wd->materials, 0);
+  s->world.set_obstacles(wd->obstacles, wd->obstacle_count);
+  s->world.substeps = wd->substeps;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(TMPM_READBACK, "cannot create stream");
+  }
+  tmpm_status st = validate_resident_world(s->world, substep_dt);
+  if (st == TMPM_OK) st = prepare(s);
+  if (st != TMPM_OK) {
+    std::string msg = g_last_error;
+    tmpm_session_destroy(s);
+    g_last_error = msg;
+    return st;
+  }
+  *out = s;
+  return TMPM_OK;
+}
+
+void tmpm_session_destroy(tmpm_session* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  collect_timers(s);
+  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+tmpm_status tmpm_session_submit_steps(tmpm_session* s, uint32_t steps) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (steps == 0 || steps > 64) return fail(TMPM_INVALID_INPUT, "steps must be within 1..=64");
+  if (s->pending + steps < s->pending) return fail(TMPM_CAPACITY, "pending substep count overflow");
+  tmpm_status st = validate_resident_world(s->world, s->substep_dt);
+  if (st != TMPM_OK) return st;
+  if (s->prepared) {
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return fail(TMPM_READBACK, "cannot select device");
+    for (uint32_t k = 0; k < steps; ++k) {
+      bool force = s->force_pending;
+      s->force_pending = false;
+      hipError_t e = enqueue_substep(s, force);
+      if (e != hipSuccess) {
+        s->valid = false;
+        return fail(TMPM_READBACK, std::string("kernel launch failed: ") + hipGetErrorString(e));
+      }
+    }
+  }
+  s->pending += steps;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_wait(tmpm_session* s) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_synchronize(tmpm_session* s) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending == 0) return TMPM_OK;
+  uint64_t steps = s->pending;
+  s->pending = 0;
+  if (s->prepared) {
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+      s->valid = false;
+      return fail(TMPM_READBACK, std::string("device failure: ") + hipGetErrorString(e));
+    }
+    collect_timers(s);
+    uint32_t counters[CNT_COUNT];
+    if (hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost) != hipSuccess) {
+      s->valid = false;
+      return fail(TMPM_READBACK, "counter readback failed");
+    }
+    tmpm_status st = status_from_error_bits(counters[CNT_ERROR]);
+    if (st != TMPM_OK) {
+      s->valid = false;
+      return st;
+    }
+    HostWorld updated;
+    st = read_back(s, updated);
+    if (st != TMPM_OK) {
+      s->valid = false;
+      return st;
+    }
+    updated.substeps = s->world.substeps + steps < s->world.substeps ? UINT64_MAX : s->world.substeps + steps;
+    s->world = updated;
+    st = maybe_grow_topology(s, counters);
+    if (st != TMPM_OK) {
+      s->valid = false;
+      return st;
+    }
+  } else {
+    HostWorld updated;
+    read_back(s, updated);
+    updated.substeps = s->world.substeps + steps;
+    s->world = updated;
+  }
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_step(tmpm_session* s, uint32_t steps) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  tmpm_status st = tmpm_session_submit_steps(s, steps);
+  if (st != TMPM_OK) return st;
+  return tmpm_session_synchronize(s);
+}
+
+// gpu.rs:2208-2240: CFL-limited substeps, each validated on the host.
+tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  if (!std::isfinite(dt) || dt <= 0.0) return fail(TMPM_INVALID_INPUT, "dt must be positive");
+  for (auto& o : s->world.obstacles)
+    if (!obstacle_valid(o.o)) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  double remaining = dt;
+  size_t count = 0;
+  double saved_dt = s->substep_dt;
+  while (remaining > 0.0) {
+    double stable = s->world.stable_timestep();
+    if (!std::isfinite(stable) || stable <= 1e-12 || count >= 100000)
+      return fail(TMPM_EXCESSIVE_SUBSTEPS, "MPM timestep is too small for the requested duration");
+    double sub = std::min(remaining, stable);
+    if (sub != s->substep_dt) {
+      s->substep_dt = sub;
+      tmpm_status st = build_scene(s);
+      if (st == TMPM_OK) st = upload_obstacles(s);
+      if (st == TMPM_OK && s->prepared) {
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TMPM_READBACK, "stream failure");
+        st = upload_scene(s);
+      }
+      if (st != TMPM_OK) {
+        s->substep_dt = saved_dt;
+        return st;
+      }
+    }
+    tmpm_status st = tmpm_session_step(s, 1);
+    if (st != TMPM_OK) return st;
+    remaining = std::max(remaining - sub, 0.0);
+    ++count;
+  }
+  return TMPM_OK;
+}
+
+uint64_t tmpm_session_pending_substeps(const tmpm_session* s) { return s ? s->pending : 0; }
+int32_t tmpm_session_valid(const tmpm_session* s) { return s && s->valid ? 1 : 0; }
+uint64_t tmpm_session_substeps(const tmpm_session* s) { return s ? s->world.substeps : 0; }
+uint64_t tmpm_session_particle_count(const tmpm_session* s) { return s ? s->world.size() : 0; }
+
+tmpm_status tmpm_session_read_particles(const tmpm_session* s, tmpm_particles* out) {
+  if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (out->count < s->world.size()) return fail(TMPM_CAPACITY, "output arrays too small");
+  return copy_particles_out(s->world, out);
+}
+
+tmpm_status tmpm_session_set_particle_force(tmpm_session* s, uint64_t index, const double force[3]) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  if (index >= s->world.size()) return fail(TMPM_INVALID_INPUT, "particle index out of range");
+  float f[4];
+  if (!finite_f32(force[0], f[0]) || !finite_f32(force[1], f[1]) || !finite_f32(force[2], f[2]) ||
+      !finite_f32(s->world.damping[index], f[3]))
+    return fail(TMPM_INVALID_INPUT, "force not representable in f32");
+  if (s->prepared) {
+    float4 v = make_float4(f[0], f[1], f[2], 0.f);
+    HIP_TRY(hipMemcpy(s->force.as<float4>() + index, &v, sizeof(v), hipMemcpyHostToDevice));
+    s->force_pending = true;
+  }
+  for (int k = 0; k < 3; ++k) s->world.force[3 * index + k] = force[k];
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_set_obstacles(tmpm_session* s, const tmpm_obstacle* obs, uint32_t count) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  for (uint32_t i = 0; i < count; ++i)
+    if (!obstacle_valid(obs[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  std::vector<tmpm_obstacle> backup;
+  for (auto& o : s->world.obstacles) backup.push_back(o.o);
+  std::vector<std::vector<double>> backup_planes;
+  for (auto& o : s->world.obstacles) backup_planes.push_back(o.planes);
+  for (size_t i = 0; i < backup.size(); ++i) backup[i].planes = backup_planes[i].empty() ? nullptr : backup_planes[i].data();
+  s->world.set_obstacles(obs, count);
+  if (s->prepared) {
+    tmpm_status st = upload_obstacles(s);
+    if (st == TMPM_OK) st = upload_scene(s);
+    if (st != TMPM_OK) {
+      std::string msg = g_last_error;
+      s->world.set_obstacles(backup.data(), (uint32_t)backup.size());
+      if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
+      g_last_error = msg;
+      return st;
+    }
+  }
+  return TMPM_OK;
+}
+
+static tmpm_status replace_world(tmpm_session* s, const HostWorld& updated) {
+  tmpm_status st = validate_resident_world(updated, s->substep_dt);
+  if (st != TMPM_OK) return st;
+  HostWorld old = s->world;
+  s->world = updated;
+  st = prepare(s);
+  if (st != TMPM_OK) {
+    std::string msg = g_last_error;
+    s->world = old;
+    tmpm_status st2 = prepare(s);
+    if (st2 != TMPM_OK) s->valid = false;
+    g_last_error = msg;
+    return st;
+  }
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_add_particles(tmpm_session* s, const tmpm_particles* particles, const tmpm_material* materials,
+                                       uint32_t material_count, uint64_t* chunk_out) {
+  if (!s || !particles) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  if (particles->count == 0 || !validate_particles(*particles, materials, material_count))
+    return fail(TMPM_INVALID_INPUT, "invalid MPM particle batch");
+  if (s->world.next_chunk_id == UINT64_MAX) return fail(TMPM_PARTICLE_CAPACITY, "chunk ids exhausted");
+  HostWorld updated = s->world;
+  uint64_t chunk = updated.next_chunk_id++;
+  updated.append(*particles, materials, chunk);
+  tmpm_status st = replace_world(s, updated);
+  if (st != TMPM_OK) return st;
+  if (chunk_out) *chunk_out = chunk;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_remove_chunk(tmpm_session* s, uint64_t chunk, uint64_t* removed_out) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  HostWorld updated = s->world;
+  size_t removed = updated.remove_chunk(chunk);
+  if (removed == 0) return fail(TMPM_UNKNOWN_CHUNK, "MPM particle chunk was not found");
+  tmpm_status st = replace_world(s, updated);
+  if (st != TMPM_OK) return st;
+  if (removed_out) *removed_out = removed;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, uint64_t output_bytes, void* stream,
+                                         uint32_t* count_out) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (count_out) *count_out = 0;
+  if (!s->prepared) return TMPM_OK;
+  uint64_t bytes = 32ull * s->n;
+  if (!device_output || output_bytes < bytes) return fail(TMPM_CAPACITY, "snapshot buffer too small");
+  hipStream_t target = stream ? static_cast<hipStream_t>(stream) : s->stream;
+  if (target != s->stream) {
+    hipEvent_t ev;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ev, s->stream));
+    HIP_TRY(hipStreamWaitEvent(target, ev, 0));
+    HIP_TRY(hipEventDestroy(ev));
+  }
+  launch_snapshot(s->parts(s->cur), static_cast<float4*>(device_output), target);
+  HIP_TRY(hipGetLastError());
+  if (target != s->stream) {
+    hipEvent_t ev;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ev, target));
+    HIP_TRY(hipStreamWaitEvent(s->stream, ev, 0));
+    HIP_TRY(hipEventDestroy(ev));
+  }
+  if (count_out) *count_out = s->n;
+  return TMPM_OK;
+}
+
+double tmpm_session_stable_timestep(const tmpm_session* s) { return s ? s->world.stable_timestep() : 0.0; }
+
+tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* out) {
+  if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  std::memset(out, 0, sizeof(*out));
+  uint32_t mass_nodes = 0;
+  if (s->prepared) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(&mass_nodes, s->counters.as<uint32_t>() + CNT_MASS_NODES, 4, hipMemcpyDeviceToHost));
+  }
+  std::vector<uint32_t> c;
+  tmpm_status st = run_diagnostic_binning(s, c);
+  if (st != TMPM_OK) return st;
+  out->live_nodes = c[CNT_LIVE_NODES];
+  out->grid_blocks = c[CNT_GRID_BLOCKS];
+  out->particle_blocks = c[CNT_PARTICLE_BLOCKS];
+  out->binned_particles = c[CNT_BINNED];
+  out->mass_nodes = mass_nodes;
+  return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+}
+
+tmpm_status tmpm_session_export_nodes(tmpm_session* s, int32_t* keys, uint64_t cap, uint64_t* count_out) {
+  if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  std::vector<uint32_t> c;
+  tmpm_status st = run_diagnostic_binning(s, c);
+  if (st != TMPM_OK) return st;
+  *count_out = 0;
+  if (!s->prepared) return TMPM_OK;
+  uint32_t ng = c[CNT_GRID_BLOCKS];
+  std::vector<int4> gk(ng);
+  std::vector<uint32_t> masks(2ull * ng);
+  HIP_TRY(hipMemcpy(gk.data(), s->grid_key.ptr, 16ull * ng, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(masks.data(), s->node_mask.ptr, 8ull * ng, hipMemcpyDeviceToHost));
+  std::vector<std::array<int32_t, 4>> nodes;
+  for (uint32_t g = 0; g < ng; ++g)
+    for (int l = 0; l < 64; ++l)
+      if (masks[2 * g + (l >> 5)] & (1u << (l & 31)))
+        nodes.push_back({gk[g].x * 4 + (l >> 4), gk[g].y * 4 + ((l >> 2) & 3), gk[g].z * 4 + (l & 3),
+                         (int32_t)((uint32_t)gk[g].w & 0xffffu)});
+  std::sort(nodes.begin(), nodes.end());
+  for (uint64_t i = 0; i < nodes.size() && i < cap; ++i)
+    for (int k = 0; k < 4; ++k) keys[4 * i + k] = nodes[i][k];
+  *count_out = nodes.size();
+  return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+}
+
+tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uint64_t cap, int64_t* particle_block,
+                                       uint64_t* count_out) {
+  if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  std::vector<uint32_t> c;
+  tmpm_status st = run_diagnostic_binning(s, c);
+  if (st != TMPM_OK) return st;
+  *count_out = 0;
+  size_t total = s->world.size();
+  if (particle_block)
+    for (size_t i = 0; i < total; ++i) particle_block[i] = -1;
+  if (!s->prepared) return TMPM_OK;
+  uint32_t ng = c[CNT_GRID_BLOCKS];
+  uint32_t nb = s->topo.bucket_mask + 1;
+  std::vector<int4> gk(ng);
+  std::vector<uint32_t> pslot(s->n), bslot(nb), bpb(nb), ins(s->n);
+  HIP_TRY(hipMemcpy(gk.data(), s->grid_key.ptr, 16ull * ng, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pslot.data(), s->pslot.ptr, 4ull * s->n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(bslot.data(), s->bucket_slot.ptr, 4ull * nb, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(bpb.data(), s->bucket_pblock.ptr, 4ull * nb, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ins.data(), s->buf[s->cur].ins_id.ptr, 4ull * s->n, hipMemcpyDeviceToHost));
+  std::vector<std::array<int32_t, 4>> keys;
+  std::vector<std::array<int32_t, 4>> pkey(s->n);
+  std::vector<char> has(s->n, 0);
+  for (uint32_t i = 0; i < s->n; ++i) {
+    uint32_t b = pslot[i];
+    if (b == kInvalid || bpb[b] == kInvalid || bslot[b] == kInvalid || bslot[b] >= ng) continue;
+    int4 k = gk[bslot[b]];
+    pkey[i] = {k.x, k.y, k.z, (int32_t)((uint32_t)k.w & 0xffffu)};
+    has[i] = 1;
+    keys.push_back(pkey[i]);
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  for (uint64_t i = 0; i < keys.size() && i < cap; ++i)
+    for (int k = 0; k < 4; ++k) block_keys[4 * i + k] = keys[i][k];
+  if (particle_block)
+    for (uint32_t i = 0; i < s->n; ++i)
+      if (has[i]) particle_block[ins[i]] = std::lower_bound(keys.begin(), keys.end(), pkey[i]) - keys.begin();
+  *count_out = keys.size();
+  return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+}
+
+tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  collect_timers(s);
+  s->kernel_ms.clear();
+  s->timing = enabled != 0;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
+                                      uint64_t* calls, uint32_t cap, uint32_t* count_out) {
+  if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  collect_timers(s);
+  std::string csv;
+  uint32_t k = 0;
+  for (auto& kv : s->kernel_ms) {
+    if (k < cap) {
+      if (avg_ms) avg_ms[k] = kv.second.second ? kv.second.first / (double)kv.second.second : 0.0;
+      if (calls) calls[k] = kv.second.second;
+      if (!csv.empty()) csv += ",";
+      csv += kv.first;
+    }
+    ++k;
+  }
+  if (names_csv && names_cap) {
+    std::strncpy(names_csv, csv.c_str(), names_cap - 1);
+    names_csv[names_cap - 1] = '\0';
+  }
+  *count_out = k;
+  return TMPM_OK;
+}
+
+double tmpm_timestep_bound(const tmpm_material* material, double density, const double velocity[3], double cell_width,
+                           double deformation_det) {
+  return timestep_bound(*material, density, velocity, cell_width, deformation_det);
+}
+int32_t tmpm_material_valid(const tmpm_material* material) { return material && material_valid(*material) ? 1 : 0; }
+int32_t tmpm_obstacle_valid(const tmpm_obstacle* obstacle) { return obstacle && obstacle_valid(*obstacle) ? 1 : 0; }
+
+}  // extern "C"

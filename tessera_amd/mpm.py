@@ -1,0 +1,939 @@
+"""Python binding of the HIP MLS-MPM backend (ctypes over include/tessera_mpm.h).
+
+The classes mirror the reference crate's public types so that code written
+against tessera-mpm reads the same:
+
+=============================  ==================================================
+reference (crates/tessera-mpm)  this module
+=============================  ==================================================
+``MaterialModel``               :class:`MaterialModel` (material.rs:28-169)
+``PlasticState``                :class:`PlasticState` (material.rs:5-24)
+``MpmParticle::new``            :meth:`MpmParticle.new` (world.rs:118-150)
+``MpmParams`` / ``WorldBounds``  :class:`MpmParams`, :class:`WorldBounds` (world.rs:10-49)
+``RigidObstacle``               :class:`RigidObstacle` (obstacle.rs:79-262)
+``MpmWorld``                    :class:`MpmWorld` (world.rs:206-303), a container
+``GpuMpmTransfers``             :class:`HipMpmTransfers` (gpu.rs:160-167, 927-992)
+``GpuMpmResidentSession``       :class:`HipMpmResidentSession` (gpu.rs:206-662)
+``GpuMpmError``                 :class:`GpuMpmError` and subclasses (gpu.rs:135-154)
+=============================  ==================================================
+
+Bulk particle data lives in :class:`ParticleArrays` (structure of arrays); the
+per-object :class:`MpmParticle` exists for small scenes and tests.  Matrices
+are numpy ``(3, 3)`` arrays indexed ``[row, column]`` like nalgebra's
+``Matrix3``.
+
+This module performs no simulation on the CPU: stepping happens in the HIP
+library, which must be built (``tessera_amd.build``) and fails loudly when it
+is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Iterable, Sequence
+
+import numpy as np
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libtessera_mpm_hip.so"
+
+# ---------------------------------------------------------------------------
+# Status codes and errors (gpu.rs:135-154, world.rs:165-181)
+# ---------------------------------------------------------------------------
+OK = 0
+STATUS_NAMES = {
+    1: "InvalidInput",
+    2: "Capacity",
+    3: "PendingSteps",
+    4: "Readback",
+    5: "ResidentUnavailable",
+    6: "ExcessiveSubsteps",
+    7: "NonFiniteState",
+    8: "UnknownChunk",
+    9: "ParticleCapacity",
+    10: "NoDevice",
+}
+
+
+class GpuMpmError(Exception):
+    """Base of all backend errors; ``status`` is the C status code."""
+
+    status = -1
+
+
+def _error_class(name: str, code: int):
+    return type(name, (GpuMpmError,), {"status": code})
+
+
+InvalidInput = _error_class("InvalidInput", 1)
+Capacity = _error_class("Capacity", 2)
+PendingSteps = _error_class("PendingSteps", 3)
+Readback = _error_class("Readback", 4)
+ResidentUnavailable = _error_class("ResidentUnavailable", 5)
+ExcessiveSubsteps = _error_class("ExcessiveSubsteps", 6)
+NonFiniteState = _error_class("NonFiniteState", 7)
+UnknownChunk = _error_class("UnknownChunk", 8)
+ParticleCapacity = _error_class("ParticleCapacity", 9)
+NoDevice = _error_class("NoDevice", 10)
+_ERRORS = {cls.status: cls for cls in (InvalidInput, Capacity, PendingSteps, Readback, ResidentUnavailable,
+                                        ExcessiveSubsteps, NonFiniteState, UnknownChunk, ParticleCapacity, NoDevice)}
+
+# ---------------------------------------------------------------------------
+# C structures (include/tessera_mpm.h)
+# ---------------------------------------------------------------------------
+_dp = C.POINTER(C.c_double)
+
+
+class CMaterial(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("p", C.c_double * 5)]
+
+
+class CParams(C.Structure):
+    _fields_ = [
+        ("cell_width", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("max_substep", C.c_double),
+        ("has_bounds", C.c_int32),
+        ("use_cpic", C.c_int32),
+        ("bound_min", C.c_double * 3),
+        ("bound_max", C.c_double * 3),
+    ]
+
+
+class CObstacle(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32),
+        ("boundary", C.c_int32),
+        ("cpic_group", C.c_int32),
+        ("plane_count", C.c_uint32),
+        ("center", C.c_double * 3),
+        ("orientation", C.c_double * 4),
+        ("linear_velocity", C.c_double * 3),
+        ("angular_velocity", C.c_double * 3),
+        ("friction", C.c_double),
+        ("dims", C.c_double * 4),
+        ("vertices", C.c_double * 9),
+        ("planes", _dp),
+    ]
+
+
+class CParticles(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint64),
+        ("position", _dp),
+        ("velocity", _dp),
+        ("deformation", _dp),
+        ("affine", _dp),
+        ("force", _dp),
+        ("mass", _dp),
+        ("rest_volume", _dp),
+        ("radius", _dp),
+        ("damping", _dp),
+        ("plastic", _dp),
+        ("material_index", C.POINTER(C.c_uint32)),
+        ("group_id", C.POINTER(C.c_uint32)),
+        ("transfer_color", C.POINTER(C.c_uint16)),
+        ("enabled", C.POINTER(C.c_uint8)),
+        ("fixed", C.POINTER(C.c_uint8)),
+        ("chunk_id", C.POINTER(C.c_uint64)),
+    ]
+
+
+class CWorld(C.Structure):
+    _fields_ = [
+        ("params", CParams),
+        ("particles", CParticles),
+        ("materials", C.POINTER(CMaterial)),
+        ("material_count", C.c_uint32),
+        ("obstacle_count", C.c_uint32),
+        ("obstacles", C.POINTER(CObstacle)),
+        ("substeps", C.c_uint64),
+    ]
+
+
+class CTopologyStats(C.Structure):
+    _fields_ = [
+        ("live_nodes", C.c_uint64),
+        ("grid_blocks", C.c_uint64),
+        ("particle_blocks", C.c_uint64),
+        ("binned_particles", C.c_uint64),
+        ("mass_nodes", C.c_uint64),
+    ]
+
+
+# ---------------------------------------------------------------------------
+# Library loading
+# ---------------------------------------------------------------------------
+_LIB = None
+
+
+def load_library(path: Path | str | None = None) -> C.CDLL:
+    """Loads the HIP library (built in-tree); raises when it is missing."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    lib_path = Path(path) if path else LIBRARY_PATH
+    if not lib_path.exists():
+        raise RuntimeError(
+            f"HIP library {lib_path} is missing; build it with `python -m tessera_amd.build` "
+            "(there is no CPU fallback)"
+        )
+    lib = C.CDLL(str(lib_path))
+    vp = C.c_void_p
+    sig = {
+        "tmpm_last_error": (C.c_char_p, []),
+        "tmpm_version": (C.c_char_p, []),
+        "tmpm_context_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
+        "tmpm_context_destroy": (None, [vp]),
+        "tmpm_session_create": (C.c_int, [vp, C.POINTER(CWorld), C.c_double, C.c_int, C.POINTER(vp)]),
+        "tmpm_session_destroy": (None, [vp]),
+        "tmpm_session_submit_steps": (C.c_int, [vp, C.c_uint32]),
+        "tmpm_session_wait": (C.c_int, [vp]),
+        "tmpm_session_synchronize": (C.c_int, [vp]),
+        "tmpm_session_step": (C.c_int, [vp, C.c_uint32]),
+        "tmpm_session_step_adaptive": (C.c_int, [vp, C.c_double]),
+        "tmpm_session_pending_substeps": (C.c_uint64, [vp]),
+        "tmpm_session_valid": (C.c_int32, [vp]),
+        "tmpm_session_substeps": (C.c_uint64, [vp]),
+        "tmpm_session_particle_count": (C.c_uint64, [vp]),
+        "tmpm_session_read_particles": (C.c_int, [vp, C.POINTER(CParticles)]),
+        "tmpm_session_set_particle_force": (C.c_int, [vp, C.c_uint64, C.c_double * 3]),
+        "tmpm_session_set_obstacles": (C.c_int, [vp, C.POINTER(CObstacle), C.c_uint32]),
+        "tmpm_session_add_particles": (
+            C.c_int, [vp, C.POINTER(CParticles), C.POINTER(CMaterial), C.c_uint32, C.POINTER(C.c_uint64)]),
+        "tmpm_session_remove_chunk": (C.c_int, [vp, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "tmpm_session_encode_snapshot": (C.c_int, [vp, vp, C.c_uint64, vp, C.POINTER(C.c_uint32)]),
+        "tmpm_session_stable_timestep": (C.c_double, [vp]),
+        "tmpm_session_topology_stats": (C.c_int, [vp, C.POINTER(CTopologyStats)]),
+        "tmpm_session_export_nodes": (C.c_int, [vp, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(C.c_uint64)]),
+        "tmpm_session_export_blocks": (
+            C.c_int, [vp, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+        "tmpm_session_set_timing": (C.c_int, [vp, C.c_int32]),
+        "tmpm_session_kernel_times": (
+            C.c_int,
+            [vp, C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32,
+             C.POINTER(C.c_uint32)]),
+        "tmpm_timestep_bound": (C.c_double, [C.POINTER(CMaterial), C.c_double, C.c_double * 3, C.c_double, C.c_double]),
+        "tmpm_material_valid": (C.c_int32, [C.POINTER(CMaterial)]),
+        "tmpm_obstacle_valid": (C.c_int32, [C.POINTER(CObstacle)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names of every function the C header declares (checked against the .so)."""
+    header = Path(__file__).resolve().parent.parent / "include" / "tessera_mpm.h"
+    import re
+
+    text = header.read_text()
+    return sorted(set(re.findall(r"\b(tmpm_[a-z_0-9]+)\s*\(", text)))
+
+
+def _check(status: int, lib=None) -> None:
+    if status == OK:
+        return
+    lib = lib or load_library()
+    msg = lib.tmpm_last_error().decode(errors="replace")
+    cls = _ERRORS.get(status, GpuMpmError)
+    raise cls(msg or STATUS_NAMES.get(status, str(status)))
+
+
+# ---------------------------------------------------------------------------
+# Domain types (mirrors of the reference's Rust types)
+# ---------------------------------------------------------------------------
+class MaterialModel:
+    """Constitutive model (material.rs:28-169); ``kind`` follows variant order."""
+
+    LINEAR_ELASTIC = 0
+    NEO_HOOKEAN = 1
+    SAND = 2
+    FLUID = 3
+    SNOW = 4
+    SAND_NEO_HOOKEAN = 5
+
+    __slots__ = ("kind", "params")
+
+    def __init__(self, kind: int, params: Sequence[float]):
+        self.kind = int(kind)
+        p = [float(v) for v in params] + [0.0] * (5 - len(params))
+        self.params = tuple(p[:5])
+
+    @classmethod
+    def elastic(cls, young_modulus: float, poisson_ratio: float) -> "MaterialModel":
+        return cls(cls.LINEAR_ELASTIC, (young_modulus, poisson_ratio))
+
+    @classmethod
+    def neo_hookean(cls, young_modulus: float, poisson_ratio: float) -> "MaterialModel":
+        return cls(cls.NEO_HOOKEAN, (young_modulus, poisson_ratio))
+
+    @classmethod
+    def sand(cls, young_modulus, poisson_ratio, friction_angle, cohesion) -> "MaterialModel":
+        return cls(cls.SAND, (young_modulus, poisson_ratio, friction_angle, cohesion))
+
+    @classmethod
+    def sand_neo_hookean(cls, young_modulus, poisson_ratio, friction_angle, cohesion) -> "MaterialModel":
+        return cls(cls.SAND_NEO_HOOKEAN, (young_modulus, poisson_ratio, friction_angle, cohesion))
+
+    @classmethod
+    def fluid(cls, bulk_modulus: float, gamma: float, viscosity: float, tensile_stiffness: float = 0.25):
+        """``MaterialModel::fluid`` (material.rs:151-158) fixes tensile stiffness 0.25."""
+        return cls(cls.FLUID, (bulk_modulus, gamma, viscosity, tensile_stiffness))
+
+    @classmethod
+    def snow(cls, young_modulus: float, poisson_ratio: float) -> "MaterialModel":
+        """Stomakhin yield box defaults (material.rs:161-169)."""
+        return cls(cls.SNOW, (young_modulus, poisson_ratio, 2.5e-2, 7.5e-3, 10.0))
+
+    def to_c(self) -> CMaterial:
+        m = CMaterial()
+        m.kind = self.kind
+        for i, v in enumerate(self.params):
+            m.p[i] = v
+        return m
+
+    def key(self) -> tuple:
+        return (self.kind, self.params)
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, MaterialModel) and self.key() == other.key()
+
+    def __hash__(self) -> int:
+        return hash(self.key())
+
+    def __repr__(self) -> str:
+        return f"MaterialModel(kind={self.kind}, params={self.params})"
+
+
+@dataclass
+class PlasticState:
+    """material.rs:5-24."""
+
+    plastic_det: float = 1.0
+    hardening: float = 0.0
+    log_volume_gain: float = 0.0
+
+
+def _vec(v) -> np.ndarray:
+    return np.array(v, dtype=np.float64).reshape(3)
+
+
+@dataclass
+class MpmParticle:
+    """One material point (world.rs:67-102)."""
+
+    position: np.ndarray
+    velocity: np.ndarray
+    deformation: np.ndarray
+    affine: np.ndarray
+    force: np.ndarray
+    mass: float
+    rest_volume: float
+    radius: float
+    material: MaterialModel
+    plastic: PlasticState
+    damping: float = 0.0
+    group_id: int = 0
+    transfer_color: int = 0
+    enabled: bool = True
+    fixed: bool = False
+
+    @classmethod
+    def new(cls, position, radius: float, density: float, material: MaterialModel) -> "MpmParticle":
+        """``MpmParticle::new`` (world.rs:120-150): rest volume (2r)^3."""
+        rest_volume = (2.0 * radius) ** 3
+        hardening = 1.0 if material.kind in (MaterialModel.SAND, MaterialModel.SAND_NEO_HOOKEAN) else 0.0
+        return cls(
+            position=_vec(position),
+            velocity=np.zeros(3),
+            deformation=np.eye(3),
+            affine=np.zeros((3, 3)),
+            force=np.zeros(3),
+            mass=density * rest_volume,
+            rest_volume=rest_volume,
+            radius=float(radius),
+            material=material,
+            plastic=PlasticState(hardening=hardening),
+        )
+
+    def clone(self) -> "MpmParticle":
+        return MpmParticle(
+            position=self.position.copy(), velocity=self.velocity.copy(), deformation=self.deformation.copy(),
+            affine=self.affine.copy(), force=self.force.copy(), mass=self.mass, rest_volume=self.rest_volume,
+            radius=self.radius, material=self.material, plastic=PlasticState(**vars(self.plastic)),
+            damping=self.damping, group_id=self.group_id, transfer_color=self.transfer_color,
+            enabled=self.enabled, fixed=self.fixed)
+
+
+class ParticleArrays:
+    """Structure-of-arrays particle storage in insertion order."""
+
+    FIELDS = ("position", "velocity", "deformation", "affine", "force", "mass", "rest_volume", "radius", "damping",
+              "plastic", "material_index", "group_id", "transfer_color", "enabled", "fixed", "chunk_id")
+
+    def __init__(self, n: int, materials: Sequence[MaterialModel] | None = None):
+        self.position = np.zeros((n, 3))
+        self.velocity = np.zeros((n, 3))
+        self.deformation = np.tile(np.eye(3), (n, 1, 1))
+        self.affine = np.zeros((n, 3, 3))
+        self.force = np.zeros((n, 3))
+        self.mass = np.zeros(n)
+        self.rest_volume = np.zeros(n)
+        self.radius = np.zeros(n)
+        self.damping = np.zeros(n)
+        self.plastic = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+        self.material_index = np.zeros(n, dtype=np.uint32)
+        self.group_id = np.zeros(n, dtype=np.uint32)
+        self.transfer_color = np.zeros(n, dtype=np.uint16)
+        self.enabled = np.ones(n, dtype=bool)
+        self.fixed = np.zeros(n, dtype=bool)
+        self.chunk_id = np.zeros(n, dtype=np.uint64)
+        self.materials: list[MaterialModel] = list(materials or [])
+
+    def __len__(self) -> int:
+        return len(self.mass)
+
+    @classmethod
+    def from_particles(cls, particles: Iterable[MpmParticle]) -> "ParticleArrays":
+        plist = list(particles)
+        out = cls(len(plist))
+        index: dict[MaterialModel, int] = {}
+        for i, p in enumerate(plist):
+            out.position[i] = p.position
+            out.velocity[i] = p.velocity
+            out.deformation[i] = p.deformation
+            out.affine[i] = p.affine
+            out.force[i] = p.force
+            out.mass[i] = p.mass
+            out.rest_volume[i] = p.rest_volume
+            out.radius[i] = p.radius
+            out.damping[i] = p.damping
+            out.plastic[i] = (p.plastic.plastic_det, p.plastic.hardening, p.plastic.log_volume_gain)
+            if p.material not in index:
+                index[p.material] = len(out.materials)
+                out.materials.append(p.material)
+            out.material_index[i] = index[p.material]
+            out.group_id[i] = p.group_id
+            out.transfer_color[i] = p.transfer_color
+            out.enabled[i] = p.enabled
+            out.fixed[i] = p.fixed
+        return out
+
+    def particle(self, i: int) -> MpmParticle:
+        return MpmParticle(
+            position=self.position[i].copy(), velocity=self.velocity[i].copy(),
+            deformation=self.deformation[i].copy(), affine=self.affine[i].copy(), force=self.force[i].copy(),
+            mass=float(self.mass[i]), rest_volume=float(self.rest_volume[i]), radius=float(self.radius[i]),
+            material=self.materials[int(self.material_index[i])],
+            plastic=PlasticState(*map(float, self.plastic[i])), damping=float(self.damping[i]),
+            group_id=int(self.group_id[i]), transfer_color=int(self.transfer_color[i]),
+            enabled=bool(self.enabled[i]), fixed=bool(self.fixed[i]))
+
+    def to_particles(self) -> list[MpmParticle]:
+        return [self.particle(i) for i in range(len(self))]
+
+    def copy(self) -> "ParticleArrays":
+        out = ParticleArrays(0, self.materials)
+        for f in self.FIELDS:
+            setattr(out, f, getattr(self, f).copy())
+        return out
+
+    def concat(self, other: "ParticleArrays") -> "ParticleArrays":
+        out = ParticleArrays(0)
+        out.materials = list(self.materials)
+        remap = []
+        for m in other.materials:
+            if m in out.materials:
+                remap.append(out.materials.index(m))
+            else:
+                remap.append(len(out.materials))
+                out.materials.append(m)
+        for f in self.FIELDS:
+            a, b = getattr(self, f), getattr(other, f)
+            if f == "material_index":
+                b = np.array(remap, dtype=np.uint32)[b] if len(b) else b
+            setattr(out, f, np.concatenate([a, b.astype(a.dtype)]))
+        return out
+
+    def select(self, mask: np.ndarray) -> "ParticleArrays":
+        out = ParticleArrays(0, self.materials)
+        for f in self.FIELDS:
+            setattr(out, f, getattr(self, f)[mask].copy())
+        return out
+
+    # -- C view -----------------------------------------------------------
+    def c_view(self):
+        """Returns (CParticles, CMaterial array, keep-alive list)."""
+        n = len(self)
+        keep = []
+
+        def arr(a, dtype):
+            b = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(b)
+            return b
+
+        cp = CParticles()
+        cp.count = n
+        cp.position = arr(self.position.reshape(n, 3), np.float64).ctypes.data_as(_dp)
+        cp.velocity = arr(self.velocity.reshape(n, 3), np.float64).ctypes.data_as(_dp)
+        # nalgebra column-major: element (r, c) at c*3 + r == row-major transpose
+        cp.deformation = arr(np.transpose(self.deformation, (0, 2, 1)).reshape(n, 9), np.float64).ctypes.data_as(_dp)
+        cp.affine = arr(np.transpose(self.affine, (0, 2, 1)).reshape(n, 9), np.float64).ctypes.data_as(_dp)
+        cp.force = arr(self.force.reshape(n, 3), np.float64).ctypes.data_as(_dp)
+        cp.mass = arr(self.mass, np.float64).ctypes.data_as(_dp)
+        cp.rest_volume = arr(self.rest_volume, np.float64).ctypes.data_as(_dp)
+        cp.radius = arr(self.radius, np.float64).ctypes.data_as(_dp)
+        cp.damping = arr(self.damping, np.float64).ctypes.data_as(_dp)
+        cp.plastic = arr(self.plastic.reshape(n, 3), np.float64).ctypes.data_as(_dp)
+        cp.material_index = arr(self.material_index, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32))
+        cp.group_id = arr(self.group_id, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32))
+        cp.transfer_color = arr(self.transfer_color, np.uint16).ctypes.data_as(C.POINTER(C.c_uint16))
+        cp.enabled = arr(self.enabled, np.uint8).ctypes.data_as(C.POINTER(C.c_uint8))
+        cp.fixed = arr(self.fixed, np.uint8).ctypes.data_as(C.POINTER(C.c_uint8))
+        cp.chunk_id = arr(self.chunk_id, np.uint64).ctypes.data_as(C.POINTER(C.c_uint64))
+        mats = (CMaterial * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i] = m.to_c()
+        keep.append(mats)
+        return cp, mats, keep
+
+
+@dataclass
+class WorldBounds:
+    """world.rs:10-25."""
+
+    min: np.ndarray
+    max: np.ndarray
+
+
+@dataclass
+class MpmParams:
+    """world.rs:27-49 defaults: h=0.1, g=(0,0,-9.81), max_substep=1e-3, no bounds."""
+
+    cell_width: float = 0.1
+    gravity: np.ndarray = field(default_factory=lambda: np.array([0.0, 0.0, -9.81]))
+    max_substep: float = 1.0 / 1000.0
+    bounds: WorldBounds | None = None
+
+    def to_c(self, use_cpic: bool = True) -> CParams:
+        p = CParams()
+        p.cell_width = self.cell_width
+        for i in range(3):
+            p.gravity[i] = float(self.gravity[i])
+        p.max_substep = self.max_substep
+        p.has_bounds = 1 if self.bounds is not None else 0
+        p.use_cpic = 1 if use_cpic else 0
+        if self.bounds is not None:
+            for i in range(3):
+                p.bound_min[i] = float(self.bounds.min[i])
+                p.bound_max[i] = float(self.bounds.max[i])
+        return p
+
+
+class ObstacleShape:
+    SPHERE, BOX, CAPSULE, CYLINDER, CONE, GROUND, TRIANGLE_PRISM, CONVEX = range(8)
+
+
+class ObstacleBoundary:
+    """obstacle.rs:65-77 (Slip is the default)."""
+
+    SLIP, STICK, SEPARATE, NON_REFLECTING, FREE_SLIP = range(5)
+
+
+def quat_from_axis_angle(axis, angle: float) -> np.ndarray:
+    """Unit quaternion (i, j, k, w) of a rotation about ``axis``."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    s = math.sin(angle / 2.0)
+    return np.array([a[0] * s, a[1] * s, a[2] * s, math.cos(angle / 2.0)])
+
+
+IDENTITY_QUAT = np.array([0.0, 0.0, 0.0, 1.0])
+
+
+@dataclass
+class RigidObstacle:
+    """A prescribed rigid obstacle (obstacle.rs:79-262)."""
+
+    shape: int
+    center: np.ndarray
+    orientation: np.ndarray = field(default_factory=lambda: IDENTITY_QUAT.copy())
+    dims: tuple = (0.0, 0.0, 0.0, 0.0)
+    vertices: np.ndarray | None = None
+    planes: np.ndarray | None = None
+    linear_velocity: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    angular_velocity: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    friction: float = 0.0
+    boundary: int = ObstacleBoundary.SLIP
+    cpic_group: int | None = None
+
+    @classmethod
+    def sphere(cls, center, radius: float) -> "RigidObstacle":
+        return cls(ObstacleShape.SPHERE, _vec(center), dims=(radius, 0.0, 0.0, 0.0))
+
+    @classmethod
+    def cuboid(cls, center, half_extents, orientation=None) -> "RigidObstacle":
+        h = _vec(half_extents)
+        return cls(ObstacleShape.BOX, _vec(center), _q(orientation), dims=(h[0], h[1], h[2], 0.0))
+
+    @classmethod
+    def capsule(cls, center, half_height, radius, orientation=None) -> "RigidObstacle":
+        return cls(ObstacleShape.CAPSULE, _vec(center), _q(orientation), dims=(half_height, radius, 0.0, 0.0))
+
+    @classmethod
+    def cylinder(cls, center, half_height, radius, orientation=None) -> "RigidObstacle":
+        return cls(ObstacleShape.CYLINDER, _vec(center), _q(orientation), dims=(half_height, radius, 0.0, 0.0))
+
+    @classmethod
+    def cone(cls, center, half_height, radius, orientation=None) -> "RigidObstacle":
+        return cls(ObstacleShape.CONE, _vec(center), _q(orientation), dims=(half_height, radius, 0.0, 0.0))
+
+    @classmethod
+    def ground(cls, center, half_extents, orientation=None) -> "RigidObstacle":
+        return cls(ObstacleShape.GROUND, _vec(center), _q(orientation),
+                   dims=(float(half_extents[0]), float(half_extents[1]), 0.0, 0.0))
+
+    @classmethod
+    def triangle_prism(cls, center, vertices, half_thickness, orientation=None) -> "RigidObstacle":
+        return cls(ObstacleShape.TRIANGLE_PRISM, _vec(center), _q(orientation), dims=(half_thickness, 0, 0, 0),
+                   vertices=np.asarray(vertices, dtype=np.float64).reshape(3, 3))
+
+    @classmethod
+    def convex(cls, center, vertices, face_normals, orientation=None) -> "RigidObstacle":
+        """obstacle.rs:211-245: planes from face normals and support offsets."""
+        verts = np.asarray(vertices, dtype=np.float64).reshape(-1, 3)
+        normals = np.asarray(face_normals, dtype=np.float64).reshape(-1, 3)
+        planes = np.array([[n[0], n[1], n[2], max(float(n @ v) for v in verts)] for n in normals])
+        if len(verts) >= 4 and np.all(np.isfinite(verts)):
+            bound = float(max(np.linalg.norm(v) for v in verts))
+        else:
+            bound = float("nan")
+        return cls(ObstacleShape.CONVEX, _vec(center), _q(orientation), dims=(bound, 0, 0, 0), planes=planes)
+
+    def to_c(self, keep: list) -> CObstacle:
+        o = CObstacle()
+        o.shape = self.shape
+        o.boundary = self.boundary
+        o.cpic_group = -1 if self.cpic_group is None else int(self.cpic_group)
+        for i in range(3):
+            o.center[i] = float(self.center[i])
+            o.linear_velocity[i] = float(self.linear_velocity[i])
+            o.angular_velocity[i] = float(self.angular_velocity[i])
+        for i in range(4):
+            o.orientation[i] = float(self.orientation[i])
+            o.dims[i] = float(self.dims[i])
+        o.friction = float(self.friction)
+        if self.vertices is not None:
+            for i, v in enumerate(np.asarray(self.vertices).reshape(9)):
+                o.vertices[i] = float(v)
+        if self.planes is not None:
+            pl = np.ascontiguousarray(self.planes, dtype=np.float64).reshape(-1)
+            keep.append(pl)
+            o.planes = pl.ctypes.data_as(_dp)
+            o.plane_count = len(pl) // 4
+        return o
+
+
+def _q(orientation) -> np.ndarray:
+    return IDENTITY_QUAT.copy() if orientation is None else np.asarray(orientation, dtype=np.float64).reshape(4)
+
+
+def obstacles_to_c(obstacles: Sequence[RigidObstacle]):
+    keep: list = []
+    arr = (CObstacle * max(1, len(obstacles)))()
+    for i, o in enumerate(obstacles):
+        arr[i] = o.to_c(keep)
+    keep.append(arr)
+    return arr, keep
+
+
+class MpmWorld:
+    """A world container (world.rs:206-221): particles in insertion order,
+    parameters, obstacles and the substep count.  Stepping is done by a
+    :class:`HipMpmResidentSession`."""
+
+    def __init__(self, particles, params: MpmParams | None = None):
+        if not isinstance(particles, ParticleArrays):
+            particles = ParticleArrays.from_particles(particles)
+        self.particles = particles
+        self.params = params or MpmParams()
+        self.obstacles: list[RigidObstacle] = []
+        self.substeps = 0
+        self.use_cpic = True
+
+    def set_obstacles(self, obstacles: Sequence[RigidObstacle]) -> None:
+        self.obstacles = list(obstacles)
+
+    def clone(self) -> "MpmWorld":
+        w = MpmWorld(self.particles.copy(), self.params)
+        w.obstacles = list(self.obstacles)
+        w.substeps = self.substeps
+        w.use_cpic = self.use_cpic
+        return w
+
+    def stable_timestep(self) -> float:
+        """``MpmWorld::stable_timestep`` (world.rs:289-303), evaluated by the library."""
+        lib = load_library()
+        p = self.particles
+        s = self.params.max_substep
+        for i in range(len(p)):
+            if not p.enabled[i] or p.fixed[i]:
+                continue
+            m = p.materials[int(p.material_index[i])].to_c()
+            v = (C.c_double * 3)(*map(float, p.velocity[i]))
+            b = lib.tmpm_timestep_bound(C.byref(m), float(p.mass[i] / p.rest_volume[i]), v, self.params.cell_width,
+                                        float(np.linalg.det(p.deformation[i])))
+            s = min(s, b)
+        return min(s, self.params.max_substep)
+
+    def c_desc(self):
+        cp, mats, keep = self.particles.c_view()
+        obs, okeep = obstacles_to_c(self.obstacles)
+        keep += okeep
+        d = CWorld()
+        d.params = self.params.to_c(self.use_cpic)
+        d.particles = cp
+        d.materials = C.cast(mats, C.POINTER(CMaterial))
+        d.material_count = len(self.particles.materials)
+        d.obstacle_count = len(self.obstacles)
+        d.obstacles = C.cast(obs, C.POINTER(CObstacle))
+        d.substeps = self.substeps
+        keep.append(d)
+        return d, keep
+
+
+# ---------------------------------------------------------------------------
+# Device context and resident session
+# ---------------------------------------------------------------------------
+class HipMpmTransfers:
+    """Device context (the reference's ``GpuMpmTransfers``)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.tmpm_context_create(int(device), C.byref(h)), self.lib)
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            self.lib.tmpm_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+P2G_MODES = {"fp32": 0, "fixed64": 1}
+
+
+class HipMpmResidentSession:
+    """Resident HIP session (the reference's ``GpuMpmResidentSession``).
+
+    ``p2g_mode`` selects FP32 atomics ("fp32") or 64-bit fixed-point
+    accumulation ("fixed64", bit-reproducible)."""
+
+    def __init__(self, transfers: HipMpmTransfers, world: MpmWorld, substep_dt: float, p2g_mode: str = "fp32"):
+        self.lib = transfers.lib
+        self.transfers = transfers
+        self._world = world.clone()
+        desc, keep = world.c_desc()
+        h = C.c_void_p()
+        _check(self.lib.tmpm_session_create(transfers.handle, C.byref(desc), float(substep_dt),
+                                            P2G_MODES[p2g_mode], C.byref(h)), self.lib)
+        del keep
+        self._h = h
+        self.substep_dt = substep_dt
+        self.p2g_mode = p2g_mode
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.tmpm_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping (gpu.rs:286-450) -----------------------------------------
+    def step(self, steps: int) -> None:
+        _check(self.lib.tmpm_session_step(self._h, int(steps)), self.lib)
+
+    def submit_steps(self, steps: int) -> None:
+        _check(self.lib.tmpm_session_submit_steps(self._h, int(steps)), self.lib)
+
+    def synchronize(self) -> None:
+        _check(self.lib.tmpm_session_synchronize(self._h), self.lib)
+
+    def wait(self) -> None:
+        """Device completion only (no readback, no validation)."""
+        _check(self.lib.tmpm_session_wait(self._h), self.lib)
+
+    def step_adaptive(self, dt: float) -> None:
+        """CFL-limited substeps (``MpmWorld::step_with_gpu_transfers``)."""
+        _check(self.lib.tmpm_session_step_adaptive(self._h, float(dt)), self.lib)
+
+    def has_pending_steps(self) -> bool:
+        return self.pending_substeps() > 0
+
+    def pending_substeps(self) -> int:
+        return int(self.lib.tmpm_session_pending_substeps(self._h))
+
+    @property
+    def valid(self) -> bool:
+        return bool(self.lib.tmpm_session_valid(self._h))
+
+    @property
+    def substeps(self) -> int:
+        return int(self.lib.tmpm_session_substeps(self._h))
+
+    def stable_timestep(self) -> float:
+        return float(self.lib.tmpm_session_stable_timestep(self._h))
+
+    # -- state ---------------------------------------------------------------
+    def read_particles(self) -> ParticleArrays:
+        n = int(self.lib.tmpm_session_particle_count(self._h))
+        out = ParticleArrays(n)
+        flat = {
+            "position": np.zeros((n, 3)), "velocity": np.zeros((n, 3)), "deformation": np.zeros((n, 9)),
+            "affine": np.zeros((n, 9)), "force": np.zeros((n, 3)), "mass": np.zeros(n), "rest_volume": np.zeros(n),
+            "radius": np.zeros(n), "damping": np.zeros(n), "plastic": np.zeros((n, 3)),
+        }
+        ints = {
+            "group_id": np.zeros(n, np.uint32), "transfer_color": np.zeros(n, np.uint16),
+            "enabled": np.zeros(n, np.uint8), "fixed": np.zeros(n, np.uint8), "chunk_id": np.zeros(n, np.uint64),
+        }
+        cp = CParticles()
+        cp.count = n
+        for k, a in flat.items():
+            setattr(cp, k, a.ctypes.data_as(_dp))
+        cp.group_id = ints["group_id"].ctypes.data_as(C.POINTER(C.c_uint32))
+        cp.transfer_color = ints["transfer_color"].ctypes.data_as(C.POINTER(C.c_uint16))
+        cp.enabled = ints["enabled"].ctypes.data_as(C.POINTER(C.c_uint8))
+        cp.fixed = ints["fixed"].ctypes.data_as(C.POINTER(C.c_uint8))
+        cp.chunk_id = ints["chunk_id"].ctypes.data_as(C.POINTER(C.c_uint64))
+        _check(self.lib.tmpm_session_read_particles(self._h, C.byref(cp)), self.lib)
+        out.position = flat["position"]
+        out.velocity = flat["velocity"]
+        out.deformation = flat["deformation"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
+        out.affine = flat["affine"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
+        out.force = flat["force"]
+        out.mass = flat["mass"]
+        out.rest_volume = flat["rest_volume"]
+        out.radius = flat["radius"]
+        out.damping = flat["damping"]
+        out.plastic = flat["plastic"]
+        out.group_id = ints["group_id"]
+        out.transfer_color = ints["transfer_color"]
+        out.enabled = ints["enabled"].astype(bool)
+        out.fixed = ints["fixed"].astype(bool)
+        out.chunk_id = ints["chunk_id"]
+        out.materials = list(self._world.particles.materials)
+        out.material_index = self._material_index(n)
+        return out
+
+    def _material_index(self, n: int) -> np.ndarray:
+        idx = self._world.particles.material_index
+        if len(idx) == n:
+            return idx.copy()
+        return np.zeros(n, dtype=np.uint32)
+
+    def world(self) -> MpmWorld:
+        """Last synchronized world (``world()``, gpu.rs:653-656)."""
+        w = MpmWorld(self.read_particles(), self._world.params)
+        w.obstacles = list(self._world.obstacles)
+        w.use_cpic = self._world.use_cpic
+        w.substeps = self.substeps
+        return w
+
+    # -- edits ----------------------------------------------------------------
+    def set_particle_force(self, index: int, force) -> None:
+        f = (C.c_double * 3)(*map(float, force))
+        _check(self.lib.tmpm_session_set_particle_force(self._h, int(index), f), self.lib)
+
+    def set_obstacles(self, obstacles: Sequence[RigidObstacle]) -> None:
+        arr, keep = obstacles_to_c(obstacles)
+        _check(self.lib.tmpm_session_set_obstacles(self._h, C.cast(arr, C.POINTER(CObstacle)), len(obstacles)),
+               self.lib)
+        self._world.obstacles = list(obstacles)
+
+    def add_particles(self, particles) -> int:
+        if not isinstance(particles, ParticleArrays):
+            particles = ParticleArrays.from_particles(particles)
+        cp, mats, keep = particles.c_view()
+        chunk = C.c_uint64()
+        _check(self.lib.tmpm_session_add_particles(self._h, C.byref(cp), C.cast(mats, C.POINTER(CMaterial)),
+                                                   len(particles.materials), C.byref(chunk)), self.lib)
+        added = particles.copy()
+        added.chunk_id[:] = chunk.value
+        self._world.particles = self._world.particles.concat(added)
+        return int(chunk.value)
+
+    def remove_chunk(self, chunk: int) -> int:
+        removed = C.c_uint64()
+        _check(self.lib.tmpm_session_remove_chunk(self._h, int(chunk), C.byref(removed)), self.lib)
+        # keep the material table aligned with the remaining particles
+        self._world.particles = self._world.particles.select(self._world.particles.chunk_id != int(chunk))
+        return int(removed.value)
+
+    def encode_particle_snapshot(self, device_ptr: int, nbytes: int, stream: int | None = None) -> int:
+        """Writes 32-byte {position, mass; velocity, volume} records (gpu.rs:595-651)."""
+        count = C.c_uint32()
+        _check(self.lib.tmpm_session_encode_snapshot(self._h, C.c_void_p(device_ptr), int(nbytes),
+                                                     C.c_void_p(stream) if stream else None, C.byref(count)),
+               self.lib)
+        return int(count.value)
+
+    # -- diagnostics ------------------------------------------------------------
+    def topology_stats(self) -> dict:
+        st = CTopologyStats()
+        _check(self.lib.tmpm_session_topology_stats(self._h, C.byref(st)), self.lib)
+        return {k: int(getattr(st, k)) for k, _ in CTopologyStats._fields_}
+
+    def export_nodes(self) -> np.ndarray:
+        count = C.c_uint64()
+        _check(self.lib.tmpm_session_export_nodes(self._h, None, 0, C.byref(count)), self.lib)
+        keys = np.zeros((max(1, count.value), 4), dtype=np.int32)
+        _check(self.lib.tmpm_session_export_nodes(self._h, keys.ctypes.data_as(C.POINTER(C.c_int32)), count.value,
+                                                  C.byref(count)), self.lib)
+        return keys[: count.value]
+
+    def export_blocks(self) -> tuple[np.ndarray, np.ndarray]:
+        n = int(self.lib.tmpm_session_particle_count(self._h))
+        count = C.c_uint64()
+        part = np.zeros(max(1, n), dtype=np.int64)
+        _check(self.lib.tmpm_session_export_blocks(self._h, None, 0, part.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                   C.byref(count)), self.lib)
+        keys = np.zeros((max(1, count.value), 4), dtype=np.int32)
+        _check(self.lib.tmpm_session_export_blocks(self._h, keys.ctypes.data_as(C.POINTER(C.c_int32)), count.value,
+                                                   part.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(count)),
+               self.lib)
+        return keys[: count.value], part[:n]
+
+    def set_timing(self, enabled: bool) -> None:
+        _check(self.lib.tmpm_session_set_timing(self._h, 1 if enabled else 0), self.lib)
+
+    def kernel_times(self) -> dict[str, tuple[float, int]]:
+        cap = 64
+        names = C.create_string_buffer(4096)
+        avg = (C.c_double * cap)()
+        calls = (C.c_uint64 * cap)()
+        count = C.c_uint32()
+        _check(self.lib.tmpm_session_kernel_times(self._h, names, 4096, avg, calls, cap, C.byref(count)), self.lib)
+        keys = names.value.decode().split(",") if names.value else []
+        return {k: (avg[i], int(calls[i])) for i, k in enumerate(keys)}

@@ -1,0 +1,129 @@
+"""Seeded synthetic scenes for the benchmark and the tests.
+
+Every generator is deterministic (numpy PCG64 via ``default_rng(seed)``).
+Units are SI and gravity points along -y.  Particle volume and mass follow
+``MpmParticle::new`` (crates/tessera-mpm/src/world.rs:120-150): rest volume
+(2r)^3 and mass density * volume.
+
+* :func:`uniform_cube`       8,192 particles uniform in [0.4, 0.6]^3, elastic
+                             E=1e4, nu=0.3, rho=1e3, h=1/64, dt=1e-4, sticky
+                             ground at y=0.05 (200 steps).
+* :func:`four_cubes`         1,048,576 particles: four jittered 64^3 lattices of
+                             edge 0.2, velocities N(0, 0.5) per component,
+                             elastic, h=1/256 (4^3-cell blocks over a 256^3
+                             domain), dt=5e-5 (1000 steps).
+* :func:`dam_break`          4,194,304 fluid particles (K=1e5, gamma=7) filling
+                             [0,0.4]x[0,0.6]x[0,1] at 2 per cell; h=0.0048559
+                             so that N, the particles per cell and a 512^3 cell
+                             domain agree; dt=2e-5 (2000 steps).
+"""
+
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .mpm import (MaterialModel, MpmParams, MpmWorld, ObstacleBoundary, ParticleArrays, RigidObstacle, WorldBounds,
+                  quat_from_axis_angle)
+
+GRAVITY_Y = np.array([0.0, -9.8, 0.0])
+
+
+@dataclass
+class Scene:
+    name: str
+    world: MpmWorld
+    dt: float
+    steps: int
+
+
+def _particles(pos: np.ndarray, vel: np.ndarray, radius: float, density: float, material: MaterialModel) -> ParticleArrays:
+    n = len(pos)
+    pa = ParticleArrays(n, [material])
+    pa.position[:] = pos
+    pa.velocity[:] = vel
+    vol = (2.0 * radius) ** 3
+    pa.rest_volume[:] = vol
+    pa.mass[:] = density * vol
+    pa.radius[:] = radius
+    return pa
+
+
+def sticky_floor(y: float = 0.05) -> RigidObstacle:
+    """Finite ground with local +z mapped to world +y, Stick response
+    (obstacle.rs:185-191, 517-519)."""
+    q = quat_from_axis_angle([1.0, 0.0, 0.0], -math.pi / 2)
+    g = RigidObstacle.ground([0.5, y, 0.5], [1.0, 1.0], q)
+    g.boundary = ObstacleBoundary.STICK
+    return g
+
+
+def uniform_cube(n: int = 8192, seed: int = 0) -> Scene:
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0.4, 0.6, size=(n, 3))
+    vol = 0.008 / n
+    radius = 0.5 * vol ** (1.0 / 3.0)
+    pa = _particles(pos, np.zeros((n, 3)), radius, 1.0e3, MaterialModel.elastic(1.0e4, 0.3))
+    params = MpmParams(cell_width=1.0 / 64.0, gravity=GRAVITY_Y.copy(), max_substep=1.0e-4,
+                       bounds=WorldBounds(np.zeros(3), np.ones(3)))
+    w = MpmWorld(pa, params)
+    w.set_obstacles([sticky_floor(0.05)])
+    return Scene("uniform_cube_8k", w, 1.0e-4, 200)
+
+
+def jittered_lattice(rng, counts, lo, hi) -> np.ndarray:
+    counts = np.asarray(counts)
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    spacing = (hi - lo) / counts
+    idx = np.stack(np.meshgrid(*[np.arange(c) for c in counts], indexing="ij"), axis=-1).reshape(-1, 3)
+    jitter = rng.uniform(-0.5, 0.5, size=idx.shape)
+    return lo + (idx + 0.5 + jitter) * spacing
+
+
+FOUR_CUBE_CENTERS = np.array([[0.3, 0.3, 0.3], [0.7, 0.3, 0.7], [0.3, 0.7, 0.7], [0.7, 0.7, 0.3]])
+
+
+def four_cubes(per_axis: int = 64, seed: int = 0, edge: float = 0.2) -> Scene:
+    rng = np.random.default_rng(seed)
+    parts = []
+    for c in FOUR_CUBE_CENTERS:
+        parts.append(jittered_lattice(rng, [per_axis] * 3, c - edge / 2, c + edge / 2))
+    pos = np.concatenate(parts)
+    n = len(pos)
+    vel = rng.normal(0.0, 0.5, size=(n, 3))
+    spacing = edge / per_axis
+    pa = _particles(pos, vel, spacing / 2, 1.0e3, MaterialModel.elastic(1.0e4, 0.3))
+    params = MpmParams(cell_width=1.0 / 256.0, gravity=GRAVITY_Y.copy(), max_substep=5.0e-5,
+                       bounds=WorldBounds(np.zeros(3), np.ones(3)))
+    return Scene(f"four_cubes_{n}", MpmWorld(pa, params), 5.0e-5, 1000)
+
+
+DAM_CELL = 0.0048559
+
+
+def dam_break(counts=(128, 128, 256), seed: int = 0) -> Scene:
+    rng = np.random.default_rng(seed)
+    extent = np.array([0.4, 0.6, 1.0])
+    pos = jittered_lattice(rng, counts, np.zeros(3), extent)
+    n = len(pos)
+    vol = float(np.prod(extent)) / n
+    radius = 0.5 * vol ** (1.0 / 3.0)
+    pa = _particles(pos, np.zeros((n, 3)), radius, 1.0e3, MaterialModel.fluid(1.0e5, 7.0, 0.0))
+    h = DAM_CELL
+    params = MpmParams(cell_width=h, gravity=GRAVITY_Y.copy(), max_substep=2.0e-5,
+                       bounds=WorldBounds(np.zeros(3), np.full(3, 512 * h)))
+    return Scene(f"dam_break_{n}", MpmWorld(pa, params), 2.0e-5, 2000)
+
+
+def record_hash(pa: ParticleArrays, count: int = 4096) -> str:
+    """SHA-256 of the first ``count`` position/velocity records (f64 bytes)."""
+    m = min(count, len(pa))
+    data = np.concatenate([pa.position[:m].ravel(), pa.velocity[:m].ravel(), pa.mass[:m], pa.rest_volume[:m]])
+    return hashlib.sha256(np.ascontiguousarray(data, dtype=np.float64).tobytes()).hexdigest()
+
+
+SCENES = {"c1": uniform_cube, "c2": four_cubes, "c3": dam_break}

@@ -1,0 +1,142 @@
+"""Benchmark-scene parity: the 8,192-particle cube against the FP64 oracle
+over its whole 200-step run, bit-exact topology and size-independent
+properties at 1,048,576 and 4,194,304 particles, and run-to-run bit
+equality of the fixed-point P2G mode."""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tessera_amd.mpm import HipMpmResidentSession, MpmParams, MpmWorld
+from tessera_amd.scenes import dam_break, four_cubes, uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def run(transfers, world, dt, steps, mode="fp32"):
+    s = HipMpmResidentSession(transfers, world, dt, mode)
+    left = steps
+    while left:
+        k = min(64, left)
+        s.submit_steps(k)
+        left -= k
+    s.synchronize()
+    return s
+
+
+def errors(ref, got):
+    def e(a, b):
+        return float(np.max(np.linalg.norm((a - b).reshape(len(a), -1), axis=1)))
+
+    return dict(position=e(ref["position"], got.position), velocity=e(ref["velocity"], got.velocity),
+                affine=e(ref["affine"], got.affine), deformation=e(ref["deformation"], got.deformation))
+
+
+# Reference tier of gpu.rs:4043-4098 (elastic particles): position 1e-5,
+# velocity 1e-4, affine 1e-3, deformation 1e-5.
+C1_TOLERANCE = dict(position=1e-5, velocity=1e-4, affine=1e-3, deformation=1e-5)
+
+
+@pytest.fixture(scope="module")
+def c1_reference():
+    scene = uniform_cube()
+    cpu = O.OracleWorld(scene.world)
+    cpu.substeps_fixed(scene.dt, scene.steps)
+    return scene, cpu.read()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_uniform_cube_200_steps_tracks_fp64_oracle(transfers, c1_reference, mode):
+    scene, ref = c1_reference
+    s = run(transfers, scene.world, scene.dt, scene.steps, mode)
+    assert s.substeps == scene.steps
+    err = errors(ref, s.read_particles())
+    print(f"uniform cube, {mode}: error vs FP64 oracle after {scene.steps} steps: {err}")
+    for k, tol in C1_TOLERANCE.items():
+        assert err[k] < tol, (k, err[k], tol)
+
+
+def test_fixed64_is_bit_reproducible_run_to_run(transfers):
+    scene = uniform_cube()
+    a = run(transfers, scene.world, scene.dt, 50, "fixed64").read_particles()
+    b = run(transfers, scene.world, scene.dt, 50, "fixed64").read_particles()
+    for f in ("position", "velocity", "affine", "deformation"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+@pytest.fixture(scope="module")
+def c2_scene():
+    return four_cubes()
+
+
+def test_four_cubes_topology_is_bit_exact_at_full_size(transfers, c2_scene):
+    scene = c2_scene
+    s = HipMpmResidentSession(transfers, scene.world, scene.dt)
+    pa = scene.world.particles
+    h = scene.world.params.cell_width
+    keys, part = s.export_blocks()
+    ref_keys, ref_part = O.block_partition(pa.position, h, pa.enabled.astype(np.uint8), pa.transfer_color)
+    assert np.array_equal(keys, ref_keys)
+    assert np.array_equal(part, ref_part)
+    nodes = s.export_nodes()
+    ref_nodes = O.node_set(pa.position, h, pa.enabled.astype(np.uint8), pa.transfer_color)
+    assert len(nodes) == len(ref_nodes)
+    assert np.array_equal(nodes, ref_nodes)
+    stats = s.topology_stats()
+    assert stats["live_nodes"] == len(ref_nodes)
+    assert stats["particle_blocks"] == len(ref_keys)
+
+
+def test_four_cubes_two_substeps_match_fp64_oracle_at_full_size(transfers, c2_scene):
+    scene = c2_scene
+    cpu = O.OracleWorld(scene.world)
+    cpu.substeps_fixed(scene.dt, 2)
+    s = run(transfers, scene.world, scene.dt, 2)
+    err = errors(cpu.read(), s.read_particles())
+    print(f"four cubes, 2 substeps, error vs FP64 oracle: {err}")
+    assert err["position"] < 1e-5 and err["velocity"] < 1e-4 and err["deformation"] < 1e-5
+
+
+def test_four_cubes_momentum_follows_gravity_impulse(transfers, c2_scene):
+    # P2G conserves momentum (partition of unity), the grid update adds g dt,
+    # G2P conserves it again; far from walls the total momentum after k
+    # substeps is p0 + M g k dt in both P2G modes.
+    scene = c2_scene
+    pa = scene.world.particles
+    m = pa.mass
+    p0 = (m[:, None] * pa.velocity).sum(axis=0)
+    k = 40
+    expected = p0 + m.sum() * scene.world.params.gravity * k * scene.dt
+    finals = {}
+    for mode in ("fp32", "fixed64"):
+        s = run(transfers, scene.world, scene.dt, k, mode)
+        got = s.read_particles()
+        p1 = (m[:, None] * got.velocity).sum(axis=0)
+        rel = np.linalg.norm(p1 - expected) / (m.sum() * 0.5)
+        print(f"four cubes {mode}: momentum defect relative to M*0.5 m/s = {rel:.3e}")
+        assert rel < 1e-5
+        finals[mode] = got
+    # the two precision modes agree to FP32 accuracy
+    d = np.max(np.linalg.norm(finals["fp32"].position - finals["fixed64"].position, axis=1))
+    assert d < 1e-6
+
+
+def test_dam_break_partition_and_mass_conservation_at_full_size(transfers):
+    scene = dam_break()
+    pa = scene.world.particles
+    s = HipMpmResidentSession(transfers, scene.world, scene.dt)
+    keys, part = s.export_blocks()
+    ref_keys, ref_part = O.block_partition(pa.position, scene.world.params.cell_width, pa.enabled.astype(np.uint8),
+                                           pa.transfer_color)
+    assert np.array_equal(keys, ref_keys) and np.array_equal(part, ref_part)
+    s.submit_steps(20)
+    s.synchronize()
+    got = s.read_particles()
+    # fluid F stays isotropic (project_deformation, material.rs:293-296)
+    off = got.deformation - np.einsum("n,ij->nij", got.deformation[:, 0, 0], np.eye(3))
+    assert np.max(np.abs(off)) < 1e-6
+    m = pa.mass
+    p = (m[:, None] * got.velocity).sum(axis=0)
+    # in 20 substeps only the floor (y=0) and the x=0, z=0 walls act; the
+    # gravity impulse bounds the total downward momentum
+    assert p[1] <= 1e-9 and p[1] >= m.sum() * scene.world.params.gravity[1] * 20 * scene.dt * 1.0001
