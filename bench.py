@@ -94,7 +94,7 @@ def main() -> int:
     times = session.kernel_times()
     session.set_timing(False)
     stats = session.topology_stats()
-    live = stats["mass_nodes"]
+    live = stats["live_nodes"]
 
     substep_ms = sum(ms for ms, _ in times.values())
     kernels = {k: {"avg_ms": ms, "calls": c} for k, (ms, c) in sorted(times.items(), key=lambda kv: -kv[1][0])}

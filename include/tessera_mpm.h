@@ -242,7 +242,6 @@ typedef struct tmpm_topology_stats {
   uint64_t grid_blocks;
   uint64_t particle_blocks;
   uint64_t binned_particles;
-  uint64_t mass_nodes;  /* nodes with positive mass in the last grid update */
 } tmpm_topology_stats;
 tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* out);
 

@@ -9,7 +9,9 @@
 //   cls[i]       u32     particle class (material, radius, damping, flags, region)
 //   ins_id[i]    u32     insertion index of the particle stored in slot i
 // Grid: 4x4x4-cell blocks; node (slot s, local l) at s*64 + l with
-// l = (x&3)*16 + (y&3)*4 + (z&3).
+// l = (x&3)*16 + (y&3)*4 + (z&3).  P2G writes one 6^3-node tile per particle
+// block (ptile); the grid update sums, in a fixed order, the up to 8 tiles
+// that cover each node (grid_rev), so no global atomics are needed.
 #pragma once
 
 #include <cstdint>
@@ -32,7 +34,6 @@ enum Counter : uint32_t {
   CNT_PARTICLE_BLOCKS = 1,
   CNT_BINNED = 2,
   CNT_ERROR = 3,
-  CNT_MASS_NODES = 4,
   CNT_LIVE_NODES = 5,
   CNT_COUNT = 8
 };
@@ -121,7 +122,8 @@ struct Topo {
   uint32_t* prank;          // particle -> rank within its block
   uint32_t* sorted;         // binned order -> particle storage index
   uint32_t* counters;
-  void* grid_acc;           // FP32: float4 per node; FIXED64: 4 x int64 per node
+  void* ptile;              // per particle block: its 6^3 P2G tile (FP32: float4; FIXED64: 4 x int64 per node)
+  uint32_t* grid_rev;       // per grid slot, 8 entries: particle block whose tile covers it at offset d
   float4* grid_vel;
   uint32_t* node_mask;      // 2 words per grid slot (diagnostic live-node set)
 };

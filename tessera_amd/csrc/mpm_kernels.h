@@ -14,7 +14,7 @@ void launch_scan(const Parts& p, const Topo& t, const DevClass* c, const DevScen
 void launch_pblock_finalize(const Topo& t, hipStream_t s);
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s);
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s);
-void launch_grid_clear(const Topo& t, int fixed64, hipStream_t s);
+void launch_rev_clear(const Topo& t, hipStream_t s);
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
                 uint32_t threads, hipStream_t s);
 void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, hipStream_t s);

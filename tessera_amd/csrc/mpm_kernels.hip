@@ -13,10 +13,13 @@
 //   5 k_pblock_finalize   per particle block: grid slots of its 8 tile blocks
 //   6 k_bin_scatter       particle -> position in block-sorted order (optionally
 //                         moving the particle state itself: periodic reorder)
-//   7 k_grid_clear        zero the P2G accumulator of the live grid slots
+//   7 k_rev_clear         clear the grid-slot -> covering-tile map
+//   5 k_pblock_finalize   (fills that map)
 //   8 k_p2g               per particle block: stress, 27-node B-spline scatter
-//                         into a 6^3 LDS tile, flush with global atomics
-//   9 k_grid_update       per node: v = p/m + g dt, walls, obstacles
+//                         into a 6^3 LDS tile accumulated in 64-bit fixed point
+//                         (integer LDS atomics), stored as the block's tile
+//   9 k_grid_update       per node: sum of the covering tiles in a fixed order,
+//                         v = p/m + g dt, walls, obstacles
 //  10 k_g2p               per particle block: 6^3 velocity tile in LDS, gather,
 //                         v/C/x/F update, material projection, validity flags
 // The reference computes the same substep with an open-addressed hash of
@@ -331,7 +334,10 @@ __global__ __launch_bounds__(256) void k_pblock_finalize(Topo t) {
     uint32_t k = q >> 3, d = q & 7;
     uint32_t b = t.pblock_bucket[k];
     uint32_t nb = t.nbr_bucket[8 * b + d];
-    t.pblock_slot8[q] = nb == kInvalid ? kInvalid : t.bucket_slot[nb];
+    uint32_t slot = nb == kInvalid ? kInvalid : t.bucket_slot[nb];
+    t.pblock_slot8[q] = slot;
+    // grid block `slot` = this block + d: exactly one particle block per (slot, d)
+    if (slot != kInvalid) t.grid_rev[8 * slot + d] = k;
   }
 }
 
@@ -378,14 +384,12 @@ __global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo
 }
 
 // ---------------------------------------------------------------------------
-// 7. zero the accumulator of live grid slots
+// 7. clear the grid-slot -> covering-tile map of the live grid slots
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grid_clear(Topo t, int fixed64) {
+__global__ __launch_bounds__(256) void k_rev_clear(Topo t) {
   uint32_t ng = t.counters[CNT_GRID_BLOCKS];
-  uint32_t words = ng * kBlockNodes * (fixed64 ? 2u : 1u);  // float4 or 2 x (2 x int64)
-  float4* acc = reinterpret_cast<float4*>(t.grid_acc);
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < words; q += gridDim.x * blockDim.x)
-    acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < 8 * ng; q += gridDim.x * blockDim.x)
+    t.grid_rev[q] = kInvalid;
 }
 
 __global__ __launch_bounds__(256) void k_mask_clear(Topo t) {
@@ -734,35 +738,33 @@ __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
   local = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
 }
 
-template <int FIXED>
-struct TileAcc;
-template <>
-struct TileAcc<0> {
-  using T = float;
-};
-template <>
-struct TileAcc<1> {
-  using T = unsigned long long;
-};
-
+// P2G accumulates in 64-bit fixed point in LDS for both precision modes:
+// gfx950's integer LDS atomics (ds_add_u64) run an order of magnitude faster
+// than its float LDS atomics (ds_add_f32), and integer sums make the
+// per-block accumulation independent of atomic order.  The FP32 mode rounds
+// each tile node to f32 when storing the tile; the FIXED64 mode stores the
+// integers and the grid update sums them exactly.
 template <int FIXED>
 __global__ __launch_bounds__(256) void k_p2g(Parts p, Topo t, const DevClass* classes, const DevScene* scenes) {
-  using T = typename TileAcc<FIXED>::T;
+  using T = unsigned long long;
   __shared__ T tile[4 * kTileNodes];  // [component][node]
   __shared__ uint32_t s_slot[8];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
-    for (int q = threadIdx.x; q < 4 * kTileNodes; q += blockDim.x) tile[q] = (T)0;
+    for (int q = threadIdx.x; q < 4 * kTileNodes; q += blockDim.x) tile[q] = 0ull;
     if (threadIdx.x < 8) s_slot[threadIdx.x] = t.pblock_slot8[8 * k + threadIdx.x];
     __syncthreads();
     const uint32_t own = s_slot[0];
+    int mass_shift = 0, mom_shift = 0;
     if (own != kInvalid) {
       const int4 key = t.grid_key[own];
       const DevScene& sc = scenes[(uint32_t)key.w >> 16];
       const float h = sc.h, inv_h = sc.inv_h;
       const float stress_scale = -sc.dt * sc.inv_d;
-      const float mscale = FIXED ? exp2f((float)sc.mass_shift) : 1.0f;
-      const float pscale = FIXED ? exp2f((float)sc.mom_shift) : 1.0f;
+      mass_shift = sc.mass_shift;
+      mom_shift = sc.mom_shift;
+      const float mscale = exp2f((float)mass_shift);
+      const float pscale = exp2f((float)mom_shift);
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
       const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
       bool bad = false, range = false;
@@ -796,9 +798,9 @@ __global__ __launch_bounds__(256) void k_p2g(Parts p, Topo t, const DevClass* cl
         const float m = pm.w;
         const float sscale = stress_scale * vv.w;
         // A = m C + stress * (-dt V0 inv_d), columns
-        float3 a0 = col(C, 0) * m + col(S, 0) * sscale;
-        float3 a1 = col(C, 1) * m + col(S, 1) * sscale;
-        float3 a2 = col(C, 2) * m + col(S, 2) * sscale;
+        const float3 a0 = col(C, 0) * m + col(S, 0) * sscale;
+        const float3 a1 = col(C, 1) * m + col(S, 1) * sscale;
+        const float3 a2 = col(C, 2) * m + col(S, 2) * sscale;
         const float3 mv = make_float3(vv.x, vv.y, vv.z) * m;
 #pragma unroll
         for (int x = 0; x < 3; ++x)
@@ -808,54 +810,37 @@ __global__ __launch_bounds__(256) void k_p2g(Parts p, Topo t, const DevClass* cl
             for (int z = 0; z < 3; ++z) {
               const float w = wx[x] * wy[y] * wz[z];
               const float dx = ((float)x - fx) * h, dy = ((float)y - fy) * h, dz = ((float)z - fz) * h;
-              float3 am = a0 * dx + a1 * dy + a2 * dz;
-              float3 mom = (mv + am) * w;
-              const float wm = w * m;
+              const float3 am = a0 * dx + a1 * dy + a2 * dz;
+              const float3 mom = (mv + am) * w;
+              const float sm = (w * m) * mscale, sx = mom.x * pscale, sy = mom.y * pscale, sz = mom.z * pscale;
+              const float lim = 35184372088832.0f;  // 2^45: |contribution| bound
+              range |= !(fabsf(sm) < lim && fabsf(sx) < lim && fabsf(sy) < lim && fabsf(sz) < lim);
               const int n = tile_index(lx + x, ly + y, lz + z);
-              if (FIXED) {
-                float sm = wm * mscale, sx = mom.x * pscale, sy = mom.y * pscale, sz = mom.z * pscale;
-                const float lim = 35184372088832.0f;  // 2^45
-                range |= !(fabsf(sm) < lim && fabsf(sx) < lim && fabsf(sy) < lim && fabsf(sz) < lim);
-                atomicAdd(&tile[n], (T)__float2ll_rn(sm));
-                atomicAdd(&tile[kTileNodes + n], (T)__float2ll_rn(sx));
-                atomicAdd(&tile[2 * kTileNodes + n], (T)__float2ll_rn(sy));
-                atomicAdd(&tile[3 * kTileNodes + n], (T)__float2ll_rn(sz));
-              } else {
-                atomicAdd((float*)&tile[n], wm);
-                atomicAdd((float*)&tile[kTileNodes + n], mom.x);
-                atomicAdd((float*)&tile[2 * kTileNodes + n], mom.y);
-                atomicAdd((float*)&tile[3 * kTileNodes + n], mom.z);
-              }
+              atomicAdd(&tile[n], (T)__float2ll_rn(sm));
+              atomicAdd(&tile[kTileNodes + n], (T)__float2ll_rn(sx));
+              atomicAdd(&tile[2 * kTileNodes + n], (T)__float2ll_rn(sy));
+              atomicAdd(&tile[3 * kTileNodes + n], (T)__float2ll_rn(sz));
             }
       }
       flag_error(t.counters, bad, ERR_INTERNAL);
-      if (FIXED) flag_error(t.counters, range, ERR_FIXED_RANGE);
+      flag_error(t.counters, range, ERR_FIXED_RANGE);
     }
     __syncthreads();
     if (own != kInvalid) {
-      for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
-        T m = tile[n];
-        if (m == (T)0 && tile[kTileNodes + n] == (T)0 && tile[2 * kTileNodes + n] == (T)0 &&
-            tile[3 * kTileNodes + n] == (T)0)
-          continue;
-        int d, local;
-        tile_node(n, d, local);
-        uint32_t slot = s_slot[d];
-        if (slot == kInvalid) continue;
-        size_t node = (size_t)slot * kBlockNodes + local;
-        if (FIXED) {
-          unsigned long long* acc = reinterpret_cast<unsigned long long*>(t.grid_acc) + node * 4;
-          atomicAdd(acc, (unsigned long long)m);
-          atomicAdd(acc + 1, (unsigned long long)tile[kTileNodes + n]);
-          atomicAdd(acc + 2, (unsigned long long)tile[2 * kTileNodes + n]);
-          atomicAdd(acc + 3, (unsigned long long)tile[3 * kTileNodes + n]);
-        } else {
-          float* acc = reinterpret_cast<float*>(t.grid_acc) + node * 4;
-          unsafeAtomicAdd(acc, (float)m);
-          unsafeAtomicAdd(acc + 1, (float)tile[kTileNodes + n]);
-          unsafeAtomicAdd(acc + 2, (float)tile[2 * kTileNodes + n]);
-          unsafeAtomicAdd(acc + 3, (float)tile[3 * kTileNodes + n]);
+      if (FIXED) {
+        longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + (size_t)k * kTileNodes * 2;
+        for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
+          out[2 * n] = make_longlong2((long long)tile[n], (long long)tile[kTileNodes + n]);
+          out[2 * n + 1] = make_longlong2((long long)tile[2 * kTileNodes + n], (long long)tile[3 * kTileNodes + n]);
         }
+      } else {
+        const float im = exp2f(-(float)mass_shift), ip = exp2f(-(float)mom_shift);
+        float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes;
+        for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x)
+          out[n] = make_float4(__ll2float_rn((long long)tile[n]) * im,
+                               __ll2float_rn((long long)tile[kTileNodes + n]) * ip,
+                               __ll2float_rn((long long)tile[2 * kTileNodes + n]) * ip,
+                               __ll2float_rn((long long)tile[3 * kTileNodes + n]) * ip);
       }
     }
     __syncthreads();
@@ -870,29 +855,48 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
   const uint32_t ng = t.counters[CNT_GRID_BLOCKS];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t waves = blockDim.x >> 6;
+  const int lx = (int)(lane >> 4), ly = (int)((lane >> 2) & 3), lz = (int)(lane & 3);
   for (uint32_t s = blockIdx.x * waves + (threadIdx.x >> 6); s < ng; s += gridDim.x * waves) {
     const int4 key = t.grid_key[s];
     const DevScene& sc = scenes[(uint32_t)key.w >> 16];
-    const size_t node = (size_t)s * kBlockNodes + lane;
-    float m;
-    float3 mom;
+    // Sum the covering tiles in the fixed order d = 0..7 (tile of block
+    // key - d, node at tile coordinates local + 4 d).
+    float m = 0.0f;
+    float3 mom = make_float3(0.f, 0.f, 0.f);
+    long long im = 0, ix = 0, iy = 0, iz = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const int dx = d >> 2, dy = (d >> 1) & 1, dz = d & 1;
+      const uint32_t b = t.grid_rev[8 * s + d];
+      const int tx = lx + 4 * dx, ty = ly + 4 * dy, tz = lz + 4 * dz;
+      if (b == kInvalid || tx >= kTileEdge || ty >= kTileEdge || tz >= kTileEdge) continue;
+      const size_t n = (size_t)b * kTileNodes + tile_index(tx, ty, tz);
+      if (FIXED) {
+        const longlong2* src = reinterpret_cast<const longlong2*>(t.ptile) + 2 * n;
+        const longlong2 u = src[0], w = src[1];
+        im += u.x;
+        ix += u.y;
+        iy += w.x;
+        iz += w.y;
+      } else {
+        const float4 a = reinterpret_cast<const float4*>(t.ptile)[n];
+        m += a.x;
+        mom.x += a.y;
+        mom.y += a.z;
+        mom.z += a.w;
+      }
+    }
     if (FIXED) {
-      const long long* acc = reinterpret_cast<const long long*>(t.grid_acc) + node * 4;
-      const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
-      m = __ll2float_rn(acc[0]) * im;
-      mom = make_float3(__ll2float_rn(acc[1]) * ip, __ll2float_rn(acc[2]) * ip, __ll2float_rn(acc[3]) * ip);
-    } else {
-      float4 a = reinterpret_cast<const float4*>(t.grid_acc)[node];
-      m = a.x;
-      mom = make_float3(a.y, a.z, a.w);
+      const float sm = exp2f(-(float)sc.mass_shift), sp = exp2f(-(float)sc.mom_shift);
+      m = __ll2float_rn(im) * sm;
+      mom = make_float3(__ll2float_rn(ix) * sp, __ll2float_rn(iy) * sp, __ll2float_rn(iz) * sp);
     }
     float3 v = make_float3(0, 0, 0);
-    const bool live = m > 0.0f;
-    if (live) {
+    if (m > 0.0f) {
       v = mom / m + make_float3(sc.gravity[0], sc.gravity[1], sc.gravity[2]) * sc.dt;
-      const int cx = key.x * kBlockCells + (int)(lane >> 4);
-      const int cy = key.y * kBlockCells + (int)((lane >> 2) & 3);
-      const int cz = key.z * kBlockCells + (int)(lane & 3);
+      const int cx = key.x * kBlockCells + lx;
+      const int cy = key.y * kBlockCells + ly;
+      const int cz = key.z * kBlockCells + lz;
       const float3 pos = make_float3((float)cx * sc.h, (float)cy * sc.h, (float)cz * sc.h);
       if (sc.has_bounds) {
         for (int a = 0; a < 3; ++a) {
@@ -909,9 +913,7 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
         if (dist <= sc.h * (1.0f + 1e-5f)) v = contact_velocity(o, pos, v, nrm);
       }
     }
-    t.grid_vel[node] = make_float4(v.x, v.y, v.z, 0.0f);
-    uint64_t lm = __ballot(live);
-    if (lane == 0 && lm) atomicAdd(&t.counters[CNT_MASS_NODES], (uint32_t)__popcll(lm));
+    t.grid_vel[(size_t)s * kBlockNodes + lane] = make_float4(v.x, v.y, v.z, 0.0f);
   }
 }
 
@@ -1152,9 +1154,8 @@ void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
 }
-void launch_grid_clear(const Topo& t, int fixed64, hipStream_t s) {
-  uint64_t words = (uint64_t)t.grid_cap * kBlockNodes * (fixed64 ? 2 : 1);
-  hipLaunchKernelGGL(k_grid_clear, dim3(grid_for(words, 256, 8192)), dim3(256), 0, s, t, fixed64);
+void launch_rev_clear(const Topo& t, hipStream_t s) {
+  hipLaunchKernelGGL(k_rev_clear, dim3(grid_for(8ull * t.grid_cap, 256, 8192)), dim3(256), 0, s, t);
 }
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
                 uint32_t threads, hipStream_t s) {

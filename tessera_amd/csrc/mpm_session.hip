@@ -392,7 +392,7 @@ struct tmpm_session {
 
   // topology
   DevBuf owner, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, chunk_sums, grid_key, pblock_bucket,
-      pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, grid_acc, grid_vel, node_mask;
+      pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask;
   Topo topo{};
   uint32_t p2g_blocks = 4096, threads_pblock = 128;
   bool topo_valid = false;  // topology arrays describe the current device state
@@ -595,7 +595,8 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->n, 1)));
   HIP_TRY(s->counters.alloc(4ull * CNT_COUNT));
   HIP_TRY(hipMemset(s->counters.ptr, 0, 4ull * CNT_COUNT));
-  HIP_TRY(s->grid_acc.alloc((size_t)grid_cap * kBlockNodes * 32));
+  HIP_TRY(s->ptile.alloc((size_t)pblock_cap * kTileNodes * (s->fixed64 ? 32 : 16)));
+  HIP_TRY(s->grid_rev.alloc(32ull * grid_cap));
   HIP_TRY(s->grid_vel.alloc((size_t)grid_cap * kBlockNodes * 16));
   HIP_TRY(s->node_mask.alloc(8ull * grid_cap));
   t.owner = s->owner.as<uint32_t>();
@@ -617,7 +618,8 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   t.prank = s->prank.as<uint32_t>();
   t.sorted = s->sorted.as<uint32_t>();
   t.counters = s->counters.as<uint32_t>();
-  t.grid_acc = s->grid_acc.ptr;
+  t.ptile = s->ptile.ptr;
+  t.grid_rev = s->grid_rev.as<uint32_t>();
   t.grid_vel = s->grid_vel.as<float4>();
   t.node_mask = s->node_mask.as<uint32_t>();
   s->p2g_blocks = std::min<uint32_t>(pblock_cap, 65535);
@@ -839,6 +841,7 @@ hipError_t enqueue_binning(tmpm_session* s, const Parts& p) {
   if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p, t, c, sc, s->stream); })) != hipSuccess) return e;
   if ((e = timed(s, "k_bin_neighbors", [&] { launch_bin_neighbors(p, t, c, sc, s->stream); })) != hipSuccess) return e;
   if ((e = timed(s, "k_scan", [&] { launch_scan(p, t, c, sc, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_rev_clear", [&] { launch_rev_clear(t, s->stream); })) != hipSuccess) return e;
   if ((e = timed(s, "k_pblock_finalize", [&] { launch_pblock_finalize(t, s->stream); })) != hipSuccess) return e;
   return hipSuccess;
 }
@@ -863,7 +866,6 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
     if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p, t, s->stream); })) != hipSuccess) return e;
   }
   s->substeps_since_reorder++;
-  if ((e = timed(s, "k_grid_clear", [&] { launch_grid_clear(t, s->fixed64, s->stream); })) != hipSuccess) return e;
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
                  [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->threads_pblock, s->stream); })) != hipSuccess)
     return e;
@@ -1321,11 +1323,6 @@ tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* ou
   if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
   if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
   std::memset(out, 0, sizeof(*out));
-  uint32_t mass_nodes = 0;
-  if (s->prepared) {
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(&mass_nodes, s->counters.as<uint32_t>() + CNT_MASS_NODES, 4, hipMemcpyDeviceToHost));
-  }
   std::vector<uint32_t> c;
   tmpm_status st = run_diagnostic_binning(s, c);
   if (st != TMPM_OK) return st;
@@ -1333,7 +1330,6 @@ tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* ou
   out->grid_blocks = c[CNT_GRID_BLOCKS];
   out->particle_blocks = c[CNT_PARTICLE_BLOCKS];
   out->binned_particles = c[CNT_BINNED];
-  out->mass_nodes = mass_nodes;
   return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
 }
 

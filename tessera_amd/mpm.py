@@ -159,7 +159,6 @@ class CTopologyStats(C.Structure):
         ("grid_blocks", C.c_uint64),
         ("particle_blocks", C.c_uint64),
         ("binned_particles", C.c_uint64),
-        ("mass_nodes", C.c_uint64),
     ]
 
 
