@@ -263,6 +263,13 @@ tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);
 
+/* Bandwidth probe (no reference counterpart): copies `bytes` bytes between
+ * two device buffers `iterations` times with a plain float4 copy kernel and
+ * reports the achieved bandwidth (read + write bytes per second, in GB/s).
+ * Used to confirm the device's achievable HBM bandwidth and to calibrate
+ * hardware byte counters against a kernel of known traffic. */
+tmpm_status tmpm_bandwidth_probe(int32_t device_ordinal, uint64_t bytes, uint32_t iterations, double* gbs_out);
+
 /* Host-side validation helpers shared with the binding layer. */
 double tmpm_timestep_bound(const tmpm_material* material, double density, const double velocity[3],
                            double cell_width, double deformation_det);

@@ -25,13 +25,13 @@ constexpr int kBlockCells = 4;        // cells per block edge
 constexpr int kBlockNodes = 64;       // nodes per block
 constexpr int kTileEdge = 6;          // P2G/G2P tile: block + 2-cell halo
 constexpr int kTileNodes = 216;
-constexpr int kScanChunk = 1024;      // buckets per scan workgroup
+constexpr int kMinBuckets = 1024;     // smallest block-hash table
 constexpr int kMaxObstacles = 256;
 
 // counters[] slots
 enum Counter : uint32_t {
-  CNT_GRID_BLOCKS = 0,
-  CNT_PARTICLE_BLOCKS = 1,
+  CNT_GRID_BLOCKS = 0,      // grid blocks of the current substep (clamped to capacity)
+  CNT_PARTICLE_BLOCKS = 1,  // particle blocks of the current substep (clamped)
   CNT_BINNED = 2,
   CNT_ERROR = 3,
   CNT_LIVE_NODES = 5,
@@ -109,8 +109,6 @@ struct Topo {
   uint32_t* bucket_slot;    // bucket -> grid slot
   uint32_t* bucket_pblock;  // bucket -> particle-block id
   uint32_t bucket_mask;     // bucket count - 1 (power of two)
-  uint32_t n_chunks;        // scan chunks over buckets
-  uint32_t* chunk_sums;     // 3 per chunk (grid, particle blocks, particles), then 3 totals
   int4* grid_key;           // slot -> (bx, by, bz, region)
   uint32_t grid_cap;
   uint32_t pblock_cap;
@@ -122,6 +120,14 @@ struct Topo {
   uint32_t* prank;          // particle -> rank within its block
   uint32_t* sorted;         // binned order -> particle storage index
   uint32_t* counters;
+  // binning lists: insert workgroup g bins particles [g * bin_span, (g + 1) * bin_span)
+  uint32_t bin_span;        // particles per binning workgroup (multiple of 256)
+  uint32_t bin_groups;      // binning workgroups
+  uint32_t* bin_stage;      // bin_span per group: buckets the group claimed (particle blocks)
+  uint32_t* bin_nstage;     // 7 * bin_span per group: neighbour-only blocks it claimed
+  uint32_t* bin_claims;     // per group: particle blocks claimed
+  uint32_t* bin_nclaims;    // per group: neighbour-only blocks claimed
+  uint32_t* bin_csum;       // per group: particles in its particle blocks
   void* ptile;              // per particle block: its 6^3 P2G tile (FP32: float4; FIXED64: 4 x int64 per node)
   uint32_t* grid_rev;       // per grid slot, 8 entries: particle block whose tile covers it at offset d
   float4* grid_vel;

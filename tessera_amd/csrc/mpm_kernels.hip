@@ -3,25 +3,24 @@
 // Substep pipeline (all counts stay on the device; every kernel is a
 // grid-stride loop over a device-side count, so the launch sequence never
 // depends on host readback):
-//   1 k_topo_clear        reset hash buckets and counters
-//   2 k_bin_insert        particle -> 4^3-cell block key, lock-free hash insert
-//                         (wave-aggregated), per-block counting-sort rank
-//   3 k_bin_neighbors     insert the 7 upper neighbours of every particle block
+//   1 k_bin_insert        particle -> 4^3-cell block key, lock-free hash insert
+//                         by the head lane of each run of equal keys, counting-
+//                         sort rank; each workgroup lists the blocks it claimed
+//   2 k_bin_blocks        particle-block ids from prefix sums of those lists;
+//                         insert the 7 upper neighbours of every particle block
 //                         (grid blocks a block's P2G tile touches)
-//   4 k_scan_*            compact buckets -> grid slots, particle-block ids,
-//                         particle offsets (three-phase scan)
-//   5 k_pblock_finalize   per particle block: grid slots of its 8 tile blocks
-//   6 k_bin_scatter       particle -> position in block-sorted order (optionally
-//                         moving the particle state itself: periodic reorder)
-//   7 k_rev_clear         clear the grid-slot -> covering-tile map
-//   5 k_pblock_finalize   (fills that map)
-//   8 k_p2g               per particle block: stress, 27-node B-spline scatter
+//   3 k_bin_offsets       counting-sort offsets, grid slots of neighbour blocks
+//   4 k_bin_scatter       per particle block: grid slots of its 8 tile blocks;
+//                         particle -> position in block-sorted order (every few
+//                         substeps the particle state itself moves: reorder)
+//   5 k_p2g               per particle block: stress, 27-node B-spline scatter
 //                         into a 6^3 LDS tile accumulated in 64-bit fixed point
 //                         (integer LDS atomics), stored as the block's tile
-//   9 k_grid_update       per node: sum of the covering tiles in a fixed order,
+//   6 k_grid_update       per node: sum of the covering tiles in a fixed order,
 //                         v = p/m + g dt, walls, obstacles
-//  10 k_g2p               per particle block: 6^3 velocity tile in LDS, gather,
-//                         v/C/x/F update, material projection, validity flags
+//   7 k_g2p               per particle block: 6^3 velocity tile in LDS, gather,
+//                         v/C/x/F update, material projection, validity flags;
+//                         clears the block hash for the next substep
 // The reference computes the same substep with an open-addressed hash of
 // all 27N stencil nodes (gpu_topology.wgsl:18-103) and per-particle CAS-loop
 // float atomics (gpu.wgsl:711-1019); here the hash holds blocks, not nodes,
@@ -117,15 +116,97 @@ __device__ __forceinline__ int4 owner_key(const Parts& p, const Topo& t, const D
 // fixed offset), so a losing thread compares against the winner's key
 // without waiting for anything to be published.  Bounded by the table size.
 __device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
-                                int4 key, uint32_t me) {
+                                int4 key, uint32_t me, bool& claimed) {
   uint32_t b = hash_key(key) & t.bucket_mask;
+  claimed = false;
   for (uint32_t probe = 0; probe <= t.bucket_mask; ++probe) {
     uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
-    if (prev == 0u) return b;
+    if (prev == 0u) {
+      claimed = true;
+      return b;
+    }
     if (key_eq(owner_key(p, t, classes, scenes, prev), key)) return b;
     b = (b + 1) & t.bucket_mask;
   }
   return kInvalid;
+}
+
+// Sums of `a[j]` over j < upto and over all j < n, for every thread of a
+// 256-thread workgroup (the per-workgroup prefix of the binning lists).
+constexpr int kBinThreads = 256;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int K>
+__device__ __forceinline__ void group_prefix(const uint32_t* const (&a)[K], uint32_t n, uint32_t upto,
+                                             uint32_t (&before)[K], uint32_t (&total)[K]) {
+  __shared__ uint32_t red[2 * K][kBinThreads / 64];
+  uint32_t bs[K], ts[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) bs[k] = ts[k] = 0;
+  for (uint32_t j = threadIdx.x; j < n; j += kBinThreads) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t c = a[k][j];
+      ts[k] += c;
+      if (j < upto) bs[k] += c;
+    }
+  }
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    bs[k] = wave_sum(bs[k]);
+    ts[k] = wave_sum(ts[k]);
+    if (lane == 0) {
+      red[2 * k][wave] = bs[k];
+      red[2 * k + 1][wave] = ts[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    before[k] = total[k] = 0;
+#pragma unroll
+    for (int w = 0; w < kBinThreads / 64; ++w) {
+      before[k] += red[2 * k][w];
+      total[k] += red[2 * k + 1][w];
+    }
+  }
+  __syncthreads();
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup; the
+// workgroup's total is returned in `sum`.
+__device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t& sum) {
+  __shared__ uint32_t ws[kBinThreads / 64];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(incl, off, 64);
+    if (lane >= (uint32_t)off) incl += u;
+  }
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  sum = 0;
+#pragma unroll
+  for (int w = 0; w < kBinThreads / 64; ++w) {
+    const uint32_t x = ws[w];
+    if ((uint32_t)w < wave) before += x;
+    sum += x;
+  }
+  __syncthreads();
+  return before + incl - v;
+}
+
+__device__ __forceinline__ void clear_rev(const Topo& t, uint32_t slot) {
+#pragma unroll
+  for (int d = 0; d < 8; ++d) t.grid_rev[8 * slot + d] = kInvalid;
 }
 
 // ---------------------------------------------------------------------------
@@ -137,198 +218,199 @@ __global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
     t.owner[b] = 0;
     t.bucket_count[b] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x < CNT_COUNT && threadIdx.x != CNT_ERROR) t.counters[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
 // 2. particle -> block insert + counting-sort rank
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
-                                                    const DevScene* scenes) {
+// Workgroup g bins particles [g * bin_span, (g + 1) * bin_span).  Lanes of a
+// wave hold consecutive particles, which are mostly block-sorted: the first
+// lane of each run of equal keys (a "head") inserts the key and reserves the
+// run's ranks with one atomic; all heads of a wave probe in parallel.  Blocks
+// this workgroup claims are listed in its part of bin_stage; ids are handed
+// out from prefix sums of the per-workgroup counts (k_bin_blocks), since
+// returning atomics on one shared counter serialize device-wide.
+__global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
+                                                            const DevScene* scenes) {
+  __shared__ uint32_t s_claims;
   const uint32_t lane = lane_id();
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t base = blockIdx.x * blockDim.x; base < p.n; base += gridDim.x * blockDim.x) {
-    uint32_t i = base + threadIdx.x;
+  if (threadIdx.x == 0) s_claims = 0;
+  __syncthreads();
+  const uint32_t begin = blockIdx.x * t.bin_span;
+  uint32_t* stage = t.bin_stage + (size_t)blockIdx.x * t.bin_span;
+  for (uint32_t r = 0; r < t.bin_span; r += kBinThreads) {
+    const uint32_t i = begin + r + threadIdx.x;
     bool active = false;
     int4 key = make_int4(0, 0, 0, 0);
     if (i < p.n) {
-      float4 pm = p.pos_mass[i];
+      const float4 pm = p.pos_mass[i];
       if (pm.w > 0.0f) {
         const DevClass& c = classes[p.cls[i]];
         int3 cell;
-        bool ok = cell3(pm, scenes[c.region >> 16].inv_h, cell);
-        if (ok) {
+        if (cell3(pm, scenes[c.region >> 16].inv_h, cell)) {
           active = true;
           key = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
         } else {
           atomicOr(&t.counters[CNT_ERROR], ERR_RESIDENT);
         }
       }
-      t.pslot[i] = kInvalid;
     }
-    uint64_t pending = __ballot(active);
-    while (pending) {
-      int leader = __ffsll((unsigned long long)pending) - 1;
-      int4 lk = make_int4(readlane_i(key.x, leader), readlane_i(key.y, leader), readlane_i(key.z, leader),
-                          readlane_i(key.w, leader));
-      bool mine = ((pending >> lane) & 1ull) && key_eq(key, lk);
-      uint64_t group = __ballot(mine);
-      uint32_t bucket = 0, first = 0;
-      if ((int)lane == leader) {
-        bucket = hash_insert(p, t, classes, scenes, key, i + 1);
-        if (bucket != kInvalid) first = atomicAdd(&t.bucket_count[bucket], (uint32_t)__popcll(group));
+    const int4 prev = make_int4(__shfl_up(key.x, 1, 64), __shfl_up(key.y, 1, 64), __shfl_up(key.z, 1, 64),
+                                __shfl_up(key.w, 1, 64));
+    const bool prev_active = __shfl_up((int)active, 1, 64) != 0;
+    const bool head = active && (lane == 0 || !prev_active || !key_eq(key, prev));
+    const uint64_t heads = __ballot(head);
+    const uint64_t cont = __ballot(active && !head);
+    uint32_t bucket = kInvalid, first = 0;
+    if (head) {
+      // run length: this lane plus the continuing lanes directly above it
+      const uint64_t above = lane == 63 ? 0ull : (cont >> (lane + 1));
+      const uint32_t len = 1u + (uint32_t)__builtin_ctzll(~above);
+      bool claimed;
+      bucket = hash_insert(p, t, classes, scenes, key, i + 1, claimed);
+      if (bucket == kInvalid) {
+        atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
+      } else {
+        if (claimed) stage[atomicAdd(&s_claims, 1u)] = bucket;
+        first = atomicAdd(&t.bucket_count[bucket], len);
       }
-      bucket = readlane_u(bucket, leader);
-      first = readlane_u(first, leader);
-      if (mine) {
-        if (bucket != kInvalid) {
-          t.pslot[i] = bucket;
-          t.prank[i] = first + (uint32_t)__popcll(group & below);
-        }
-      }
-      if (bucket == kInvalid && (int)lane == leader) atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
-      pending &= ~group;
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// 3. neighbour blocks of every particle block
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bin_neighbors(Parts p, Topo t, const DevClass* classes,
-                                                       const DevScene* scenes) {
-  uint32_t nb = t.bucket_mask + 1;
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-    uint32_t o = t.owner[b];
-    if (o == 0 || o > p.n) continue;  // empty, or claimed by a neighbour insert of this pass
-    int4 k = particle_key(p, classes, scenes, o - 1);
-    t.nbr_bucket[8 * b] = b;
-    for (uint32_t d = 1; d < 8; ++d) {
-      int4 kk = make_int4(k.x + (int)(d >> 2), k.y + (int)((d >> 1) & 1), k.z + (int)(d & 1), k.w);
-      uint32_t got = hash_insert(p, t, classes, scenes, kk, p.n + 1 + 8 * b + d);
-      if (got == kInvalid) atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
-      t.nbr_bucket[8 * b + d] = got;
+    // every active lane belongs to the run of the nearest head at or below it
+    const uint64_t upto = heads & ((2ull << lane) - 1ull);
+    const int hl = upto ? 63 - __clzll((long long)upto) : (int)lane;
+    bucket = __shfl(bucket, hl, 64);
+    first = __shfl(first, hl, 64);
+    if (i < p.n) {
+      const bool ok = active && bucket != kInvalid;
+      t.pslot[i] = ok ? bucket : kInvalid;
+      if (ok) t.prank[i] = first + (lane - (uint32_t)hl);
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// 4. three-phase scan over buckets
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void bucket_flags(const Parts& p, const Topo& t, uint32_t b, uint32_t& g, uint32_t& pb,
-                                             uint32_t& cnt) {
-  uint32_t o = t.owner[b];
-  g = o != 0 ? 1u : 0u;
-  pb = (o != 0 && o <= p.n) ? 1u : 0u;
-  cnt = pb ? t.bucket_count[b] : 0u;
-}
-
-// Workgroup-wide inclusive scan of 3 values per thread (256 threads).
-__device__ __forceinline__ void block_scan3(uint32_t v[3], uint32_t total[3]) {
-  __shared__ uint32_t s[3][256];
-  for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    uint32_t add[3] = {0, 0, 0};
-    if (threadIdx.x >= (unsigned)off)
-      for (int k = 0; k < 3; ++k) add[k] = s[k][threadIdx.x - off];
-    __syncthreads();
-    for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += add[k];
-    __syncthreads();
-  }
-  for (int k = 0; k < 3; ++k) {
-    v[k] = s[k][threadIdx.x];
-    total[k] = s[k][255];
   }
   __syncthreads();
+  if (threadIdx.x == 0) t.bin_claims[blockIdx.x] = s_claims;
 }
 
-__global__ __launch_bounds__(256) void k_scan_reduce(Parts p, Topo t) {
-  for (uint32_t c = blockIdx.x; c < t.n_chunks; c += gridDim.x) {
-    uint32_t acc[3] = {0, 0, 0};
-    for (int q = 0; q < kScanChunk / 256; ++q) {
-      uint32_t b = c * kScanChunk + threadIdx.x * (kScanChunk / 256) + q;
-      uint32_t g, pb, cnt;
-      bucket_flags(p, t, b, g, pb, cnt);
-      acc[0] += g;
-      acc[1] += pb;
-      acc[2] += cnt;
+// ---------------------------------------------------------------------------
+// 3. particle-block ids, grid slots of particle blocks, neighbour inserts
+// ---------------------------------------------------------------------------
+// Workgroup g numbers the blocks listed by insert workgroup g after those of
+// workgroups < g.  A particle block's grid slot is its particle-block id.
+// It then inserts the 7 upper neighbours of each of its blocks (the grid
+// blocks its 6^3 P2G tile touches) and lists the ones it claims.
+__global__ __launch_bounds__(kBinThreads) void k_bin_blocks(Parts p, Topo t, const DevClass* classes,
+                                                            const DevScene* scenes) {
+  __shared__ uint32_t s_claims;
+  __shared__ uint32_t s_csum[kBinThreads / 64];
+  const uint32_t g = blockIdx.x;
+  if (threadIdx.x == 0) s_claims = 0;
+  const uint32_t* const lists[1] = {t.bin_claims};
+  uint32_t before[1], total[1];
+  group_prefix<1>(lists, t.bin_groups, g, before, total);
+  const uint32_t len = t.bin_claims[g];
+  const uint32_t* stage = t.bin_stage + (size_t)g * t.bin_span;
+  uint32_t* nstage = t.bin_nstage + (size_t)g * 7 * t.bin_span;
+  uint32_t csum = 0;
+  bool over = false;
+  for (uint32_t j = threadIdx.x; j < len; j += kBinThreads) {
+    const uint32_t b = stage[j], pb = before[0] + j;
+    if (pb < t.pblock_cap) {
+      t.bucket_pblock[b] = pb;
+      t.bucket_slot[b] = pb;
+      t.pblock_bucket[pb] = b;
+      t.grid_key[pb] = particle_key(p, classes, scenes, t.owner[b] - 1);
+      clear_rev(t, pb);
+      csum += t.bucket_count[b];
+    } else {
+      t.bucket_pblock[b] = kInvalid;
+      t.bucket_slot[b] = kInvalid;
+      over = true;
     }
-    uint32_t tot[3];
-    block_scan3(acc, tot);
-    if (threadIdx.x == 0)
-      for (int k = 0; k < 3; ++k) t.chunk_sums[3 * c + k] = tot[k];
   }
-}
-
-__global__ __launch_bounds__(256) void k_scan_top(Topo t) {
-  // single workgroup: exclusive scan of chunk sums in place; totals appended
-  uint32_t carry[3] = {0, 0, 0};
-  for (uint32_t base = 0; base < t.n_chunks; base += 256) {
-    uint32_t c = base + threadIdx.x;
-    uint32_t v[3] = {0, 0, 0};
-    if (c < t.n_chunks)
-      for (int k = 0; k < 3; ++k) v[k] = t.chunk_sums[3 * c + k];
-    uint32_t incl[3] = {v[0], v[1], v[2]}, tot[3];
-    block_scan3(incl, tot);
-    if (c < t.n_chunks)
-      for (int k = 0; k < 3; ++k) t.chunk_sums[3 * c + k] = carry[k] + incl[k] - v[k];
-    for (int k = 0; k < 3; ++k) carry[k] += tot[k];
+  for (uint32_t q = threadIdx.x; q < 7 * len; q += kBinThreads) {
+    const uint32_t j = q / 7u, d = q % 7u + 1u;
+    const uint32_t b = stage[j];
+    if (d == 1) t.nbr_bucket[8 * b] = b;
+    if (before[0] + j >= t.pblock_cap) continue;
+    int4 k = particle_key(p, classes, scenes, t.owner[b] - 1);
+    k.x += (int)(d >> 2);
+    k.y += (int)((d >> 1) & 1);
+    k.z += (int)(d & 1);
+    bool claimed;
+    const uint32_t got = hash_insert(p, t, classes, scenes, k, p.n + 1 + 8 * b + d, claimed);
+    if (got == kInvalid) {
+      over = true;
+    } else if (claimed) {
+      t.bucket_pblock[got] = kInvalid;
+      nstage[atomicAdd(&s_claims, 1u)] = got;
+    }
+    t.nbr_bucket[8 * b + d] = got;
   }
+  flag_error(t.counters, over, ERR_TOPOLOGY);
+  csum = wave_sum(csum);
+  if ((threadIdx.x & 63) == 0) s_csum[threadIdx.x >> 6] = csum;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 0; k < 3; ++k) t.chunk_sums[3 * t.n_chunks + k] = carry[k];
-    uint32_t err = 0;
-    if (carry[0] > t.grid_cap || carry[1] > t.pblock_cap) err |= ERR_TOPOLOGY;
-    t.counters[CNT_GRID_BLOCKS] = carry[0] < t.grid_cap ? carry[0] : t.grid_cap;
-    t.counters[CNT_PARTICLE_BLOCKS] = carry[1] < t.pblock_cap ? carry[1] : t.pblock_cap;
-    t.counters[CNT_BINNED] = carry[2];
-    if (err) atomicOr(&t.counters[CNT_ERROR], err);
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kBinThreads / 64; ++w) c += s_csum[w];
+    t.bin_csum[g] = c;
+    t.bin_nclaims[g] = s_claims;
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_apply(Parts p, Topo t, const DevClass* classes,
-                                                    const DevScene* scenes) {
-  for (uint32_t c = blockIdx.x; c < t.n_chunks; c += gridDim.x) {
-    constexpr int per = kScanChunk / 256;
-    uint32_t fl[per][3];
-    uint32_t acc[3] = {0, 0, 0};
-    for (int q = 0; q < per; ++q) {
-      uint32_t b = c * kScanChunk + threadIdx.x * per + q;
-      bucket_flags(p, t, b, fl[q][0], fl[q][1], fl[q][2]);
-      for (int k = 0; k < 3; ++k) acc[k] += fl[q][k];
+// ---------------------------------------------------------------------------
+// 4. counting-sort offsets and grid slots of neighbour-only blocks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBinThreads) void k_bin_offsets(Parts p, Topo t, const DevClass* classes,
+                                                             const DevScene* scenes) {
+  const uint32_t g = blockIdx.x;
+  const uint32_t* const lists[3] = {t.bin_claims, t.bin_nclaims, t.bin_csum};
+  uint32_t before[3], total[3];
+  group_prefix<3>(lists, t.bin_groups, g, before, total);
+  const uint32_t npb = min(total[0], t.pblock_cap);
+  // particle offsets of this group's particle blocks, in id order
+  const uint32_t len = t.bin_claims[g];
+  const uint32_t* stage = t.bin_stage + (size_t)g * t.bin_span;
+  uint32_t carry = before[2];
+  for (uint32_t c0 = 0; c0 < len; c0 += kBinThreads) {
+    const uint32_t j = c0 + threadIdx.x, pb = before[0] + j;
+    const bool mine = j < len && pb < t.pblock_cap;
+    const uint32_t cnt = mine ? t.bucket_count[stage[j]] : 0u;
+    uint32_t sum;
+    const uint32_t excl = group_exclusive_scan(cnt, sum);
+    if (mine) {
+      t.pblock_start[pb] = carry + excl;
+      t.pblock_len[pb] = cnt;
     }
-    uint32_t incl[3] = {acc[0], acc[1], acc[2]}, tot[3];
-    block_scan3(incl, tot);
-    uint32_t run[3];
-    for (int k = 0; k < 3; ++k) run[k] = t.chunk_sums[3 * c + k] + incl[k] - acc[k];
-    for (int q = 0; q < per; ++q) {
-      uint32_t b = c * kScanChunk + threadIdx.x * per + q;
-      uint32_t slot = kInvalid, pbid = kInvalid;
-      if (fl[q][0]) {
-        if (run[0] < t.grid_cap) {
-          slot = run[0];
-          t.grid_key[slot] = owner_key(p, t, classes, scenes, t.owner[b]);
-        }
-      }
-      if (fl[q][1]) {
-        if (run[1] < t.pblock_cap) {
-          pbid = run[1];
-          t.pblock_bucket[pbid] = b;
-          t.pblock_start[pbid] = run[2];
-          t.pblock_len[pbid] = fl[q][2];
-        }
-      }
+    carry += sum;
+  }
+  // grid slots of the neighbour-only blocks this group claimed
+  const uint32_t nlen = t.bin_nclaims[g];
+  const uint32_t* nstage = t.bin_nstage + (size_t)g * 7 * t.bin_span;
+  bool over = false;
+  for (uint32_t j = threadIdx.x; j < nlen; j += kBinThreads) {
+    const uint32_t b = nstage[j], slot = npb + before[1] + j;
+    if (slot < t.grid_cap) {
       t.bucket_slot[b] = slot;
-      t.bucket_pblock[b] = pbid;
-      for (int k = 0; k < 3; ++k) run[k] += fl[q][k];
+      t.grid_key[slot] = owner_key(p, t, classes, scenes, t.owner[b]);
+      clear_rev(t, slot);
+    } else {
+      t.bucket_slot[b] = kInvalid;
+      over = true;
     }
+  }
+  flag_error(t.counters, over, ERR_TOPOLOGY);
+  if (g == 0 && threadIdx.x == 0) {
+    t.counters[CNT_PARTICLE_BLOCKS] = npb;
+    t.counters[CNT_GRID_BLOCKS] = min(npb + total[1], t.grid_cap);
+    t.counters[CNT_BINNED] = total[2];
   }
 }
 
 // ---------------------------------------------------------------------------
 // 5. per particle block: grid slots of the 8 tile blocks
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pblock_finalize(Topo t) {
+__device__ __forceinline__ void pblock_finalize(const Topo& t) {
   uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npb * 8; q += gridDim.x * blockDim.x) {
     uint32_t k = q >> 3, d = q & 7;
@@ -345,6 +427,7 @@ __global__ __launch_bounds__(256) void k_pblock_finalize(Topo t) {
 // 6. scatter particle indices into block-sorted order (optionally the state)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
+  pblock_finalize(t);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
     uint32_t b = t.pslot[i];
     if (b == kInvalid) continue;
@@ -358,6 +441,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
 // the destination arrays `q`; sorted[] becomes the identity.  Particles that
 // were not binned (disabled) keep their slots in the tail [binned, n).
 __global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo t, uint32_t tail_begin) {
+  pblock_finalize(t);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
     uint32_t dst = kInvalid;
     uint32_t b = t.pslot[i];
@@ -384,14 +468,8 @@ __global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo
 }
 
 // ---------------------------------------------------------------------------
-// 7. clear the grid-slot -> covering-tile map of the live grid slots
+// Diagnostics: live-node masks
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rev_clear(Topo t) {
-  uint32_t ng = t.counters[CNT_GRID_BLOCKS];
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < 8 * ng; q += gridDim.x * blockDim.x)
-    t.grid_rev[q] = kInvalid;
-}
-
 __global__ __launch_bounds__(256) void k_mask_clear(Topo t) {
   uint32_t ng = t.counters[CNT_GRID_BLOCKS];
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * ng; q += gridDim.x * blockDim.x)
@@ -433,28 +511,45 @@ __device__ __forceinline__ bool regular_corotated(const M3& F) {
   float det = dot3(a, cross3(b, c));
   float nrm = sqrtf(dot3(a, a) + dot3(b, b) + dot3(c, c));
   if (fabsf(det) < 1e-5f || nrm < 1e-5f || nrm > 20.0f) return false;
-  float3 i0 = cross3(b, c) / det, i1 = cross3(c, a) / det, i2 = cross3(a, b) / det;
+  const float id = 1.0f / det;
+  float3 i0 = cross3(b, c) * id, i1 = cross3(c, a) * id, i2 = cross3(a, b) * id;
   float inrm = sqrtf(dot3(i0, i0) + dot3(i1, i1) + dot3(i2, i2));
   return nrm * inrm <= 100.0f;
 }
 
-// Newton iteration R <- (R + R^-T) / 2 (gpu.wgsl:467-482).
+// Newton iteration R <- (R + R^-T) / 2 (gpu.wgsl:467-482), at most 10 steps.
+// The gate above bounds the condition number by 100, where the iteration
+// converges quadratically; the wave leaves the loop once no lane's step
+// moved an entry by more than 1e-6 (the next step would be below f32
+// resolution).  Near F = I that is 3-4 steps instead of 10.
 __device__ void polar_newton(const M3& F, M3& R) {
   R = F;
   for (int it = 0; it < 10; ++it) {
     float3 a = col(R, 0), b = col(R, 1), c = col(R, 2);
     float det = dot3(a, cross3(b, c));
-    float inv = 1.0f / det;
+    float inv = __builtin_amdgcn_rcpf(det);
     float3 t0 = cross3(b, c) * inv, t1 = cross3(c, a) * inv, t2 = cross3(a, b) * inv;
-    R.a[0] = 0.5f * (a.x + t0.x); R.a[1] = 0.5f * (a.y + t0.y); R.a[2] = 0.5f * (a.z + t0.z);
-    R.a[3] = 0.5f * (b.x + t1.x); R.a[4] = 0.5f * (b.y + t1.y); R.a[5] = 0.5f * (b.z + t1.z);
-    R.a[6] = 0.5f * (c.x + t2.x); R.a[7] = 0.5f * (c.y + t2.y); R.a[8] = 0.5f * (c.z + t2.z);
+    float delta = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float n0 = 0.5f * (comp(a, q) + comp(t0, q));
+      const float n1 = 0.5f * (comp(b, q) + comp(t1, q));
+      const float n2 = 0.5f * (comp(c, q) + comp(t2, q));
+      delta = fmaxf(delta, fmaxf(fabsf(n0 - R.a[q]), fmaxf(fabsf(n1 - R.a[3 + q]), fabsf(n2 - R.a[6 + q]))));
+      R.a[q] = n0;
+      R.a[3 + q] = n1;
+      R.a[6 + q] = n2;
+    }
+    if (__all(delta <= 1e-6f)) break;
   }
 }
 
 // FP64 fallback for near-degenerate F: one-sided Jacobi SVD, R = U V^T (the
 // rotation the reference CPU takes from nalgebra's SVD, material.rs:447-450).
-__device__ __noinline__ void polar_fp64(const M3& Ff, M3& R) {
+// Taken and returned by value: a reference would make every caller keep F in
+// scratch memory for this rarely taken call.
+__device__ __noinline__ M3 polar_fp64(M3 Ff) {
+  M3 R;
   double A[3][3], V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) A[r][c] = (double)Ff.at(r, c);
@@ -508,6 +603,7 @@ __device__ __noinline__ void polar_fp64(const M3& Ff, M3& R) {
   }
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) R.at(r, c) = (float)(U[r][0] * V[c][0] + U[r][1] * V[c][1] + U[r][2] * V[c][2]);
+  return R;
 }
 
 // Kirchhoff stress (material.rs:234-284; f32 op order of gpu.wgsl:638-699).
@@ -534,7 +630,7 @@ __device__ void kirchhoff(const DevClass& c, const M3& F, const M3& C, M3& S) {
   } else {
     M3 R;
     if (regular_corotated(F)) polar_newton(F, R);
-    else polar_fp64(F, R);
+    else R = polar_fp64(F);
     float iso = c.lambda * (j - 1.0f) * j;
     float k = 2.0f * c.mu;
     for (int r = 0; r < 3; ++r)
@@ -738,41 +834,91 @@ __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
   local = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
 }
 
-// P2G accumulates in 64-bit fixed point in LDS for both precision modes:
-// gfx950's integer LDS atomics (ds_add_u64) run an order of magnitude faster
-// than its float LDS atomics (ds_add_f32), and integer sums make the
-// per-block accumulation independent of atomic order.  The FP32 mode rounds
-// each tile node to f32 when storing the tile; the FIXED64 mode stores the
-// integers and the grid update sums them exactly.
-template <int FIXED>
-__global__ __launch_bounds__(256) void k_p2g(Parts p, Topo t, const DevClass* classes, const DevScene* scenes) {
-  using T = unsigned long long;
-  __shared__ T tile[4 * kTileNodes];  // [component][node]
+// P2G scatter.  One workgroup per particle block: every particle computes
+// its stress and scatters its 27 quadratic B-spline contributions into a 6^3
+// LDS tile.  Contributions are accumulated as fixed-point integers in both
+// precision modes: gfx950's float LDS atomics run at ~1/30 the rate of its
+// integer ones, and integer sums do not depend on atomic order.
+//
+// ACC = 1 (default): each scaled contribution s (|s| < 2^46, checked) is
+// converted to the 64-bit two's-complement integer floor(s) from
+// v = s * 2^-32: hi = floor(v), lo = floor((v - rint(v)) * 2^32) mod 2^32.
+// v - rint(v) is exact (|v| < 2^14), and both halves round the same way, so
+// hi * 2^32 + lo is floor(s) for either sign; one ds_add_u64 adds it.
+// ACC = 0: s is split exactly into hi * 2^20 + lo, two 32-bit integers added
+// with ds_add_u32; chunks of kP2GChunk particles keep both sums inside 32
+// bits.
+// FP32 mode converts each node's 64-bit sum to f32, FIXED64 mode stores the
+// integers and the grid update sums them exactly.  Blocks of more than one
+// chunk add their later chunks to the stored tile in chunk order, so both
+// modes stay deterministic.
+constexpr int kP2GChunk = 1024;
+
+__device__ __forceinline__ void split_fixed(float s, int& hi, int& lo) {
+  const float hf = truncf(s * 0x1p-20f);
+  const float r = fmaf(hf, -0x1p20f, s);  // exact
+  hi = (int)hf;
+  lo = (int)rintf(r);
+}
+
+// v = s * 2^-32 -> floor(s) as a 64-bit integer (see above).
+__device__ __forceinline__ unsigned long long fixed64_of(float v) {
+  // No contraction: the subtraction must see the same rounded v as floor(v)
+  // (fused into the caller's product it would see the unrounded one).
+#pragma clang fp contract(off)
+  const float d = (v - rintf(v)) * 0x1p32f;  // exact, in [-2^31, 2^31]
+  int hi, lo;  // v_cvt_flr_i32_f32: floor to integer in one instruction
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(v));
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(lo) : "v"(d));
+  return ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Node sum q of the LDS tile in either accumulation layout.
+template <int ACC>
+__device__ __forceinline__ long long tile_sum(const unsigned long long* acc, int q) {
+  if constexpr (ACC) {
+    return (long long)acc[q];
+  } else {
+    const int* hi = reinterpret_cast<const int*>(acc);
+    return (long long)hi[q] * (1ll << 20) + (long long)hi[4 * kTileNodes + q];
+  }
+}
+
+template <int FIXED, int THREADS, int ACC>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g(Parts p, Topo t,
+                                                                                         const DevClass* classes,
+                                                                                         const DevScene* scenes) {
+  __shared__ unsigned long long tacc[4 * kTileNodes];
+  int* const thi = reinterpret_cast<int*>(tacc);  // ACC = 0 view: hi halves, then lo halves
+  int* const tlo = thi + 4 * kTileNodes;
   __shared__ uint32_t s_slot[8];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
+  const int tid = threadIdx.x;
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
-    for (int q = threadIdx.x; q < 4 * kTileNodes; q += blockDim.x) tile[q] = 0ull;
-    if (threadIdx.x < 8) s_slot[threadIdx.x] = t.pblock_slot8[8 * k + threadIdx.x];
+    if (tid < 8) s_slot[tid] = t.pblock_slot8[8 * k + tid];
+    for (int q = tid; q < 4 * kTileNodes; q += THREADS) tacc[q] = 0;
     __syncthreads();
     const uint32_t own = s_slot[0];
-    int mass_shift = 0, mom_shift = 0;
-    if (own != kInvalid) {
-      const int4 key = t.grid_key[own];
-      const DevScene& sc = scenes[(uint32_t)key.w >> 16];
-      const float h = sc.h, inv_h = sc.inv_h;
-      const float stress_scale = -sc.dt * sc.inv_d;
-      mass_shift = sc.mass_shift;
-      mom_shift = sc.mom_shift;
-      const float mscale = exp2f((float)mass_shift);
-      const float pscale = exp2f((float)mom_shift);
-      const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
-      const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
-      bool bad = false, range = false;
-      for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
+    if (own == kInvalid) {  // grid capacity exhausted (flagged by the scan)
+      __syncthreads();
+      continue;
+    }
+    const int4 key = t.grid_key[own];
+    const DevScene& sc = scenes[(uint32_t)key.w >> 16];
+    const float h = sc.h, inv_h = sc.inv_h;
+    const float stress_scale = -sc.dt * sc.inv_d;
+    const float mscale = exp2f((float)sc.mass_shift), pscale = exp2f((float)sc.mom_shift);
+    const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
+    const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
+    bool bad = false, range = false;
+    const uint32_t chunk = ACC ? max(len, 1u) : (uint32_t)kP2GChunk;
+    for (uint32_t c0 = 0; c0 < len; c0 += chunk) {
+      const uint32_t c1 = min(len, c0 + chunk);
+      for (uint32_t j = c0 + tid; j < c1; j += THREADS) {
         const uint32_t i = t.sorted[start + j];
         const float4 pm = p.pos_mass[i];
         const float4 vv = p.vel_vol[i];
-        const DevClass& cl = classes[p.cls[i]];
+        const DevClass& cls = classes[p.cls[i]];
         M3 F, C;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
@@ -794,55 +940,111 @@ __global__ __launch_bounds__(256) void k_p2g(Parts p, Topo t, const DevClass* cl
         quadratic_weights(fy, wy);
         quadratic_weights(fz, wz);
         M3 S;
-        kirchhoff(cl, F, C, S);
+        kirchhoff(cls, F, C, S);
         const float m = pm.w;
         const float sscale = stress_scale * vv.w;
-        // A = m C + stress * (-dt V0 inv_d), columns
-        const float3 a0 = col(C, 0) * m + col(S, 0) * sscale;
-        const float3 a1 = col(C, 1) * m + col(S, 1) * sscale;
-        const float3 a2 = col(C, 2) * m + col(S, 2) * sscale;
-        const float3 mv = make_float3(vv.x, vv.y, vv.z) * m;
+        // node o receives w_o (m v + A (o - fx) h) with A = m C + stress (-dt V0 inv_d):
+        // pre-scaled as q + o.x a0 + o.y a1 + o.z a2 (a = h A columns, q = m v - h A fx)
+        const float3 a0 = (col(C, 0) * m + col(S, 0) * sscale) * (h * pscale);
+        const float3 a1 = (col(C, 1) * m + col(S, 1) * sscale) * (h * pscale);
+        const float3 a2 = (col(C, 2) * m + col(S, 2) * sscale) * (h * pscale);
+        const float3 q = make_float3(vv.x, vv.y, vv.z) * (m * pscale) - (a0 * fx + a1 * fy + a2 * fz);
+        const float ms = m * mscale;
+        // weights are at most 1, so |contribution| <= |q| + 2 (|a0| + |a1| + |a2|)
+        const float bound = fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fabsf(q.z)) +
+                            2.0f * (fmaxf(fmaxf(fabsf(a0.x), fabsf(a0.y)), fabsf(a0.z)) +
+                                    fmaxf(fmaxf(fabsf(a1.x), fabsf(a1.y)), fabsf(a1.z)) +
+                                    fmaxf(fmaxf(fabsf(a2.x), fabsf(a2.y)), fabsf(a2.z)));
+        constexpr float kFixedBound = ACC ? 0x1p46f : 0x1p40f;  // per-contribution limit (units)
+        range |= !(ms < kFixedBound && bound < kFixedBound);
+        if constexpr (ACC) {
+          const float wz2[3] = {wz[0] * 0x1p-32f, wz[1] * 0x1p-32f, wz[2] * 0x1p-32f};
 #pragma unroll
-        for (int x = 0; x < 3; ++x)
+          for (int x = 0; x < 3; ++x) {
+            const float3 qx = q + a0 * (float)x;
 #pragma unroll
-          for (int y = 0; y < 3; ++y)
+            for (int y = 0; y < 3; ++y) {
+              const float3 qxy = qx + a1 * (float)y;
+              const float wxy = wx[x] * wy[y];
 #pragma unroll
-            for (int z = 0; z < 3; ++z) {
-              const float w = wx[x] * wy[y] * wz[z];
-              const float dx = ((float)x - fx) * h, dy = ((float)y - fy) * h, dz = ((float)z - fz) * h;
-              const float3 am = a0 * dx + a1 * dy + a2 * dz;
-              const float3 mom = (mv + am) * w;
-              const float sm = (w * m) * mscale, sx = mom.x * pscale, sy = mom.y * pscale, sz = mom.z * pscale;
-              const float lim = 35184372088832.0f;  // 2^45: |contribution| bound
-              range |= !(fabsf(sm) < lim && fabsf(sx) < lim && fabsf(sy) < lim && fabsf(sz) < lim);
-              const int n = tile_index(lx + x, ly + y, lz + z);
-              atomicAdd(&tile[n], (T)__float2ll_rn(sm));
-              atomicAdd(&tile[kTileNodes + n], (T)__float2ll_rn(sx));
-              atomicAdd(&tile[2 * kTileNodes + n], (T)__float2ll_rn(sy));
-              atomicAdd(&tile[3 * kTileNodes + n], (T)__float2ll_rn(sz));
+              for (int z = 0; z < 3; ++z) {
+                const float w = wxy * wz2[z];
+                const float3 mom = qxy + a2 * (float)z;
+                const int n = tile_index(lx + x, ly + y, lz + z);
+                atomicAdd(&tacc[n], fixed64_of(ms * w));
+                atomicAdd(&tacc[kTileNodes + n], fixed64_of(mom.x * w));
+                atomicAdd(&tacc[2 * kTileNodes + n], fixed64_of(mom.y * w));
+                atomicAdd(&tacc[3 * kTileNodes + n], fixed64_of(mom.z * w));
+              }
             }
+          }
+        } else {
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            const float3 qx = q + a0 * (float)x;
+#pragma unroll
+            for (int y = 0; y < 3; ++y) {
+              const float3 qxy = qx + a1 * (float)y;
+              const float wxy = wx[x] * wy[y];
+#pragma unroll
+              for (int z = 0; z < 3; ++z) {
+                const float w = wxy * wz[z];
+                const float3 mom = (qxy + a2 * (float)z) * w;
+                const int n = tile_index(lx + x, ly + y, lz + z);
+                int hi, lo;
+                split_fixed(w * ms, hi, lo);
+                atomicAdd(&thi[n], hi);
+                atomicAdd(&tlo[n], lo);
+                split_fixed(mom.x, hi, lo);
+                atomicAdd(&thi[kTileNodes + n], hi);
+                atomicAdd(&tlo[kTileNodes + n], lo);
+                split_fixed(mom.y, hi, lo);
+                atomicAdd(&thi[2 * kTileNodes + n], hi);
+                atomicAdd(&tlo[2 * kTileNodes + n], lo);
+                split_fixed(mom.z, hi, lo);
+                atomicAdd(&thi[3 * kTileNodes + n], hi);
+                atomicAdd(&tlo[3 * kTileNodes + n], lo);
+              }
+            }
+          }
+        }
       }
-      flag_error(t.counters, bad, ERR_INTERNAL);
-      flag_error(t.counters, range, ERR_FIXED_RANGE);
-    }
-    __syncthreads();
-    if (own != kInvalid) {
+      __syncthreads();
+      // this chunk's exact 64-bit node sums go to the tile (added after the first chunk)
+      const bool first = c0 == 0, last = c1 == len;
       if (FIXED) {
         longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + (size_t)k * kTileNodes * 2;
-        for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
-          out[2 * n] = make_longlong2((long long)tile[n], (long long)tile[kTileNodes + n]);
-          out[2 * n + 1] = make_longlong2((long long)tile[2 * kTileNodes + n], (long long)tile[3 * kTileNodes + n]);
+        for (int n = tid; n < kTileNodes; n += THREADS) {
+          longlong2 a = make_longlong2(tile_sum<ACC>(tacc, n), tile_sum<ACC>(tacc, kTileNodes + n));
+          longlong2 b = make_longlong2(tile_sum<ACC>(tacc, 2 * kTileNodes + n), tile_sum<ACC>(tacc, 3 * kTileNodes + n));
+          if (!first) {
+            const longlong2 a0 = out[2 * n], b0 = out[2 * n + 1];
+            a.x += a0.x; a.y += a0.y; b.x += b0.x; b.y += b0.y;
+          }
+          out[2 * n] = a;
+          out[2 * n + 1] = b;
         }
       } else {
-        const float im = exp2f(-(float)mass_shift), ip = exp2f(-(float)mom_shift);
+        const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
         float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes;
-        for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x)
-          out[n] = make_float4(__ll2float_rn((long long)tile[n]) * im,
-                               __ll2float_rn((long long)tile[kTileNodes + n]) * ip,
-                               __ll2float_rn((long long)tile[2 * kTileNodes + n]) * ip,
-                               __ll2float_rn((long long)tile[3 * kTileNodes + n]) * ip);
+        for (int n = tid; n < kTileNodes; n += THREADS) {
+          float4 v = make_float4(__ll2float_rn(tile_sum<ACC>(tacc, n)) * im,
+                                 __ll2float_rn(tile_sum<ACC>(tacc, kTileNodes + n)) * ip,
+                                 __ll2float_rn(tile_sum<ACC>(tacc, 2 * kTileNodes + n)) * ip,
+                                 __ll2float_rn(tile_sum<ACC>(tacc, 3 * kTileNodes + n)) * ip);
+          if (!first) {
+            const float4 o = out[n];
+            v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+          }
+          out[n] = v;
+        }
       }
+      if (!last)
+        for (int q = tid; q < 4 * kTileNodes; q += THREADS) tacc[q] = 0;
+      __syncthreads();
     }
+    flag_error(t.counters, bad, ERR_INTERNAL);
+    flag_error(t.counters, range, ERR_FIXED_RANGE);
     __syncthreads();
   }
 }
@@ -925,6 +1127,15 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
+  // The block hash is not read after the scatter of this substep: clear it
+  // here for the next substep's binning (saves a launch).
+  {
+    const uint32_t nb = t.bucket_mask + 1;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+      t.owner[b] = 0;
+      t.bucket_count[b] = 0;
+    }
+  }
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
     if (threadIdx.x < 8) s_slot[threadIdx.x] = t.pblock_slot8[8 * k + threadIdx.x];
     __syncthreads();
@@ -973,22 +1184,33 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
         quadratic_weights(fx, wx);
         quadratic_weights(fy, wy);
         quadratic_weights(fz, wz);
+        // v = sum w v_i; C = sum v_i (w (o - fx) h inv_d)^T, with h inv_d applied once
         float3 v = make_float3(0, 0, 0), c0 = v, c1 = v, c2 = v;
+        const float ex[3] = {-fx, 1.0f - fx, 2.0f - fx};
+        const float ey[3] = {-fy, 1.0f - fy, 2.0f - fy};
+        const float ez[3] = {-fz, 1.0f - fz, 2.0f - fz};
+        const int n0 = tile_index(lx, ly, lz);
+#pragma unroll 1
+        for (int x = 0; x < 3; ++x) {
 #pragma unroll
-        for (int x = 0; x < 3; ++x)
-#pragma unroll
-          for (int y = 0; y < 3; ++y)
+          for (int y = 0; y < 3; ++y) {
+            const float wxy = wx[x] * wy[y];
 #pragma unroll
             for (int z = 0; z < 3; ++z) {
-              const float w = wx[x] * wy[y] * wz[z];
-              const int n = tile_index(lx + x, ly + y, lz + z);
-              const float3 nv = make_float3(vt[n], vt[kTileNodes + n], vt[2 * kTileNodes + n]);
-              const float dx = ((float)x - fx) * h, dy = ((float)y - fy) * h, dz = ((float)z - fz) * h;
-              v = v + nv * w;
-              c0 = c0 + nv * (dx * w * inv_d);
-              c1 = c1 + nv * (dy * w * inv_d);
-              c2 = c2 + nv * (dz * w * inv_d);
+              const int n = n0 + (x * kTileEdge + y) * kTileEdge + z;
+              const float w = wxy * wz[z];
+              const float3 nvw = make_float3(vt[n], vt[kTileNodes + n], vt[2 * kTileNodes + n]) * w;
+              v = v + nvw;
+              c0 = c0 + nvw * ex[x];
+              c1 = c1 + nvw * ey[y];
+              c2 = c2 + nvw * ez[z];
             }
+          }
+        }
+        const float hd = h * inv_d;
+        c0 = c0 * hd;
+        c1 = c1 * hd;
+        c2 = c2 * hd;
         if (sc.apply_force) {
           const uint32_t ii = p.ins_id[i];
           const float4 f = p.force[ii];
@@ -1121,6 +1343,16 @@ __global__ __launch_bounds__(256) void k_live_count(Topo t) {
 }
 
 // ---------------------------------------------------------------------------
+// Bandwidth probe: a plain float4 copy of known size (reads n*16 B, writes
+// n*16 B), used to measure achievable HBM bandwidth and to calibrate
+// hardware byte counters.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------
 static inline uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t max_blocks) {
@@ -1134,19 +1366,13 @@ void launch_topo_clear(const Topo& t, hipStream_t s) {
   hipLaunchKernelGGL(k_topo_clear, dim3(grid_for(t.bucket_mask + 1ull, 256, 4096)), dim3(256), 0, s, t);
 }
 void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_insert, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t, c, sc);
+  hipLaunchKernelGGL(k_bin_insert, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
 }
-void launch_bin_neighbors(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_neighbors, dim3(grid_for(t.bucket_mask + 1ull, 256, 8192)), dim3(256), 0, s, p, t, c, sc);
+void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_blocks, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
 }
-void launch_scan(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
-  uint32_t g = grid_for(t.n_chunks, 1, 4096);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(256), 0, s, p, t);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, t);
-  hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(256), 0, s, p, t, c, sc);
-}
-void launch_pblock_finalize(const Topo& t, hipStream_t s) {
-  hipLaunchKernelGGL(k_pblock_finalize, dim3(grid_for(8ull * t.pblock_cap, 256, 4096)), dim3(256), 0, s, t);
+void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_offsets, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
 }
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t);
@@ -1154,15 +1380,23 @@ void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
 }
-void launch_rev_clear(const Topo& t, hipStream_t s) {
-  hipLaunchKernelGGL(k_rev_clear, dim3(grid_for(8ull * t.grid_cap, 256, 8192)), dim3(256), 0, s, t);
-}
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
-                uint32_t threads, hipStream_t s) {
-  if (fixed64)
-    hipLaunchKernelGGL(k_p2g<1>, dim3(blocks), dim3(threads), 0, s, p, t, c, sc);
-  else
-    hipLaunchKernelGGL(k_p2g<0>, dim3(blocks), dim3(threads), 0, s, p, t, c, sc);
+                uint32_t threads, int acc, hipStream_t s) {
+  // threads per workgroup: 64, 128 or 256 (anything else runs 128); acc: 64-bit (1) or split 32-bit (0) LDS sums
+#define TMPM_P2G(F, T, A) hipLaunchKernelGGL((k_p2g<F, T, A>), dim3(blocks), dim3(T), 0, s, p, t, c, sc)
+#define TMPM_P2G_T(F, A)               \
+  do {                                 \
+    if (threads == 64) TMPM_P2G(F, 64, A);        \
+    else if (threads == 256) TMPM_P2G(F, 256, A); \
+    else TMPM_P2G(F, 128, A);          \
+  } while (0)
+  if (fixed64) {
+    if (acc) TMPM_P2G_T(1, 1); else TMPM_P2G_T(1, 0);
+  } else {
+    if (acc) TMPM_P2G_T(0, 1); else TMPM_P2G_T(0, 0);
+  }
+#undef TMPM_P2G_T
+#undef TMPM_P2G
 }
 void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, hipStream_t s) {
   uint32_t blocks = grid_for(t.grid_cap, 4, 16384);
@@ -1182,6 +1416,10 @@ void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const D
   hipLaunchKernelGGL(k_mask_clear, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);
   hipLaunchKernelGGL(k_live_mask, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t, c, sc);
   hipLaunchKernelGGL(k_live_count, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);
+}
+
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256 * 8, 16384)), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace tmpm

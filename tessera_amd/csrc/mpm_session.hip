@@ -15,6 +15,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -391,11 +392,14 @@ struct tmpm_session {
   uint32_t reorder_interval = 8;
 
   // topology
-  DevBuf owner, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, chunk_sums, grid_key, pblock_bucket,
-      pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask;
+  DevBuf owner, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, grid_key, pblock_bucket,
+      pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask,
+    bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum;
   Topo topo{};
-  uint32_t p2g_blocks = 4096, threads_pblock = 128;
+  uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 128;
+  int p2g_acc = 1;
   bool topo_valid = false;  // topology arrays describe the current device state
+  bool table_dirty = true;  // block hash needs clearing before the next binning
 
   // timing
   bool timing = false;
@@ -543,13 +547,18 @@ tmpm_status build_scene(tmpm_session* s) {
   for (int k = 0; k < 3; ++k)
     if (!finite_f32(p.gravity[k], sc.gravity[k])) return fail(TMPM_INVALID_INPUT, "gravity not representable");
   sc.has_bounds = p.has_bounds ? 1 : 0;
-  // Fixed-point exponents: one contribution stays below 2^45 units (checked
-  // per contribution), so up to 2^18 contributions per node cannot wrap the
-  // int64 accumulator.  Momentum is bounded by mass x 64 cells per substep.
+  // Fixed-point exponents: a particle's mass maps to at most 2^44 units
+  // (2^38 with the split 32-bit accumulation) and every contribution must
+  // stay below 2^46 units (2^40; checked per particle); momentum is scaled
+  // for mass x 64 cells per substep.  An int64 node sum then holds 2^17
+  // contributions (2^23) before it could wrap.  The 64-bit conversion rounds
+  // down, so its unit must be fine enough that the -1/2 unit mean error per
+  // contribution stays far below f32 resolution.
   if (s->max_mass > 0.0) {
     double vbound = 64.0 * p.cell_width / s->substep_dt;
-    int ms = (int)std::floor(std::log2(std::ldexp(1.0, 44) / s->max_mass));
-    int ps = (int)std::floor(std::log2(std::ldexp(1.0, 44) / (s->max_mass * vbound)));
+    const int top = s->p2g_acc ? 44 : 38;
+    int ms = (int)std::floor(std::log2(std::ldexp(1.0, top) / s->max_mass));
+    int ps = (int)std::floor(std::log2(std::ldexp(1.0, top) / (s->max_mass * vbound)));
     sc.mass_shift = std::max(-120, std::min(120, ms));
     sc.mom_shift = std::max(-120, std::min(120, ps));
   }
@@ -572,10 +581,28 @@ tmpm_status build_scene(tmpm_session* s) {
   return TMPM_OK;
 }
 
+// Launch-shape overrides for tuning sweeps, read once per session.
+void read_tuning_env(tmpm_session* s) {
+  if (const char* v = std::getenv("TMPM_PBLOCK_THREADS")) {
+    int th = std::atoi(v);
+    if (th == 64 || th == 128 || th == 192 || th == 256) s->threads_pblock = (uint32_t)th;
+  }
+  if (const char* v = std::getenv("TMPM_P2G_THREADS")) {
+    int th = std::atoi(v);
+    if (th == 64 || th == 128 || th == 256) s->p2g_threads = (uint32_t)th;
+  }
+  if (const char* v = std::getenv("TMPM_P2G_ACC")) s->p2g_acc = std::atoi(v) != 0;
+  if (const char* v = std::getenv("TMPM_REORDER_INTERVAL")) {
+    int r = std::atoi(v);
+    if (r >= 1 && r <= 1000000) s->reorder_interval = (uint32_t)r;
+  }
+}
+
 // Allocates the topology for `pblocks` particle blocks and `gblocks` grid
 // blocks of capacity.
 tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_cap) {
-  uint32_t buckets = std::max<uint32_t>(next_pow2(2ull * grid_cap), (uint32_t)kScanChunk);
+  pblock_cap = std::min(pblock_cap, grid_cap);  // a particle block's grid slot is its id
+  uint32_t buckets = std::max<uint32_t>(next_pow2(2ull * grid_cap), (uint32_t)kMinBuckets);
   if ((uint64_t)s->n + 8ull * buckets + 1 >= 0xffffffffull) return fail(TMPM_CAPACITY, "topology id space exhausted");
   Topo& t = s->topo;
   HIP_TRY(s->owner.alloc(4ull * buckets));
@@ -583,8 +610,6 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->nbr_bucket.alloc(32ull * buckets));
   HIP_TRY(s->bucket_slot.alloc(4ull * buckets));
   HIP_TRY(s->bucket_pblock.alloc(4ull * buckets));
-  uint32_t chunks = buckets / kScanChunk;
-  HIP_TRY(s->chunk_sums.alloc(12ull * (chunks + 1)));
   HIP_TRY(s->grid_key.alloc(16ull * grid_cap));
   HIP_TRY(s->pblock_bucket.alloc(4ull * pblock_cap));
   HIP_TRY(s->pblock_slot8.alloc(32ull * pblock_cap));
@@ -595,6 +620,23 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->n, 1)));
   HIP_TRY(s->counters.alloc(4ull * CNT_COUNT));
   HIP_TRY(hipMemset(s->counters.ptr, 0, 4ull * CNT_COUNT));
+  // Binning workgroups: at most ~1024 (each one's id prefix sums over all of
+  // them), each over a whole number of 256-particle rounds.
+  const uint64_t nparts = std::max<uint32_t>(s->n, 1);
+  const uint32_t rounds = (uint32_t)((nparts + 256ull * 1024 - 1) / (256ull * 1024));
+  t.bin_span = 256u * rounds;
+  t.bin_groups = (uint32_t)((nparts + t.bin_span - 1) / t.bin_span);
+  const uint64_t staged = (uint64_t)t.bin_groups * t.bin_span;
+  HIP_TRY(s->bin_stage.alloc(4ull * staged));
+  HIP_TRY(s->bin_nstage.alloc(28ull * staged));
+  HIP_TRY(s->bin_claims.alloc(4ull * t.bin_groups));
+  HIP_TRY(s->bin_nclaims.alloc(4ull * t.bin_groups));
+  HIP_TRY(s->bin_csum.alloc(4ull * t.bin_groups));
+  t.bin_stage = s->bin_stage.as<uint32_t>();
+  t.bin_nstage = s->bin_nstage.as<uint32_t>();
+  t.bin_claims = s->bin_claims.as<uint32_t>();
+  t.bin_nclaims = s->bin_nclaims.as<uint32_t>();
+  t.bin_csum = s->bin_csum.as<uint32_t>();
   HIP_TRY(s->ptile.alloc((size_t)pblock_cap * kTileNodes * (s->fixed64 ? 32 : 16)));
   HIP_TRY(s->grid_rev.alloc(32ull * grid_cap));
   HIP_TRY(s->grid_vel.alloc((size_t)grid_cap * kBlockNodes * 16));
@@ -605,8 +647,6 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   t.bucket_slot = s->bucket_slot.as<uint32_t>();
   t.bucket_pblock = s->bucket_pblock.as<uint32_t>();
   t.bucket_mask = buckets - 1;
-  t.n_chunks = chunks;
-  t.chunk_sums = s->chunk_sums.as<uint32_t>();
   t.grid_key = s->grid_key.as<int4>();
   t.grid_cap = grid_cap;
   t.pblock_cap = pblock_cap;
@@ -624,6 +664,7 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   t.node_mask = s->node_mask.as<uint32_t>();
   s->p2g_blocks = std::min<uint32_t>(pblock_cap, 65535);
   s->topo_valid = false;
+  s->table_dirty = true;
   return TMPM_OK;
 }
 
@@ -837,12 +878,14 @@ hipError_t enqueue_binning(tmpm_session* s, const Parts& p) {
   const DevScene* sc = s->scene_plain.as<DevScene>();
   Topo& t = s->topo;
   hipError_t e;
-  if ((e = timed(s, "k_topo_clear", [&] { launch_topo_clear(t, s->stream); })) != hipSuccess) return e;
+  if (s->table_dirty) {
+    if ((e = timed(s, "k_topo_clear", [&] { launch_topo_clear(t, s->stream); })) != hipSuccess) return e;
+    s->table_dirty = false;
+  }
+  if ((uint64_t)t.bin_groups * t.bin_span < p.n) return hipErrorInvalidValue;  // lists sized for fewer particles
   if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p, t, c, sc, s->stream); })) != hipSuccess) return e;
-  if ((e = timed(s, "k_bin_neighbors", [&] { launch_bin_neighbors(p, t, c, sc, s->stream); })) != hipSuccess) return e;
-  if ((e = timed(s, "k_scan", [&] { launch_scan(p, t, c, sc, s->stream); })) != hipSuccess) return e;
-  if ((e = timed(s, "k_rev_clear", [&] { launch_rev_clear(t, s->stream); })) != hipSuccess) return e;
-  if ((e = timed(s, "k_pblock_finalize", [&] { launch_pblock_finalize(t, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_bin_blocks", [&] { launch_bin_blocks(p, t, c, sc, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_bin_offsets", [&] { launch_bin_offsets(p, t, c, sc, s->stream); })) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -867,7 +910,7 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   }
   s->substeps_since_reorder++;
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
-                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->threads_pblock, s->stream); })) != hipSuccess)
+                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_acc, s->stream); })) != hipSuccess)
     return e;
   if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, s->stream); })) != hipSuccess)
     return e;
@@ -981,6 +1024,8 @@ tmpm_status run_diagnostic_binning(tmpm_session* s, std::vector<uint32_t>& count
   bool was = s->timing;
   s->timing = false;
   HIP_TRY(enqueue_binning(s, p));
+  launch_bin_scatter(p, s->topo, s->stream);  // fills the per-block tile slots
+  s->table_dirty = true;
   launch_live_nodes(p, s->topo, s->classes.as<DevClass>(), s->scene_plain.as<DevScene>(), s->stream);
   s->timing = was;
   HIP_TRY(hipGetLastError());
@@ -1038,6 +1083,7 @@ tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* wd, do
   s->ctx = ctx;
   s->substep_dt = substep_dt;
   s->fixed64 = mode == TMPM_P2G_FIXED64;
+  read_tuning_env(s);
   s->world.params = wd->params;
   s->world.append(wd->particles, wd->materials, 0);
   s->world.set_obstacles(wd->obstacles, wd->obstacle_count);
@@ -1432,6 +1478,33 @@ tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t
     names_csv[names_cap - 1] = '\0';
   }
   *count_out = k;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_bandwidth_probe(int32_t device_ordinal, uint64_t bytes, uint32_t iterations, double* gbs_out) {
+  if (!gbs_out || bytes < 16 || iterations == 0) return fail(TMPM_INVALID_INPUT, "invalid probe arguments");
+  if (hipSetDevice(device_ordinal) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
+  size_t n = bytes / 16;
+  DevBuf a, b;
+  HIP_TRY(a.alloc(n * 16));
+  HIP_TRY(b.alloc(n * 16));
+  HIP_TRY(hipMemset(a.ptr, 0, n * 16));
+  hipStream_t st;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  launch_copy(a.as<float4>(), b.as<float4>(), n, st);  // warm up
+  HIP_TRY(hipEventRecord(e0, st));
+  for (uint32_t i = 0; i < iterations; ++i) launch_copy(a.as<float4>(), b.as<float4>(), n, st);
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  *gbs_out = 2.0 * (double)n * 16.0 * iterations / (ms * 1e-3) / 1e9;
   return TMPM_OK;
 }
 

@@ -214,6 +214,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
             C.c_int,
             [vp, C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32,
              C.POINTER(C.c_uint32)]),
+        "tmpm_bandwidth_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
         "tmpm_timestep_bound": (C.c_double, [C.POINTER(CMaterial), C.c_double, C.c_double * 3, C.c_double, C.c_double]),
         "tmpm_material_valid": (C.c_int32, [C.POINTER(CMaterial)]),
         "tmpm_obstacle_valid": (C.c_int32, [C.POINTER(CObstacle)]),
@@ -936,3 +937,11 @@ class HipMpmResidentSession:
         _check(self.lib.tmpm_session_kernel_times(self._h, names, 4096, avg, calls, cap, C.byref(count)), self.lib)
         keys = names.value.decode().split(",") if names.value else []
         return {k: (avg[i], int(calls[i])) for i, k in enumerate(keys)}
+
+
+def bandwidth_probe(device: int = 0, nbytes: int = 1 << 30, iterations: int = 20) -> float:
+    """Achieved copy bandwidth (read + write GB/s) of a plain float4 copy."""
+    lib = load_library()
+    out = C.c_double()
+    _check(lib.tmpm_bandwidth_probe(int(device), int(nbytes), int(iterations), C.byref(out)), lib)
+    return float(out.value)
