@@ -12,7 +12,9 @@ ends with a device wait; readback and validation follow outside it.
 Printed JSON fields beyond the common contract:
   roofline      dominant kernel's algorithmic bytes per launch over its
                 average duration (HIP events on the session stream), against
-                MI355X's 8 TB/s HBM peak; plus the whole-substep figure
+                MI355X's 8 TB/s HBM peak; plus the whole-substep figure.
+                `traffic` is that kernel's measured HBM bytes per launch from
+                the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
   cpu_baseline  the FP64 C++ restatement of the reference CPU substep
                 (oracle/, one thread) timed on a bounded sample of the same
                 scene on this host
@@ -39,12 +41,71 @@ P2G_BYTES_PARTICLE, P2G_BYTES_NODE = 104, 16
 G2P_BYTES_PARTICLE, G2P_BYTES_NODE = 96, 16
 
 
+TRAFFIC_TABLE = ROOT / "profiles" / "substep_four_cubes_1M_hbm_traffic.md"
+
+
+def measured_traffic(kernel: str, mode: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (already scaled by the copy-kernel calibration), or None if it has no row for this kernel."""
+    if not TRAFFIC_TABLE.exists():
+        return None
+    prefix = {"k_p2g": "tmpm::k_p2g<0", "k_p2g_fixed64": "tmpm::k_p2g<1"}.get(kernel, "tmpm::" + kernel)
+    for line in TRAFFIC_TABLE.read_text().splitlines():
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].startswith(prefix):
+            try:
+                return float(cells[3]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def submit(session, steps: int) -> None:
     left = steps
     while left > 0:
         k = min(64, left)
         session.submit_steps(k)
         left -= k
+
+
+def batch_leg(transfers, mode: str, steps: int) -> dict:
+    """256 independent scenes (N_i log-uniform in [2048, 65536]) advanced as one batch, against
+    the same scenes advanced one resident session at a time (each timed from a device wait to a
+    device wait; sessions created beforehand)."""
+    from tessera_amd.mpm import HipMpmBatch, HipMpmResidentSession
+    from tessera_amd.scenes import batch_scenes
+
+    scenes = batch_scenes(256, seed=0)
+    total = sum(len(sc.world.particles) for sc in scenes)
+    b = HipMpmBatch(transfers, [sc.world for sc in scenes], [sc.dt for sc in scenes], mode)
+    submit(b, 8)
+    b.wait()
+    t0 = time.perf_counter()
+    submit(b, steps)
+    b.wait()
+    t_batch = time.perf_counter() - t0
+    b.synchronize()
+    failed = sum(1 for k in range(len(scenes)) if b.scene_error(k) is not None)
+    b.close()
+    sessions = [HipMpmResidentSession(transfers, sc.world, sc.dt, mode) for sc in scenes]
+    t_alone = 0.0
+    for s in sessions:
+        submit(s, 8)
+        s.wait()
+        t0 = time.perf_counter()
+        submit(s, steps)
+        s.wait()
+        t_alone += time.perf_counter() - t0
+    for s in sessions:
+        s.synchronize()
+        s.close()
+    value = total * steps / t_batch
+    alone = total * steps / t_alone
+    print(f"bench: batch of {len(scenes)} scenes ({total} particles): {value:.4e} vs one session at a time "
+          f"{alone:.4e} particle-updates/s", file=sys.stderr, flush=True)
+    return {"workload": "batch_256_scenes", "scenes": len(scenes), "particles": total, "steps": steps,
+            "value": value, "unit": "particle-updates/s", "ms_per_step": t_batch / steps * 1e3,
+            "one_session_at_a_time": alone, "speedup": value / alone, "failed_scenes": failed}
 
 
 def main() -> int:
@@ -57,6 +118,7 @@ def main() -> int:
     ap.add_argument("--cpu-substeps", type=int, default=2, help="substeps of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=64, help="substeps of the batched-scenes leg (0: skip)")
     args = ap.parse_args()
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -124,6 +186,10 @@ def main() -> int:
         extra_modes[other] = {"value": n * k2 / (tb - ta), "steps": k2}
         s2.close()
 
+    batch = None
+    if args.batch_steps > 0:
+        batch = batch_leg(transfers, args.mode, args.batch_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -170,7 +236,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dominant, args.mode),
+            "traffic_source": str(TRAFFIC_TABLE.relative_to(ROOT)),
             "algorithmic_bytes": alg[dominant],
             "avg_ms": dom_ms,
             "live_nodes": live,
@@ -182,6 +249,7 @@ def main() -> int:
         "event_substep_ms": substep_ms,
         "other_modes": extra_modes,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -255,6 +255,43 @@ tmpm_status tmpm_session_export_nodes(tmpm_session* s, int32_t* keys, uint64_t c
 tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uint64_t cap, int64_t* particle_block,
                                        uint64_t* count_out);
 
+/* ---------------------------------------------------------------------------
+ * Batches.  The reference has one world per GpuMpmResidentSession and no
+ * batch type; these entry points are the batch/serving boundary the
+ * replacement adds (many independent scenes advanced together, one kernel
+ * launch per stage per substep for all of them).  A batch is a session
+ * handle: tmpm_session_submit_steps / _wait / _synchronize / _step /
+ * _pending_substeps / _valid / _set_timing / _kernel_times / _destroy apply to
+ * all its scenes.  Per scene it keeps the resident-session semantics of
+ * GpuMpmResidentSession (gpu.rs:206-662): its own parameters, substep dt,
+ * obstacles and insertion order; transactional edits; and a failure after
+ * submission (CFL, particle out of the grid, non-finite state, fixed-point
+ * range) stops that scene only, keeping its last synchronized world
+ * readable; tmpm_batch_scene_status reports it.  Grid or hash capacity and
+ * device failures invalidate the whole batch.
+ * ------------------------------------------------------------------------- */
+/* New batch from `scene_count` worlds, each with its substep dt (each dt
+ * validated like GpuMpmResidentSession::new, gpu.rs:253-280). */
+tmpm_status tmpm_batch_create(tmpm_context* ctx, const tmpm_world_desc* worlds, const double* substep_dt,
+                              uint32_t scene_count, tmpm_p2g_mode mode, tmpm_session** out);
+/* Scene slots (including removed ones, whose status is INVALID_INPUT). */
+uint32_t tmpm_batch_scene_count(const tmpm_session* s);
+/* TMPM_OK while the scene steps; otherwise the error that stopped it. */
+tmpm_status tmpm_batch_scene_status(const tmpm_session* s, uint32_t scene);
+uint64_t tmpm_batch_scene_particle_count(const tmpm_session* s, uint32_t scene);
+uint64_t tmpm_batch_scene_substeps(const tmpm_session* s, uint32_t scene);
+/* `world()` of one scene (gpu.rs:653-656): its last synchronized state. */
+tmpm_status tmpm_batch_read_particles(const tmpm_session* s, uint32_t scene, tmpm_particles* out);
+/* `set_particle_force` / `set_obstacles` of one scene (gpu.rs:457-529). */
+tmpm_status tmpm_batch_set_particle_force(tmpm_session* s, uint32_t scene, uint64_t index, const double force[3]);
+tmpm_status tmpm_batch_set_obstacles(tmpm_session* s, uint32_t scene, const tmpm_obstacle* obstacles,
+                                     uint32_t count);
+/* Admits a new scene (serving): it takes the first free slot. */
+tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, double substep_dt,
+                                 uint32_t* scene_out);
+/* Retires a scene; its slot becomes free. */
+tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene);
+
 /* Per-kernel average durations (milliseconds) recorded with HIP events on the
  * session stream over the substeps since the last reset; names are the
  * kernel names.  Timing is off unless enabled (it adds one event pair per

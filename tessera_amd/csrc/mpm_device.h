@@ -27,6 +27,7 @@ constexpr int kTileEdge = 6;          // P2G/G2P tile: block + 2-cell halo
 constexpr int kTileNodes = 216;
 constexpr int kMinBuckets = 1024;     // smallest block-hash table
 constexpr int kMaxObstacles = 256;
+constexpr int32_t kNoKey = INT32_MIN;  // no block key has x = INT32_MIN (cells are >> 2)
 
 // counters[] slots
 enum Counter : uint32_t {
@@ -50,7 +51,7 @@ enum DevMaterial : int32_t { DM_COROTATED = 0, DM_NEO_HOOKEAN = 1, DM_FLUID = 2 
 
 struct DevClass {
   int32_t kind;
-  uint32_t region;   // (scene << 16) | transfer colour
+  uint32_t region;   // (scene << 16) | transfer colour; the block key's 4th field
   uint32_t fixed;
   float radius;
   float damping;
@@ -104,6 +105,7 @@ struct Parts {
 
 struct Topo {
   uint32_t* owner;          // bucket -> owner id (0 empty, 1..n particle, > n neighbour insert)
+  int4* bkey;               // bucket -> key, written by the claimer; x = kNoKey until then
   uint32_t* bucket_count;   // particles binned into the bucket's block
   uint32_t* nbr_bucket;     // 8 per bucket: buckets of block + {0,1}^3
   uint32_t* bucket_slot;    // bucket -> grid slot
@@ -120,6 +122,7 @@ struct Topo {
   uint32_t* prank;          // particle -> rank within its block
   uint32_t* sorted;         // binned order -> particle storage index
   uint32_t* counters;
+  uint32_t* scene_err;      // per scene: error bits of its particles and blocks
   // binning lists: insert workgroup g bins particles [g * bin_span, (g + 1) * bin_span)
   uint32_t bin_span;        // particles per binning workgroup (multiple of 256)
   uint32_t bin_groups;      // binning workgroups

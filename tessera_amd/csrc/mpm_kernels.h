@@ -14,7 +14,7 @@ void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const 
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s);
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s);
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
-                uint32_t threads, int acc, hipStream_t s);
+                uint32_t threads, int copies, hipStream_t s);
 void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, hipStream_t s);
 void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
                 uint32_t blocks, uint32_t threads, hipStream_t s);

@@ -51,6 +51,16 @@ __device__ __forceinline__ void flag_error(uint32_t* counters, bool cond, uint32
   if (m != 0 && lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1)) atomicOr(&counters[CNT_ERROR], bit);
 }
 
+// Error of one scene's particles or blocks (`scene` uniform over the wave):
+// recorded for the scene and in the session-wide word.
+__device__ __forceinline__ void flag_scene_error(const Topo& t, uint32_t scene, bool cond, uint32_t bit) {
+  uint64_t m = __ballot(cond);
+  if (m != 0 && lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1)) {
+    atomicOr(&t.counters[CNT_ERROR], bit);
+    atomicOr(&t.scene_err[scene], bit);
+  }
+}
+
 // Cell key of one coordinate: floor(x * inv_h - 0.5) with both operations
 // rounded to f32 and no fused multiply-add (gpu.rs:1145, the host packing of
 // the reference).  Valid when finite and within [i32::MIN, i32::MAX - 2].
@@ -114,7 +124,9 @@ __device__ __forceinline__ int4 owner_key(const Parts& p, const Topo& t, const D
 // Lock-free linear-probing insert.  Owners refer to keys that are immutable
 // during the kernel (particle positions, or a particle block's key plus a
 // fixed offset), so a losing thread compares against the winner's key
-// without waiting for anything to be published.  Bounded by the table size.
+// without waiting for anything to be published.  The winner also stores the
+// key in bkey; a loser that already sees it there skips recomputing the key
+// from the owner (a chain of dependent loads).  Bounded by the table size.
 __device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
                                 int4 key, uint32_t me, bool& claimed) {
   uint32_t b = hash_key(key) & t.bucket_mask;
@@ -123,9 +135,12 @@ __device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* c
     uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
     if (prev == 0u) {
       claimed = true;
+      t.bkey[b] = key;
       return b;
     }
-    if (key_eq(owner_key(p, t, classes, scenes, prev), key)) return b;
+    int4 k = t.bkey[b];
+    if (k.x == kNoKey) k = owner_key(p, t, classes, scenes, prev);
+    if (key_eq(k, key)) return b;
     b = (b + 1) & t.bucket_mask;
   }
   return kInvalid;
@@ -217,6 +232,7 @@ __global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
     t.owner[b] = 0;
     t.bucket_count[b] = 0;
+    t.bkey[b].x = kNoKey;
   }
 }
 
@@ -238,21 +254,33 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
   __syncthreads();
   const uint32_t begin = blockIdx.x * t.bin_span;
   uint32_t* stage = t.bin_stage + (size_t)blockIdx.x * t.bin_span;
+  // the next round's particle loads are issued before this round's atomics
+  float4 pm_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t cls_next = 0;
+  if (begin + threadIdx.x < p.n) {
+    pm_next = p.pos_mass[begin + threadIdx.x];
+    cls_next = p.cls[begin + threadIdx.x];
+  }
   for (uint32_t r = 0; r < t.bin_span; r += kBinThreads) {
     const uint32_t i = begin + r + threadIdx.x;
+    const float4 pm = pm_next;
+    const uint32_t ci = cls_next;
+    const uint32_t inext = i + kBinThreads;
+    if (r + kBinThreads < t.bin_span && inext < p.n) {
+      pm_next = p.pos_mass[inext];
+      cls_next = p.cls[inext];
+    }
     bool active = false;
     int4 key = make_int4(0, 0, 0, 0);
-    if (i < p.n) {
-      const float4 pm = p.pos_mass[i];
-      if (pm.w > 0.0f) {
-        const DevClass& c = classes[p.cls[i]];
-        int3 cell;
-        if (cell3(pm, scenes[c.region >> 16].inv_h, cell)) {
-          active = true;
-          key = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
-        } else {
-          atomicOr(&t.counters[CNT_ERROR], ERR_RESIDENT);
-        }
+    if (i < p.n && pm.w > 0.0f) {
+      const DevClass& c = classes[ci];
+      int3 cell;
+      if (cell3(pm, scenes[c.region >> 16].inv_h, cell)) {
+        active = true;
+        key = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
+      } else {
+        atomicOr(&t.counters[CNT_ERROR], ERR_RESIDENT);
+        atomicOr(&t.scene_err[c.region >> 16], ERR_RESIDENT);
       }
     }
     const int4 prev = make_int4(__shfl_up(key.x, 1, 64), __shfl_up(key.y, 1, 64), __shfl_up(key.z, 1, 64),
@@ -317,7 +345,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_blocks(Parts p, Topo t, con
       t.bucket_pblock[b] = pb;
       t.bucket_slot[b] = pb;
       t.pblock_bucket[pb] = b;
-      t.grid_key[pb] = particle_key(p, classes, scenes, t.owner[b] - 1);
+      t.grid_key[pb] = t.bkey[b];
       clear_rev(t, pb);
       csum += t.bucket_count[b];
     } else {
@@ -331,7 +359,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_blocks(Parts p, Topo t, con
     const uint32_t b = stage[j];
     if (d == 1) t.nbr_bucket[8 * b] = b;
     if (before[0] + j >= t.pblock_cap) continue;
-    int4 k = particle_key(p, classes, scenes, t.owner[b] - 1);
+    int4 k = t.bkey[b];
     k.x += (int)(d >> 2);
     k.y += (int)((d >> 1) & 1);
     k.z += (int)(d & 1);
@@ -392,7 +420,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_offsets(Parts p, Topo t, co
     const uint32_t b = nstage[j], slot = npb + before[1] + j;
     if (slot < t.grid_cap) {
       t.bucket_slot[b] = slot;
-      t.grid_key[slot] = owner_key(p, t, classes, scenes, t.owner[b]);
+      t.grid_key[slot] = t.bkey[b];
       clear_rev(t, slot);
     } else {
       t.bucket_slot[b] = kInvalid;
@@ -836,30 +864,27 @@ __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
 
 // P2G scatter.  One workgroup per particle block: every particle computes
 // its stress and scatters its 27 quadratic B-spline contributions into a 6^3
-// LDS tile.  Contributions are accumulated as fixed-point integers in both
-// precision modes: gfx950's float LDS atomics run at ~1/30 the rate of its
-// integer ones, and integer sums do not depend on atomic order.
+// LDS tile.  Contributions are accumulated as 64-bit fixed-point integers in
+// both precision modes: gfx950's float LDS atomics run at ~1/30 the rate of
+// its integer ones, and integer sums do not depend on atomic order.
 //
-// ACC = 1 (default): each scaled contribution s (|s| < 2^46, checked) is
-// converted to the 64-bit two's-complement integer floor(s) from
-// v = s * 2^-32: hi = floor(v), lo = floor((v - rint(v)) * 2^32) mod 2^32.
-// v - rint(v) is exact (|v| < 2^14), and both halves round the same way, so
-// hi * 2^32 + lo is floor(s) for either sign; one ds_add_u64 adds it.
-// ACC = 0: s is split exactly into hi * 2^20 + lo, two 32-bit integers added
-// with ds_add_u32; chunks of kP2GChunk particles keep both sums inside 32
-// bits.
-// FP32 mode converts each node's 64-bit sum to f32, FIXED64 mode stores the
-// integers and the grid update sums them exactly.  Blocks of more than one
-// chunk add their later chunks to the stored tile in chunk order, so both
-// modes stay deterministic.
-constexpr int kP2GChunk = 1024;
-
-__device__ __forceinline__ void split_fixed(float s, int& hi, int& lo) {
-  const float hf = truncf(s * 0x1p-20f);
-  const float r = fmaf(hf, -0x1p20f, s);  // exact
-  hi = (int)hf;
-  lo = (int)rintf(r);
-}
+// Each scaled contribution s (|s| < 2^45 units, checked per particle) is
+// converted to floor(s) as a two's-complement int64 from v = s * 2^-32:
+// hi = floor(v), lo = floor((v - rint(v)) * 2^32) mod 2^32.  v - rint(v) is
+// exact (|v| < 2^13) and both halves round the same way, so hi * 2^32 + lo is
+// floor(s) for either sign; one ds_add_u64 adds it.  The session scales mass
+// to at most 2^43 units, so the mean -1/2 unit rounding error per
+// contribution is ~2^-44 of the largest particle mass.  A block of at most
+// kP2GMaxBlock particles keeps a tile sum below 2^59, and the FIXED64 grid
+// update's sum of 8 tiles below 2^62; larger blocks raise Capacity.
+//
+// COPIES = 2 gives lanes of even and odd index separate tiles: particles of
+// one cell are often adjacent in storage, and LDS serializes atomics of one
+// instruction that hit the same address.  The copies are summed in a fixed
+// order.  FP32 mode converts each node's sum to f32; FIXED64 mode stores the
+// integers and the grid update sums them exactly.
+constexpr uint32_t kP2GMaxBlock = 1u << 14;
+constexpr float kFixedBound = 0x1p45f;
 
 // v = s * 2^-32 -> floor(s) as a 64-bit integer (see above).
 __device__ __forceinline__ unsigned long long fixed64_of(float v) {
@@ -873,33 +898,22 @@ __device__ __forceinline__ unsigned long long fixed64_of(float v) {
   return ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// Node sum q of the LDS tile in either accumulation layout.
-template <int ACC>
-__device__ __forceinline__ long long tile_sum(const unsigned long long* acc, int q) {
-  if constexpr (ACC) {
-    return (long long)acc[q];
-  } else {
-    const int* hi = reinterpret_cast<const int*>(acc);
-    return (long long)hi[q] * (1ll << 20) + (long long)hi[4 * kTileNodes + q];
-  }
-}
-
-template <int FIXED, int THREADS, int ACC>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g(Parts p, Topo t,
+template <int FIXED, int THREADS, int COPIES>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS == 64 && COPIES == 2 ? 3 : 4))) void k_p2g(Parts p, Topo t,
                                                                                          const DevClass* classes,
                                                                                          const DevScene* scenes) {
-  __shared__ unsigned long long tacc[4 * kTileNodes];
-  int* const thi = reinterpret_cast<int*>(tacc);  // ACC = 0 view: hi halves, then lo halves
-  int* const tlo = thi + 4 * kTileNodes;
+  constexpr int kTile = 4 * kTileNodes;  // m, px, py, pz planes of one tile copy
+  __shared__ unsigned long long tacc[COPIES * kTile];
   __shared__ uint32_t s_slot[8];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   const int tid = threadIdx.x;
+  unsigned long long* const mine = tacc + (COPIES == 2 ? (lane_id() & 1) * kTile : 0);
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
     if (tid < 8) s_slot[tid] = t.pblock_slot8[8 * k + tid];
-    for (int q = tid; q < 4 * kTileNodes; q += THREADS) tacc[q] = 0;
+    for (int q = tid; q < COPIES * kTile; q += THREADS) tacc[q] = 0;
     __syncthreads();
     const uint32_t own = s_slot[0];
-    if (own == kInvalid) {  // grid capacity exhausted (flagged by the scan)
+    if (own == kInvalid) {  // grid capacity exhausted (flagged by binning)
       __syncthreads();
       continue;
     }
@@ -910,141 +924,93 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     const float mscale = exp2f((float)sc.mass_shift), pscale = exp2f((float)sc.mom_shift);
     const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
     const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
-    bool bad = false, range = false;
-    const uint32_t chunk = ACC ? max(len, 1u) : (uint32_t)kP2GChunk;
-    for (uint32_t c0 = 0; c0 < len; c0 += chunk) {
-      const uint32_t c1 = min(len, c0 + chunk);
-      for (uint32_t j = c0 + tid; j < c1; j += THREADS) {
-        const uint32_t i = t.sorted[start + j];
-        const float4 pm = p.pos_mass[i];
-        const float4 vv = p.vel_vol[i];
-        const DevClass& cls = classes[p.cls[i]];
-        M3 F, C;
+    bool bad = false, range = len > kP2GMaxBlock;
+    for (uint32_t j = tid; j < len; j += THREADS) {
+      const uint32_t i = t.sorted[start + j];
+      const float4 pm = p.pos_mass[i];
+      const float4 vv = p.vel_vol[i];
+      const DevClass& cls = classes[p.cls[i]];
+      M3 F, C;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-          F.a[q] = p.F[q * p.stride + i];
-          C.a[q] = p.C[q * p.stride + i];
-        }
-        int3 cell;
-        cell3(pm, inv_h, cell);
-        const int lx = cell.x - ox, ly = cell.y - oy, lz = cell.z - oz;
-        if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
-          bad = true;
-          continue;
-        }
-        const float fx = pm.x * inv_h - (float)cell.x;
-        const float fy = pm.y * inv_h - (float)cell.y;
-        const float fz = pm.z * inv_h - (float)cell.z;
-        float wx[3], wy[3], wz[3];
-        quadratic_weights(fx, wx);
-        quadratic_weights(fy, wy);
-        quadratic_weights(fz, wz);
-        M3 S;
-        kirchhoff(cls, F, C, S);
-        const float m = pm.w;
-        const float sscale = stress_scale * vv.w;
-        // node o receives w_o (m v + A (o - fx) h) with A = m C + stress (-dt V0 inv_d):
-        // pre-scaled as q + o.x a0 + o.y a1 + o.z a2 (a = h A columns, q = m v - h A fx)
-        const float3 a0 = (col(C, 0) * m + col(S, 0) * sscale) * (h * pscale);
-        const float3 a1 = (col(C, 1) * m + col(S, 1) * sscale) * (h * pscale);
-        const float3 a2 = (col(C, 2) * m + col(S, 2) * sscale) * (h * pscale);
-        const float3 q = make_float3(vv.x, vv.y, vv.z) * (m * pscale) - (a0 * fx + a1 * fy + a2 * fz);
-        const float ms = m * mscale;
-        // weights are at most 1, so |contribution| <= |q| + 2 (|a0| + |a1| + |a2|)
-        const float bound = fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fabsf(q.z)) +
-                            2.0f * (fmaxf(fmaxf(fabsf(a0.x), fabsf(a0.y)), fabsf(a0.z)) +
-                                    fmaxf(fmaxf(fabsf(a1.x), fabsf(a1.y)), fabsf(a1.z)) +
-                                    fmaxf(fmaxf(fabsf(a2.x), fabsf(a2.y)), fabsf(a2.z)));
-        constexpr float kFixedBound = ACC ? 0x1p46f : 0x1p40f;  // per-contribution limit (units)
-        range |= !(ms < kFixedBound && bound < kFixedBound);
-        if constexpr (ACC) {
-          const float wz2[3] = {wz[0] * 0x1p-32f, wz[1] * 0x1p-32f, wz[2] * 0x1p-32f};
+      for (int q = 0; q < 9; ++q) {
+        F.a[q] = p.F[q * p.stride + i];
+        C.a[q] = p.C[q * p.stride + i];
+      }
+      int3 cell;
+      cell3(pm, inv_h, cell);
+      const int lx = cell.x - ox, ly = cell.y - oy, lz = cell.z - oz;
+      if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
+        bad = true;
+        continue;
+      }
+      const float fx = pm.x * inv_h - (float)cell.x;
+      const float fy = pm.y * inv_h - (float)cell.y;
+      const float fz = pm.z * inv_h - (float)cell.z;
+      float wx[3], wy[3], wz[3];
+      quadratic_weights(fx, wx);
+      quadratic_weights(fy, wy);
+      quadratic_weights(fz, wz);
+      M3 S;
+      kirchhoff(cls, F, C, S);
+      const float m = pm.w;
+      const float sscale = stress_scale * vv.w;
+      // node o receives w_o (m v + A (o - fx) h) with A = m C + stress (-dt V0 inv_d):
+      // pre-scaled as q + o.x a0 + o.y a1 + o.z a2 (a = h A columns, q = m v - h A fx)
+      const float3 a0 = (col(C, 0) * m + col(S, 0) * sscale) * (h * pscale);
+      const float3 a1 = (col(C, 1) * m + col(S, 1) * sscale) * (h * pscale);
+      const float3 a2 = (col(C, 2) * m + col(S, 2) * sscale) * (h * pscale);
+      const float3 q = make_float3(vv.x, vv.y, vv.z) * (m * pscale) - (a0 * fx + a1 * fy + a2 * fz);
+      const float ms = m * mscale;
+      // weights are at most 1, so |contribution| <= |q| + 2 (|a0| + |a1| + |a2|)
+      const float bound = fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fabsf(q.z)) +
+                          2.0f * (fmaxf(fmaxf(fabsf(a0.x), fabsf(a0.y)), fabsf(a0.z)) +
+                                  fmaxf(fmaxf(fabsf(a1.x), fabsf(a1.y)), fabsf(a1.z)) +
+                                  fmaxf(fmaxf(fabsf(a2.x), fabsf(a2.y)), fabsf(a2.z)));
+      range |= !(ms < kFixedBound && bound < kFixedBound);
+      // the 2^-32 of v = s * 2^-32 rides on the z weights
+      const float wz2[3] = {wz[0] * 0x1p-32f, wz[1] * 0x1p-32f, wz[2] * 0x1p-32f};
 #pragma unroll
-          for (int x = 0; x < 3; ++x) {
-            const float3 qx = q + a0 * (float)x;
+      for (int x = 0; x < 3; ++x) {
+        const float3 qx = q + a0 * (float)x;
 #pragma unroll
-            for (int y = 0; y < 3; ++y) {
-              const float3 qxy = qx + a1 * (float)y;
-              const float wxy = wx[x] * wy[y];
+        for (int y = 0; y < 3; ++y) {
+          const float3 qxy = qx + a1 * (float)y;
+          const float wxy = wx[x] * wy[y];
 #pragma unroll
-              for (int z = 0; z < 3; ++z) {
-                const float w = wxy * wz2[z];
-                const float3 mom = qxy + a2 * (float)z;
-                const int n = tile_index(lx + x, ly + y, lz + z);
-                atomicAdd(&tacc[n], fixed64_of(ms * w));
-                atomicAdd(&tacc[kTileNodes + n], fixed64_of(mom.x * w));
-                atomicAdd(&tacc[2 * kTileNodes + n], fixed64_of(mom.y * w));
-                atomicAdd(&tacc[3 * kTileNodes + n], fixed64_of(mom.z * w));
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int x = 0; x < 3; ++x) {
-            const float3 qx = q + a0 * (float)x;
-#pragma unroll
-            for (int y = 0; y < 3; ++y) {
-              const float3 qxy = qx + a1 * (float)y;
-              const float wxy = wx[x] * wy[y];
-#pragma unroll
-              for (int z = 0; z < 3; ++z) {
-                const float w = wxy * wz[z];
-                const float3 mom = (qxy + a2 * (float)z) * w;
-                const int n = tile_index(lx + x, ly + y, lz + z);
-                int hi, lo;
-                split_fixed(w * ms, hi, lo);
-                atomicAdd(&thi[n], hi);
-                atomicAdd(&tlo[n], lo);
-                split_fixed(mom.x, hi, lo);
-                atomicAdd(&thi[kTileNodes + n], hi);
-                atomicAdd(&tlo[kTileNodes + n], lo);
-                split_fixed(mom.y, hi, lo);
-                atomicAdd(&thi[2 * kTileNodes + n], hi);
-                atomicAdd(&tlo[2 * kTileNodes + n], lo);
-                split_fixed(mom.z, hi, lo);
-                atomicAdd(&thi[3 * kTileNodes + n], hi);
-                atomicAdd(&tlo[3 * kTileNodes + n], lo);
-              }
-            }
+          for (int z = 0; z < 3; ++z) {
+            const float w = wxy * wz2[z];
+            const float3 mom = qxy + a2 * (float)z;
+            const int n = tile_index(lx + x, ly + y, lz + z);
+            atomicAdd(&mine[n], fixed64_of(ms * w));
+            atomicAdd(&mine[kTileNodes + n], fixed64_of(mom.x * w));
+            atomicAdd(&mine[2 * kTileNodes + n], fixed64_of(mom.y * w));
+            atomicAdd(&mine[3 * kTileNodes + n], fixed64_of(mom.z * w));
           }
         }
       }
-      __syncthreads();
-      // this chunk's exact 64-bit node sums go to the tile (added after the first chunk)
-      const bool first = c0 == 0, last = c1 == len;
-      if (FIXED) {
-        longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + (size_t)k * kTileNodes * 2;
-        for (int n = tid; n < kTileNodes; n += THREADS) {
-          longlong2 a = make_longlong2(tile_sum<ACC>(tacc, n), tile_sum<ACC>(tacc, kTileNodes + n));
-          longlong2 b = make_longlong2(tile_sum<ACC>(tacc, 2 * kTileNodes + n), tile_sum<ACC>(tacc, 3 * kTileNodes + n));
-          if (!first) {
-            const longlong2 a0 = out[2 * n], b0 = out[2 * n + 1];
-            a.x += a0.x; a.y += a0.y; b.x += b0.x; b.y += b0.y;
-          }
-          out[2 * n] = a;
-          out[2 * n + 1] = b;
-        }
-      } else {
-        const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
-        float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes;
-        for (int n = tid; n < kTileNodes; n += THREADS) {
-          float4 v = make_float4(__ll2float_rn(tile_sum<ACC>(tacc, n)) * im,
-                                 __ll2float_rn(tile_sum<ACC>(tacc, kTileNodes + n)) * ip,
-                                 __ll2float_rn(tile_sum<ACC>(tacc, 2 * kTileNodes + n)) * ip,
-                                 __ll2float_rn(tile_sum<ACC>(tacc, 3 * kTileNodes + n)) * ip);
-          if (!first) {
-            const float4 o = out[n];
-            v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
-          }
-          out[n] = v;
-        }
-      }
-      if (!last)
-        for (int q = tid; q < 4 * kTileNodes; q += THREADS) tacc[q] = 0;
-      __syncthreads();
     }
+    __syncthreads();
     flag_error(t.counters, bad, ERR_INTERNAL);
-    flag_error(t.counters, range, ERR_FIXED_RANGE);
+    flag_scene_error(t, (uint32_t)key.w >> 16, range, ERR_FIXED_RANGE);
+    auto node_sum = [&](int q) -> long long {
+      unsigned long long v = tacc[q];
+      if (COPIES == 2) v += tacc[kTile + q];
+      return (long long)v;
+    };
+    if (FIXED) {
+      longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + (size_t)k * kTileNodes * 2;
+      for (int n = tid; n < kTileNodes; n += THREADS) {
+        out[2 * n] = make_longlong2(node_sum(n), node_sum(kTileNodes + n));
+        out[2 * n + 1] = make_longlong2(node_sum(2 * kTileNodes + n), node_sum(3 * kTileNodes + n));
+      }
+    } else {
+      const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
+      float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes;
+      for (int n = tid; n < kTileNodes; n += THREADS)
+        out[n] = make_float4(__ll2float_rn(node_sum(n)) * im, __ll2float_rn(node_sum(kTileNodes + n)) * ip,
+                             __ll2float_rn(node_sum(2 * kTileNodes + n)) * ip,
+                             __ll2float_rn(node_sum(3 * kTileNodes + n)) * ip);
+    }
     __syncthreads();
   }
 }
@@ -1134,6 +1100,7 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
       t.owner[b] = 0;
       t.bucket_count[b] = 0;
+      t.bkey[b].x = kNoKey;
     }
   }
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
@@ -1287,7 +1254,7 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
 #pragma unroll
         for (int q = 0; q < 9; ++q) p.F[q * p.stride + i] = Fn.a[q];
       }
-      flag_error(t.counters, failed, ERR_RESIDENT);
+      flag_scene_error(t, (uint32_t)key.w >> 16, failed, ERR_RESIDENT);
       flag_error(t.counters, bad, ERR_INTERNAL);
     }
     __syncthreads();
@@ -1381,19 +1348,18 @@ void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint
   hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
 }
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
-                uint32_t threads, int acc, hipStream_t s) {
-  // threads per workgroup: 64, 128 or 256 (anything else runs 128); acc: 64-bit (1) or split 32-bit (0) LDS sums
-#define TMPM_P2G(F, T, A) hipLaunchKernelGGL((k_p2g<F, T, A>), dim3(blocks), dim3(T), 0, s, p, t, c, sc)
-#define TMPM_P2G_T(F, A)               \
-  do {                                 \
-    if (threads == 64) TMPM_P2G(F, 64, A);        \
-    else if (threads == 256) TMPM_P2G(F, 256, A); \
-    else TMPM_P2G(F, 128, A);          \
+                uint32_t threads, int copies, hipStream_t s) {
+  // threads per workgroup: 64 or 128 (anything else runs 128); LDS tile copies: 1 or 2
+#define TMPM_P2G(F, T, K) hipLaunchKernelGGL((k_p2g<F, T, K>), dim3(blocks), dim3(T), 0, s, p, t, c, sc)
+#define TMPM_P2G_T(F, K)                  \
+  do {                                    \
+    if (threads == 64) TMPM_P2G(F, 64, K); \
+    else TMPM_P2G(F, 128, K);              \
   } while (0)
   if (fixed64) {
-    if (acc) TMPM_P2G_T(1, 1); else TMPM_P2G_T(1, 0);
+    if (copies == 2) TMPM_P2G_T(1, 2); else TMPM_P2G_T(1, 1);
   } else {
-    if (acc) TMPM_P2G_T(0, 1); else TMPM_P2G_T(0, 0);
+    if (copies == 2) TMPM_P2G_T(0, 2); else TMPM_P2G_T(0, 1);
   }
 #undef TMPM_P2G_T
 #undef TMPM_P2G

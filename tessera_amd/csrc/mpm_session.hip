@@ -366,15 +366,44 @@ struct tmpm_context {
   hipDeviceProp_t props{};
 };
 
+// One independent world of a session.  A plain session has one scene; a
+// batch steps several together, one launch per stage per substep, with the
+// scene index in every block key's region so blocks never mix scenes.
+struct SceneState {
+  HostWorld world;           // last synchronized state, insertion order
+  double substep_dt = 0;
+  double max_mass = 0;       // largest enabled particle mass (fixed-point scales)
+  bool force_pending = false;
+  bool active = true;        // false: removed from the batch (slot free)
+  tmpm_status status = TMPM_OK;  // error that stopped this scene in a batch
+  DevScene dev{};
+  uint32_t first = 0;        // insertion index of its particle 0 on the device
+  uint32_t count = 0;        // particles on the device
+  double stable = -1.0;      // world.stable_timestep(), cached; < 0 when stale
+
+  bool stepping() const { return active && status == TMPM_OK; }
+  double stable_timestep() {
+    if (stable < 0.0) stable = world.stable_timestep();
+    return stable;
+  }
+  void set_world(const HostWorld& w) {
+    world = w;
+    stable = -1.0;
+  }
+};
+
 struct tmpm_session {
   tmpm_context* ctx = nullptr;
   hipStream_t stream = nullptr;
-  HostWorld world;
-  double substep_dt = 0;
+  std::vector<SceneState> scenes;
+  bool batch = false;
   int fixed64 = 0;
   bool valid = true;
   uint64_t pending = 0;
   bool prepared = false;
+
+  HostWorld& world() { return scenes[0].world; }
+  const HostWorld& world() const { return scenes[0].world; }
 
   // device state
   uint32_t n = 0;          // particles in device storage
@@ -382,22 +411,19 @@ struct tmpm_session {
   ParticleBuffers buf[2];
   int cur = 0;
   DevBuf force;            // float4 per particle, insertion order
-  DevBuf classes, scene_plain, scene_force, obstacles;
-  uint32_t obstacle_cap = 0;
+  DevBuf classes, scene_plain, scene_force, obstacles, scene_err;
+  uint32_t obstacle_cap = 0, scene_cap = 0;
   std::vector<DevClass> host_classes;
-  DevScene host_scene{};
-  bool force_pending = false;
-  double max_mass = 0.0;
   uint32_t substeps_since_reorder = 1u << 30;
   uint32_t reorder_interval = 8;
 
   // topology
-  DevBuf owner, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, grid_key, pblock_bucket,
+  DevBuf owner, bkey, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, grid_key, pblock_bucket,
       pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask,
     bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum;
   Topo topo{};
   uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 128;
-  int p2g_acc = 1;
+  int p2g_copies = 2;
   bool topo_valid = false;  // topology arrays describe the current device state
   bool table_dirty = true;  // block hash needs clearing before the next binning
 
@@ -470,9 +496,11 @@ uint32_t next_pow2(uint64_t v) {
 }
 
 // gpu.rs:664-677
-tmpm_status validate_resident_world(const HostWorld& w, double substep_dt) {
+tmpm_status validate_resident_world(SceneState& scene) {
+  const HostWorld& w = scene.world;
   if (!params_valid(w.params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
-  if (!std::isfinite(substep_dt) || substep_dt <= 0.0 || substep_dt > w.stable_timestep())
+  const double dt = scene.substep_dt;
+  if (!std::isfinite(dt) || dt <= 0.0 || dt > scene.stable_timestep())
     return fail(TMPM_INVALID_INPUT, "substep dt is not positive or exceeds the stable timestep");
   for (auto& o : w.obstacles)
     if (!obstacle_valid(o.o)) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
@@ -496,19 +524,27 @@ bool pack_obstacle(const tmpm_obstacle& o, DevObstacle& d) {
   return ok;
 }
 
+// Obstacles of all active scenes, concatenated; each scene keeps its range.
 tmpm_status upload_obstacles(tmpm_session* s) {
   std::vector<DevObstacle> packed;
-  for (auto& ob : s->world.obstacles) {
-    const tmpm_obstacle& o = ob.o;
-    if (o.shape == TMPM_SHAPE_TRIANGLE_PRISM || o.shape == TMPM_SHAPE_CONVEX)
-      return fail(TMPM_RESIDENT_UNAVAILABLE, "triangle-prism and convex obstacles are not supported by the HIP backend");
-    if (o.cpic_group >= 0 && s->world.params.use_cpic)
-      return fail(TMPM_RESIDENT_UNAVAILABLE, "CPIC obstacle groups are not supported by the HIP backend");
-    DevObstacle d;
-    if (!pack_obstacle(o, d)) return fail(TMPM_INVALID_INPUT, "obstacle is not representable in f32");
-    packed.push_back(d);
+  for (auto& scene : s->scenes) {
+    scene.dev.obstacle_begin = (int32_t)packed.size();
+    scene.dev.obstacle_count = 0;
+    if (!scene.stepping()) continue;
+    for (auto& ob : scene.world.obstacles) {
+      const tmpm_obstacle& o = ob.o;
+      if (o.shape == TMPM_SHAPE_TRIANGLE_PRISM || o.shape == TMPM_SHAPE_CONVEX)
+        return fail(TMPM_RESIDENT_UNAVAILABLE,
+                    "triangle-prism and convex obstacles are not supported by the HIP backend");
+      if (o.cpic_group >= 0 && scene.world.params.use_cpic)
+        return fail(TMPM_RESIDENT_UNAVAILABLE, "CPIC obstacle groups are not supported by the HIP backend");
+      DevObstacle d;
+      if (!pack_obstacle(o, d)) return fail(TMPM_INVALID_INPUT, "obstacle is not representable in f32");
+      packed.push_back(d);
+    }
+    scene.dev.obstacle_count = (int32_t)(packed.size() - scene.dev.obstacle_begin);
+    if (scene.dev.obstacle_count > kMaxObstacles) return fail(TMPM_CAPACITY, "too many obstacles");
   }
-  if (packed.size() > (size_t)kMaxObstacles) return fail(TMPM_CAPACITY, "too many obstacles");
   if (packed.size() > s->obstacle_cap || !s->obstacles.ptr) {
     s->obstacle_cap = std::max<uint32_t>(16, (uint32_t)packed.size());
     HIP_TRY(s->obstacles.alloc(sizeof(DevObstacle) * s->obstacle_cap));
@@ -516,26 +552,40 @@ tmpm_status upload_obstacles(tmpm_session* s) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (!packed.empty())
     HIP_TRY(hipMemcpy(s->obstacles.ptr, packed.data(), packed.size() * sizeof(DevObstacle), hipMemcpyHostToDevice));
-  s->host_scene.obstacle_begin = 0;
-  s->host_scene.obstacle_count = (int32_t)packed.size();
   return TMPM_OK;
 }
 
+// Scene tables: `plain` for ordinary substeps, `force` (apply_force set for
+// scenes with a one-shot force edit) for the first substep after an edit.
 tmpm_status upload_scene(tmpm_session* s) {
-  DevScene plain = s->host_scene, forced = s->host_scene;
-  plain.apply_force = 0;
-  forced.apply_force = 1;
-  HIP_TRY(hipMemcpy(s->scene_plain.ptr, &plain, sizeof(DevScene), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(s->scene_force.ptr, &forced, sizeof(DevScene), hipMemcpyHostToDevice));
+  const size_t n = s->scenes.size();
+  if (n > s->scene_cap || !s->scene_plain.ptr) {
+    s->scene_cap = (uint32_t)std::max<size_t>(n, 1);
+    HIP_TRY(s->scene_plain.alloc(sizeof(DevScene) * s->scene_cap));
+    HIP_TRY(s->scene_force.alloc(sizeof(DevScene) * s->scene_cap));
+    HIP_TRY(s->scene_err.alloc(4ull * s->scene_cap));
+    HIP_TRY(hipMemset(s->scene_err.ptr, 0, 4ull * s->scene_cap));
+  }
+  std::vector<DevScene> plain(n), forced(n);
+  for (size_t k = 0; k < n; ++k) {
+    plain[k] = forced[k] = s->scenes[k].dev;
+    plain[k].apply_force = 0;
+    forced[k].apply_force = s->scenes[k].force_pending ? 1 : 0;
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(s->scene_plain.ptr, plain.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->scene_force.ptr, forced.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
+  s->topo.scene_err = s->scene_err.as<uint32_t>();
   return TMPM_OK;
 }
 
 // Scene constants; f32 packing as in gpu.rs:1118-1127, 1407-1432.
-tmpm_status build_scene(tmpm_session* s) {
-  const tmpm_params& p = s->world.params;
-  DevScene& sc = s->host_scene;
+// Packs scene k's parameters (keeps its obstacle range).
+tmpm_status build_scene(SceneState& scene) {
+  const tmpm_params& p = scene.world.params;
+  DevScene& sc = scene.dev;
   float h, step, inv_d;
-  if (!finite_f32(p.cell_width, h) || !finite_f32(s->substep_dt, step) ||
+  if (!finite_f32(p.cell_width, h) || !finite_f32(scene.substep_dt, step) ||
       !finite_f32(4.0 / (p.cell_width * p.cell_width), inv_d) || h <= 0.0f || step <= 0.0f)
     return fail(TMPM_INVALID_INPUT, "cell width or dt not representable in f32");
   float inv_h = 1.0f / h;
@@ -547,18 +597,16 @@ tmpm_status build_scene(tmpm_session* s) {
   for (int k = 0; k < 3; ++k)
     if (!finite_f32(p.gravity[k], sc.gravity[k])) return fail(TMPM_INVALID_INPUT, "gravity not representable");
   sc.has_bounds = p.has_bounds ? 1 : 0;
-  // Fixed-point exponents: a particle's mass maps to at most 2^44 units
-  // (2^38 with the split 32-bit accumulation) and every contribution must
-  // stay below 2^46 units (2^40; checked per particle); momentum is scaled
-  // for mass x 64 cells per substep.  An int64 node sum then holds 2^17
-  // contributions (2^23) before it could wrap.  The 64-bit conversion rounds
-  // down, so its unit must be fine enough that the -1/2 unit mean error per
-  // contribution stays far below f32 resolution.
-  if (s->max_mass > 0.0) {
-    double vbound = 64.0 * p.cell_width / s->substep_dt;
-    const int top = s->p2g_acc ? 44 : 38;
-    int ms = (int)std::floor(std::log2(std::ldexp(1.0, top) / s->max_mass));
-    int ps = (int)std::floor(std::log2(std::ldexp(1.0, top) / (s->max_mass * vbound)));
+  // Fixed-point exponents: a particle's mass maps to at most 2^43 units and
+  // every contribution must stay below 2^45 units (checked per particle);
+  // momentum is scaled for mass x 64 cells per substep.  The unit is fine
+  // enough that P2G's round-down conversion (-1/2 unit mean error per
+  // contribution) stays far below f32 resolution.
+  if (scene.max_mass > 0.0) {
+    double vbound = 64.0 * p.cell_width / scene.substep_dt;
+    const int top = 43;
+    int ms = (int)std::floor(std::log2(std::ldexp(1.0, top) / scene.max_mass));
+    int ps = (int)std::floor(std::log2(std::ldexp(1.0, top) / (scene.max_mass * vbound)));
     sc.mass_shift = std::max(-120, std::min(120, ms));
     sc.mom_shift = std::max(-120, std::min(120, ps));
   }
@@ -589,9 +637,9 @@ void read_tuning_env(tmpm_session* s) {
   }
   if (const char* v = std::getenv("TMPM_P2G_THREADS")) {
     int th = std::atoi(v);
-    if (th == 64 || th == 128 || th == 256) s->p2g_threads = (uint32_t)th;
+    if (th == 64 || th == 128) s->p2g_threads = (uint32_t)th;
   }
-  if (const char* v = std::getenv("TMPM_P2G_ACC")) s->p2g_acc = std::atoi(v) != 0;
+  if (const char* v = std::getenv("TMPM_P2G_COPIES")) s->p2g_copies = std::atoi(v) == 1 ? 1 : 2;
   if (const char* v = std::getenv("TMPM_REORDER_INTERVAL")) {
     int r = std::atoi(v);
     if (r >= 1 && r <= 1000000) s->reorder_interval = (uint32_t)r;
@@ -606,6 +654,7 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   if ((uint64_t)s->n + 8ull * buckets + 1 >= 0xffffffffull) return fail(TMPM_CAPACITY, "topology id space exhausted");
   Topo& t = s->topo;
   HIP_TRY(s->owner.alloc(4ull * buckets));
+  HIP_TRY(s->bkey.alloc(16ull * buckets));
   HIP_TRY(s->bucket_count.alloc(4ull * buckets));
   HIP_TRY(s->nbr_bucket.alloc(32ull * buckets));
   HIP_TRY(s->bucket_slot.alloc(4ull * buckets));
@@ -620,10 +669,10 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->n, 1)));
   HIP_TRY(s->counters.alloc(4ull * CNT_COUNT));
   HIP_TRY(hipMemset(s->counters.ptr, 0, 4ull * CNT_COUNT));
-  // Binning workgroups: at most ~1024 (each one's id prefix sums over all of
+  // Binning workgroups: at most ~2048 (each one's id prefix sums over all of
   // them), each over a whole number of 256-particle rounds.
   const uint64_t nparts = std::max<uint32_t>(s->n, 1);
-  const uint32_t rounds = (uint32_t)((nparts + 256ull * 1024 - 1) / (256ull * 1024));
+  const uint32_t rounds = (uint32_t)((nparts + 256ull * 2048 - 1) / (256ull * 2048));
   t.bin_span = 256u * rounds;
   t.bin_groups = (uint32_t)((nparts + t.bin_span - 1) / t.bin_span);
   const uint64_t staged = (uint64_t)t.bin_groups * t.bin_span;
@@ -642,6 +691,7 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->grid_vel.alloc((size_t)grid_cap * kBlockNodes * 16));
   HIP_TRY(s->node_mask.alloc(8ull * grid_cap));
   t.owner = s->owner.as<uint32_t>();
+  t.bkey = s->bkey.as<int4>();
   t.bucket_count = s->bucket_count.as<uint32_t>();
   t.nbr_bucket = s->nbr_bucket.as<uint32_t>();
   t.bucket_slot = s->bucket_slot.as<uint32_t>();
@@ -670,8 +720,8 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
 
 // Counts particle blocks and grid blocks (block + its 7 upper neighbours) of
 // the enabled particles on the host, f32 cell rule.
-void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& region, float inv_h, uint64_t& pblocks,
-                  uint64_t& gblocks) {
+void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& region,
+                  const std::vector<float>& inv_h, uint64_t& pblocks, uint64_t& gblocks) {
   struct K {
     int32_t x, y, z;
     uint32_t r;
@@ -688,7 +738,7 @@ void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& 
   for (size_t i = 0; i < n; ++i) {
     int32_t c[3];
     bool ok = true;
-    for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h, c[a]) && ok;
+    for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h[i], c[a]) && ok;
     if (!ok) continue;
     K k{c[0] >> 2, c[1] >> 2, c[2] >> 2, region[i]};
     if (pb.emplace(k, 1).second)
@@ -698,44 +748,92 @@ void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& 
   gblocks = gb.size();
 }
 
-// Upload the world to the device (gpu.rs:1091-1600 `prepare`, resident).
+// Packs a particle's class (material constants, radius, damping, fixed flag,
+// region).  `scene` goes into the region's high 16 bits.
+tmpm_status pack_class(const HostWorld& w, size_t i, uint32_t scene, float rad, float damp, DevClass& c) {
+  const tmpm_material& m = w.material[i];
+  c = DevClass{};
+  c.region = (scene << 16) | w.color[i];
+  c.fixed = w.fixed[i] ? 1u : 0u;
+  c.radius = rad;
+  c.damping = damp;
+  if (!w.enabled[i]) return TMPM_OK;
+  switch (m.kind) {
+    case TMPM_MATERIAL_LINEAR_ELASTIC:
+    case TMPM_MATERIAL_NEO_HOOKEAN: {
+      double lambda, mu;
+      lame(m.p[0], m.p[1], lambda, mu);
+      if (!finite_f32(lambda, c.lambda) || !finite_f32(mu, c.mu))
+        return fail(TMPM_INVALID_INPUT, "Lame parameters not representable");
+      c.kind = m.kind == TMPM_MATERIAL_LINEAR_ELASTIC ? DM_COROTATED : DM_NEO_HOOKEAN;
+      c.wave_numerator = c.lambda + 2.0f * c.mu;
+      return TMPM_OK;
+    }
+    case TMPM_MATERIAL_FLUID:
+      c.kind = DM_FLUID;
+      if (!finite_f32(m.p[0], c.bulk) || !finite_f32(m.p[1], c.gamma) || !finite_f32(m.p[2], c.viscosity) ||
+          !finite_f32(m.p[3], c.tensile))
+        return fail(TMPM_INVALID_INPUT, "fluid parameters not representable");
+      c.wave_numerator = c.bulk * c.gamma;
+      return TMPM_OK;
+    default:
+      return fail(TMPM_RESIDENT_UNAVAILABLE, "plastic materials (sand, snow) are not supported by the HIP backend");
+  }
+}
+
+// Upload all active scenes to the device (gpu.rs:1091-1600 `prepare`,
+// resident).  Device insertion order is scene by scene; storage order puts
+// the enabled particles of all scenes first.
 tmpm_status prepare(tmpm_session* s) {
-  HostWorld& w = s->world;
   s->prepared = false;
   s->topo_valid = false;
-  tmpm_status st = build_scene(s);
-  if (st != TMPM_OK) return st;
-  const float inv_h = s->host_scene.inv_h;
-  const float inv_d = s->host_scene.inv_d;
-  (void)inv_d;
-  size_t total = w.size();
+  if (s->scenes.size() > 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
+  size_t total = 0;
+  for (auto& scene : s->scenes) {
+    scene.first = (uint32_t)std::min<size_t>(total, UINT32_MAX);
+    scene.count = scene.stepping() ? (uint32_t)std::min<size_t>(scene.world.size(), UINT32_MAX) : 0;
+    total += scene.count;
+    scene.max_mass = 0.0;
+    for (size_t i = 0; i < scene.count; ++i)
+      if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
+    tmpm_status st = build_scene(scene);
+    if (st != TMPM_OK) return st;
+  }
   if (total >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
+  // global insertion index -> (scene, index)
+  std::vector<uint32_t> scene_of(total);
+  for (uint32_t k = 0; k < s->scenes.size(); ++k)
+    for (uint32_t i = 0; i < s->scenes[k].count; ++i) scene_of[s->scenes[k].first + i] = k;
+  auto local = [&](size_t g) { return g - s->scenes[scene_of[g]].first; };
   // storage order: enabled particles first (insertion order), then disabled
   std::vector<uint32_t> order;
   order.reserve(total);
-  for (size_t i = 0; i < total; ++i)
-    if (w.enabled[i]) order.push_back((uint32_t)i);
+  for (size_t g = 0; g < total; ++g)
+    if (s->scenes[scene_of[g]].world.enabled[local(g)]) order.push_back((uint32_t)g);
   uint32_t n_enabled = (uint32_t)order.size();
-  for (size_t i = 0; i < total; ++i)
-    if (!w.enabled[i]) order.push_back((uint32_t)i);
+  for (size_t g = 0; g < total; ++g)
+    if (!s->scenes[scene_of[g]].world.enabled[local(g)]) order.push_back((uint32_t)g);
 
   std::vector<float4> pm(total), vv(total);
   std::vector<float> Cf(9 * total), Ff(9 * total);
   std::vector<uint32_t> cls(total), ins(total);
   std::vector<float> pos32(3 * (size_t)n_enabled);
   std::vector<uint32_t> region(n_enabled);
+  std::vector<float> inv_h_of(n_enabled);
   std::map<std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t>, uint32_t>
       class_ids;
   s->host_classes.clear();
-  double max_mass = 0;
   for (size_t slot = 0; slot < total; ++slot) {
-    size_t i = order[slot];
+    const size_t g = order[slot];
+    const uint32_t k = scene_of[g];
+    const HostWorld& w = s->scenes[k].world;
+    const size_t i = local(g);
     bool en = w.enabled[i] != 0;
     float f[16];
     bool ok = true;
-    for (int k = 0; k < 3; ++k) ok = finite_f32(w.pos[3 * i + k], f[k]) && ok;
+    for (int a = 0; a < 3; ++a) ok = finite_f32(w.pos[3 * i + a], f[a]) && ok;
     ok = finite_f32(en ? w.mass[i] : 0.0, f[3]) && ok;
-    for (int k = 0; k < 3; ++k) ok = finite_f32(w.vel[3 * i + k], f[4 + k]) && ok;
+    for (int a = 0; a < 3; ++a) ok = finite_f32(w.vel[3 * i + a], f[4 + a]) && ok;
     ok = finite_f32(w.rest_volume[i], f[7]) && ok;
     float rad, damp;
     ok = finite_f32(w.radius[i], rad) && finite_f32(w.damping[i], damp) && ok;
@@ -744,48 +842,24 @@ tmpm_status prepare(tmpm_session* s) {
     if (rad <= 0.0f || w.damping[i] < 0.0) return fail(TMPM_INVALID_INPUT, "radius or damping invalid");
     pm[slot] = make_float4(f[0], f[1], f[2], f[3]);
     vv[slot] = make_float4(f[4], f[5], f[6], f[7]);
-    for (int k = 0; k < 9; ++k) {
-      if (!finite_f32(w.C[9 * i + k], Cf[9 * slot + k]) || !finite_f32(w.F[9 * i + k], Ff[9 * slot + k]))
+    for (int q = 0; q < 9; ++q) {
+      if (!finite_f32(w.C[9 * i + q], Cf[9 * slot + q]) || !finite_f32(w.F[9 * i + q], Ff[9 * slot + q]))
         return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
     }
-    ins[slot] = (uint32_t)i;
-    // class
-    const tmpm_material& m = w.material[i];
-    DevClass c{};
-    c.region = w.color[i];
-    c.fixed = w.fixed[i] ? 1u : 0u;
-    c.radius = rad;
-    c.damping = damp;
+    ins[slot] = (uint32_t)g;
+    DevClass c;
+    tmpm_status st = pack_class(w, i, k, rad, damp, c);
+    if (st != TMPM_OK) return st;
     if (en) {
-      switch (m.kind) {
-        case TMPM_MATERIAL_LINEAR_ELASTIC:
-        case TMPM_MATERIAL_NEO_HOOKEAN: {
-          double lambda, mu;
-          lame(m.p[0], m.p[1], lambda, mu);
-          if (!finite_f32(lambda, c.lambda) || !finite_f32(mu, c.mu))
-            return fail(TMPM_INVALID_INPUT, "Lame parameters not representable");
-          c.kind = m.kind == TMPM_MATERIAL_LINEAR_ELASTIC ? DM_COROTATED : DM_NEO_HOOKEAN;
-          c.wave_numerator = c.lambda + 2.0f * c.mu;
-          break;
-        }
-        case TMPM_MATERIAL_FLUID:
-          c.kind = DM_FLUID;
-          if (!finite_f32(m.p[0], c.bulk) || !finite_f32(m.p[1], c.gamma) || !finite_f32(m.p[2], c.viscosity) ||
-              !finite_f32(m.p[3], c.tensile))
-            return fail(TMPM_INVALID_INPUT, "fluid parameters not representable");
-          c.wave_numerator = c.bulk * c.gamma;
-          break;
-        default:
-          return fail(TMPM_RESIDENT_UNAVAILABLE, "plastic materials (sand, snow) are not supported by the HIP backend");
-      }
-      max_mass = std::max(max_mass, w.mass[i]);
       // base cell must be representable (gpu.rs:1143-1153)
+      const float inv_h = s->scenes[k].dev.inv_h;
       for (int a = 0; a < 3; ++a) {
         int32_t cc;
         if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
         pos32[3 * slot + a] = f[a];
       }
       region[slot] = c.region;
+      inv_h_of[slot] = inv_h;
     }
     auto key = std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
                                c.region, c.fixed);
@@ -806,9 +880,6 @@ tmpm_status prepare(tmpm_session* s) {
     // gpu.rs:1358-1360: no live node, nothing to prepare; steps only count.
     return TMPM_OK;
   }
-  s->max_mass = max_mass;
-  st = build_scene(s);
-  if (st != TMPM_OK) return st;
 
   // device particle buffers
   for (int b = 0; b < 2; ++b) {
@@ -825,41 +896,40 @@ tmpm_status prepare(tmpm_session* s) {
   ParticleBuffers& pb = s->buf[0];
   HIP_TRY(hipMemcpy(pb.pos_mass.ptr, pm.data(), 16ull * total, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.vel_vol.ptr, vv.data(), 16ull * total, hipMemcpyHostToDevice));
-  // SoA matrices: component k of particle slot at [k * total + slot]
+  // SoA matrices: component q of particle slot at [q * total + slot]
   std::vector<float> soa(9 * total);
   for (size_t slot = 0; slot < total; ++slot)
-    for (int k = 0; k < 9; ++k) soa[k * total + slot] = Cf[9 * slot + k];
+    for (int q = 0; q < 9; ++q) soa[q * total + slot] = Cf[9 * slot + q];
   HIP_TRY(hipMemcpy(pb.C.ptr, soa.data(), 36ull * total, hipMemcpyHostToDevice));
   for (size_t slot = 0; slot < total; ++slot)
-    for (int k = 0; k < 9; ++k) soa[k * total + slot] = Ff[9 * slot + k];
+    for (int q = 0; q < 9; ++q) soa[q * total + slot] = Ff[9 * slot + q];
   HIP_TRY(hipMemcpy(pb.F.ptr, soa.data(), 36ull * total, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.cls.ptr, cls.data(), 4ull * total, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.ins_id.ptr, ins.data(), 4ull * total, hipMemcpyHostToDevice));
   // forces (insertion order)
   std::vector<float4> fz(total);
-  bool any_force = false;
-  for (size_t i = 0; i < total; ++i) {
+  for (auto& scene : s->scenes) scene.force_pending = false;
+  for (size_t g = 0; g < total; ++g) {
+    SceneState& scene = s->scenes[scene_of[g]];
+    const double* fd = &scene.world.force[3 * local(g)];
     float fx, fy, fzv;
-    if (!finite_f32(w.force[3 * i], fx) || !finite_f32(w.force[3 * i + 1], fy) || !finite_f32(w.force[3 * i + 2], fzv))
+    if (!finite_f32(fd[0], fx) || !finite_f32(fd[1], fy) || !finite_f32(fd[2], fzv))
       return fail(TMPM_INVALID_INPUT, "force not representable in f32");
-    fz[i] = make_float4(fx, fy, fzv, 0.f);
-    any_force = any_force || fx != 0.f || fy != 0.f || fzv != 0.f;
+    fz[g] = make_float4(fx, fy, fzv, 0.f);
+    if (fx != 0.f || fy != 0.f || fzv != 0.f) scene.force_pending = true;
   }
   HIP_TRY(hipMemcpy(s->force.ptr, fz.data(), 16ull * total, hipMemcpyHostToDevice));
-  s->force_pending = any_force;
   HIP_TRY(s->classes.alloc(sizeof(DevClass) * std::max<size_t>(1, s->host_classes.size())));
   HIP_TRY(hipMemcpy(s->classes.ptr, s->host_classes.data(), sizeof(DevClass) * s->host_classes.size(),
                     hipMemcpyHostToDevice));
-  HIP_TRY(s->scene_plain.alloc(sizeof(DevScene)));
-  HIP_TRY(s->scene_force.alloc(sizeof(DevScene)));
-  st = upload_obstacles(s);
+  tmpm_status st = upload_obstacles(s);
   if (st != TMPM_OK) return st;
-  HIP_TRY(hipStreamSynchronize(s->stream));
   st = upload_scene(s);
   if (st != TMPM_OK) return st;
+  HIP_TRY(hipMemset(s->scene_err.ptr, 0, 4ull * s->scene_cap));
   // topology capacity from the initial block counts, with headroom
   uint64_t pblocks = 0, gblocks = 0;
-  count_blocks(pos32, region, inv_h, pblocks, gblocks);
+  count_blocks(pos32, region, inv_h_of, pblocks, gblocks);
   uint64_t pcap = std::max<uint64_t>(1024, 4 * pblocks);
   uint64_t gcap = std::max<uint64_t>(4096, 4 * gblocks);
   pcap = std::min<uint64_t>(pcap, n_enabled);
@@ -910,7 +980,7 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   }
   s->substeps_since_reorder++;
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
-                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_acc, s->stream); })) != hipSuccess)
+                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_copies, s->stream); })) != hipSuccess)
     return e;
   if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, s->stream); })) != hipSuccess)
     return e;
@@ -931,19 +1001,27 @@ tmpm_status status_from_error_bits(uint32_t bits) {
   return TMPM_OK;
 }
 
-// Copies device state into a new world (gpu.rs:2295-2329 apply_gpu_transfers).
-tmpm_status read_back(tmpm_session* s, HostWorld& out) {
-  out = s->world;
-  if (!s->prepared) {
-    for (size_t i = 0; i < out.size(); ++i) {
-      out.force[3 * i] = out.force[3 * i + 1] = out.force[3 * i + 2] = 0.0;
-      if (out.fixed[i] && out.enabled[i]) {
-        for (int k = 0; k < 3; ++k) out.vel[3 * i + k] = 0.0;
-        for (int k = 0; k < 9; ++k) out.C[9 * i + k] = 0.0;
+// Copies device state into new worlds (gpu.rs:2295-2329 apply_gpu_transfers):
+// out[k] is scene k's updated world and st[k] TMPM_NON_FINITE_STATE if one of
+// its particles is not finite.  Scenes that are not stepping are left as is.
+tmpm_status read_back(tmpm_session* s, std::vector<HostWorld>& out, std::vector<tmpm_status>& st) {
+  const size_t ns = s->scenes.size();
+  out.assign(ns, HostWorld());
+  st.assign(ns, TMPM_OK);
+  for (size_t k = 0; k < ns; ++k) {
+    if (!s->scenes[k].stepping()) continue;
+    HostWorld& w = out[k];
+    w = s->scenes[k].world;
+    // forces are one-shot; fixed particles end with zero v and C (world.rs:419-430)
+    for (size_t i = 0; i < w.size(); ++i) {
+      w.force[3 * i] = w.force[3 * i + 1] = w.force[3 * i + 2] = 0.0;
+      if (w.fixed[i] && w.enabled[i]) {
+        for (int q = 0; q < 3; ++q) w.vel[3 * i + q] = 0.0;
+        for (int q = 0; q < 9; ++q) w.C[9 * i + q] = 0.0;
       }
     }
-    return TMPM_OK;
   }
+  if (!s->prepared) return TMPM_OK;
   size_t n = s->n;
   const ParticleBuffers& b = s->buf[s->cur];
   std::vector<float4> pm(n), vv(n);
@@ -954,30 +1032,33 @@ tmpm_status read_back(tmpm_session* s, HostWorld& out) {
   HIP_TRY(hipMemcpy(C.data(), b.C.ptr, 36 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(F.data(), b.F.ptr, 36 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(ins.data(), b.ins_id.ptr, 4 * n, hipMemcpyDeviceToHost));
+  // insertion index -> scene: scenes occupy consecutive ranges
+  std::vector<std::pair<uint32_t, uint32_t>> ranges;  // (first, scene)
+  for (uint32_t k = 0; k < ns; ++k)
+    if (s->scenes[k].count) ranges.emplace_back(s->scenes[k].first, k);
   for (size_t slot = 0; slot < n; ++slot) {
-    size_t i = ins[slot];
-    if (i >= out.size()) return fail(TMPM_READBACK, "corrupt particle order");
-    out.force[3 * i] = out.force[3 * i + 1] = out.force[3 * i + 2] = 0.0;
-    if (!out.enabled[i]) continue;
-    if (out.fixed[i]) {
-      for (int k = 0; k < 3; ++k) out.vel[3 * i + k] = 0.0;
-      for (int k = 0; k < 9; ++k) out.C[9 * i + k] = 0.0;
-      continue;
-    }
+    const uint32_t g = ins[slot];
+    auto it = std::upper_bound(ranges.begin(), ranges.end(), std::make_pair(g, UINT32_MAX));
+    if (it == ranges.begin()) return fail(TMPM_READBACK, "corrupt particle order");
+    const uint32_t k = (it - 1)->second;
+    const size_t i = g - s->scenes[k].first;
+    HostWorld& w = out[k];
+    if (i >= w.size() || i >= s->scenes[k].count) return fail(TMPM_READBACK, "corrupt particle order");
+    if (!w.enabled[i] || w.fixed[i]) continue;
     const float4 p = pm[slot], v = vv[slot];
     double xs[3] = {p.x, p.y, p.z}, vs[3] = {v.x, v.y, v.z};
     bool finite = true;
-    for (int k = 0; k < 3; ++k) {
-      out.pos[3 * i + k] = xs[k];
-      out.vel[3 * i + k] = vs[k];
-      finite = finite && std::isfinite(xs[k]) && std::isfinite(vs[k]);
+    for (int q = 0; q < 3; ++q) {
+      w.pos[3 * i + q] = xs[q];
+      w.vel[3 * i + q] = vs[q];
+      finite = finite && std::isfinite(xs[q]) && std::isfinite(vs[q]);
     }
-    for (int k = 0; k < 9; ++k) {
-      out.C[9 * i + k] = C[k * n + slot];
-      out.F[9 * i + k] = F[k * n + slot];
-      finite = finite && std::isfinite(out.C[9 * i + k]) && std::isfinite(out.F[9 * i + k]);
+    for (int q = 0; q < 9; ++q) {
+      w.C[9 * i + q] = C[q * n + slot];
+      w.F[9 * i + q] = F[q * n + slot];
+      finite = finite && std::isfinite(w.C[9 * i + q]) && std::isfinite(w.F[9 * i + q]);
     }
-    if (!finite) return fail(TMPM_NON_FINITE_STATE, "MPM integration produced a non-finite state");
+    if (!finite) st[k] = TMPM_NON_FINITE_STATE;
   }
   return TMPM_OK;
 }
@@ -1068,31 +1149,56 @@ tmpm_status tmpm_context_create(int32_t device_ordinal, tmpm_context** out) {
 
 void tmpm_context_destroy(tmpm_context* ctx) { delete ctx; }
 
-tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* wd, double substep_dt, tmpm_p2g_mode mode,
-                                tmpm_session** out) {
-  if (!ctx || !wd || !out) return fail(TMPM_INVALID_INPUT, "null argument");
-  *out = nullptr;
-  if (mode != TMPM_P2G_FP32 && mode != TMPM_P2G_FIXED64) return fail(TMPM_INVALID_INPUT, "unknown P2G mode");
+}  // extern "C"
+
+namespace {
+
+tmpm_status check_world_desc(const tmpm_world_desc* wd) {
+  if (!wd) return fail(TMPM_INVALID_INPUT, "null world");
   if (!params_valid(wd->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
   if (!validate_particles(wd->particles, wd->materials, wd->material_count))
     return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
   for (uint32_t i = 0; i < wd->obstacle_count; ++i)
     if (!obstacle_valid(wd->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  return TMPM_OK;
+}
+
+void load_scene(SceneState& scene, const tmpm_world_desc* wd, double substep_dt) {
+  HostWorld w;
+  w.params = wd->params;
+  w.append(wd->particles, wd->materials, 0);
+  w.set_obstacles(wd->obstacles, wd->obstacle_count);
+  w.substeps = wd->substeps;
+  scene = SceneState();
+  scene.set_world(w);
+  scene.substep_dt = substep_dt;
+}
+
+tmpm_status create_session(tmpm_context* ctx, const tmpm_world_desc* worlds, const double* dts, uint32_t count,
+                           tmpm_p2g_mode mode, bool batch, tmpm_session** out) {
+  if (!ctx || !out || (count > 0 && (!worlds || !dts))) return fail(TMPM_INVALID_INPUT, "null argument");
+  *out = nullptr;
+  if (mode != TMPM_P2G_FP32 && mode != TMPM_P2G_FIXED64) return fail(TMPM_INVALID_INPUT, "unknown P2G mode");
+  if (count > 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
+  for (uint32_t k = 0; k < count; ++k) {
+    tmpm_status st = check_world_desc(&worlds[k]);
+    if (st != TMPM_OK) return st;
+  }
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
   tmpm_session* s = new tmpm_session();
   s->ctx = ctx;
-  s->substep_dt = substep_dt;
+  s->batch = batch;
   s->fixed64 = mode == TMPM_P2G_FIXED64;
   read_tuning_env(s);
-  s->world.params = wd->params;
-  s->world.append(wd->particles, wd->materials, 0);
-  s->world.set_obstacles(wd->obstacles, wd->obstacle_count);
-  s->world.substeps = wd->substeps;
+  s->scenes.resize(count);
+  for (uint32_t k = 0; k < count; ++k) load_scene(s->scenes[k], &worlds[k], dts[k]);
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return fail(TMPM_READBACK, "cannot create stream");
   }
-  tmpm_status st = validate_resident_world(s->world, substep_dt);
+  tmpm_status st = TMPM_OK;
+  for (auto& scene : s->scenes)
+    if (st == TMPM_OK) st = validate_resident_world(scene);
   if (st == TMPM_OK) st = prepare(s);
   if (st != TMPM_OK) {
     std::string msg = g_last_error;
@@ -1102,6 +1208,89 @@ tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* wd, do
   }
   *out = s;
   return TMPM_OK;
+}
+
+// Common checks of an edit: the session (or the batch scene) must be usable.
+tmpm_status check_editable(tmpm_session* s) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  return TMPM_OK;
+}
+
+tmpm_status check_scene(const tmpm_session* s, uint32_t scene) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (scene >= s->scenes.size() || !s->scenes[scene].active) return fail(TMPM_INVALID_INPUT, "no such scene");
+  return TMPM_OK;
+}
+
+// Re-uploads every stepping scene after a host-side change; on failure the
+// previous scene states are restored (edits are transactional).
+tmpm_status reprepare(tmpm_session* s, std::vector<SceneState>& previous) {
+  tmpm_status st = prepare(s);
+  if (st != TMPM_OK) {
+    std::string msg = g_last_error;
+    s->scenes.swap(previous);
+    if (prepare(s) != TMPM_OK) s->valid = false;
+    g_last_error = msg;
+  }
+  return st;
+}
+
+tmpm_status set_scene_force(tmpm_session* s, uint32_t k, uint64_t index, const double force[3]) {
+  SceneState& scene = s->scenes[k];
+  if (!scene.stepping()) return fail(scene.status == TMPM_OK ? TMPM_INVALID_INPUT : scene.status, "scene is stopped");
+  if (!force) return fail(TMPM_INVALID_INPUT, "null force");
+  if (index >= scene.world.size()) return fail(TMPM_INVALID_INPUT, "particle index out of range");
+  float f[4];
+  if (!finite_f32(force[0], f[0]) || !finite_f32(force[1], f[1]) || !finite_f32(force[2], f[2]) ||
+      !finite_f32(scene.world.damping[index], f[3]))
+    return fail(TMPM_INVALID_INPUT, "force not representable in f32");
+  if (s->prepared) {
+    float4 v = make_float4(f[0], f[1], f[2], 0.f);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->force.as<float4>() + scene.first + index, &v, sizeof(v), hipMemcpyHostToDevice));
+    const bool was = scene.force_pending;
+    scene.force_pending = true;
+    if (!was) {
+      tmpm_status st = upload_scene(s);
+      if (st != TMPM_OK) return st;
+    }
+  }
+  for (int q = 0; q < 3; ++q) scene.world.force[3 * index + q] = force[q];
+  return TMPM_OK;
+}
+
+tmpm_status set_scene_obstacles(tmpm_session* s, uint32_t k, const tmpm_obstacle* obs, uint32_t count) {
+  SceneState& scene = s->scenes[k];
+  if (!scene.stepping()) return fail(scene.status == TMPM_OK ? TMPM_INVALID_INPUT : scene.status, "scene is stopped");
+  if (count > 0 && !obs) return fail(TMPM_INVALID_INPUT, "null obstacles");
+  for (uint32_t i = 0; i < count; ++i)
+    if (!obstacle_valid(obs[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  HostWorld backup = scene.world;
+  scene.world.set_obstacles(obs, count);
+  if (s->prepared) {
+    tmpm_status st = upload_obstacles(s);
+    if (st == TMPM_OK) st = upload_scene(s);
+    if (st != TMPM_OK) {
+      std::string msg = g_last_error;
+      scene.world = backup;
+      if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
+      g_last_error = msg;
+      return st;
+    }
+  }
+  return TMPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* wd, double substep_dt, tmpm_p2g_mode mode,
+                                tmpm_session** out) {
+  if (!wd) return fail(TMPM_INVALID_INPUT, "null argument");
+  return create_session(ctx, wd, &substep_dt, 1, mode, false, out);
 }
 
 void tmpm_session_destroy(tmpm_session* s) {
@@ -1118,13 +1307,19 @@ tmpm_status tmpm_session_submit_steps(tmpm_session* s, uint32_t steps) {
   if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
   if (steps == 0 || steps > 64) return fail(TMPM_INVALID_INPUT, "steps must be within 1..=64");
   if (s->pending + steps < s->pending) return fail(TMPM_CAPACITY, "pending substep count overflow");
-  tmpm_status st = validate_resident_world(s->world, s->substep_dt);
-  if (st != TMPM_OK) return st;
+  for (auto& scene : s->scenes) {
+    if (!scene.stepping()) continue;
+    tmpm_status st = validate_resident_world(scene);
+    if (st != TMPM_OK) return st;
+  }
   if (s->prepared) {
     if (hipSetDevice(s->ctx->device) != hipSuccess) return fail(TMPM_READBACK, "cannot select device");
     for (uint32_t k = 0; k < steps; ++k) {
-      bool force = s->force_pending;
-      s->force_pending = false;
+      bool force = false;
+      for (auto& scene : s->scenes) {
+        force = force || scene.force_pending;
+        scene.force_pending = false;
+      }
       hipError_t e = enqueue_substep(s, force);
       if (e != hipSuccess) {
         s->valid = false;
@@ -1146,8 +1341,11 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
   if (!s) return fail(TMPM_INVALID_INPUT, "null session");
   if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
   if (s->pending == 0) return TMPM_OK;
-  uint64_t steps = s->pending;
+  const uint64_t steps = s->pending;
   s->pending = 0;
+  const size_t ns = s->scenes.size();
+  std::vector<uint32_t> scene_bits(ns, 0);
+  uint32_t counters[CNT_COUNT] = {};
   if (s->prepared) {
     hipError_t e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
@@ -1155,34 +1353,61 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
       return fail(TMPM_READBACK, std::string("device failure: ") + hipGetErrorString(e));
     }
     collect_timers(s);
-    uint32_t counters[CNT_COUNT];
-    if (hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(scene_bits.data(), s->scene_err.ptr, 4 * ns, hipMemcpyDeviceToHost) != hipSuccess) {
       s->valid = false;
       return fail(TMPM_READBACK, "counter readback failed");
     }
-    tmpm_status st = status_from_error_bits(counters[CNT_ERROR]);
+    // Capacity and internal errors concern every scene: the session fails.
+    // A single-scene session also fails on its scene's errors; a batch stops
+    // only the scenes concerned.
+    const uint32_t global = s->batch ? (counters[CNT_ERROR] & (ERR_TOPOLOGY | ERR_INTERNAL)) : counters[CNT_ERROR];
+    tmpm_status st = status_from_error_bits(global);
     if (st != TMPM_OK) {
       s->valid = false;
       return st;
     }
-    HostWorld updated;
-    st = read_back(s, updated);
+  }
+  std::vector<HostWorld> updated;
+  std::vector<tmpm_status> scene_st;
+  tmpm_status st = read_back(s, updated, scene_st);
+  if (st != TMPM_OK) {
+    s->valid = false;
+    return st;
+  }
+  if (!s->batch && scene_st[0] != TMPM_OK) {
+    s->valid = false;
+    return fail(scene_st[0], "MPM integration produced a non-finite state");
+  }
+  bool stopped = false;
+  for (size_t k = 0; k < ns; ++k) {
+    SceneState& scene = s->scenes[k];
+    if (!scene.stepping()) continue;
+    tmpm_status sst = scene_bits[k] ? status_from_error_bits(scene_bits[k]) : scene_st[k];
+    if (sst != TMPM_OK) {
+      scene.status = sst;  // the last synchronized world stays readable
+      stopped = true;
+      continue;
+    }
+    const uint64_t before = scene.world.substeps;
+    updated[k].substeps = before + steps < before ? UINT64_MAX : before + steps;
+    scene.set_world(updated[k]);
+  }
+  if (stopped) {
+    // drop the stopped scenes' particles from the device
+    st = prepare(s);
     if (st != TMPM_OK) {
       s->valid = false;
       return st;
     }
-    updated.substeps = s->world.substeps + steps < s->world.substeps ? UINT64_MAX : s->world.substeps + steps;
-    s->world = updated;
+    return TMPM_OK;
+  }
+  if (s->prepared) {
     st = maybe_grow_topology(s, counters);
     if (st != TMPM_OK) {
       s->valid = false;
       return st;
     }
-  } else {
-    HostWorld updated;
-    read_back(s, updated);
-    updated.substeps = s->world.substeps + steps;
-    s->world = updated;
   }
   return TMPM_OK;
 }
@@ -1197,34 +1422,33 @@ tmpm_status tmpm_session_step(tmpm_session* s, uint32_t steps) {
 
 // gpu.rs:2208-2240: CFL-limited substeps, each validated on the host.
 tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  if (s->batch) return fail(TMPM_INVALID_INPUT, "adaptive stepping takes a single-scene session");
   if (!std::isfinite(dt) || dt <= 0.0) return fail(TMPM_INVALID_INPUT, "dt must be positive");
-  for (auto& o : s->world.obstacles)
+  SceneState& scene = s->scenes[0];
+  for (auto& o : scene.world.obstacles)
     if (!obstacle_valid(o.o)) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
   double remaining = dt;
   size_t count = 0;
-  double saved_dt = s->substep_dt;
+  const double saved_dt = scene.substep_dt;
   while (remaining > 0.0) {
-    double stable = s->world.stable_timestep();
+    double stable = scene.stable_timestep();
     if (!std::isfinite(stable) || stable <= 1e-12 || count >= 100000)
       return fail(TMPM_EXCESSIVE_SUBSTEPS, "MPM timestep is too small for the requested duration");
     double sub = std::min(remaining, stable);
-    if (sub != s->substep_dt) {
-      s->substep_dt = sub;
-      tmpm_status st = build_scene(s);
-      if (st == TMPM_OK) st = upload_obstacles(s);
-      if (st == TMPM_OK && s->prepared) {
-        if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TMPM_READBACK, "stream failure");
-        st = upload_scene(s);
-      }
+    if (sub != scene.substep_dt) {
+      scene.substep_dt = sub;
+      st = build_scene(scene);
+      if (st == TMPM_OK && s->prepared) st = upload_obstacles(s);
+      if (st == TMPM_OK && s->prepared) st = upload_scene(s);
       if (st != TMPM_OK) {
-        s->substep_dt = saved_dt;
+        scene.substep_dt = saved_dt;
+        (void)build_scene(scene);
         return st;
       }
     }
-    tmpm_status st = tmpm_session_step(s, 1);
+    st = tmpm_session_step(s, 1);
     if (st != TMPM_OK) return st;
     remaining = std::max(remaining - sub, 0.0);
     ++count;
@@ -1234,104 +1458,149 @@ tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt) {
 
 uint64_t tmpm_session_pending_substeps(const tmpm_session* s) { return s ? s->pending : 0; }
 int32_t tmpm_session_valid(const tmpm_session* s) { return s && s->valid ? 1 : 0; }
-uint64_t tmpm_session_substeps(const tmpm_session* s) { return s ? s->world.substeps : 0; }
-uint64_t tmpm_session_particle_count(const tmpm_session* s) { return s ? s->world.size() : 0; }
+uint64_t tmpm_session_substeps(const tmpm_session* s) { return s && !s->scenes.empty() ? s->world().substeps : 0; }
+uint64_t tmpm_session_particle_count(const tmpm_session* s) { return s && !s->scenes.empty() ? s->world().size() : 0; }
 
 tmpm_status tmpm_session_read_particles(const tmpm_session* s, tmpm_particles* out) {
-  if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (out->count < s->world.size()) return fail(TMPM_CAPACITY, "output arrays too small");
-  return copy_particles_out(s->world, out);
+  if (!s || !out || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (out->count < s->world().size()) return fail(TMPM_CAPACITY, "output arrays too small");
+  return copy_particles_out(s->world(), out);
 }
 
 tmpm_status tmpm_session_set_particle_force(tmpm_session* s, uint64_t index, const double force[3]) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  if (index >= s->world.size()) return fail(TMPM_INVALID_INPUT, "particle index out of range");
-  float f[4];
-  if (!finite_f32(force[0], f[0]) || !finite_f32(force[1], f[1]) || !finite_f32(force[2], f[2]) ||
-      !finite_f32(s->world.damping[index], f[3]))
-    return fail(TMPM_INVALID_INPUT, "force not representable in f32");
-  if (s->prepared) {
-    float4 v = make_float4(f[0], f[1], f[2], 0.f);
-    HIP_TRY(hipMemcpy(s->force.as<float4>() + index, &v, sizeof(v), hipMemcpyHostToDevice));
-    s->force_pending = true;
-  }
-  for (int k = 0; k < 3; ++k) s->world.force[3 * index + k] = force[k];
-  return TMPM_OK;
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  return set_scene_force(s, 0, index, force);
 }
 
 tmpm_status tmpm_session_set_obstacles(tmpm_session* s, const tmpm_obstacle* obs, uint32_t count) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  for (uint32_t i = 0; i < count; ++i)
-    if (!obstacle_valid(obs[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
-  std::vector<tmpm_obstacle> backup;
-  for (auto& o : s->world.obstacles) backup.push_back(o.o);
-  std::vector<std::vector<double>> backup_planes;
-  for (auto& o : s->world.obstacles) backup_planes.push_back(o.planes);
-  for (size_t i = 0; i < backup.size(); ++i) backup[i].planes = backup_planes[i].empty() ? nullptr : backup_planes[i].data();
-  s->world.set_obstacles(obs, count);
-  if (s->prepared) {
-    tmpm_status st = upload_obstacles(s);
-    if (st == TMPM_OK) st = upload_scene(s);
-    if (st != TMPM_OK) {
-      std::string msg = g_last_error;
-      s->world.set_obstacles(backup.data(), (uint32_t)backup.size());
-      if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
-      g_last_error = msg;
-      return st;
-    }
-  }
-  return TMPM_OK;
-}
-
-static tmpm_status replace_world(tmpm_session* s, const HostWorld& updated) {
-  tmpm_status st = validate_resident_world(updated, s->substep_dt);
+  tmpm_status st = check_editable(s);
   if (st != TMPM_OK) return st;
-  HostWorld old = s->world;
-  s->world = updated;
-  st = prepare(s);
-  if (st != TMPM_OK) {
-    std::string msg = g_last_error;
-    s->world = old;
-    tmpm_status st2 = prepare(s);
-    if (st2 != TMPM_OK) s->valid = false;
-    g_last_error = msg;
-    return st;
-  }
-  return TMPM_OK;
+  return set_scene_obstacles(s, 0, obs, count);
 }
 
 tmpm_status tmpm_session_add_particles(tmpm_session* s, const tmpm_particles* particles, const tmpm_material* materials,
                                        uint32_t material_count, uint64_t* chunk_out) {
-  if (!s || !particles) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  if (!particles) return fail(TMPM_INVALID_INPUT, "null argument");
   if (particles->count == 0 || !validate_particles(*particles, materials, material_count))
     return fail(TMPM_INVALID_INPUT, "invalid MPM particle batch");
-  if (s->world.next_chunk_id == UINT64_MAX) return fail(TMPM_PARTICLE_CAPACITY, "chunk ids exhausted");
-  HostWorld updated = s->world;
-  uint64_t chunk = updated.next_chunk_id++;
+  SceneState& scene = s->scenes[0];
+  if (scene.world.next_chunk_id == UINT64_MAX) return fail(TMPM_PARTICLE_CAPACITY, "chunk ids exhausted");
+  std::vector<SceneState> previous = s->scenes;
+  HostWorld updated = scene.world;
+  const uint64_t chunk = updated.next_chunk_id++;
   updated.append(*particles, materials, chunk);
-  tmpm_status st = replace_world(s, updated);
+  SceneState trial = scene;
+  trial.set_world(updated);
+  st = validate_resident_world(trial);
+  if (st != TMPM_OK) return st;
+  scene = trial;
+  st = reprepare(s, previous);
   if (st != TMPM_OK) return st;
   if (chunk_out) *chunk_out = chunk;
   return TMPM_OK;
 }
 
 tmpm_status tmpm_session_remove_chunk(tmpm_session* s, uint64_t chunk, uint64_t* removed_out) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  HostWorld updated = s->world;
-  size_t removed = updated.remove_chunk(chunk);
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  SceneState& scene = s->scenes[0];
+  std::vector<SceneState> previous = s->scenes;
+  HostWorld updated = scene.world;
+  const size_t removed = updated.remove_chunk(chunk);
   if (removed == 0) return fail(TMPM_UNKNOWN_CHUNK, "MPM particle chunk was not found");
-  tmpm_status st = replace_world(s, updated);
+  SceneState trial = scene;
+  trial.set_world(updated);
+  st = validate_resident_world(trial);
+  if (st != TMPM_OK) return st;
+  scene = trial;
+  st = reprepare(s, previous);
   if (st != TMPM_OK) return st;
   if (removed_out) *removed_out = removed;
   return TMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: several independent scenes stepped together
+// ---------------------------------------------------------------------------
+tmpm_status tmpm_batch_create(tmpm_context* ctx, const tmpm_world_desc* worlds, const double* substep_dt,
+                              uint32_t scene_count, tmpm_p2g_mode mode, tmpm_session** out) {
+  return create_session(ctx, worlds, substep_dt, scene_count, mode, true, out);
+}
+
+uint32_t tmpm_batch_scene_count(const tmpm_session* s) { return s ? (uint32_t)s->scenes.size() : 0; }
+
+tmpm_status tmpm_batch_scene_status(const tmpm_session* s, uint32_t scene) {
+  tmpm_status st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return s->scenes[scene].status;
+}
+
+uint64_t tmpm_batch_scene_particle_count(const tmpm_session* s, uint32_t scene) {
+  return check_scene(s, scene) == TMPM_OK ? s->scenes[scene].world.size() : 0;
+}
+
+uint64_t tmpm_batch_scene_substeps(const tmpm_session* s, uint32_t scene) {
+  return check_scene(s, scene) == TMPM_OK ? s->scenes[scene].world.substeps : 0;
+}
+
+tmpm_status tmpm_batch_read_particles(const tmpm_session* s, uint32_t scene, tmpm_particles* out) {
+  tmpm_status st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  if (!out) return fail(TMPM_INVALID_INPUT, "null argument");
+  const HostWorld& w = s->scenes[scene].world;
+  if (out->count < w.size()) return fail(TMPM_CAPACITY, "output arrays too small");
+  return copy_particles_out(w, out);
+}
+
+tmpm_status tmpm_batch_set_particle_force(tmpm_session* s, uint32_t scene, uint64_t index, const double force[3]) {
+  tmpm_status st = check_editable(s);
+  if (st == TMPM_OK) st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return set_scene_force(s, scene, index, force);
+}
+
+tmpm_status tmpm_batch_set_obstacles(tmpm_session* s, uint32_t scene, const tmpm_obstacle* obs, uint32_t count) {
+  tmpm_status st = check_editable(s);
+  if (st == TMPM_OK) st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return set_scene_obstacles(s, scene, obs, count);
+}
+
+tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, double substep_dt,
+                                 uint32_t* scene_out) {
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
+  st = check_world_desc(world);
+  if (st != TMPM_OK) return st;
+  SceneState scene;
+  load_scene(scene, world, substep_dt);
+  st = validate_resident_world(scene);
+  if (st != TMPM_OK) return st;
+  uint32_t slot = 0;
+  while (slot < s->scenes.size() && s->scenes[slot].active) ++slot;
+  if (slot >= 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
+  std::vector<SceneState> previous = s->scenes;
+  if (slot == s->scenes.size()) s->scenes.push_back(scene);
+  else s->scenes[slot] = scene;
+  st = reprepare(s, previous);
+  if (st != TMPM_OK) return st;
+  if (scene_out) *scene_out = slot;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene) {
+  tmpm_status st = check_editable(s);
+  if (st == TMPM_OK) st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
+  std::vector<SceneState> previous = s->scenes;
+  s->scenes[scene] = SceneState();
+  s->scenes[scene].active = false;
+  return reprepare(s, previous);
 }
 
 tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, uint64_t output_bytes, void* stream,
@@ -1363,7 +1632,9 @@ tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, u
   return TMPM_OK;
 }
 
-double tmpm_session_stable_timestep(const tmpm_session* s) { return s ? s->world.stable_timestep() : 0.0; }
+double tmpm_session_stable_timestep(const tmpm_session* s) {
+  return s && !s->scenes.empty() ? s->world().stable_timestep() : 0.0;
+}
 
 tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* out) {
   if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
@@ -1413,7 +1684,7 @@ tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uin
   tmpm_status st = run_diagnostic_binning(s, c);
   if (st != TMPM_OK) return st;
   *count_out = 0;
-  size_t total = s->world.size();
+  size_t total = s->scenes.empty() ? 0 : s->world().size();
   if (particle_block)
     for (size_t i = 0; i < total; ++i) particle_block[i] = -1;
   if (!s->prepared) return TMPM_OK;

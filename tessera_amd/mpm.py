@@ -214,6 +214,17 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
             C.c_int,
             [vp, C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32,
              C.POINTER(C.c_uint32)]),
+        "tmpm_batch_create": (
+            C.c_int, [vp, C.POINTER(CWorld), C.POINTER(C.c_double), C.c_uint32, C.c_int, C.POINTER(vp)]),
+        "tmpm_batch_scene_count": (C.c_uint32, [vp]),
+        "tmpm_batch_scene_status": (C.c_int, [vp, C.c_uint32]),
+        "tmpm_batch_scene_particle_count": (C.c_uint64, [vp, C.c_uint32]),
+        "tmpm_batch_scene_substeps": (C.c_uint64, [vp, C.c_uint32]),
+        "tmpm_batch_read_particles": (C.c_int, [vp, C.c_uint32, C.POINTER(CParticles)]),
+        "tmpm_batch_set_particle_force": (C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_double * 3]),
+        "tmpm_batch_set_obstacles": (C.c_int, [vp, C.c_uint32, C.POINTER(CObstacle), C.c_uint32]),
+        "tmpm_batch_add_scene": (C.c_int, [vp, C.POINTER(CWorld), C.c_double, C.POINTER(C.c_uint32)]),
+        "tmpm_batch_remove_scene": (C.c_int, [vp, C.c_uint32]),
         "tmpm_bandwidth_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
         "tmpm_timestep_bound": (C.c_double, [C.POINTER(CMaterial), C.c_double, C.c_double * 3, C.c_double, C.c_double]),
         "tmpm_material_valid": (C.c_int32, [C.POINTER(CMaterial)]),
@@ -740,6 +751,46 @@ class HipMpmTransfers:
 P2G_MODES = {"fp32": 0, "fixed64": 1}
 
 
+def _read_particle_arrays(n: int, read, lib) -> ParticleArrays:
+    """Reads n particles through `read(CParticles*)` into a ParticleArrays (row-major matrices)."""
+    out = ParticleArrays(n)
+    flat = {
+        "position": np.zeros((n, 3)), "velocity": np.zeros((n, 3)), "deformation": np.zeros((n, 9)),
+        "affine": np.zeros((n, 9)), "force": np.zeros((n, 3)), "mass": np.zeros(n), "rest_volume": np.zeros(n),
+        "radius": np.zeros(n), "damping": np.zeros(n), "plastic": np.zeros((n, 3)),
+    }
+    ints = {
+        "group_id": np.zeros(n, np.uint32), "transfer_color": np.zeros(n, np.uint16),
+        "enabled": np.zeros(n, np.uint8), "fixed": np.zeros(n, np.uint8), "chunk_id": np.zeros(n, np.uint64),
+    }
+    cp = CParticles()
+    cp.count = n
+    for k, a in flat.items():
+        setattr(cp, k, a.ctypes.data_as(_dp))
+    cp.group_id = ints["group_id"].ctypes.data_as(C.POINTER(C.c_uint32))
+    cp.transfer_color = ints["transfer_color"].ctypes.data_as(C.POINTER(C.c_uint16))
+    cp.enabled = ints["enabled"].ctypes.data_as(C.POINTER(C.c_uint8))
+    cp.fixed = ints["fixed"].ctypes.data_as(C.POINTER(C.c_uint8))
+    cp.chunk_id = ints["chunk_id"].ctypes.data_as(C.POINTER(C.c_uint64))
+    _check(read(C.byref(cp)), lib)
+    out.position = flat["position"]
+    out.velocity = flat["velocity"]
+    out.deformation = flat["deformation"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
+    out.affine = flat["affine"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
+    out.force = flat["force"]
+    out.mass = flat["mass"]
+    out.rest_volume = flat["rest_volume"]
+    out.radius = flat["radius"]
+    out.damping = flat["damping"]
+    out.plastic = flat["plastic"]
+    out.group_id = ints["group_id"]
+    out.transfer_color = ints["transfer_color"]
+    out.enabled = ints["enabled"].astype(bool)
+    out.fixed = ints["fixed"].astype(bool)
+    out.chunk_id = ints["chunk_id"]
+    return out
+
+
 class HipMpmResidentSession:
     """Resident HIP session (the reference's ``GpuMpmResidentSession``).
 
@@ -808,41 +859,7 @@ class HipMpmResidentSession:
     # -- state ---------------------------------------------------------------
     def read_particles(self) -> ParticleArrays:
         n = int(self.lib.tmpm_session_particle_count(self._h))
-        out = ParticleArrays(n)
-        flat = {
-            "position": np.zeros((n, 3)), "velocity": np.zeros((n, 3)), "deformation": np.zeros((n, 9)),
-            "affine": np.zeros((n, 9)), "force": np.zeros((n, 3)), "mass": np.zeros(n), "rest_volume": np.zeros(n),
-            "radius": np.zeros(n), "damping": np.zeros(n), "plastic": np.zeros((n, 3)),
-        }
-        ints = {
-            "group_id": np.zeros(n, np.uint32), "transfer_color": np.zeros(n, np.uint16),
-            "enabled": np.zeros(n, np.uint8), "fixed": np.zeros(n, np.uint8), "chunk_id": np.zeros(n, np.uint64),
-        }
-        cp = CParticles()
-        cp.count = n
-        for k, a in flat.items():
-            setattr(cp, k, a.ctypes.data_as(_dp))
-        cp.group_id = ints["group_id"].ctypes.data_as(C.POINTER(C.c_uint32))
-        cp.transfer_color = ints["transfer_color"].ctypes.data_as(C.POINTER(C.c_uint16))
-        cp.enabled = ints["enabled"].ctypes.data_as(C.POINTER(C.c_uint8))
-        cp.fixed = ints["fixed"].ctypes.data_as(C.POINTER(C.c_uint8))
-        cp.chunk_id = ints["chunk_id"].ctypes.data_as(C.POINTER(C.c_uint64))
-        _check(self.lib.tmpm_session_read_particles(self._h, C.byref(cp)), self.lib)
-        out.position = flat["position"]
-        out.velocity = flat["velocity"]
-        out.deformation = flat["deformation"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
-        out.affine = flat["affine"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
-        out.force = flat["force"]
-        out.mass = flat["mass"]
-        out.rest_volume = flat["rest_volume"]
-        out.radius = flat["radius"]
-        out.damping = flat["damping"]
-        out.plastic = flat["plastic"]
-        out.group_id = ints["group_id"]
-        out.transfer_color = ints["transfer_color"]
-        out.enabled = ints["enabled"].astype(bool)
-        out.fixed = ints["fixed"].astype(bool)
-        out.chunk_id = ints["chunk_id"]
+        out = _read_particle_arrays(n, lambda cp: self.lib.tmpm_session_read_particles(self._h, cp), self.lib)
         out.materials = list(self._world.particles.materials)
         out.material_index = self._material_index(n)
         return out
@@ -937,6 +954,130 @@ class HipMpmResidentSession:
         _check(self.lib.tmpm_session_kernel_times(self._h, names, 4096, avg, calls, cap, C.byref(count)), self.lib)
         keys = names.value.decode().split(",") if names.value else []
         return {k: (avg[i], int(calls[i])) for i, k in enumerate(keys)}
+
+
+class HipMpmBatch:
+    """Independent worlds advanced together, one kernel launch per stage per
+    substep for all of them (the batch/serving boundary; the reference steps
+    one ``GpuMpmResidentSession`` per world).  Each scene keeps the resident
+    session's semantics; a failure after submission stops only its scene
+    (``scene_error``), whose last synchronized world stays readable."""
+
+    def __init__(self, transfers: HipMpmTransfers, worlds: Sequence[MpmWorld], substep_dts: Sequence[float],
+                 p2g_mode: str = "fp32"):
+        if len(worlds) != len(substep_dts):
+            raise InvalidInput("one substep dt per world")
+        self.lib = transfers.lib
+        self.transfers = transfers
+        descs = (CWorld * max(1, len(worlds)))()
+        keep = []
+        for k, w in enumerate(worlds):
+            d, kk = w.c_desc()
+            descs[k] = d
+            keep.append(kk)
+        dts = (C.c_double * max(1, len(worlds)))(*map(float, substep_dts))
+        h = C.c_void_p()
+        _check(self.lib.tmpm_batch_create(transfers.handle, descs, dts, len(worlds), P2G_MODES[p2g_mode],
+                                          C.byref(h)), self.lib)
+        del keep
+        self._h = h
+        self._worlds = {k: w.clone() for k, w in enumerate(worlds)}
+        self.p2g_mode = p2g_mode
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.tmpm_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping (all scenes) ----------------------------------------------
+    def submit_steps(self, steps: int) -> None:
+        _check(self.lib.tmpm_session_submit_steps(self._h, int(steps)), self.lib)
+
+    def synchronize(self) -> None:
+        _check(self.lib.tmpm_session_synchronize(self._h), self.lib)
+
+    def wait(self) -> None:
+        _check(self.lib.tmpm_session_wait(self._h), self.lib)
+
+    def step(self, steps: int) -> None:
+        _check(self.lib.tmpm_session_step(self._h, int(steps)), self.lib)
+
+    def pending_substeps(self) -> int:
+        return int(self.lib.tmpm_session_pending_substeps(self._h))
+
+    @property
+    def valid(self) -> bool:
+        return bool(self.lib.tmpm_session_valid(self._h))
+
+    # -- scenes -------------------------------------------------------------
+    def scene_count(self) -> int:
+        return int(self.lib.tmpm_batch_scene_count(self._h))
+
+    def scenes(self) -> list[int]:
+        """Live scene slots."""
+        return sorted(self._worlds)
+
+    def scene_error(self, scene: int):
+        """None while the scene steps, else the GpuMpmError that stopped it."""
+        st = int(self.lib.tmpm_batch_scene_status(self._h, int(scene)))
+        if st == OK:
+            return None
+        return _ERRORS.get(st, GpuMpmError)(STATUS_NAMES.get(st, str(st)))
+
+    def substeps(self, scene: int) -> int:
+        return int(self.lib.tmpm_batch_scene_substeps(self._h, int(scene)))
+
+    def read_particles(self, scene: int) -> ParticleArrays:
+        n = int(self.lib.tmpm_batch_scene_particle_count(self._h, int(scene)))
+        out = _read_particle_arrays(n, lambda cp: self.lib.tmpm_batch_read_particles(self._h, int(scene), cp),
+                                    self.lib)
+        src = self._worlds[int(scene)].particles
+        out.materials = list(src.materials)
+        out.material_index = src.material_index.copy() if len(src.material_index) == n else np.zeros(n, np.uint32)
+        return out
+
+    def world(self, scene: int) -> MpmWorld:
+        base = self._worlds[int(scene)]
+        w = MpmWorld(self.read_particles(scene), base.params)
+        w.obstacles = list(base.obstacles)
+        w.use_cpic = base.use_cpic
+        w.substeps = self.substeps(scene)
+        return w
+
+    def set_particle_force(self, scene: int, index: int, force) -> None:
+        f = (C.c_double * 3)(*map(float, force))
+        _check(self.lib.tmpm_batch_set_particle_force(self._h, int(scene), int(index), f), self.lib)
+
+    def set_obstacles(self, scene: int, obstacles: Sequence[RigidObstacle]) -> None:
+        arr, keep = obstacles_to_c(obstacles)
+        _check(self.lib.tmpm_batch_set_obstacles(self._h, int(scene), C.cast(arr, C.POINTER(CObstacle)),
+                                                 len(obstacles)), self.lib)
+        self._worlds[int(scene)].obstacles = list(obstacles)
+
+    def add_scene(self, world: MpmWorld, substep_dt: float) -> int:
+        desc, keep = world.c_desc()
+        slot = C.c_uint32()
+        _check(self.lib.tmpm_batch_add_scene(self._h, C.byref(desc), float(substep_dt), C.byref(slot)), self.lib)
+        del keep
+        self._worlds[int(slot.value)] = world.clone()
+        return int(slot.value)
+
+    def remove_scene(self, scene: int) -> None:
+        _check(self.lib.tmpm_batch_remove_scene(self._h, int(scene)), self.lib)
+        self._worlds.pop(int(scene), None)
+
+    # -- diagnostics ----------------------------------------------------------
+    def set_timing(self, enabled: bool) -> None:
+        _check(self.lib.tmpm_session_set_timing(self._h, 1 if enabled else 0), self.lib)
+
+    def kernel_times(self) -> dict[str, tuple[float, int]]:
+        return HipMpmResidentSession.kernel_times(self)  # same session-level entry point
 
 
 def bandwidth_probe(device: int = 0, nbytes: int = 1 << 30, iterations: int = 20) -> float:

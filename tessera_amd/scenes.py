@@ -16,6 +16,11 @@ Units are SI and gravity points along -y.  Particle volume and mass follow
                              [0,0.4]x[0,0.6]x[0,1] at 2 per cell; h=0.0048559
                              so that N, the particles per cell and a 512^3 cell
                              domain agree; dt=2e-5 (2000 steps).
+* :func:`batch_scenes`       256 independent scenes, N_i log-uniform in
+                             [2048, 65536] (sum ~4.7M), h_i in {1/64 .. 1/128},
+                             a jittered cube of 8 particles per cell with
+                             N(0, 0.3) velocities, elastic (corotated or
+                             neo-Hookean) or fluid, dt=5e-5 each.
 """
 
 from __future__ import annotations
@@ -119,6 +124,38 @@ def dam_break(counts=(128, 128, 256), seed: int = 0) -> Scene:
     return Scene(f"dam_break_{n}", MpmWorld(pa, params), 2.0e-5, 2000)
 
 
+BATCH_CELLS = (1.0 / 64, 1.0 / 80, 1.0 / 96, 1.0 / 112, 1.0 / 128)
+
+
+def batch_scene(rng, n_target: int, index: int) -> Scene:
+    """One scene of a batch: a jittered cube of ~n_target particles at 8 per cell."""
+    h = BATCH_CELLS[int(rng.integers(len(BATCH_CELLS)))]
+    per_axis = max(2, int(round((n_target) ** (1.0 / 3.0))))
+    spacing = h / 2.0
+    edge = per_axis * spacing
+    lo = rng.uniform(0.2, 0.8 - edge, size=3)
+    pos = jittered_lattice(rng, [per_axis] * 3, lo, lo + edge)
+    n = len(pos)
+    vel = rng.normal(0.0, 0.3, size=(n, 3))
+    kind = int(rng.integers(3))
+    if kind == 0:
+        material = MaterialModel.elastic(float(rng.uniform(5e3, 5e4)), 0.3)
+    elif kind == 1:
+        material = MaterialModel.neo_hookean(float(rng.uniform(5e3, 5e4)), 0.3)
+    else:
+        material = MaterialModel.fluid(1.0e5, 7.0, 0.0)
+    pa = _particles(pos, vel, spacing / 2, 1.0e3, material)
+    params = MpmParams(cell_width=h, gravity=GRAVITY_Y.copy(), max_substep=5.0e-5,
+                       bounds=WorldBounds(np.zeros(3), np.ones(3)))
+    return Scene(f"batch_{index}_{n}", MpmWorld(pa, params), 5.0e-5, 100)
+
+
+def batch_scenes(count: int = 256, seed: int = 0, n_min: int = 2048, n_max: int = 65536) -> list[Scene]:
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(n_min), math.log(n_max), size=count))
+    return [batch_scene(rng, int(n), k) for k, n in enumerate(sizes)]
+
+
 def record_hash(pa: ParticleArrays, count: int = 4096) -> str:
     """SHA-256 of the first ``count`` position/velocity records (f64 bytes)."""
     m = min(count, len(pa))
@@ -126,4 +163,4 @@ def record_hash(pa: ParticleArrays, count: int = 4096) -> str:
     return hashlib.sha256(np.ascontiguousarray(data, dtype=np.float64).tobytes()).hexdigest()
 
 
-SCENES = {"c1": uniform_cube, "c2": four_cubes, "c3": dam_break}
+SCENES = {"c1": uniform_cube, "c2": four_cubes, "c3": dam_break, "c4": batch_scenes}
