@@ -256,6 +256,61 @@ tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uin
                                        uint64_t* count_out);
 
 /* ---------------------------------------------------------------------------
+ * Obstacle reactions and rigid coupling
+ * ------------------------------------------------------------------------- */
+/* Reactions (ObstacleReaction, world.rs:190-204): for every obstacle, the
+ * linear impulse and the angular impulse about its center that the MPM
+ * transferred to it, from grid-node projection and particle collision
+ * projection (step_with_obstacle_reactions, world.rs:309-330; the resident
+ * GPU path's reaction passes, gpu.wgsl:1021-1167).  When enabled, each
+ * synchronize replaces the readback with the sums over the substeps it
+ * synchronized.  A scene may have at most 8 obstacles while reactions are
+ * on (Capacity otherwise). */
+tmpm_status tmpm_session_set_reactions(tmpm_session* s, int32_t enabled);
+/* 6 doubles per obstacle in obstacle order: linear xyz, angular xyz. */
+tmpm_status tmpm_session_read_reactions(const tmpm_session* s, double* out, uint32_t obstacle_cap,
+                                        uint32_t* count_out);
+
+/* Rigid body state (GpuRigidBodyState, phys/src/gpu_rigid_state.rs:17-28):
+ * zero inverse mass is static or kinematic; orientation is XYZW; inertia is
+ * the body-frame inverse principal inertia. */
+typedef struct tmpm_rigid_body {
+  double position[3];
+  double inverse_mass;
+  double orientation[4];
+  double linear_velocity[3];
+  double angular_velocity[3];
+  double inverse_inertia[3];
+} tmpm_rigid_body;
+
+/* One per obstacle, in obstacle order (the coupler's Mapping,
+ * gpu_rigid_sync.rs:15-30): the body that drives it (UINT32_MAX: none) and
+ * its pose in that body's frame. */
+typedef struct tmpm_rigid_mapping {
+  uint32_t body;
+  uint32_t reserved;
+  double local_center[3];
+  double local_orientation[4];
+} tmpm_rigid_mapping;
+
+/* Couples rigid bodies to the session's obstacles, device-resident, as
+ * GpuRigidMpmCoupler's MPM-owned step does (gpu_rigid_sync.rs:187-213): before
+ * every substep each mapped obstacle takes its body's pose and velocity
+ * (gpu_rigid_sync.wgsl:43-62); after it each body takes the substep's
+ * reactions of its obstacles and, when `integrate` is set, gravity, the speed
+ * cap (0: none) and its pose update (gpu_rigid_reaction.wgsl:72-150).
+ * Reactions are computed while bodies are set.  count 0 removes the
+ * coupling.  Bodies and obstacle poses are read back at synchronize. */
+tmpm_status tmpm_session_set_rigid_bodies(tmpm_session* s, const tmpm_rigid_body* bodies, uint32_t body_count,
+                                          const tmpm_rigid_mapping* mappings, uint32_t mapping_count,
+                                          const double gravity[3], double max_linear_speed, int32_t integrate);
+tmpm_status tmpm_session_read_rigid_bodies(const tmpm_session* s, tmpm_rigid_body* out, uint32_t cap,
+                                           uint32_t* count_out);
+/* Obstacles as last synchronized (poses of mapped obstacles follow their
+ * bodies); planes of convex obstacles are not returned (NULL). */
+tmpm_status tmpm_session_read_obstacles(const tmpm_session* s, tmpm_obstacle* out, uint32_t cap, uint32_t* count_out);
+
+/* ---------------------------------------------------------------------------
  * Batches.  The reference has one world per GpuMpmResidentSession and no
  * batch type; these entry points are the batch/serving boundary the
  * replacement adds (many independent scenes advanced together, one kernel
@@ -286,6 +341,17 @@ tmpm_status tmpm_batch_read_particles(const tmpm_session* s, uint32_t scene, tmp
 tmpm_status tmpm_batch_set_particle_force(tmpm_session* s, uint32_t scene, uint64_t index, const double force[3]);
 tmpm_status tmpm_batch_set_obstacles(tmpm_session* s, uint32_t scene, const tmpm_obstacle* obstacles,
                                      uint32_t count);
+/* Per-scene forms of the reaction and rigid-coupling entry points above. */
+tmpm_status tmpm_batch_read_reactions(const tmpm_session* s, uint32_t scene, double* out, uint32_t obstacle_cap,
+                                      uint32_t* count_out);
+tmpm_status tmpm_batch_set_rigid_bodies(tmpm_session* s, uint32_t scene, const tmpm_rigid_body* bodies,
+                                        uint32_t body_count, const tmpm_rigid_mapping* mappings,
+                                        uint32_t mapping_count, const double gravity[3], double max_linear_speed,
+                                        int32_t integrate);
+tmpm_status tmpm_batch_read_rigid_bodies(const tmpm_session* s, uint32_t scene, tmpm_rigid_body* out, uint32_t cap,
+                                         uint32_t* count_out);
+tmpm_status tmpm_batch_read_obstacles(const tmpm_session* s, uint32_t scene, tmpm_obstacle* out, uint32_t cap,
+                                      uint32_t* count_out);
 /* Admits a new scene (serving): it takes the first free slot. */
 tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, double substep_dt,
                                  uint32_t* scene_out);

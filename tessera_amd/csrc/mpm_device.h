@@ -73,7 +73,11 @@ struct DevScene {
   int32_t apply_force;     // read the force array in this substep's G2P
   int32_t mass_shift;      // fixed-point scale exponents
   int32_t mom_shift;
-  int32_t pad;
+  int32_t body_begin;      // rigid bodies of this scene: [body_begin, body_begin + body_count)
+  int32_t body_count;
+  int32_t rigid_integrate; // MPM-owned rigid step: gravity, speed cap and pose update
+  float rigid_gravity[3];
+  float rigid_max_speed;   // 0: no cap
 };
 
 struct DevObstacle {
@@ -89,7 +93,36 @@ struct DevObstacle {
   float ang[3];
   float pad3;
   float dims[4];
+  // reactions: contributions are added as int64 floor(value * 2^shift)
+  float rscale_lin, rscale_ang;  // 2^(shift - 32), the fixed64_of input scale
+  int32_t rshift_lin, rshift_ang;
 };
+
+// Rigid body state (GpuRigidBodyState, phys/src/gpu_rigid_state.rs:17-28):
+// position and inverse mass, XYZW orientation, world velocities, body-frame
+// inverse principal inertia.  Zero inverse mass: static or kinematic.
+struct DevRigid {
+  float pos[3];
+  float inv_mass;
+  float q[4];
+  float lin[3];
+  float pad0;
+  float ang[3];
+  float pad1;
+  float inv_inertia[3];
+  float pad2;
+};
+
+// Obstacle -> rigid body mapping (gpu_rigid_sync.rs:15-30): body index in the
+// session's body array (-1: obstacle not driven), pose in the body frame.
+struct DevMapping {
+  int32_t body;
+  float local_center[3];
+  float local_q[4];
+};
+
+constexpr int kReactBins = 16;       // int64 accumulators per obstacle component
+constexpr int kMaxReactObstacles = 8;  // per scene, when reactions are on
 
 struct Parts {
   float4* pos_mass;
@@ -131,6 +164,12 @@ struct Topo {
   uint32_t* bin_claims;     // per group: particle blocks claimed
   uint32_t* bin_nclaims;    // per group: neighbour-only blocks claimed
   uint32_t* bin_csum;       // per group: particles in its particle blocks
+  // reactions (when enabled): [obstacle][bin][6] int64, then per obstacle the
+  // substep's reaction (6 floats) and the sum since the last readback (6 doubles)
+  long long* react_bins;
+  float* react_sub;
+  double* react_total;
+  uint32_t obstacle_total;
   void* ptile;              // per particle block: its 6^3 P2G tile (FP32: float4; FIXED64: 4 x int64 per node)
   uint32_t* grid_rev;       // per grid slot, 8 entries: particle block whose tile covers it at offset d
   float4* grid_vel;

@@ -15,9 +15,14 @@ void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s);
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s);
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
                 uint32_t threads, int copies, hipStream_t s);
-void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, hipStream_t s);
+void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react,
+                        hipStream_t s);
 void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                uint32_t blocks, uint32_t threads, hipStream_t s);
+                uint32_t blocks, uint32_t threads, int react, hipStream_t s);
+void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s);
+void launch_rigid_sync(const DevRigid* bodies, const DevMapping* maps, DevObstacle* o, uint32_t count, hipStream_t s);
+void launch_rigid_apply(const Topo& t, const DevScene* sc, DevRigid* bodies, const DevMapping* maps,
+                        const DevObstacle* o, const uint32_t* body_scene, uint32_t count, hipStream_t s);
 void launch_snapshot(const Parts& p, float4* out, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);

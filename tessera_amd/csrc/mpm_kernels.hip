@@ -547,28 +547,34 @@ __device__ __forceinline__ bool regular_corotated(const M3& F) {
 
 // Newton iteration R <- (R + R^-T) / 2 (gpu.wgsl:467-482), at most 10 steps.
 // The gate above bounds the condition number by 100, where the iteration
-// converges quadratically; the wave leaves the loop once no lane's step
-// moved an entry by more than 1e-6 (the next step would be below f32
-// resolution).  Near F = I that is 3-4 steps instead of 10.
+// converges quadratically.  A lane stops updating once its last step moved
+// no entry by more than 1e-6 (the next step would be below f32 resolution),
+// so its result depends on its own F only, not on the other particles of
+// its wavefront; the wavefront leaves the loop when every lane has stopped.
+// Near F = I that is 3-4 steps instead of 10.
 __device__ void polar_newton(const M3& F, M3& R) {
   R = F;
+  bool done = false;
   for (int it = 0; it < 10; ++it) {
-    float3 a = col(R, 0), b = col(R, 1), c = col(R, 2);
-    float det = dot3(a, cross3(b, c));
-    float inv = __builtin_amdgcn_rcpf(det);
-    float3 t0 = cross3(b, c) * inv, t1 = cross3(c, a) * inv, t2 = cross3(a, b) * inv;
-    float delta = 0.0f;
+    if (!done) {
+      float3 a = col(R, 0), b = col(R, 1), c = col(R, 2);
+      float det = dot3(a, cross3(b, c));
+      float inv = __builtin_amdgcn_rcpf(det);
+      float3 t0 = cross3(b, c) * inv, t1 = cross3(c, a) * inv, t2 = cross3(a, b) * inv;
+      float delta = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const float n0 = 0.5f * (comp(a, q) + comp(t0, q));
-      const float n1 = 0.5f * (comp(b, q) + comp(t1, q));
-      const float n2 = 0.5f * (comp(c, q) + comp(t2, q));
-      delta = fmaxf(delta, fmaxf(fabsf(n0 - R.a[q]), fmaxf(fabsf(n1 - R.a[3 + q]), fabsf(n2 - R.a[6 + q]))));
-      R.a[q] = n0;
-      R.a[3 + q] = n1;
-      R.a[6 + q] = n2;
+      for (int q = 0; q < 3; ++q) {
+        const float n0 = 0.5f * (comp(a, q) + comp(t0, q));
+        const float n1 = 0.5f * (comp(b, q) + comp(t1, q));
+        const float n2 = 0.5f * (comp(c, q) + comp(t2, q));
+        delta = fmaxf(delta, fmaxf(fabsf(n0 - R.a[q]), fmaxf(fabsf(n1 - R.a[3 + q]), fabsf(n2 - R.a[6 + q]))));
+        R.a[q] = n0;
+        R.a[3 + q] = n1;
+        R.a[6 + q] = n2;
+      }
+      done = delta <= 1e-6f;
     }
-    if (__all(delta <= 1e-6f)) break;
+    if (__all(done)) break;
   }
 }
 
@@ -1018,7 +1024,35 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
 // ---------------------------------------------------------------------------
 // 9. grid update (world.rs:401-417; gpu.wgsl:772-819)
 // ---------------------------------------------------------------------------
-template <int FIXED>
+// Reactions (ObstacleReaction::record, world.rs:198-204): an obstacle
+// receives the linear impulse J = (v_before - v_after) m and the angular
+// impulse (point - center) x J of every node (gpu.wgsl:1056-1108) and
+// particle (gpu.wgsl:1110-1142) it deflects.  A wavefront sums its lanes in a
+// fixed butterfly order and adds the six sums to the obstacle's int64 bins,
+// so the totals do not depend on scheduling; k_react_finalize folds the bins.
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Called by every lane of a wavefront (uniform control flow).
+__device__ __forceinline__ void add_reaction(const Topo& t, const DevObstacle& o, uint32_t oidx, uint32_t bin,
+                                             float3 lin, float3 ang) {
+  const float c[6] = {wave_sum_f(lin.x), wave_sum_f(lin.y), wave_sum_f(lin.z),
+                      wave_sum_f(ang.x), wave_sum_f(ang.y), wave_sum_f(ang.z)};
+  if (lane_id() == 0) {
+    unsigned long long* dst =
+        reinterpret_cast<unsigned long long*>(t.react_bins) + ((size_t)oidx * kReactBins + bin) * 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicAdd(&dst[k], fixed64_of(c[k] * o.rscale_lin));
+      atomicAdd(&dst[3 + k], fixed64_of(c[3 + k] * o.rscale_ang));
+    }
+  }
+}
+
+template <int FIXED, int REACT>
 __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* scenes, const DevObstacle* obstacles) {
   const uint32_t ng = t.counters[CNT_GRID_BLOCKS];
   const uint32_t lane = threadIdx.x & 63;
@@ -1060,25 +1094,40 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
       mom = make_float3(__ll2float_rn(ix) * sp, __ll2float_rn(iy) * sp, __ll2float_rn(iz) * sp);
     }
     float3 v = make_float3(0, 0, 0);
-    if (m > 0.0f) {
+    const bool live = m > 0.0f;
+    const int cx = key.x * kBlockCells + lx;
+    const int cy = key.y * kBlockCells + ly;
+    const int cz = key.z * kBlockCells + lz;
+    const float3 pos = make_float3((float)cx * sc.h, (float)cy * sc.h, (float)cz * sc.h);
+    if (live) {
       v = mom / m + make_float3(sc.gravity[0], sc.gravity[1], sc.gravity[2]) * sc.dt;
-      const int cx = key.x * kBlockCells + lx;
-      const int cy = key.y * kBlockCells + ly;
-      const int cz = key.z * kBlockCells + lz;
-      const float3 pos = make_float3((float)cx * sc.h, (float)cy * sc.h, (float)cz * sc.h);
       if (sc.has_bounds) {
         for (int a = 0; a < 3; ++a) {
           float pa = comp(pos, a), va = comp(v, a);
           if ((pa <= sc.bmin[a] + sc.h && va < 0.0f) || (pa >= sc.bmax[a] - sc.h && va > 0.0f)) set_comp(v, a, 0.0f);
         }
       }
-      for (int oi = 0; oi < sc.obstacle_count; ++oi) {
-        const DevObstacle& o = obstacles[sc.obstacle_begin + oi];
-        if (!obstacle_may_contact(o, pos, sc.h)) continue;
-        float dist;
-        float3 nrm;
+    }
+    for (int oi = 0; oi < sc.obstacle_count; ++oi) {  // uniform over the wavefront (one scene)
+      const uint32_t oidx = sc.obstacle_begin + oi;
+      const DevObstacle& o = obstacles[oidx];
+      bool contact = false;
+      float dist = 0.0f;
+      float3 nrm = make_float3(0.f, 0.f, 0.f);
+      const float3 before = v;
+      if (live && obstacle_may_contact(o, pos, sc.h)) {
         obstacle_surface(o, pos, dist, nrm);
-        if (dist <= sc.h * (1.0f + 1e-5f)) v = contact_velocity(o, pos, v, nrm);
+        if (dist <= sc.h * (1.0f + 1e-5f)) {
+          contact = true;
+          v = contact_velocity(o, pos, v, nrm);
+        }
+      }
+      if (REACT && __any(contact)) {
+        const float3 J = contact ? (before - v) * m : make_float3(0.f, 0.f, 0.f);
+        const float3 point = pos - nrm * dist;
+        const float3 L = contact ? cross3(point - make_float3(o.center[0], o.center[1], o.center[2]), J)
+                                 : make_float3(0.f, 0.f, 0.f);
+        add_reaction(t, o, oidx, s % kReactBins, J, L);
       }
     }
     t.grid_vel[(size_t)s * kBlockNodes + lane] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -1088,10 +1137,15 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // ---------------------------------------------------------------------------
 // 10. G2P (world.rs:419-464; gpu.wgsl:821-1019)
 // ---------------------------------------------------------------------------
+// REACT = 1 runs with one 64-lane wavefront per workgroup: each lane adds its
+// particles' obstacle impulses into its own LDS column (no atomics), and the
+// columns are reduced in a fixed order at the end of the block.
+template <int REACT>
 __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* classes, const DevScene* scenes,
                                              const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
+  __shared__ float racc[REACT ? kMaxReactObstacles * 6 * 64 : 1];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   // The block hash is not read after the scatter of this substep: clear it
   // here for the next substep's binning (saves a launch).
@@ -1122,6 +1176,9 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
       const int4 key = t.grid_key[own];
       const DevScene& sc = scenes[(uint32_t)key.w >> 16];
       const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
+      const int nobs = REACT ? min(sc.obstacle_count, kMaxReactObstacles) : 0;
+      if (REACT)
+        for (int q = 0; q < nobs * 6; ++q) racc[q * 64 + threadIdx.x] = 0.0f;
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
       const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
       bool failed = false, bad = false;
@@ -1208,7 +1265,19 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
           obstacle_surface(o, x, dist, nrm);
           if (dist < radius) {
             x = x + nrm * (radius - dist);
+            const float3 before = v;
             v = contact_velocity(o, x, v, nrm);
+            if (REACT && oi < nobs) {
+              const float3 J = (before - v) * pm.w;
+              const float3 L = cross3(x - nrm * radius - make_float3(o.center[0], o.center[1], o.center[2]), J);
+              float* acc = racc + oi * 6 * 64 + threadIdx.x;
+              acc[0] += J.x;
+              acc[64] += J.y;
+              acc[128] += J.z;
+              acc[192] += L.x;
+              acc[256] += L.y;
+              acc[320] += L.z;
+            }
           }
         }
         // F <- (I + C dt) F
@@ -1256,8 +1325,149 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
       }
       flag_scene_error(t, (uint32_t)key.w >> 16, failed, ERR_RESIDENT);
       flag_error(t.counters, bad, ERR_INTERNAL);
+      if (REACT) {
+        for (int oi = 0; oi < nobs; ++oi) {
+          const float* acc = racc + oi * 6 * 64 + threadIdx.x;
+          const float3 J = make_float3(acc[0], acc[64], acc[128]);
+          const float3 L = make_float3(acc[192], acc[256], acc[320]);
+          const bool any = J.x != 0.f || J.y != 0.f || J.z != 0.f || L.x != 0.f || L.y != 0.f || L.z != 0.f;
+          if (__any(any)) add_reaction(t, obstacles[sc.obstacle_begin + oi], sc.obstacle_begin + oi, k % kReactBins, J, L);
+        }
+      }
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reactions and rigid coupling
+// ---------------------------------------------------------------------------
+// Folds each obstacle's bins (exact int64 sums) into this substep's reaction
+// and the running total since the last readback, and clears the bins.
+__global__ __launch_bounds__(64) void k_react_finalize(Topo t, const DevObstacle* obstacles) {
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < t.obstacle_total; o += gridDim.x * blockDim.x) {
+    const DevObstacle& ob = obstacles[o];
+    long long* bins = t.react_bins + (size_t)o * kReactBins * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      long long sum = 0;
+      for (int b = 0; b < kReactBins; ++b) {
+        sum += bins[b * 6 + c];
+        bins[b * 6 + c] = 0;
+      }
+      const double value = ldexp((double)sum, -(c < 3 ? ob.rshift_lin : ob.rshift_ang));
+      t.react_sub[o * 6 + c] = (float)value;
+      t.react_total[o * 6 + c] += value;
+    }
+  }
+}
+
+__device__ __forceinline__ float4 quat_mul(float4 a, float4 b) {
+  const float3 av = make_float3(a.x, a.y, a.z), bv = make_float3(b.x, b.y, b.z);
+  const float3 v = bv * a.w + av * b.w + cross3(av, bv);
+  return make_float4(v.x, v.y, v.z, a.w * b.w - dot3(av, bv));
+}
+
+// Obstacle pose from its rigid body (gpu_rigid_sync.wgsl:43-62): center =
+// body position + R local_center, orientation = q_body q_local, surface
+// velocity of the body at the center.
+__global__ __launch_bounds__(64) void k_rigid_sync(const DevRigid* bodies, const DevMapping* maps,
+                                                  DevObstacle* obstacles, uint32_t count) {
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < count; o += gridDim.x * blockDim.x) {
+    const DevMapping mp = maps[o];
+    if (mp.body < 0) continue;
+    const DevRigid st = bodies[mp.body];
+    const float3 offset = qrot(st.q, make_float3(mp.local_center[0], mp.local_center[1], mp.local_center[2]));
+    DevObstacle& ob = obstacles[o];
+    ob.center[0] = st.pos[0] + offset.x;
+    ob.center[1] = st.pos[1] + offset.y;
+    ob.center[2] = st.pos[2] + offset.z;
+    const float4 q = quat_mul(make_float4(st.q[0], st.q[1], st.q[2], st.q[3]),
+                              make_float4(mp.local_q[0], mp.local_q[1], mp.local_q[2], mp.local_q[3]));
+    ob.q[0] = q.x;
+    ob.q[1] = q.y;
+    ob.q[2] = q.z;
+    ob.q[3] = q.w;
+    const float3 w = make_float3(st.ang[0], st.ang[1], st.ang[2]);
+    const float3 lv = make_float3(st.lin[0], st.lin[1], st.lin[2]) + cross3(w, offset);
+    ob.lin[0] = lv.x;
+    ob.lin[1] = lv.y;
+    ob.lin[2] = lv.z;
+    ob.ang[0] = w.x;
+    ob.ang[1] = w.y;
+    ob.ang[2] = w.z;
+  }
+}
+
+// Applies this substep's reactions of a body's obstacles to it
+// (gpu_rigid_reaction.wgsl:72-150): J / m, R diag(I^-1) R^T L, then, for an
+// MPM-owned step, gravity, the speed cap and the pose update.
+__global__ __launch_bounds__(64) void k_rigid_apply(Topo t, const DevScene* scenes, DevRigid* bodies,
+                                                   const DevMapping* maps, const DevObstacle* obstacles,
+                                                   const uint32_t* body_scene, uint32_t count) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < count; b += gridDim.x * blockDim.x) {
+    DevRigid st = bodies[b];
+    const float inv_mass = st.inv_mass;
+    if (inv_mass <= 0.0f) continue;
+    const DevScene& sc = scenes[body_scene[b]];
+    const float3 pos = make_float3(st.pos[0], st.pos[1], st.pos[2]);
+    float3 J = make_float3(0.f, 0.f, 0.f), L = J;
+    bool mapped = false;
+    for (int oi = 0; oi < sc.obstacle_count; ++oi) {
+      const uint32_t o = sc.obstacle_begin + oi;
+      if (maps[o].body != (int32_t)b) continue;
+      mapped = true;
+      const float* r = t.react_sub + o * 6;
+      const float3 imp = make_float3(r[0], r[1], r[2]);
+      J = J + imp;
+      L = L + make_float3(r[3], r[4], r[5]) +
+          cross3(make_float3(obstacles[o].center[0], obstacles[o].center[1], obstacles[o].center[2]) - pos, imp);
+    }
+    const bool finite = fabsf(J.x) < 1e30f && fabsf(J.y) < 1e30f && fabsf(J.z) < 1e30f && fabsf(L.x) < 1e30f &&
+                        fabsf(L.y) < 1e30f && fabsf(L.z) < 1e30f;
+    if (!finite || !mapped) continue;
+    if (dot3(J, J) == 0.0f && dot3(L, L) == 0.0f && !sc.rigid_integrate) continue;
+    const float qc[4] = {-st.q[0], -st.q[1], -st.q[2], st.q[3]};
+    const float3 torque_body = qrot(qc, L);
+    const float3 dw = qrot(st.q, make_float3(torque_body.x * st.inv_inertia[0], torque_body.y * st.inv_inertia[1],
+                                             torque_body.z * st.inv_inertia[2]));
+    float3 lin = make_float3(st.lin[0], st.lin[1], st.lin[2]) + J * inv_mass;
+    const float3 ang = make_float3(st.ang[0], st.ang[1], st.ang[2]) + dw;
+    if (sc.rigid_integrate)
+      lin = lin + make_float3(sc.rigid_gravity[0], sc.rigid_gravity[1], sc.rigid_gravity[2]) * sc.dt;
+    if (sc.rigid_max_speed > 0.0f) {
+      const float speed = len3(lin);
+      if (speed > sc.rigid_max_speed) lin = lin * (sc.rigid_max_speed / speed);
+    }
+    st.lin[0] = lin.x;
+    st.lin[1] = lin.y;
+    st.lin[2] = lin.z;
+    st.ang[0] = ang.x;
+    st.ang[1] = ang.y;
+    st.ang[2] = ang.z;
+    if (sc.rigid_integrate) {
+      st.pos[0] += lin.x * sc.dt;
+      st.pos[1] += lin.y * sc.dt;
+      st.pos[2] += lin.z * sc.dt;
+      // q <- normalize(exp(w dt) q), exp as in quat_from_rotation
+      const float3 rot = ang * sc.dt;
+      const float angle = len3(rot);
+      float4 dq;
+      if (angle < 1e-5f) {
+        const float n = sqrtf(0.25f * dot3(rot, rot) + 1.0f);
+        dq = make_float4(0.5f * rot.x / n, 0.5f * rot.y / n, 0.5f * rot.z / n, 1.0f / n);
+      } else {
+        const float sh = sinf(0.5f * angle) / angle;
+        dq = make_float4(rot.x * sh, rot.y * sh, rot.z * sh, cosf(0.5f * angle));
+      }
+      float4 q = quat_mul(dq, make_float4(st.q[0], st.q[1], st.q[2], st.q[3]));
+      const float qn = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+      st.q[0] = q.x / qn;
+      st.q[1] = q.y / qn;
+      st.q[2] = q.z / qn;
+      st.q[3] = q.w / qn;
+    }
+    bodies[b] = st;
   }
 }
 
@@ -1364,16 +1574,34 @@ void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene
 #undef TMPM_P2G_T
 #undef TMPM_P2G
 }
-void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, hipStream_t s) {
+void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react,
+                        hipStream_t s) {
   uint32_t blocks = grid_for(t.grid_cap, 4, 16384);
-  if (fixed64)
-    hipLaunchKernelGGL(k_grid_update<1>, dim3(blocks), dim3(256), 0, s, t, sc, o);
-  else
-    hipLaunchKernelGGL(k_grid_update<0>, dim3(blocks), dim3(256), 0, s, t, sc, o);
+  if (fixed64) {
+    if (react) hipLaunchKernelGGL((k_grid_update<1, 1>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+    else hipLaunchKernelGGL((k_grid_update<1, 0>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+  } else {
+    if (react) hipLaunchKernelGGL((k_grid_update<0, 1>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+    else hipLaunchKernelGGL((k_grid_update<0, 0>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+  }
 }
 void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                uint32_t blocks, uint32_t threads, hipStream_t s) {
-  hipLaunchKernelGGL(k_g2p, dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+                uint32_t blocks, uint32_t threads, int react, hipStream_t s) {
+  if (react)  // one wavefront per workgroup: the lane-private reaction columns assume 64 lanes
+    hipLaunchKernelGGL(k_g2p<1>, dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
+  else
+    hipLaunchKernelGGL(k_g2p<0>, dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+}
+void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s) {
+  hipLaunchKernelGGL(k_react_finalize, dim3(grid_for(t.obstacle_total, 64, 1024)), dim3(64), 0, s, t, o);
+}
+void launch_rigid_sync(const DevRigid* bodies, const DevMapping* maps, DevObstacle* o, uint32_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_rigid_sync, dim3(grid_for(count, 64, 1024)), dim3(64), 0, s, bodies, maps, o, count);
+}
+void launch_rigid_apply(const Topo& t, const DevScene* sc, DevRigid* bodies, const DevMapping* maps,
+                        const DevObstacle* o, const uint32_t* body_scene, uint32_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_rigid_apply, dim3(grid_for(count, 64, 1024)), dim3(64), 0, s, t, sc, bodies, maps, o,
+                     body_scene, count);
 }
 void launch_snapshot(const Parts& p, float4* out, hipStream_t s) {
   hipLaunchKernelGGL(k_snapshot, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, out);

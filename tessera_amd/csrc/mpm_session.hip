@@ -380,6 +380,13 @@ struct SceneState {
   uint32_t first = 0;        // insertion index of its particle 0 on the device
   uint32_t count = 0;        // particles on the device
   double stable = -1.0;      // world.stable_timestep(), cached; < 0 when stale
+  // rigid coupling (bodies and one mapping per obstacle) and reactions
+  std::vector<tmpm_rigid_body> bodies;
+  std::vector<tmpm_rigid_mapping> maps;
+  double rigid_gravity[3] = {0.0, 0.0, 0.0};
+  double rigid_max_speed = 0.0;
+  bool rigid_integrate = false;
+  std::vector<double> reactions;  // 6 per obstacle, substeps of the last synchronize
 
   bool stepping() const { return active && status == TMPM_OK; }
   double stable_timestep() {
@@ -413,6 +420,11 @@ struct tmpm_session {
   DevBuf force;            // float4 per particle, insertion order
   DevBuf classes, scene_plain, scene_force, obstacles, scene_err;
   uint32_t obstacle_cap = 0, scene_cap = 0;
+  // reactions and rigid coupling
+  bool reactions_on = false;
+  DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
+  uint32_t obstacle_total = 0, react_cap = 0, body_total = 0;
+  bool react_active() const { return reactions_on || body_total > 0; }
   std::vector<DevClass> host_classes;
   uint32_t substeps_since_reorder = 1u << 30;
   uint32_t reorder_interval = 8;
@@ -422,7 +434,7 @@ struct tmpm_session {
       pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask,
     bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum;
   Topo topo{};
-  uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 128;
+  uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 64;
   int p2g_copies = 2;
   bool topo_valid = false;  // topology arrays describe the current device state
   bool table_dirty = true;  // block hash needs clearing before the next binning
@@ -524,13 +536,79 @@ bool pack_obstacle(const tmpm_obstacle& o, DevObstacle& d) {
   return ok;
 }
 
+bool pack_rigid(const tmpm_rigid_body& b, DevRigid& d) {
+  std::memset(&d, 0, sizeof(d));
+  bool ok = finite_f32(b.inverse_mass, d.inv_mass) && b.inverse_mass >= 0.0;
+  for (int k = 0; k < 3; ++k) {
+    ok = finite_f32(b.position[k], d.pos[k]) && ok;
+    ok = finite_f32(b.linear_velocity[k], d.lin[k]) && ok;
+    ok = finite_f32(b.angular_velocity[k], d.ang[k]) && ok;
+    ok = finite_f32(b.inverse_inertia[k], d.inv_inertia[k]) && b.inverse_inertia[k] >= 0.0 && ok;
+  }
+  double qn = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    ok = finite_f32(b.orientation[k], d.q[k]) && ok;
+    qn += b.orientation[k] * b.orientation[k];
+  }
+  return ok && std::fabs(qn - 1.0) < 1e-3;
+}
+
+// Rigid bodies of all stepping scenes, concatenated; each scene keeps its range.
+tmpm_status upload_rigid(tmpm_session* s) {
+  std::vector<DevRigid> packed;
+  std::vector<uint32_t> owner;
+  for (uint32_t k = 0; k < s->scenes.size(); ++k) {
+    SceneState& scene = s->scenes[k];
+    scene.dev.body_begin = (int32_t)packed.size();
+    scene.dev.body_count = 0;
+    scene.dev.rigid_integrate = 0;
+    if (!scene.stepping()) continue;
+    for (const auto& b : scene.bodies) {
+      DevRigid d;
+      if (!pack_rigid(b, d)) return fail(TMPM_INVALID_INPUT, "rigid body state not representable");
+      packed.push_back(d);
+      owner.push_back(k);
+    }
+    scene.dev.body_count = (int32_t)scene.bodies.size();
+    scene.dev.rigid_integrate = scene.rigid_integrate ? 1 : 0;
+    for (int a = 0; a < 3; ++a)
+      if (!finite_f32(scene.rigid_gravity[a], scene.dev.rigid_gravity[a]))
+        return fail(TMPM_INVALID_INPUT, "rigid gravity not representable");
+    if (!finite_f32(scene.rigid_max_speed, scene.dev.rigid_max_speed) || scene.rigid_max_speed < 0.0)
+      return fail(TMPM_INVALID_INPUT, "rigid speed cap invalid");
+  }
+  s->body_total = (uint32_t)packed.size();
+  if (packed.empty()) return TMPM_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(s->rigid.alloc(sizeof(DevRigid) * packed.size()));
+  HIP_TRY(s->body_scene.alloc(4 * owner.size()));
+  HIP_TRY(hipMemcpy(s->rigid.ptr, packed.data(), sizeof(DevRigid) * packed.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->body_scene.ptr, owner.data(), 4 * owner.size(), hipMemcpyHostToDevice));
+  return TMPM_OK;
+}
+
+// Largest distance from an obstacle's center to a point within one cell of
+// it: the lever arm that bounds its angular reaction contributions.
+double obstacle_reach(const tmpm_obstacle& o, double h) {
+  double r = 0.0;
+  for (int k = 0; k < 4; ++k) r += std::fabs(o.dims[k]);
+  for (int k = 0; k < 9; ++k) r = std::max(r, std::fabs(o.vertices[k]));
+  return 2.0 * r + 2.0 * h;
+}
+
 // Obstacles of all active scenes, concatenated; each scene keeps its range.
 tmpm_status upload_obstacles(tmpm_session* s) {
   std::vector<DevObstacle> packed;
+  std::vector<DevMapping> maps;
   for (auto& scene : s->scenes) {
     scene.dev.obstacle_begin = (int32_t)packed.size();
     scene.dev.obstacle_count = 0;
     if (!scene.stepping()) continue;
+    if (s->react_active() && scene.world.obstacles.size() > (size_t)kMaxReactObstacles)
+      return fail(TMPM_CAPACITY, "at most 8 obstacles per scene while reactions are computed");
+    const bool mapped = !scene.bodies.empty();
+    if (mapped && scene.maps.size() != scene.world.obstacles.size())
+      return fail(TMPM_INVALID_INPUT, "one rigid mapping per obstacle is required");
     for (auto& ob : scene.world.obstacles) {
       const tmpm_obstacle& o = ob.o;
       if (o.shape == TMPM_SHAPE_TRIANGLE_PRISM || o.shape == TMPM_SHAPE_CONVEX)
@@ -540,7 +618,29 @@ tmpm_status upload_obstacles(tmpm_session* s) {
         return fail(TMPM_RESIDENT_UNAVAILABLE, "CPIC obstacle groups are not supported by the HIP backend");
       DevObstacle d;
       if (!pack_obstacle(o, d)) return fail(TMPM_INVALID_INPUT, "obstacle is not representable in f32");
+      // reaction units: momentum at the scene's P2G scale; angular impulse
+      // scaled down by the lever-arm bound so both stay well inside int64
+      const int lin = std::max(-90, std::min(150, scene.dev.mom_shift));
+      const int ang = std::max(-90, std::min(150, lin - (int)std::ceil(std::log2(obstacle_reach(o, scene.world.params.cell_width)))));
+      d.rshift_lin = lin;
+      d.rshift_ang = ang;
+      d.rscale_lin = std::ldexp(1.0f, lin - 32);
+      d.rscale_ang = std::ldexp(1.0f, ang - 32);
+      DevMapping m{};
+      m.body = -1;
+      if (mapped) {
+        const tmpm_rigid_mapping& src = scene.maps[packed.size() - scene.dev.obstacle_begin];
+        if (src.body != UINT32_MAX) {
+          if (src.body >= scene.bodies.size()) return fail(TMPM_INVALID_INPUT, "rigid mapping names no body");
+          m.body = scene.dev.body_begin + (int32_t)src.body;
+        }
+        bool ok = true;
+        for (int k = 0; k < 3; ++k) ok = finite_f32(src.local_center[k], m.local_center[k]) && ok;
+        for (int k = 0; k < 4; ++k) ok = finite_f32(src.local_orientation[k], m.local_q[k]) && ok;
+        if (!ok) return fail(TMPM_INVALID_INPUT, "rigid mapping not representable");
+      }
       packed.push_back(d);
+      maps.push_back(m);
     }
     scene.dev.obstacle_count = (int32_t)(packed.size() - scene.dev.obstacle_begin);
     if (scene.dev.obstacle_count > kMaxObstacles) return fail(TMPM_CAPACITY, "too many obstacles");
@@ -548,10 +648,29 @@ tmpm_status upload_obstacles(tmpm_session* s) {
   if (packed.size() > s->obstacle_cap || !s->obstacles.ptr) {
     s->obstacle_cap = std::max<uint32_t>(16, (uint32_t)packed.size());
     HIP_TRY(s->obstacles.alloc(sizeof(DevObstacle) * s->obstacle_cap));
+    HIP_TRY(s->rigid_maps.alloc(sizeof(DevMapping) * s->obstacle_cap));
   }
   HIP_TRY(hipStreamSynchronize(s->stream));
-  if (!packed.empty())
+  if (!packed.empty()) {
     HIP_TRY(hipMemcpy(s->obstacles.ptr, packed.data(), packed.size() * sizeof(DevObstacle), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->rigid_maps.ptr, maps.data(), maps.size() * sizeof(DevMapping), hipMemcpyHostToDevice));
+  }
+  s->obstacle_total = (uint32_t)packed.size();
+  s->topo.obstacle_total = s->obstacle_total;
+  if (s->react_active() && s->obstacle_total > 0) {
+    if (s->obstacle_total > s->react_cap || !s->react_bins.ptr) {
+      s->react_cap = std::max<uint32_t>(16, s->obstacle_total);
+      HIP_TRY(s->react_bins.alloc(8ull * kReactBins * 6 * s->react_cap));
+      HIP_TRY(s->react_sub.alloc(4ull * 6 * s->react_cap));
+      HIP_TRY(s->react_total.alloc(8ull * 6 * s->react_cap));
+    }
+    HIP_TRY(hipMemset(s->react_bins.ptr, 0, 8ull * kReactBins * 6 * s->react_cap));
+    HIP_TRY(hipMemset(s->react_sub.ptr, 0, 4ull * 6 * s->react_cap));
+    HIP_TRY(hipMemset(s->react_total.ptr, 0, 8ull * 6 * s->react_cap));
+    s->topo.react_bins = s->react_bins.as<long long>();
+    s->topo.react_sub = s->react_sub.as<float>();
+    s->topo.react_total = s->react_total.as<double>();
+  }
   return TMPM_OK;
 }
 
@@ -793,6 +912,7 @@ tmpm_status prepare(tmpm_session* s) {
     scene.first = (uint32_t)std::min<size_t>(total, UINT32_MAX);
     scene.count = scene.stepping() ? (uint32_t)std::min<size_t>(scene.world.size(), UINT32_MAX) : 0;
     total += scene.count;
+    if (!scene.stepping()) continue;
     scene.max_mass = 0.0;
     for (size_t i = 0; i < scene.count; ++i)
       if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
@@ -922,7 +1042,9 @@ tmpm_status prepare(tmpm_session* s) {
   HIP_TRY(s->classes.alloc(sizeof(DevClass) * std::max<size_t>(1, s->host_classes.size())));
   HIP_TRY(hipMemcpy(s->classes.ptr, s->host_classes.data(), sizeof(DevClass) * s->host_classes.size(),
                     hipMemcpyHostToDevice));
-  tmpm_status st = upload_obstacles(s);
+  tmpm_status st = upload_rigid(s);
+  if (st != TMPM_OK) return st;
+  st = upload_obstacles(s);
   if (st != TMPM_OK) return st;
   st = upload_scene(s);
   if (st != TMPM_OK) return st;
@@ -965,7 +1087,15 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   const DevClass* c = s->classes.as<DevClass>();
   const DevScene* sc = (apply_force ? s->scene_force : s->scene_plain).as<DevScene>();
   const DevObstacle* ob = s->obstacles.as<DevObstacle>();
-  hipError_t e = enqueue_binning(s, p);
+  const int react = s->react_active() && s->obstacle_total > 0 ? 1 : 0;
+  hipError_t e;
+  if (s->body_total > 0 && s->obstacle_total > 0 &&
+      (e = timed(s, "k_rigid_sync", [&] {
+         launch_rigid_sync(s->rigid.as<DevRigid>(), s->rigid_maps.as<DevMapping>(), s->obstacles.as<DevObstacle>(),
+                           s->obstacle_total, s->stream);
+       })) != hipSuccess)
+    return e;
+  e = enqueue_binning(s, p);
   if (e != hipSuccess) return e;
   if (s->substeps_since_reorder >= s->reorder_interval) {
     Parts q = s->parts(1 - s->cur);
@@ -982,10 +1112,21 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
                  [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_copies, s->stream); })) != hipSuccess)
     return e;
-  if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, s->stream); })) != hipSuccess)
+  if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, react, s->stream); })) !=
+      hipSuccess)
     return e;
   // The P2G kernel template takes 256-thread launch bounds; G2P uses the same.
-  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, s->stream); })) != hipSuccess) return e;
+  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, react, s->stream); })) !=
+      hipSuccess)
+    return e;
+  if (react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, ob, s->stream); })) != hipSuccess)
+    return e;
+  if (react && s->body_total > 0 &&
+      (e = timed(s, "k_rigid_apply", [&] {
+         launch_rigid_apply(t, sc, s->rigid.as<DevRigid>(), s->rigid_maps.as<DevMapping>(), ob,
+                            s->body_scene.as<uint32_t>(), s->body_total, s->stream);
+       })) != hipSuccess)
+    return e;
   s->topo_valid = false;
   return hipSuccess;
 }
@@ -1152,6 +1293,123 @@ void tmpm_context_destroy(tmpm_context* ctx) { delete ctx; }
 }  // extern "C"
 
 namespace {
+
+// Reactions, rigid bodies and the poses of body-driven obstacles, read back
+// for the scenes whose substeps were committed.
+tmpm_status read_coupling(tmpm_session* s, const std::vector<char>& commit) {
+  if (!s->prepared || s->obstacle_total == 0) return TMPM_OK;
+  if (s->react_active()) {
+    std::vector<double> total(6ull * s->obstacle_total);
+    HIP_TRY(hipMemcpy(total.data(), s->react_total.ptr, 8ull * total.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemset(s->react_total.ptr, 0, 8ull * total.size()));
+    for (size_t k = 0; k < s->scenes.size(); ++k) {
+      SceneState& scene = s->scenes[k];
+      if (!commit[k]) continue;
+      const size_t b = 6ull * scene.dev.obstacle_begin, n = 6ull * scene.dev.obstacle_count;
+      scene.reactions.assign(total.begin() + b, total.begin() + b + n);
+    }
+  }
+  if (s->body_total > 0) {
+    std::vector<DevRigid> rb(s->body_total);
+    std::vector<DevObstacle> ob(s->obstacle_total);
+    HIP_TRY(hipMemcpy(rb.data(), s->rigid.ptr, sizeof(DevRigid) * rb.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ob.data(), s->obstacles.ptr, sizeof(DevObstacle) * ob.size(), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < s->scenes.size(); ++k) {
+      SceneState& scene = s->scenes[k];
+      if (!commit[k] || scene.bodies.empty()) continue;
+      for (size_t i = 0; i < scene.bodies.size(); ++i) {
+        const DevRigid& d = rb[scene.dev.body_begin + i];
+        tmpm_rigid_body& b = scene.bodies[i];
+        for (int a = 0; a < 3; ++a) {
+          b.position[a] = d.pos[a];
+          b.linear_velocity[a] = d.lin[a];
+          b.angular_velocity[a] = d.ang[a];
+        }
+        for (int a = 0; a < 4; ++a) b.orientation[a] = d.q[a];
+      }
+      for (size_t j = 0; j < scene.world.obstacles.size() && j < scene.maps.size(); ++j) {
+        if (scene.maps[j].body == UINT32_MAX) continue;
+        const DevObstacle& d = ob[scene.dev.obstacle_begin + j];
+        tmpm_obstacle& o = scene.world.obstacles[j].o;
+        for (int a = 0; a < 3; ++a) {
+          o.center[a] = d.center[a];
+          o.linear_velocity[a] = d.lin[a];
+          o.angular_velocity[a] = d.ang[a];
+        }
+        for (int a = 0; a < 4; ++a) o.orientation[a] = d.q[a];
+      }
+    }
+  }
+  return TMPM_OK;
+}
+
+tmpm_status read_scene_reactions(const SceneState& scene, double* out, uint32_t cap, uint32_t* count_out) {
+  const uint32_t n = (uint32_t)scene.world.obstacles.size();
+  if (count_out) *count_out = n;
+  if (cap < n || (n > 0 && !out)) return fail(TMPM_CAPACITY, "reaction output too small");
+  for (uint32_t i = 0; i < 6 * n; ++i) out[i] = i < scene.reactions.size() ? scene.reactions[i] : 0.0;
+  return TMPM_OK;
+}
+
+tmpm_status read_scene_bodies(const SceneState& scene, tmpm_rigid_body* out, uint32_t cap, uint32_t* count_out) {
+  const uint32_t n = (uint32_t)scene.bodies.size();
+  if (count_out) *count_out = n;
+  if (cap < n || (n > 0 && !out)) return fail(TMPM_CAPACITY, "rigid body output too small");
+  for (uint32_t i = 0; i < n; ++i) out[i] = scene.bodies[i];
+  return TMPM_OK;
+}
+
+tmpm_status read_scene_obstacles(const SceneState& scene, tmpm_obstacle* out, uint32_t cap, uint32_t* count_out) {
+  const uint32_t n = (uint32_t)scene.world.obstacles.size();
+  if (count_out) *count_out = n;
+  if (cap < n || (n > 0 && !out)) return fail(TMPM_CAPACITY, "obstacle output too small");
+  for (uint32_t i = 0; i < n; ++i) {
+    out[i] = scene.world.obstacles[i].o;
+    out[i].planes = nullptr;
+  }
+  return TMPM_OK;
+}
+
+tmpm_status set_scene_rigid(tmpm_session* s, uint32_t k, const tmpm_rigid_body* bodies, uint32_t body_count,
+                            const tmpm_rigid_mapping* maps, uint32_t map_count, const double gravity[3],
+                            double max_speed, int32_t integrate) {
+  SceneState& scene = s->scenes[k];
+  if (!scene.stepping()) return fail(scene.status == TMPM_OK ? TMPM_INVALID_INPUT : scene.status, "scene is stopped");
+  if ((body_count > 0 && (!bodies || !maps || !gravity)) ) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (body_count > 0 && map_count != scene.world.obstacles.size())
+    return fail(TMPM_INVALID_INPUT, "one rigid mapping per obstacle is required");
+  if (body_count > 0) {
+    if (!std::isfinite(max_speed) || max_speed < 0.0) return fail(TMPM_INVALID_INPUT, "invalid speed cap");
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(gravity[a])) return fail(TMPM_INVALID_INPUT, "invalid gravity");
+    for (uint32_t i = 0; i < body_count; ++i) {
+      DevRigid d;
+      if (!pack_rigid(bodies[i], d)) return fail(TMPM_INVALID_INPUT, "invalid rigid body");
+    }
+    for (uint32_t j = 0; j < map_count; ++j)
+      if (maps[j].body != UINT32_MAX && maps[j].body >= body_count)
+        return fail(TMPM_INVALID_INPUT, "rigid mapping names no body");
+  }
+  std::vector<SceneState> previous = s->scenes;
+  scene.bodies.assign(bodies, bodies + body_count);
+  scene.maps.assign(maps, maps + (body_count > 0 ? map_count : 0));
+  if (body_count > 0) {
+    for (int a = 0; a < 3; ++a) scene.rigid_gravity[a] = gravity[a];
+    scene.rigid_max_speed = max_speed;
+    scene.rigid_integrate = integrate != 0;
+  }
+  if (!s->prepared) return TMPM_OK;
+  tmpm_status st = upload_rigid(s);
+  if (st == TMPM_OK) st = upload_obstacles(s);
+  if (st == TMPM_OK) st = upload_scene(s);
+  if (st != TMPM_OK) {
+    std::string msg = g_last_error;
+    s->scenes.swap(previous);
+    if (upload_rigid(s) != TMPM_OK || upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
+    g_last_error = msg;
+  }
+  return st;
+}
 
 tmpm_status check_world_desc(const tmpm_world_desc* wd) {
   if (!wd) return fail(TMPM_INVALID_INPUT, "null world");
@@ -1380,6 +1638,7 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
     return fail(scene_st[0], "MPM integration produced a non-finite state");
   }
   bool stopped = false;
+  std::vector<char> commit(ns, 0);
   for (size_t k = 0; k < ns; ++k) {
     SceneState& scene = s->scenes[k];
     if (!scene.stepping()) continue;
@@ -1392,6 +1651,12 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
     const uint64_t before = scene.world.substeps;
     updated[k].substeps = before + steps < before ? UINT64_MAX : before + steps;
     scene.set_world(updated[k]);
+    commit[k] = 1;
+  }
+  st = read_coupling(s, commit);
+  if (st != TMPM_OK) {
+    s->valid = false;
+    return st;
   }
   if (stopped) {
     // drop the stopped scenes' particles from the device
@@ -1590,6 +1855,78 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
   if (st != TMPM_OK) return st;
   if (scene_out) *scene_out = slot;
   return TMPM_OK;
+}
+
+tmpm_status tmpm_session_set_reactions(tmpm_session* s, int32_t enabled) {
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  const bool was = s->reactions_on;
+  s->reactions_on = enabled != 0;
+  if (!s->prepared || was == s->reactions_on) return TMPM_OK;
+  st = upload_obstacles(s);
+  if (st == TMPM_OK) st = upload_scene(s);
+  if (st != TMPM_OK) {
+    std::string msg = g_last_error;
+    s->reactions_on = was;
+    if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
+    g_last_error = msg;
+  }
+  return st;
+}
+
+tmpm_status tmpm_session_read_reactions(const tmpm_session* s, double* out, uint32_t cap, uint32_t* count_out) {
+  if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
+  return read_scene_reactions(s->scenes[0], out, cap, count_out);
+}
+
+tmpm_status tmpm_session_set_rigid_bodies(tmpm_session* s, const tmpm_rigid_body* bodies, uint32_t body_count,
+                                          const tmpm_rigid_mapping* mappings, uint32_t mapping_count,
+                                          const double gravity[3], double max_linear_speed, int32_t integrate) {
+  tmpm_status st = check_editable(s);
+  if (st != TMPM_OK) return st;
+  return set_scene_rigid(s, 0, bodies, body_count, mappings, mapping_count, gravity, max_linear_speed, integrate);
+}
+
+tmpm_status tmpm_session_read_rigid_bodies(const tmpm_session* s, tmpm_rigid_body* out, uint32_t cap,
+                                           uint32_t* count_out) {
+  if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
+  return read_scene_bodies(s->scenes[0], out, cap, count_out);
+}
+
+tmpm_status tmpm_session_read_obstacles(const tmpm_session* s, tmpm_obstacle* out, uint32_t cap, uint32_t* count_out) {
+  if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
+  return read_scene_obstacles(s->scenes[0], out, cap, count_out);
+}
+
+tmpm_status tmpm_batch_read_reactions(const tmpm_session* s, uint32_t scene, double* out, uint32_t cap,
+                                      uint32_t* count_out) {
+  tmpm_status st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return read_scene_reactions(s->scenes[scene], out, cap, count_out);
+}
+
+tmpm_status tmpm_batch_set_rigid_bodies(tmpm_session* s, uint32_t scene, const tmpm_rigid_body* bodies,
+                                        uint32_t body_count, const tmpm_rigid_mapping* mappings,
+                                        uint32_t mapping_count, const double gravity[3], double max_linear_speed,
+                                        int32_t integrate) {
+  tmpm_status st = check_editable(s);
+  if (st == TMPM_OK) st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return set_scene_rigid(s, scene, bodies, body_count, mappings, mapping_count, gravity, max_linear_speed, integrate);
+}
+
+tmpm_status tmpm_batch_read_rigid_bodies(const tmpm_session* s, uint32_t scene, tmpm_rigid_body* out, uint32_t cap,
+                                         uint32_t* count_out) {
+  tmpm_status st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return read_scene_bodies(s->scenes[scene], out, cap, count_out);
+}
+
+tmpm_status tmpm_batch_read_obstacles(const tmpm_session* s, uint32_t scene, tmpm_obstacle* out, uint32_t cap,
+                                      uint32_t* count_out) {
+  tmpm_status st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  return read_scene_obstacles(s->scenes[scene], out, cap, count_out);
 }
 
 tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene) {
