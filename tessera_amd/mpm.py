@@ -162,6 +162,26 @@ class CTopologyStats(C.Structure):
     ]
 
 
+class CRigidBody(C.Structure):
+    _fields_ = [
+        ("position", C.c_double * 3),
+        ("inverse_mass", C.c_double),
+        ("orientation", C.c_double * 4),
+        ("linear_velocity", C.c_double * 3),
+        ("angular_velocity", C.c_double * 3),
+        ("inverse_inertia", C.c_double * 3),
+    ]
+
+
+class CRigidMapping(C.Structure):
+    _fields_ = [
+        ("body", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("local_center", C.c_double * 3),
+        ("local_orientation", C.c_double * 4),
+    ]
+
+
 # ---------------------------------------------------------------------------
 # Library loading
 # ---------------------------------------------------------------------------
@@ -225,6 +245,22 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_batch_set_obstacles": (C.c_int, [vp, C.c_uint32, C.POINTER(CObstacle), C.c_uint32]),
         "tmpm_batch_add_scene": (C.c_int, [vp, C.POINTER(CWorld), C.c_double, C.POINTER(C.c_uint32)]),
         "tmpm_batch_remove_scene": (C.c_int, [vp, C.c_uint32]),
+        "tmpm_session_set_reactions": (C.c_int, [vp, C.c_int32]),
+        "tmpm_session_read_reactions": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.c_uint32)]),
+        "tmpm_session_set_rigid_bodies": (
+            C.c_int, [vp, C.POINTER(CRigidBody), C.c_uint32, C.POINTER(CRigidMapping), C.c_uint32, C.c_double * 3,
+                      C.c_double, C.c_int32]),
+        "tmpm_session_read_rigid_bodies": (C.c_int, [vp, C.POINTER(CRigidBody), C.c_uint32, C.POINTER(C.c_uint32)]),
+        "tmpm_session_read_obstacles": (C.c_int, [vp, C.POINTER(CObstacle), C.c_uint32, C.POINTER(C.c_uint32)]),
+        "tmpm_batch_read_reactions": (
+            C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.c_uint32)]),
+        "tmpm_batch_set_rigid_bodies": (
+            C.c_int, [vp, C.c_uint32, C.POINTER(CRigidBody), C.c_uint32, C.POINTER(CRigidMapping), C.c_uint32,
+                      C.c_double * 3, C.c_double, C.c_int32]),
+        "tmpm_batch_read_rigid_bodies": (
+            C.c_int, [vp, C.c_uint32, C.POINTER(CRigidBody), C.c_uint32, C.POINTER(C.c_uint32)]),
+        "tmpm_batch_read_obstacles": (
+            C.c_int, [vp, C.c_uint32, C.POINTER(CObstacle), C.c_uint32, C.POINTER(C.c_uint32)]),
         "tmpm_bandwidth_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
         "tmpm_timestep_bound": (C.c_double, [C.POINTER(CMaterial), C.c_double, C.c_double * 3, C.c_double, C.c_double]),
         "tmpm_material_valid": (C.c_int32, [C.POINTER(CMaterial)]),
@@ -723,6 +759,100 @@ class MpmWorld:
 # ---------------------------------------------------------------------------
 # Device context and resident session
 # ---------------------------------------------------------------------------
+@dataclass
+class RigidBody:
+    """Rigid body state (GpuRigidBodyState, phys/src/gpu_rigid_state.rs:17-28); zero inverse
+    mass is static or kinematic, orientation is XYZW, inertia the body-frame inverse principal."""
+
+    position: np.ndarray
+    inverse_mass: float
+    orientation: np.ndarray = field(default_factory=lambda: IDENTITY_QUAT.copy())
+    linear_velocity: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    angular_velocity: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    inverse_inertia: np.ndarray = field(default_factory=lambda: np.zeros(3))
+
+    @classmethod
+    def box(cls, center, half_extents, density: float, orientation=None) -> "RigidBody":
+        """Solid box: mass and principal inertia from the box formulas."""
+        h = _vec(half_extents)
+        mass = density * 8.0 * h[0] * h[1] * h[2]
+        inertia = mass / 3.0 * np.array([h[1] ** 2 + h[2] ** 2, h[0] ** 2 + h[2] ** 2, h[0] ** 2 + h[1] ** 2])
+        return cls(_vec(center), 1.0 / mass, _q(orientation), inverse_inertia=1.0 / inertia)
+
+    def to_c(self) -> "CRigidBody":
+        c = CRigidBody()
+        c.position[:] = list(map(float, self.position))
+        c.inverse_mass = float(self.inverse_mass)
+        c.orientation[:] = list(map(float, self.orientation))
+        c.linear_velocity[:] = list(map(float, self.linear_velocity))
+        c.angular_velocity[:] = list(map(float, self.angular_velocity))
+        c.inverse_inertia[:] = list(map(float, self.inverse_inertia))
+        return c
+
+    @classmethod
+    def from_c(cls, c: "CRigidBody") -> "RigidBody":
+        return cls(np.array(c.position[:]), float(c.inverse_mass), np.array(c.orientation[:]),
+                   np.array(c.linear_velocity[:]), np.array(c.angular_velocity[:]), np.array(c.inverse_inertia[:]))
+
+    def as_dict(self) -> dict:
+        return {"position": np.array(self.position, float), "inverse_mass": float(self.inverse_mass),
+                "orientation": np.array(self.orientation, float),
+                "linear_velocity": np.array(self.linear_velocity, float),
+                "angular_velocity": np.array(self.angular_velocity, float),
+                "inverse_inertia": np.array(self.inverse_inertia, float)}
+
+
+@dataclass
+class RigidMapping:
+    """Obstacle -> body mapping (gpu_rigid_sync.rs:15-30); body None: the obstacle is not driven."""
+
+    body: int | None
+    local_center: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    local_orientation: np.ndarray = field(default_factory=lambda: IDENTITY_QUAT.copy())
+
+    def to_c(self) -> "CRigidMapping":
+        c = CRigidMapping()
+        c.body = 0xFFFFFFFF if self.body is None else int(self.body)
+        c.local_center[:] = list(map(float, self.local_center))
+        c.local_orientation[:] = list(map(float, self.local_orientation))
+        return c
+
+    def as_dict(self) -> dict:
+        return {"body": self.body, "local_center": np.array(self.local_center, float),
+                "local_orientation": np.array(self.local_orientation, float)}
+
+
+def _rigid_args(bodies, mappings):
+    cb = (CRigidBody * max(1, len(bodies)))(*[b.to_c() for b in bodies])
+    cm = (CRigidMapping * max(1, len(mappings)))(*[m.to_c() for m in mappings])
+    return cb, cm
+
+
+def _read_n(fn, ctype, cap_hint: int):
+    count = C.c_uint32()
+    cap = max(1, cap_hint)
+    while True:
+        arr = (ctype * cap)()
+        st = fn(arr, cap, C.byref(count))
+        if st == OK:
+            return arr, int(count.value)
+        if count.value > cap:
+            cap = int(count.value)
+            continue
+        _check(st)
+
+
+def _obstacle_from_c(c: "CObstacle") -> "RigidObstacle":
+    o = RigidObstacle(int(c.shape), np.array(c.center[:]), np.array(c.orientation[:]), tuple(c.dims[:]))
+    o.vertices = np.array(c.vertices[:]).reshape(3, 3) if c.shape == ObstacleShape.TRIANGLE_PRISM else None
+    o.linear_velocity = np.array(c.linear_velocity[:])
+    o.angular_velocity = np.array(c.angular_velocity[:])
+    o.friction = float(c.friction)
+    o.boundary = int(c.boundary)
+    o.cpic_group = None if c.cpic_group < 0 else int(c.cpic_group)
+    return o
+
+
 class HipMpmTransfers:
     """Device context (the reference's ``GpuMpmTransfers``)."""
 
@@ -916,6 +1046,34 @@ class HipMpmResidentSession:
                self.lib)
         return int(count.value)
 
+    # -- reactions and rigid coupling ----------------------------------------------
+    def set_reactions(self, enabled: bool) -> None:
+        """Compute obstacle reactions (step_with_obstacle_reactions, world.rs:309-330)."""
+        _check(self.lib.tmpm_session_set_reactions(self._h, 1 if enabled else 0), self.lib)
+
+    def reactions(self) -> np.ndarray:
+        """(obstacles, 6): linear and angular impulse of the last synchronized substeps."""
+        arr, n = _read_n(lambda a, c, k: self.lib.tmpm_session_read_reactions(self._h, a, c // 6, k), C.c_double,
+                         6 * max(1, len(self._world.obstacles)))
+        return np.array(arr[: 6 * n]).reshape(n, 6)
+
+    def set_rigid_bodies(self, bodies, mappings, gravity=(0.0, -9.8, 0.0), max_linear_speed: float = 0.0,
+                         integrate: bool = True) -> None:
+        """Device-resident MPM-owned rigid coupling (gpu_rigid_sync.rs:187-213)."""
+        cb, cm = _rigid_args(bodies, mappings)
+        g = (C.c_double * 3)(*map(float, gravity))
+        _check(self.lib.tmpm_session_set_rigid_bodies(self._h, cb, len(bodies), cm, len(mappings), g,
+                                                      float(max_linear_speed), 1 if integrate else 0), self.lib)
+
+    def rigid_bodies(self) -> list:
+        arr, n = _read_n(lambda a, c, k: self.lib.tmpm_session_read_rigid_bodies(self._h, a, c, k), CRigidBody, 8)
+        return [RigidBody.from_c(arr[i]) for i in range(n)]
+
+    def obstacles(self) -> list:
+        arr, n = _read_n(lambda a, c, k: self.lib.tmpm_session_read_obstacles(self._h, a, c, k), CObstacle,
+                         len(self._world.obstacles))
+        return [_obstacle_from_c(arr[i]) for i in range(n)]
+
     # -- diagnostics ------------------------------------------------------------
     def topology_stats(self) -> dict:
         st = CTopologyStats()
@@ -1067,6 +1225,31 @@ class HipMpmBatch:
         del keep
         self._worlds[int(slot.value)] = world.clone()
         return int(slot.value)
+
+    def set_reactions(self, enabled: bool) -> None:
+        _check(self.lib.tmpm_session_set_reactions(self._h, 1 if enabled else 0), self.lib)
+
+    def reactions(self, scene: int) -> np.ndarray:
+        arr, n = _read_n(lambda a, c, k: self.lib.tmpm_batch_read_reactions(self._h, int(scene), a, c // 6, k),
+                         C.c_double, 6 * max(1, len(self._worlds[int(scene)].obstacles)))
+        return np.array(arr[: 6 * n]).reshape(n, 6)
+
+    def set_rigid_bodies(self, scene: int, bodies, mappings, gravity=(0.0, -9.8, 0.0),
+                         max_linear_speed: float = 0.0, integrate: bool = True) -> None:
+        cb, cm = _rigid_args(bodies, mappings)
+        g = (C.c_double * 3)(*map(float, gravity))
+        _check(self.lib.tmpm_batch_set_rigid_bodies(self._h, int(scene), cb, len(bodies), cm, len(mappings), g,
+                                                    float(max_linear_speed), 1 if integrate else 0), self.lib)
+
+    def rigid_bodies(self, scene: int) -> list:
+        arr, n = _read_n(lambda a, c, k: self.lib.tmpm_batch_read_rigid_bodies(self._h, int(scene), a, c, k),
+                         CRigidBody, 8)
+        return [RigidBody.from_c(arr[i]) for i in range(n)]
+
+    def obstacles(self, scene: int) -> list:
+        arr, n = _read_n(lambda a, c, k: self.lib.tmpm_batch_read_obstacles(self._h, int(scene), a, c, k), CObstacle,
+                         len(self._worlds[int(scene)].obstacles))
+        return [_obstacle_from_c(arr[i]) for i in range(n)]
 
     def remove_scene(self, scene: int) -> None:
         _check(self.lib.tmpm_batch_remove_scene(self._h, int(scene)), self.lib)
