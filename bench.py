@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -108,6 +109,29 @@ def batch_leg(transfers, mode: str, steps: int) -> dict:
             "one_session_at_a_time": alone, "speedup": value / alone, "failed_scenes": failed}
 
 
+def serving_leg(transfers, mode: str, seconds: float, single_rate: float) -> dict:
+    """Continuous batching: Poisson arrivals offering 80% of the measured single-scene rate,
+    N log-uniform in [4096, 524288], 1-4 rigid boxes per scene, horizons U{50..2000} substeps,
+    admission under an HBM budget; runs until every request has completed."""
+    from tessera_amd.scenes import serving_requests
+    from tessera_amd.serving import Server
+
+    mean_n = (524288 - 4096) / math.log(524288 / 4096)
+    work = mean_n * 1025.0  # particle-updates per request
+    rate = 0.8 * single_rate / work
+    count = max(1, int(round(rate * seconds)))
+    reqs = serving_requests(count, rate, seed=7)
+    server = Server(transfers, hbm_budget=16 << 30, steps_per_tick=32, mode=mode, keep_worlds=False)
+    report = server.run(reqs)
+    server.close()
+    report.update({"workload": "serving_poisson", "arrival_rate_per_s": rate, "offered_fraction": 0.8,
+                   "steps_per_tick": 32, "hbm_budget_bytes": 16 << 30})
+    print(f"bench: serving {report['completed']}/{count} requests in {report['wall_s']:.1f}s: "
+          f"{report['value']:.4e} particle-updates/s, step latency p50 {report['step_latency_ms_p50']:.3f} ms "
+          f"p99 {report['step_latency_ms_p99']:.3f} ms", file=sys.stderr, flush=True)
+    return report
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +143,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=64, help="substeps of the batched-scenes leg (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg (0: skip)")
     args = ap.parse_args()
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -190,6 +215,10 @@ def main() -> int:
     if args.batch_steps > 0:
         batch = batch_leg(transfers, args.mode, args.batch_steps)
 
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_leg(transfers, args.mode, args.serve_seconds, value)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -250,6 +279,7 @@ def main() -> int:
         "other_modes": extra_modes,
         "cpu_baseline": cpu,
         "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

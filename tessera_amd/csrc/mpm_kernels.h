@@ -24,6 +24,10 @@ void launch_rigid_sync(const DevRigid* bodies, const DevMapping* maps, DevObstac
 void launch_rigid_apply(const Topo& t, const DevScene* sc, DevRigid* bodies, const DevMapping* maps,
                         const DevObstacle* o, const uint32_t* body_scene, uint32_t count, hipStream_t s);
 void launch_snapshot(const Parts& p, float4* out, hipStream_t s);
+void launch_snapshot_disabled(const float4* rec, uint32_t count, float4* out, hipStream_t s);
+void launch_retire(const Parts& p, uint32_t first, uint32_t count, hipStream_t s);
+void launch_renumber(uint32_t* ins_id, uint32_t n, uint32_t first, uint32_t count, hipStream_t s);
+void launch_gather_scene(const Parts& p, uint32_t first, uint32_t count, float* out, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);
 

@@ -493,6 +493,13 @@ __global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo
     q.cls[dst] = p.cls[i];
     q.ins_id[dst] = p.ins_id[i];
   }
+  // Slots past the sorted particles hold no particle after the move (retired
+  // particles were not binned): mark them so that nothing bins or exports them.
+  const uint32_t binned = min(t.counters[CNT_BINNED], tail_begin);
+  for (uint32_t i = binned + blockIdx.x * blockDim.x + threadIdx.x; i < tail_begin; i += gridDim.x * blockDim.x) {
+    q.pos_mass[i].w = 0.0f;
+    q.ins_id[i] = kInvalid;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1477,8 +1484,63 @@ __global__ __launch_bounds__(64) void k_rigid_apply(Topo t, const DevScene* scen
 __global__ __launch_bounds__(256) void k_snapshot(Parts p, float4* out) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
     uint32_t dst = p.ins_id[i];
+    if (dst == kInvalid) continue;  // retired slot
     out[2 * dst] = p.pos_mass[i];
     out[2 * dst + 1] = p.vel_vol[i];
+  }
+}
+
+// Records of disabled particles (kept on the host; mass 0): {pos, 0; vel, vol}
+// at their insertion index, stored as 3 float4 {x, y, z, index bits; ...}.
+__global__ __launch_bounds__(256) void k_snapshot_disabled(const float4* rec, uint32_t count, float4* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const float4 a = rec[2 * i], b = rec[2 * i + 1];
+    const uint32_t dst = __float_as_uint(a.w);
+    out[2 * dst] = make_float4(a.x, a.y, a.z, 0.0f);
+    out[2 * dst + 1] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batches: retiring a scene, compacting insertion indices, per-scene gather
+// ---------------------------------------------------------------------------
+// Particles of insertion range [first, first + count) leave the simulation:
+// mass 0 (binning skips them; the next reorder drops them) and no index.
+__global__ __launch_bounds__(256) void k_retire(Parts p, uint32_t first, uint32_t count) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    const uint32_t g = p.ins_id[i];
+    if (g != kInvalid && g - first < count) {
+      p.pos_mass[i].w = 0.0f;
+      p.ins_id[i] = kInvalid;
+    }
+  }
+}
+
+// Insertion indices above a removed range move down by its length.
+__global__ __launch_bounds__(256) void k_renumber(uint32_t* ins_id, uint32_t n, uint32_t first, uint32_t count) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t g = ins_id[i];
+    if (g == kInvalid) continue;
+    if (g - first < count) ins_id[i] = kInvalid;
+    else if (g >= first + count) ins_id[i] = g - count;
+  }
+}
+
+// One scene's particles in insertion order, 26 floats each: x, y, z, mass,
+// vx, vy, vz, volume, C (9, column-major), F (9).
+__global__ __launch_bounds__(256) void k_gather_scene(Parts p, uint32_t first, uint32_t count, float* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    const uint32_t g = p.ins_id[i];
+    if (g == kInvalid || g - first >= count) continue;
+    float* r = out + (size_t)(g - first) * 26;
+    const float4 a = p.pos_mass[i], b = p.vel_vol[i];
+    r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+    r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      r[8 + q] = p.C[q * p.stride + i];
+      r[17 + q] = p.F[q * p.stride + i];
+    }
   }
 }
 
@@ -1605,6 +1667,18 @@ void launch_rigid_apply(const Topo& t, const DevScene* sc, DevRigid* bodies, con
 }
 void launch_snapshot(const Parts& p, float4* out, hipStream_t s) {
   hipLaunchKernelGGL(k_snapshot, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, out);
+}
+void launch_snapshot_disabled(const float4* rec, uint32_t count, float4* out, hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_snapshot_disabled, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s, rec, count, out);
+}
+void launch_retire(const Parts& p, uint32_t first, uint32_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_retire, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, first, count);
+}
+void launch_renumber(uint32_t* ins_id, uint32_t n, uint32_t first, uint32_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_renumber, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, ins_id, n, first, count);
+}
+void launch_gather_scene(const Parts& p, uint32_t first, uint32_t count, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_scene, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, first, count, out);
 }
 void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_mask_clear, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);

@@ -380,6 +380,7 @@ struct SceneState {
   uint32_t first = 0;        // insertion index of its particle 0 on the device
   uint32_t count = 0;        // particles on the device
   double stable = -1.0;      // world.stable_timestep(), cached; < 0 when stale
+  bool device_newer = false; // batch: the device holds substeps not yet read into `world`
   // rigid coupling (bodies and one mapping per obstacle) and reactions
   std::vector<tmpm_rigid_body> bodies;
   std::vector<tmpm_rigid_mapping> maps;
@@ -412,11 +413,26 @@ struct tmpm_session {
   HostWorld& world() { return scenes[0].world; }
   const HostWorld& world() const { return scenes[0].world; }
 
-  // device state
-  uint32_t n = 0;          // particles in device storage
-  uint32_t n_enabled = 0;  // enabled particles occupy storage [0, n_enabled)
+  // device state: enabled particles only (disabled ones never change and stay
+  // on the host); SoA matrices use stride `cap`
+  uint32_t n = 0;          // particle slots in use (a retired scene's slots stay until the next reorder)
+  uint32_t n_enabled = 0;  // particles in device storage at the last upload
+  uint32_t cap = 0;        // particle capacity of the buffers (SoA stride)
+  uint32_t id_space = 0;   // insertion indices in use, all scenes, disabled particles included
   ParticleBuffers buf[2];
   int cur = 0;
+  bool compacted_last = false;  // the last substep reordered: slots past the binned count are free
+  // batches: state at the last synchronize, read per scene on demand
+  ParticleBuffers synced;
+  uint32_t synced_n = 0;
+  DevBuf staging;
+  DevBuf disabled_rec;     // snapshot records of disabled particles (2 float4 each)
+  std::vector<float4> disabled_host;
+  uint32_t force_cap = 0;
+  uint32_t last_pblocks = 0, last_gblocks = 0;  // block counts of the last synchronized substep
+  uint32_t topo_cap = 0;   // particle capacity the topology's per-particle arrays were sized for
+  std::map<std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t>, uint32_t>
+      class_ids;
   DevBuf force;            // float4 per particle, insertion order
   DevBuf classes, scene_plain, scene_force, obstacles, scene_err;
   uint32_t obstacle_cap = 0, scene_cap = 0;
@@ -456,7 +472,7 @@ struct tmpm_session {
     p.force = force.as<float4>();
     p.ins_id = b.ins_id.as<uint32_t>();
     p.n = n;
-    p.stride = n;
+    p.stride = cap;
     return p;
   }
 };
@@ -770,7 +786,7 @@ void read_tuning_env(tmpm_session* s) {
 tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_cap) {
   pblock_cap = std::min(pblock_cap, grid_cap);  // a particle block's grid slot is its id
   uint32_t buckets = std::max<uint32_t>(next_pow2(2ull * grid_cap), (uint32_t)kMinBuckets);
-  if ((uint64_t)s->n + 8ull * buckets + 1 >= 0xffffffffull) return fail(TMPM_CAPACITY, "topology id space exhausted");
+  if ((uint64_t)s->cap + 8ull * buckets + 1 >= 0xffffffffull) return fail(TMPM_CAPACITY, "topology id space exhausted");
   Topo& t = s->topo;
   HIP_TRY(s->owner.alloc(4ull * buckets));
   HIP_TRY(s->bkey.alloc(16ull * buckets));
@@ -783,14 +799,15 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->pblock_slot8.alloc(32ull * pblock_cap));
   HIP_TRY(s->pblock_start.alloc(4ull * pblock_cap));
   HIP_TRY(s->pblock_len.alloc(4ull * pblock_cap));
-  HIP_TRY(s->pslot.alloc(4ull * std::max<uint32_t>(s->n, 1)));
-  HIP_TRY(s->prank.alloc(4ull * std::max<uint32_t>(s->n, 1)));
-  HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->n, 1)));
+  s->topo_cap = s->cap;
+  HIP_TRY(s->pslot.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
+  HIP_TRY(s->prank.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
+  HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
   HIP_TRY(s->counters.alloc(4ull * CNT_COUNT));
   HIP_TRY(hipMemset(s->counters.ptr, 0, 4ull * CNT_COUNT));
   // Binning workgroups: at most ~2048 (each one's id prefix sums over all of
   // them), each over a whole number of 256-particle rounds.
-  const uint64_t nparts = std::max<uint32_t>(s->n, 1);
+  const uint64_t nparts = std::max<uint32_t>(s->cap, 1);
   const uint32_t rounds = (uint32_t)((nparts + 256ull * 2048 - 1) / (256ull * 2048));
   t.bin_span = 256u * rounds;
   t.bin_groups = (uint32_t)((nparts + t.bin_span - 1) / t.bin_span);
@@ -900,17 +917,201 @@ tmpm_status pack_class(const HostWorld& w, size_t i, uint32_t scene, float rad, 
   }
 }
 
+// Host image of some scenes' enabled particles, ready for upload.
+struct PackedParticles {
+  std::vector<float4> pm, vv;
+  std::vector<float> C, F;  // 9 per particle, particle-major
+  std::vector<uint32_t> cls, ins;
+  std::vector<float> pos32, inv_h;  // for counting blocks
+  std::vector<uint32_t> region;
+  std::vector<float4> disabled;     // snapshot records of disabled particles
+  size_t size() const { return pm.size(); }
+};
+
+uint32_t intern_class(tmpm_session* s, const DevClass& c) {
+  // runs of particles usually share a class: compare with the last one first
+  static thread_local const tmpm_session* last_s = nullptr;
+  static thread_local uint32_t last_id = 0;
+  if (last_s == s && last_id < s->host_classes.size() &&
+      std::memcmp(&s->host_classes[last_id], &c, sizeof(DevClass)) == 0)
+    return last_id;
+  auto key = std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
+                             c.region, c.fixed);
+  auto it = s->class_ids.find(key);
+  uint32_t id;
+  if (it != s->class_ids.end()) {
+    id = it->second;
+  } else {
+    id = (uint32_t)s->host_classes.size();
+    s->class_ids.emplace(key, id);
+    s->host_classes.push_back(c);
+  }
+  last_s = s;
+  last_id = id;
+  return id;
+}
+
+// Appends scene k's enabled particles to `out` (insertion index first + i)
+// and records its disabled ones for snapshots.
+tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
+  SceneState& scene = s->scenes[k];
+  const HostWorld& w = scene.world;
+  const float inv_h = scene.dev.inv_h;
+  for (size_t i = 0; i < w.size(); ++i) {
+    const bool en = w.enabled[i] != 0;
+    float f[8];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) ok = finite_f32(w.pos[3 * i + a], f[a]) && ok;
+    ok = finite_f32(en ? w.mass[i] : 0.0, f[3]) && ok;
+    for (int a = 0; a < 3; ++a) ok = finite_f32(w.vel[3 * i + a], f[4 + a]) && ok;
+    ok = finite_f32(w.rest_volume[i], f[7]) && ok;
+    float rad, damp;
+    ok = finite_f32(w.radius[i], rad) && finite_f32(w.damping[i], damp) && ok;
+    if (!ok) return fail(TMPM_INVALID_INPUT, "particle state not representable in f32");
+    if (en && (f[3] <= 0.0f || f[7] <= 0.0f))
+      return fail(TMPM_INVALID_INPUT, "enabled particle mass or volume underflows f32");
+    if (rad <= 0.0f || w.damping[i] < 0.0) return fail(TMPM_INVALID_INPUT, "radius or damping invalid");
+    const uint32_t g = scene.first + (uint32_t)i;
+    if (!en) {
+      out.disabled.push_back(make_float4(f[0], f[1], f[2], __builtin_bit_cast(float, g)));
+      out.disabled.push_back(make_float4(f[4], f[5], f[6], f[7]));
+      continue;
+    }
+    float cf[9], ff[9];
+    for (int q = 0; q < 9; ++q)
+      if (!finite_f32(w.C[9 * i + q], cf[q]) || !finite_f32(w.F[9 * i + q], ff[q]))
+        return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
+    DevClass c;
+    tmpm_status st = pack_class(w, i, k, rad, damp, c);
+    if (st != TMPM_OK) return st;
+    // base cell must be representable (gpu.rs:1143-1153)
+    for (int a = 0; a < 3; ++a) {
+      int32_t cc;
+      if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
+      out.pos32.push_back(f[a]);
+    }
+    out.region.push_back(c.region);
+    out.inv_h.push_back(inv_h);
+    out.pm.push_back(make_float4(f[0], f[1], f[2], f[3]));
+    out.vv.push_back(make_float4(f[4], f[5], f[6], f[7]));
+    out.C.insert(out.C.end(), cf, cf + 9);
+    out.F.insert(out.F.end(), ff, ff + 9);
+    out.cls.push_back(intern_class(s, c));
+    out.ins.push_back(g);
+  }
+  return TMPM_OK;
+}
+
+tmpm_status alloc_particles(ParticleBuffers& pb, uint32_t cap) {
+  HIP_TRY(pb.pos_mass.alloc(16ull * cap));
+  HIP_TRY(pb.vel_vol.alloc(16ull * cap));
+  HIP_TRY(pb.C.alloc(36ull * cap));
+  HIP_TRY(pb.F.alloc(36ull * cap));
+  HIP_TRY(pb.cls.alloc(4ull * cap));
+  HIP_TRY(pb.ins_id.alloc(4ull * cap));
+  return TMPM_OK;
+}
+
+// Copies the first n slots of `src` (stride src_cap) into `dst` (stride dst_cap).
+tmpm_status copy_particles(ParticleBuffers& dst, uint32_t dst_cap, const ParticleBuffers& src, uint32_t src_cap,
+                           uint32_t n, hipStream_t st) {
+  if (n == 0) return TMPM_OK;
+  HIP_TRY(hipMemcpyAsync(dst.pos_mass.ptr, src.pos_mass.ptr, 16ull * n, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(dst.vel_vol.ptr, src.vel_vol.ptr, 16ull * n, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(dst.cls.ptr, src.cls.ptr, 4ull * n, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(dst.ins_id.ptr, src.ins_id.ptr, 4ull * n, hipMemcpyDeviceToDevice, st));
+  for (int q = 0; q < 9; ++q) {
+    HIP_TRY(hipMemcpyAsync(dst.C.as<float>() + (size_t)q * dst_cap, src.C.as<float>() + (size_t)q * src_cap, 4ull * n,
+                           hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dst.F.as<float>() + (size_t)q * dst_cap, src.F.as<float>() + (size_t)q * src_cap, 4ull * n,
+                           hipMemcpyDeviceToDevice, st));
+  }
+  return TMPM_OK;
+}
+
+// Writes packed particles into the current buffer at slots [at, at + size).
+tmpm_status upload_particles(tmpm_session* s, const PackedParticles& pk, uint32_t at) {
+  const size_t n = pk.size();
+  if (n == 0) return TMPM_OK;
+  ParticleBuffers& pb = s->buf[s->cur];
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(pb.pos_mass.as<float4>() + at, pk.pm.data(), 16ull * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pb.vel_vol.as<float4>() + at, pk.vv.data(), 16ull * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pb.cls.as<uint32_t>() + at, pk.cls.data(), 4ull * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pb.ins_id.as<uint32_t>() + at, pk.ins.data(), 4ull * n, hipMemcpyHostToDevice));
+  std::vector<float> col(n);
+  for (int q = 0; q < 9; ++q) {
+    for (size_t i = 0; i < n; ++i) col[i] = pk.C[9 * i + q];
+    HIP_TRY(hipMemcpy(pb.C.as<float>() + (size_t)q * s->cap + at, col.data(), 4ull * n, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < n; ++i) col[i] = pk.F[9 * i + q];
+    HIP_TRY(hipMemcpy(pb.F.as<float>() + (size_t)q * s->cap + at, col.data(), 4ull * n, hipMemcpyHostToDevice));
+  }
+  return TMPM_OK;
+}
+
+// One-shot forces of scene k's particles, at their insertion indices.
+tmpm_status upload_scene_forces(tmpm_session* s, uint32_t k) {
+  SceneState& scene = s->scenes[k];
+  const HostWorld& w = scene.world;
+  if (w.size() == 0) return TMPM_OK;
+  std::vector<float4> fz(w.size());
+  scene.force_pending = false;
+  for (size_t i = 0; i < w.size(); ++i) {
+    float fx, fy, fzv;
+    if (!finite_f32(w.force[3 * i], fx) || !finite_f32(w.force[3 * i + 1], fy) || !finite_f32(w.force[3 * i + 2], fzv))
+      return fail(TMPM_INVALID_INPUT, "force not representable in f32");
+    fz[i] = make_float4(fx, fy, fzv, 0.f);
+    if (w.enabled[i] && (fx != 0.f || fy != 0.f || fzv != 0.f)) scene.force_pending = true;
+  }
+  HIP_TRY(hipMemcpy(s->force.as<float4>() + scene.first, fz.data(), 16ull * w.size(), hipMemcpyHostToDevice));
+  return TMPM_OK;
+}
+
+// Grows the force array (indexed by insertion index) to `need`, keeping its contents.
+tmpm_status ensure_force_capacity(tmpm_session* s, uint32_t need) {
+  if (need <= s->force_cap && s->force.ptr) return TMPM_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const uint32_t cap = std::max<uint32_t>(std::max<uint32_t>(need, s->force_cap + s->force_cap / 2), 1024);
+  DevBuf grown;
+  HIP_TRY(grown.alloc(16ull * cap));
+  HIP_TRY(hipMemset(grown.ptr, 0, 16ull * cap));
+  if (s->force.ptr && s->force_cap)
+    HIP_TRY(hipMemcpy(grown.ptr, s->force.ptr, 16ull * s->force_cap, hipMemcpyDeviceToDevice));
+  std::swap(s->force.ptr, grown.ptr);
+  std::swap(s->force.bytes, grown.bytes);
+  s->force_cap = cap;
+  return TMPM_OK;
+}
+
+tmpm_status upload_disabled(tmpm_session* s) {
+  if (s->disabled_host.empty()) return TMPM_OK;
+  HIP_TRY(s->disabled_rec.alloc(16ull * s->disabled_host.size()));
+  HIP_TRY(hipMemcpy(s->disabled_rec.ptr, s->disabled_host.data(), 16ull * s->disabled_host.size(),
+                    hipMemcpyHostToDevice));
+  return TMPM_OK;
+}
+
+tmpm_status upload_classes(tmpm_session* s) {
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(s->classes.alloc(sizeof(DevClass) * std::max<size_t>(1, s->host_classes.size())));
+  if (!s->host_classes.empty())
+    HIP_TRY(hipMemcpy(s->classes.ptr, s->host_classes.data(), sizeof(DevClass) * s->host_classes.size(),
+                      hipMemcpyHostToDevice));
+  return TMPM_OK;
+}
+
 // Upload all active scenes to the device (gpu.rs:1091-1600 `prepare`,
-// resident).  Device insertion order is scene by scene; storage order puts
-// the enabled particles of all scenes first.
+// resident).  Insertion indices run scene by scene; the device holds the
+// enabled particles in insertion order, batches with room to admit more.
 tmpm_status prepare(tmpm_session* s) {
   s->prepared = false;
   s->topo_valid = false;
   if (s->scenes.size() > 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
-  size_t total = 0;
+  uint64_t total = 0;
   for (auto& scene : s->scenes) {
-    scene.first = (uint32_t)std::min<size_t>(total, UINT32_MAX);
+    scene.first = (uint32_t)total;
     scene.count = scene.stepping() ? (uint32_t)std::min<size_t>(scene.world.size(), UINT32_MAX) : 0;
+    scene.device_newer = false;
     total += scene.count;
     if (!scene.stepping()) continue;
     scene.max_mass = 0.0;
@@ -920,146 +1121,61 @@ tmpm_status prepare(tmpm_session* s) {
     if (st != TMPM_OK) return st;
   }
   if (total >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
-  // global insertion index -> (scene, index)
-  std::vector<uint32_t> scene_of(total);
-  for (uint32_t k = 0; k < s->scenes.size(); ++k)
-    for (uint32_t i = 0; i < s->scenes[k].count; ++i) scene_of[s->scenes[k].first + i] = k;
-  auto local = [&](size_t g) { return g - s->scenes[scene_of[g]].first; };
-  // storage order: enabled particles first (insertion order), then disabled
-  std::vector<uint32_t> order;
-  order.reserve(total);
-  for (size_t g = 0; g < total; ++g)
-    if (s->scenes[scene_of[g]].world.enabled[local(g)]) order.push_back((uint32_t)g);
-  uint32_t n_enabled = (uint32_t)order.size();
-  for (size_t g = 0; g < total; ++g)
-    if (!s->scenes[scene_of[g]].world.enabled[local(g)]) order.push_back((uint32_t)g);
-
-  std::vector<float4> pm(total), vv(total);
-  std::vector<float> Cf(9 * total), Ff(9 * total);
-  std::vector<uint32_t> cls(total), ins(total);
-  std::vector<float> pos32(3 * (size_t)n_enabled);
-  std::vector<uint32_t> region(n_enabled);
-  std::vector<float> inv_h_of(n_enabled);
-  std::map<std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t>, uint32_t>
-      class_ids;
+  s->id_space = (uint32_t)total;
   s->host_classes.clear();
-  for (size_t slot = 0; slot < total; ++slot) {
-    const size_t g = order[slot];
-    const uint32_t k = scene_of[g];
-    const HostWorld& w = s->scenes[k].world;
-    const size_t i = local(g);
-    bool en = w.enabled[i] != 0;
-    float f[16];
-    bool ok = true;
-    for (int a = 0; a < 3; ++a) ok = finite_f32(w.pos[3 * i + a], f[a]) && ok;
-    ok = finite_f32(en ? w.mass[i] : 0.0, f[3]) && ok;
-    for (int a = 0; a < 3; ++a) ok = finite_f32(w.vel[3 * i + a], f[4 + a]) && ok;
-    ok = finite_f32(w.rest_volume[i], f[7]) && ok;
-    float rad, damp;
-    ok = finite_f32(w.radius[i], rad) && finite_f32(w.damping[i], damp) && ok;
-    if (!ok) return fail(TMPM_INVALID_INPUT, "particle state not representable in f32");
-    if (en && (f[3] <= 0.0f || f[7] <= 0.0f)) return fail(TMPM_INVALID_INPUT, "enabled particle mass or volume underflows f32");
-    if (rad <= 0.0f || w.damping[i] < 0.0) return fail(TMPM_INVALID_INPUT, "radius or damping invalid");
-    pm[slot] = make_float4(f[0], f[1], f[2], f[3]);
-    vv[slot] = make_float4(f[4], f[5], f[6], f[7]);
-    for (int q = 0; q < 9; ++q) {
-      if (!finite_f32(w.C[9 * i + q], Cf[9 * slot + q]) || !finite_f32(w.F[9 * i + q], Ff[9 * slot + q]))
-        return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
-    }
-    ins[slot] = (uint32_t)g;
-    DevClass c;
-    tmpm_status st = pack_class(w, i, k, rad, damp, c);
+  s->class_ids.clear();
+  PackedParticles pk;
+  for (uint32_t k = 0; k < s->scenes.size(); ++k) {
+    if (!s->scenes[k].stepping()) continue;
+    tmpm_status st = pack_scene(s, k, pk);
     if (st != TMPM_OK) return st;
-    if (en) {
-      // base cell must be representable (gpu.rs:1143-1153)
-      const float inv_h = s->scenes[k].dev.inv_h;
-      for (int a = 0; a < 3; ++a) {
-        int32_t cc;
-        if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
-        pos32[3 * slot + a] = f[a];
-      }
-      region[slot] = c.region;
-      inv_h_of[slot] = inv_h;
-    }
-    auto key = std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
-                               c.region, c.fixed);
-    auto it = class_ids.find(key);
-    uint32_t id;
-    if (it == class_ids.end()) {
-      id = (uint32_t)s->host_classes.size();
-      class_ids.emplace(key, id);
-      s->host_classes.push_back(c);
-    } else {
-      id = it->second;
-    }
-    cls[slot] = id;
   }
-  s->n = (uint32_t)total;
-  s->n_enabled = n_enabled;
-  if (n_enabled == 0) {
+  s->n = (uint32_t)pk.size();
+  s->n_enabled = s->n;
+  s->synced_n = 0;
+  s->disabled_host = pk.disabled;
+  if (s->n == 0) {
     // gpu.rs:1358-1360: no live node, nothing to prepare; steps only count.
     return TMPM_OK;
   }
-
-  // device particle buffers
+  s->cap = s->batch ? std::max<uint32_t>(s->n + s->n / 4, 4096) : s->n;
   for (int b = 0; b < 2; ++b) {
-    ParticleBuffers& pb = s->buf[b];
-    HIP_TRY(pb.pos_mass.alloc(16ull * total));
-    HIP_TRY(pb.vel_vol.alloc(16ull * total));
-    HIP_TRY(pb.C.alloc(36ull * total));
-    HIP_TRY(pb.F.alloc(36ull * total));
-    HIP_TRY(pb.cls.alloc(4ull * total));
-    HIP_TRY(pb.ins_id.alloc(4ull * total));
+    tmpm_status st = alloc_particles(s->buf[b], s->cap);
+    if (st != TMPM_OK) return st;
   }
-  HIP_TRY(s->force.alloc(16ull * total));
+  if (s->batch) {
+    tmpm_status st = alloc_particles(s->synced, s->cap);
+    if (st != TMPM_OK) return st;
+  }
   s->cur = 0;
-  ParticleBuffers& pb = s->buf[0];
-  HIP_TRY(hipMemcpy(pb.pos_mass.ptr, pm.data(), 16ull * total, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pb.vel_vol.ptr, vv.data(), 16ull * total, hipMemcpyHostToDevice));
-  // SoA matrices: component q of particle slot at [q * total + slot]
-  std::vector<float> soa(9 * total);
-  for (size_t slot = 0; slot < total; ++slot)
-    for (int q = 0; q < 9; ++q) soa[q * total + slot] = Cf[9 * slot + q];
-  HIP_TRY(hipMemcpy(pb.C.ptr, soa.data(), 36ull * total, hipMemcpyHostToDevice));
-  for (size_t slot = 0; slot < total; ++slot)
-    for (int q = 0; q < 9; ++q) soa[q * total + slot] = Ff[9 * slot + q];
-  HIP_TRY(hipMemcpy(pb.F.ptr, soa.data(), 36ull * total, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pb.cls.ptr, cls.data(), 4ull * total, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pb.ins_id.ptr, ins.data(), 4ull * total, hipMemcpyHostToDevice));
-  // forces (insertion order)
-  std::vector<float4> fz(total);
-  for (auto& scene : s->scenes) scene.force_pending = false;
-  for (size_t g = 0; g < total; ++g) {
-    SceneState& scene = s->scenes[scene_of[g]];
-    const double* fd = &scene.world.force[3 * local(g)];
-    float fx, fy, fzv;
-    if (!finite_f32(fd[0], fx) || !finite_f32(fd[1], fy) || !finite_f32(fd[2], fzv))
-      return fail(TMPM_INVALID_INPUT, "force not representable in f32");
-    fz[g] = make_float4(fx, fy, fzv, 0.f);
-    if (fx != 0.f || fy != 0.f || fzv != 0.f) scene.force_pending = true;
-  }
-  HIP_TRY(hipMemcpy(s->force.ptr, fz.data(), 16ull * total, hipMemcpyHostToDevice));
-  HIP_TRY(s->classes.alloc(sizeof(DevClass) * std::max<size_t>(1, s->host_classes.size())));
-  HIP_TRY(hipMemcpy(s->classes.ptr, s->host_classes.data(), sizeof(DevClass) * s->host_classes.size(),
-                    hipMemcpyHostToDevice));
-  tmpm_status st = upload_rigid(s);
+  tmpm_status st = upload_particles(s, pk, 0);
   if (st != TMPM_OK) return st;
-  st = upload_obstacles(s);
+  s->force_cap = 0;
+  s->force.release();
+  st = ensure_force_capacity(s, s->batch ? std::max<uint32_t>(s->id_space + s->id_space / 4, 4096) : s->id_space);
   if (st != TMPM_OK) return st;
-  st = upload_scene(s);
-  if (st != TMPM_OK) return st;
+  for (uint32_t k = 0; k < s->scenes.size(); ++k)
+    if (s->scenes[k].stepping() && (st = upload_scene_forces(s, k)) != TMPM_OK) return st;
+  if ((st = upload_disabled(s)) != TMPM_OK) return st;
+  if ((st = upload_classes(s)) != TMPM_OK) return st;
+  if ((st = upload_rigid(s)) != TMPM_OK) return st;
+  if ((st = upload_obstacles(s)) != TMPM_OK) return st;
+  if ((st = upload_scene(s)) != TMPM_OK) return st;
   HIP_TRY(hipMemset(s->scene_err.ptr, 0, 4ull * s->scene_cap));
   // topology capacity from the initial block counts, with headroom
   uint64_t pblocks = 0, gblocks = 0;
-  count_blocks(pos32, region, inv_h_of, pblocks, gblocks);
+  count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
+  s->last_pblocks = (uint32_t)pblocks;
+  s->last_gblocks = (uint32_t)gblocks;
   uint64_t pcap = std::max<uint64_t>(1024, 4 * pblocks);
   uint64_t gcap = std::max<uint64_t>(4096, 4 * gblocks);
-  pcap = std::min<uint64_t>(pcap, n_enabled);
+  pcap = std::min<uint64_t>(pcap, s->cap);
   pcap = std::max<uint64_t>(pcap, 1);
   if (gcap > (1ull << 26)) return fail(TMPM_CAPACITY, "grid exceeds the block capacity");
   st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap);
   if (st != TMPM_OK) return st;
   s->substeps_since_reorder = 1u << 30;
+  s->compacted_last = false;
   s->prepared = true;
   return TMPM_OK;
 }
@@ -1099,14 +1215,16 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   if (e != hipSuccess) return e;
   if (s->substeps_since_reorder >= s->reorder_interval) {
     Parts q = s->parts(1 - s->cur);
-    if ((e = timed(s, "k_bin_scatter_move", [&] { launch_bin_scatter_move(p, q, t, s->n_enabled, s->stream); })) !=
+    if ((e = timed(s, "k_bin_scatter_move", [&] { launch_bin_scatter_move(p, q, t, s->n, s->stream); })) !=
         hipSuccess)
       return e;
+    s->compacted_last = true;
     s->cur = 1 - s->cur;
     p = q;
     s->substeps_since_reorder = 0;
   } else {
     if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p, t, s->stream); })) != hipSuccess) return e;
+    s->compacted_last = false;
   }
   s->substeps_since_reorder++;
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
@@ -1170,8 +1288,10 @@ tmpm_status read_back(tmpm_session* s, std::vector<HostWorld>& out, std::vector<
   std::vector<uint32_t> ins(n);
   HIP_TRY(hipMemcpy(pm.data(), b.pos_mass.ptr, 16 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(vv.data(), b.vel_vol.ptr, 16 * n, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(C.data(), b.C.ptr, 36 * n, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(F.data(), b.F.ptr, 36 * n, hipMemcpyDeviceToHost));
+  for (int q = 0; q < 9; ++q) {
+    HIP_TRY(hipMemcpy(C.data() + (size_t)q * n, b.C.as<float>() + (size_t)q * s->cap, 4 * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(F.data() + (size_t)q * n, b.F.as<float>() + (size_t)q * s->cap, 4 * n, hipMemcpyDeviceToHost));
+  }
   HIP_TRY(hipMemcpy(ins.data(), b.ins_id.ptr, 4 * n, hipMemcpyDeviceToHost));
   // insertion index -> scene: scenes occupy consecutive ranges
   std::vector<std::pair<uint32_t, uint32_t>> ranges;  // (first, scene)
@@ -1179,6 +1299,7 @@ tmpm_status read_back(tmpm_session* s, std::vector<HostWorld>& out, std::vector<
     if (s->scenes[k].count) ranges.emplace_back(s->scenes[k].first, k);
   for (size_t slot = 0; slot < n; ++slot) {
     const uint32_t g = ins[slot];
+    if (g == kInvalid) continue;  // retired slot
     auto it = std::upper_bound(ranges.begin(), ranges.end(), std::make_pair(g, UINT32_MAX));
     if (it == ranges.begin()) return fail(TMPM_READBACK, "corrupt particle order");
     const uint32_t k = (it - 1)->second;
@@ -1204,12 +1325,62 @@ tmpm_status read_back(tmpm_session* s, std::vector<HostWorld>& out, std::vector<
   return TMPM_OK;
 }
 
+// Batches: brings scene k's host world up to its last synchronize by
+// gathering its particles from the synchronized copy on the device.
+tmpm_status materialize(tmpm_session* s, uint32_t k) {
+  SceneState& scene = s->scenes[k];
+  if (!scene.device_newer) return TMPM_OK;
+  if (!s->prepared || s->synced_n == 0 || scene.count == 0) {
+    scene.device_newer = false;
+    return TMPM_OK;
+  }
+  const size_t bytes = 104ull * scene.count;
+  if (s->staging.bytes < bytes) HIP_TRY(s->staging.alloc(bytes));
+  Parts sp = s->parts(0);
+  sp.pos_mass = s->synced.pos_mass.as<float4>();
+  sp.vel_vol = s->synced.vel_vol.as<float4>();
+  sp.C = s->synced.C.as<float>();
+  sp.F = s->synced.F.as<float>();
+  sp.cls = s->synced.cls.as<uint32_t>();
+  sp.ins_id = s->synced.ins_id.as<uint32_t>();
+  sp.n = s->synced_n;
+  launch_gather_scene(sp, scene.first, scene.count, s->staging.as<float>(), s->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<float> rec(26ull * scene.count);
+  HIP_TRY(hipMemcpyAsync(rec.data(), s->staging.ptr, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HostWorld w = scene.world;
+  for (size_t i = 0; i < w.size(); ++i) {
+    w.force[3 * i] = w.force[3 * i + 1] = w.force[3 * i + 2] = 0.0;
+    if (!w.enabled[i]) continue;
+    if (w.fixed[i]) {
+      for (int q = 0; q < 3; ++q) w.vel[3 * i + q] = 0.0;
+      for (int q = 0; q < 9; ++q) w.C[9 * i + q] = 0.0;
+      continue;
+    }
+    const float* r = &rec[26 * i];
+    for (int q = 0; q < 3; ++q) {
+      w.pos[3 * i + q] = r[q];
+      w.vel[3 * i + q] = r[4 + q];
+    }
+    for (int q = 0; q < 9; ++q) {
+      w.C[9 * i + q] = r[8 + q];
+      w.F[9 * i + q] = r[17 + q];
+    }
+  }
+  const uint64_t substeps = scene.world.substeps;
+  scene.set_world(w);
+  scene.world.substeps = substeps;
+  scene.device_newer = false;
+  return TMPM_OK;
+}
+
 // Grows the topology when the last substep used more than half of it.
 tmpm_status maybe_grow_topology(tmpm_session* s, const uint32_t* counters) {
   uint32_t g = counters[CNT_GRID_BLOCKS], pbk = counters[CNT_PARTICLE_BLOCKS];
   if (2ull * g <= s->topo.grid_cap && 2ull * pbk <= s->topo.pblock_cap) return TMPM_OK;
   uint64_t gcap = std::max<uint64_t>(s->topo.grid_cap, 4ull * g);
-  uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(s->topo.pblock_cap, 4ull * pbk), s->n_enabled);
+  uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(s->topo.pblock_cap, 4ull * pbk), std::max<uint32_t>(s->cap, 1));
   return alloc_topology(s, (uint32_t)pcap, (uint32_t)std::min<uint64_t>(gcap, 1ull << 26));
 }
 
@@ -1390,7 +1561,12 @@ tmpm_status set_scene_rigid(tmpm_session* s, uint32_t k, const tmpm_rigid_body* 
       if (maps[j].body != UINT32_MAX && maps[j].body >= body_count)
         return fail(TMPM_INVALID_INPUT, "rigid mapping names no body");
   }
-  std::vector<SceneState> previous = s->scenes;
+  // rollback keeps only this scene's coupling (copying whole scenes would copy their particles)
+  auto saved_bodies = scene.bodies;
+  auto saved_maps = scene.maps;
+  double saved_gravity[3] = {scene.rigid_gravity[0], scene.rigid_gravity[1], scene.rigid_gravity[2]};
+  const double saved_speed = scene.rigid_max_speed;
+  const bool saved_integrate = scene.rigid_integrate;
   scene.bodies.assign(bodies, bodies + body_count);
   scene.maps.assign(maps, maps + (body_count > 0 ? map_count : 0));
   if (body_count > 0) {
@@ -1404,7 +1580,11 @@ tmpm_status set_scene_rigid(tmpm_session* s, uint32_t k, const tmpm_rigid_body* 
   if (st == TMPM_OK) st = upload_scene(s);
   if (st != TMPM_OK) {
     std::string msg = g_last_error;
-    s->scenes.swap(previous);
+    scene.bodies.swap(saved_bodies);
+    scene.maps.swap(saved_maps);
+    for (int a = 0; a < 3; ++a) scene.rigid_gravity[a] = saved_gravity[a];
+    scene.rigid_max_speed = saved_speed;
+    scene.rigid_integrate = saved_integrate;
     if (upload_rigid(s) != TMPM_OK || upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
     g_last_error = msg;
   }
@@ -1485,6 +1665,19 @@ tmpm_status check_scene(const tmpm_session* s, uint32_t scene) {
 // Re-uploads every stepping scene after a host-side change; on failure the
 // previous scene states are restored (edits are transactional).
 tmpm_status reprepare(tmpm_session* s, std::vector<SceneState>& previous) {
+  // prepare uploads host worlds: bring lazily synchronized scenes up to date first
+  for (uint32_t k = 0; k < previous.size() && k < s->scenes.size(); ++k) {
+    if (!previous[k].device_newer) continue;
+    std::swap(s->scenes[k], previous[k]);
+    tmpm_status st = materialize(s, k);
+    std::swap(s->scenes[k], previous[k]);
+    if (st != TMPM_OK) return st;
+    if (s->scenes[k].active && s->scenes[k].first == previous[k].first) {
+      s->scenes[k].world = previous[k].world;
+      s->scenes[k].stable = -1.0;
+      s->scenes[k].device_newer = false;
+    }
+  }
   tmpm_status st = prepare(s);
   if (st != TMPM_OK) {
     std::string msg = g_last_error;
@@ -1525,14 +1718,20 @@ tmpm_status set_scene_obstacles(tmpm_session* s, uint32_t k, const tmpm_obstacle
   if (count > 0 && !obs) return fail(TMPM_INVALID_INPUT, "null obstacles");
   for (uint32_t i = 0; i < count; ++i)
     if (!obstacle_valid(obs[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
-  HostWorld backup = scene.world;
+  std::vector<tmpm_obstacle> backup;
+  std::vector<std::vector<double>> backup_planes;
+  for (auto& o : scene.world.obstacles) {
+    backup.push_back(o.o);
+    backup_planes.push_back(o.planes);
+  }
+  for (size_t i = 0; i < backup.size(); ++i) backup[i].planes = backup_planes[i].empty() ? nullptr : backup_planes[i].data();
   scene.world.set_obstacles(obs, count);
   if (s->prepared) {
     tmpm_status st = upload_obstacles(s);
     if (st == TMPM_OK) st = upload_scene(s);
     if (st != TMPM_OK) {
       std::string msg = g_last_error;
-      scene.world = backup;
+      scene.world.set_obstacles(backup.data(), (uint32_t)backup.size());
       if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
       g_last_error = msg;
       return st;
@@ -1625,47 +1824,83 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
       s->valid = false;
       return st;
     }
+    s->last_pblocks = counters[CNT_PARTICLE_BLOCKS];
+    s->last_gblocks = counters[CNT_GRID_BLOCKS];
   }
-  std::vector<HostWorld> updated;
-  std::vector<tmpm_status> scene_st;
-  tmpm_status st = read_back(s, updated, scene_st);
-  if (st != TMPM_OK) {
-    s->valid = false;
-    return st;
-  }
-  if (!s->batch && scene_st[0] != TMPM_OK) {
-    s->valid = false;
-    return fail(scene_st[0], "MPM integration produced a non-finite state");
-  }
-  bool stopped = false;
   std::vector<char> commit(ns, 0);
-  for (size_t k = 0; k < ns; ++k) {
-    SceneState& scene = s->scenes[k];
-    if (!scene.stepping()) continue;
-    tmpm_status sst = scene_bits[k] ? status_from_error_bits(scene_bits[k]) : scene_st[k];
-    if (sst != TMPM_OK) {
-      scene.status = sst;  // the last synchronized world stays readable
-      stopped = true;
-      continue;
+  tmpm_status st = TMPM_OK;
+  if (!s->batch) {
+    // eager: the world is read back at every synchronize
+    std::vector<HostWorld> updated;
+    std::vector<tmpm_status> scene_st;
+    st = read_back(s, updated, scene_st);
+    if (st != TMPM_OK) {
+      s->valid = false;
+      return st;
     }
+    if (scene_st[0] != TMPM_OK) {
+      s->valid = false;
+      return fail(scene_st[0], "MPM integration produced a non-finite state");
+    }
+    SceneState& scene = s->scenes[0];
     const uint64_t before = scene.world.substeps;
-    updated[k].substeps = before + steps < before ? UINT64_MAX : before + steps;
-    scene.set_world(updated[k]);
-    commit[k] = 1;
+    updated[0].substeps = before + steps < before ? UINT64_MAX : before + steps;
+    scene.set_world(updated[0]);
+    commit[0] = 1;
+  } else {
+    // lazy: stopped scenes take their state of the previous synchronize from
+    // the device copy; the others only advance their substep counts and are
+    // read on demand from the new copy
+    bool stopped = false;
+    for (size_t k = 0; k < ns; ++k) {
+      SceneState& scene = s->scenes[k];
+      if (!scene.stepping()) continue;
+      if (scene_bits[k] == 0) {
+        const uint64_t before = scene.world.substeps;
+        scene.world.substeps = before + steps < before ? UINT64_MAX : before + steps;
+        if (s->prepared && scene.count) scene.device_newer = true;
+        commit[k] = 1;
+        continue;
+      }
+      st = materialize(s, (uint32_t)k);
+      if (st != TMPM_OK) {
+        s->valid = false;
+        return st;
+      }
+      scene.status = status_from_error_bits(scene_bits[k]);  // the last synchronized world stays readable
+      stopped = true;
+      if (s->prepared && scene.count) {
+        launch_retire(s->parts(s->cur), scene.first, scene.count, s->stream);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+    if (stopped) {
+      s->substeps_since_reorder = s->reorder_interval;  // the next substep drops the retired slots
+      st = upload_rigid(s);
+      if (st == TMPM_OK) st = upload_obstacles(s);
+      if (st == TMPM_OK) st = upload_scene(s);
+      if (st != TMPM_OK) {
+        s->valid = false;
+        return st;
+      }
+      HIP_TRY(hipMemset(s->scene_err.ptr, 0, 4ull * s->scene_cap));
+      uint32_t err = counters[CNT_ERROR] & ~(ERR_RESIDENT | ERR_FIXED_RANGE);
+      HIP_TRY(hipMemcpy(s->counters.as<uint32_t>() + CNT_ERROR, &err, 4, hipMemcpyHostToDevice));
+    }
+    if (s->prepared) {
+      if (s->compacted_last && !stopped) s->n = std::min<uint32_t>(s->n, counters[CNT_BINNED]);
+      st = copy_particles(s->synced, s->cap, s->buf[s->cur], s->cap, s->n, s->stream);
+      if (st != TMPM_OK) {
+        s->valid = false;
+        return st;
+      }
+      s->synced_n = s->n;
+    }
   }
   st = read_coupling(s, commit);
   if (st != TMPM_OK) {
     s->valid = false;
     return st;
-  }
-  if (stopped) {
-    // drop the stopped scenes' particles from the device
-    st = prepare(s);
-    if (st != TMPM_OK) {
-      s->valid = false;
-      return st;
-    }
-    return TMPM_OK;
   }
   if (s->prepared) {
     st = maybe_grow_topology(s, counters);
@@ -1815,6 +2050,10 @@ tmpm_status tmpm_batch_read_particles(const tmpm_session* s, uint32_t scene, tmp
   tmpm_status st = check_scene(s, scene);
   if (st != TMPM_OK) return st;
   if (!out) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+  // the world of the last synchronize, read from the device on first use
+  st = materialize(const_cast<tmpm_session*>(s), scene);
+  if (st != TMPM_OK) return st;
   const HostWorld& w = s->scenes[scene].world;
   if (out->count < w.size()) return fail(TMPM_CAPACITY, "output arrays too small");
   return copy_particles_out(w, out);
@@ -1839,21 +2078,172 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
   tmpm_status st = check_editable(s);
   if (st != TMPM_OK) return st;
   if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
-  st = check_world_desc(world);
-  if (st != TMPM_OK) return st;
+  // one pass over the particles: load, then validate what was loaded
+  if (!world) return fail(TMPM_INVALID_INPUT, "null world");
+  if (!params_valid(world->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
+  for (uint32_t i = 0; i < world->obstacle_count; ++i)
+    if (!obstacle_valid(world->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  const tmpm_particles& pp = world->particles;
+  if (pp.count > 0 && (!pp.position || !pp.mass || !pp.rest_volume || !pp.radius || world->material_count == 0))
+    return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  for (uint64_t i = 0; pp.material_index && i < pp.count; ++i)
+    if (pp.material_index[i] >= world->material_count) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
   SceneState scene;
   load_scene(scene, world, substep_dt);
+  for (size_t i = 0; i < scene.world.size(); ++i)
+    if (!scene.world.particle_valid(i)) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
   st = validate_resident_world(scene);
   if (st != TMPM_OK) return st;
   uint32_t slot = 0;
   while (slot < s->scenes.size() && s->scenes[slot].active) ++slot;
   if (slot >= 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
-  std::vector<SceneState> previous = s->scenes;
-  if (slot == s->scenes.size()) s->scenes.push_back(scene);
-  else s->scenes[slot] = scene;
-  st = reprepare(s, previous);
+  if (!s->prepared) {
+    std::vector<SceneState> previous = s->scenes;
+    if (slot == s->scenes.size()) s->scenes.push_back(scene);
+    else s->scenes[slot] = scene;
+    st = reprepare(s, previous);
+    if (st != TMPM_OK) return st;
+    if (scene_out) *scene_out = slot;
+    return TMPM_OK;
+  }
+  // Admission on the device: the new particles go after the current slots
+  // and the new insertion indices after the current ones; nothing already on
+  // the device moves.
+  if ((uint64_t)s->id_space + scene.world.size() >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
+  scene.first = s->id_space;
+  scene.count = (uint32_t)scene.world.size();
+  for (size_t i = 0; i < scene.count; ++i)
+    if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
+  st = build_scene(scene);
   if (st != TMPM_OK) return st;
+  const size_t saved_classes = s->host_classes.size();
+  const bool appended = slot == s->scenes.size();
+  if (appended) s->scenes.push_back(std::move(scene));
+  else s->scenes[slot] = std::move(scene);
+  PackedParticles pk;
+  st = pack_scene(s, slot, pk);
+  auto undo = [&]() {
+    if (appended) s->scenes.pop_back();
+    else {
+      s->scenes[slot] = SceneState();
+      s->scenes[slot].active = false;
+    }
+    for (size_t c = saved_classes; c < s->host_classes.size(); ++c) {
+      const DevClass& d = s->host_classes[c];
+      s->class_ids.erase(std::make_tuple(d.kind, d.radius, d.damping, d.lambda, d.mu, d.bulk, d.gamma, d.viscosity,
+                                         d.tensile, d.region, d.fixed));
+    }
+    s->host_classes.resize(saved_classes);
+  };
+  if (st != TMPM_OK) {
+    undo();
+    return st;
+  }
+  // from here on device state changes; a failure invalidates the batch
+  auto broken = [&](tmpm_status code) {
+    s->valid = false;
+    return code;
+  };
+  const uint64_t need = (uint64_t)s->n + pk.size();
+  if (need > s->cap) {
+    const uint32_t cap = (uint32_t)std::max<uint64_t>(need + need / 4, (uint64_t)s->cap + s->cap / 2);
+    ParticleBuffers grown, spare, synced;
+    if (alloc_particles(grown, cap) != TMPM_OK || alloc_particles(spare, cap) != TMPM_OK ||
+        alloc_particles(synced, cap) != TMPM_OK)
+      return broken(TMPM_CAPACITY);
+    if (copy_particles(grown, cap, s->buf[s->cur], s->cap, s->n, s->stream) != TMPM_OK ||
+        copy_particles(synced, cap, s->synced, s->cap, s->synced_n, s->stream) != TMPM_OK)
+      return broken(TMPM_READBACK);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return broken(fail(TMPM_READBACK, "device copy failed"));
+    auto swap_bufs = [](ParticleBuffers& a, ParticleBuffers& b) {
+      std::swap(a.pos_mass.ptr, b.pos_mass.ptr); std::swap(a.pos_mass.bytes, b.pos_mass.bytes);
+      std::swap(a.vel_vol.ptr, b.vel_vol.ptr); std::swap(a.vel_vol.bytes, b.vel_vol.bytes);
+      std::swap(a.C.ptr, b.C.ptr); std::swap(a.C.bytes, b.C.bytes);
+      std::swap(a.F.ptr, b.F.ptr); std::swap(a.F.bytes, b.F.bytes);
+      std::swap(a.cls.ptr, b.cls.ptr); std::swap(a.cls.bytes, b.cls.bytes);
+      std::swap(a.ins_id.ptr, b.ins_id.ptr); std::swap(a.ins_id.bytes, b.ins_id.bytes);
+    };
+    swap_bufs(s->buf[s->cur], grown);
+    swap_bufs(s->buf[1 - s->cur], spare);
+    swap_bufs(s->synced, synced);
+    s->cap = cap;
+  }
+  if ((st = upload_particles(s, pk, s->n)) != TMPM_OK) return broken(st);
+  s->n += (uint32_t)pk.size();
+  s->n_enabled = s->n;
+  s->id_space += s->scenes[slot].count;
+  if ((st = ensure_force_capacity(s, s->id_space + s->id_space / 4)) != TMPM_OK) return broken(st);
+  if ((st = upload_scene_forces(s, slot)) != TMPM_OK) return broken(st);
+  s->disabled_host.insert(s->disabled_host.end(), pk.disabled.begin(), pk.disabled.end());
+  if ((st = upload_disabled(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_classes(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_rigid(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_obstacles(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_scene(s)) != TMPM_OK) return broken(st);
+  // topology: room for the blocks this scene adds (and the per-particle arrays)
+  uint64_t pblocks = 0, gblocks = 0;
+  count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
+  s->last_pblocks += (uint32_t)pblocks;
+  s->last_gblocks += (uint32_t)gblocks;
+  const Topo& t = s->topo;
+  if (2ull * s->last_pblocks > t.pblock_cap || 2ull * s->last_gblocks > t.grid_cap || s->topo_cap < s->cap) {
+    const uint64_t gcap = std::min<uint64_t>(std::max<uint64_t>(t.grid_cap, 4ull * s->last_gblocks), 1ull << 26);
+    const uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(t.pblock_cap, 4ull * s->last_pblocks), s->cap);
+    if ((st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap)) != TMPM_OK) return broken(st);
+  }
   if (scene_out) *scene_out = slot;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene) {
+  tmpm_status st = check_editable(s);
+  if (st == TMPM_OK) st = check_scene(s, scene);
+  if (st != TMPM_OK) return st;
+  if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
+  SceneState& gone = s->scenes[scene];
+  const uint32_t first = gone.first, count = gone.count;
+  if (s->prepared && count > 0) {
+    // Retirement on the device: the scene's particles lose their mass (the
+    // next reorder drops them) and the insertion indices above its range
+    // close the gap, here and in the synchronized copy.
+    Parts p = s->parts(s->cur);
+    launch_retire(p, first, count, s->stream);
+    launch_renumber(p.ins_id, s->n, first, count, s->stream);
+    launch_renumber(s->synced.ins_id.as<uint32_t>(), s->synced_n, first, count, s->stream);
+    HIP_TRY(hipGetLastError());
+    const uint32_t above = s->id_space - (first + count);
+    if (above > 0) {
+      if (s->staging.bytes < 16ull * above) HIP_TRY(s->staging.alloc(16ull * above));
+      HIP_TRY(hipMemcpyAsync(s->staging.ptr, s->force.as<float4>() + first + count, 16ull * above,
+                             hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(s->force.as<float4>() + first, s->staging.ptr, 16ull * above, hipMemcpyDeviceToDevice,
+                             s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->substeps_since_reorder = s->reorder_interval;
+  }
+  for (auto& other : s->scenes)
+    if (other.active && &other != &gone && other.first > first) other.first -= count;
+  s->id_space -= count;
+  // snapshot records of disabled particles: drop the scene's, shift the others
+  std::vector<float4> kept;
+  for (size_t i = 0; i + 1 < s->disabled_host.size(); i += 2) {
+    float4 a = s->disabled_host[i];
+    uint32_t g = __builtin_bit_cast(uint32_t, a.w);
+    if (g - first < count) continue;
+    if (g >= first + count) a.w = __builtin_bit_cast(float, g - count);
+    kept.push_back(a);
+    kept.push_back(s->disabled_host[i + 1]);
+  }
+  s->disabled_host.swap(kept);
+  s->scenes[scene] = SceneState();
+  s->scenes[scene].active = false;
+  if (!s->prepared) return TMPM_OK;
+  if ((st = upload_disabled(s)) != TMPM_OK || (st = upload_rigid(s)) != TMPM_OK ||
+      (st = upload_obstacles(s)) != TMPM_OK || (st = upload_scene(s)) != TMPM_OK) {
+    s->valid = false;
+    return st;
+  }
   return TMPM_OK;
 }
 
@@ -1929,16 +2319,6 @@ tmpm_status tmpm_batch_read_obstacles(const tmpm_session* s, uint32_t scene, tmp
   return read_scene_obstacles(s->scenes[scene], out, cap, count_out);
 }
 
-tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene) {
-  tmpm_status st = check_editable(s);
-  if (st == TMPM_OK) st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
-  std::vector<SceneState> previous = s->scenes;
-  s->scenes[scene] = SceneState();
-  s->scenes[scene].active = false;
-  return reprepare(s, previous);
-}
 
 tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, uint64_t output_bytes, void* stream,
                                          uint32_t* count_out) {
@@ -1946,7 +2326,7 @@ tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, u
   if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
   if (count_out) *count_out = 0;
   if (!s->prepared) return TMPM_OK;
-  uint64_t bytes = 32ull * s->n;
+  uint64_t bytes = 32ull * s->id_space;
   if (!device_output || output_bytes < bytes) return fail(TMPM_CAPACITY, "snapshot buffer too small");
   hipStream_t target = stream ? static_cast<hipStream_t>(stream) : s->stream;
   if (target != s->stream) {
@@ -1957,6 +2337,8 @@ tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, u
     HIP_TRY(hipEventDestroy(ev));
   }
   launch_snapshot(s->parts(s->cur), static_cast<float4*>(device_output), target);
+  launch_snapshot_disabled(s->disabled_rec.as<float4>(), (uint32_t)(s->disabled_host.size() / 2),
+                           static_cast<float4*>(device_output), target);
   HIP_TRY(hipGetLastError());
   if (target != s->stream) {
     hipEvent_t ev;
@@ -1965,7 +2347,7 @@ tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, u
     HIP_TRY(hipStreamWaitEvent(s->stream, ev, 0));
     HIP_TRY(hipEventDestroy(ev));
   }
-  if (count_out) *count_out = s->n;
+  if (count_out) *count_out = s->id_space;
   return TMPM_OK;
 }
 

@@ -16,6 +16,14 @@ Units are SI and gravity points along -y.  Particle volume and mass follow
                              [0,0.4]x[0,0.6]x[0,1] at 2 per cell; h=0.0048559
                              so that N, the particles per cell and a 512^3 cell
                              domain agree; dt=2e-5 (2000 steps).
+* :func:`serving_requests`   a Poisson stream of requests: N log-uniform in
+                             [4096, 524288] at 8 particles per cell (h the
+                             largest of 1/64, 1/96, 1/128 that keeps the cube
+                             within 0.4 m) on a sticky floor, elastic or
+                             fluid, 1-4 falling rigid boxes (edge U[0.05,
+                             0.15], density U[500, 5000]) driven by the MPM's
+                             reactions, horizons uniform in {50..2000}
+                             substeps of dt=5e-5.
 * :func:`batch_scenes`       256 independent scenes, N_i log-uniform in
                              [2048, 65536] (sum ~4.7M), h_i in {1/64 .. 1/128},
                              a jittered cube of 8 particles per cell with
@@ -154,6 +162,52 @@ def batch_scenes(count: int = 256, seed: int = 0, n_min: int = 2048, n_max: int 
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(n_min), math.log(n_max), size=count))
     return [batch_scene(rng, int(n), k) for k, n in enumerate(sizes)]
+
+
+def serving_request(rng, n_target: int, name: str, horizon: int, arrival: float, max_boxes: int = 4):
+    """One serving request: particle cube on a sticky floor with falling rigid boxes."""
+    from .mpm import RigidBody, RigidMapping
+    from .serving import Request
+
+    cells = (n_target / 8.0) ** (1.0 / 3.0)
+    h = next((c for c in (1 / 64, 1 / 96, 1 / 128) if cells * c <= 0.4), 1 / 128)
+    per_axis = max(2, int(round(n_target ** (1.0 / 3.0))))
+    spacing = h / 2
+    edge = per_axis * spacing
+    lo = np.array([0.5 - edge / 2, 0.06, 0.5 - edge / 2])
+    pos = jittered_lattice(rng, [per_axis] * 3, lo, lo + edge)
+    vel = rng.normal(0.0, 0.1, size=pos.shape)
+    material = MaterialModel.elastic(float(rng.uniform(1e4, 5e4)), 0.3) if rng.integers(2) == 0 else \
+        MaterialModel.fluid(1.0e5, 7.0, 0.0)
+    pa = _particles(pos, vel, spacing / 2, 1.0e3, material)
+    params = MpmParams(cell_width=h, gravity=GRAVITY_Y.copy(), max_substep=5.0e-5,
+                       bounds=WorldBounds(np.zeros(3), np.ones(3)))
+    w = MpmWorld(pa, params)
+    obstacles = [sticky_floor(0.05)]
+    bodies, maps = [], [RigidMapping(None)]
+    top = float(pos[:, 1].max())
+    for b in range(int(rng.integers(1, max_boxes + 1))):
+        half = 0.5 * float(rng.uniform(0.05, 0.15))
+        center = [0.5 + rng.uniform(-0.3, 0.3) * edge, top + half + 0.5 * h + 0.02 * b,
+                  0.5 + rng.uniform(-0.3, 0.3) * edge]
+        body = RigidBody.box(center, [half, half, half], float(rng.uniform(500.0, 5000.0)))
+        body.linear_velocity = np.array([0.0, -0.5, 0.0])
+        bodies.append(body)
+        obstacles.append(RigidObstacle.cuboid(center, [half, half, half]))
+        maps.append(RigidMapping(len(bodies) - 1))
+    w.set_obstacles(obstacles)
+    return Request(name, w, 5.0e-5, int(horizon), float(arrival), bodies, maps)
+
+
+def serving_requests(count: int, rate: float, seed: int = 0, n_min: int = 4096, n_max: int = 524288,
+                     horizons=(50, 2000), max_boxes: int = 4):
+    """`count` requests with Poisson arrivals at `rate` per second."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    sizes = np.exp(rng.uniform(math.log(n_min), math.log(n_max), size=count))
+    hz = rng.integers(horizons[0], horizons[1] + 1, size=count)
+    return [serving_request(rng, int(n), f"req{k}", int(hz[k]), float(arrivals[k]), max_boxes)
+            for k, n in enumerate(sizes)]
 
 
 def record_hash(pa: ParticleArrays, count: int = 4096) -> str:
