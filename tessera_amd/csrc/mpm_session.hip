@@ -13,13 +13,16 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 #include <vector>
@@ -175,14 +178,58 @@ inline bool cell_f32(float x, float inv_h, int32_t& c) {
 // ---------------------------------------------------------------------------
 // Host world (f64, insertion order) — `MpmWorld` (world.rs:206-221).
 // ---------------------------------------------------------------------------
+// Runs fn(begin, end) over [0, n) in up to 16 host threads (chunks of at
+// least min_chunk items; small ranges run inline).  Used for the per-particle
+// host work of uploads and admissions.
+template <class F>
+void parallel_for(size_t n, F&& fn, size_t min_chunk = 16384) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t chunks = std::min<size_t>(hw, (n + min_chunk - 1) / min_chunk);
+  if (chunks <= 1) {
+    fn((size_t)0, n);
+    return;
+  }
+  const size_t step = (n + chunks - 1) / chunks;
+  std::vector<std::thread> pool;
+  for (size_t c = 1; c < chunks; ++c) {
+    const size_t b = c * step, e = std::min(n, b + step);
+    if (b < e) pool.emplace_back([&fn, b, e] { fn(b, e); });
+  }
+  fn((size_t)0, std::min(n, step));
+  for (auto& t : pool) t.join();
+}
+
 struct OwnedObstacle {
   tmpm_obstacle o;
   std::vector<double> planes;
 };
 
+// Allocator whose value-less construct default-initializes: resizing a
+// vector of doubles leaves the new elements unwritten, so a parallel fill
+// is the first touch of their pages.
+template <class T>
+struct DefaultInit : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = DefaultInit<U>;
+  };
+  DefaultInit() = default;
+  template <class U>
+  DefaultInit(const DefaultInit<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+using DVec = std::vector<double, DefaultInit<double>>;
+
 struct HostWorld {
   tmpm_params params{};
-  std::vector<double> pos, vel, F, C, force, mass, rest_volume, radius, damping, plastic;
+  DVec pos, vel, F, C, force, mass, rest_volume, radius, damping, plastic;
   std::vector<tmpm_material> material;
   std::vector<uint32_t> group_id;
   std::vector<uint16_t> color;
@@ -193,6 +240,23 @@ struct HostWorld {
   uint64_t next_chunk_id = 1;
 
   size_t size() const { return mass.size(); }
+
+  // Empties the world but keeps every vector's memory, for reuse by a later
+  // admission (no fresh pages to fault in).
+  void clear_keep() {
+    for (auto* v : {&pos, &vel, &F, &C, &force, &mass, &rest_volume, &radius, &damping, &plastic}) v->clear();
+    material.clear();
+    group_id.clear();
+    color.clear();
+    enabled.clear();
+    fixed.clear();
+    chunk.clear();
+    obstacles.clear();
+    params = tmpm_params{};
+    substeps = 0;
+    next_chunk_id = 1;
+  }
+  size_t capacity_bytes() const { return pos.capacity() * 34 * sizeof(double) / 3; }
 
   void set_obstacles(const tmpm_obstacle* obs, uint32_t n) {
     obstacles.clear();
@@ -205,6 +269,9 @@ struct HostWorld {
     }
   }
   HostWorld() = default;
+  // moves keep every vector's buffer, so obstacle plane pointers stay valid
+  HostWorld(HostWorld&&) noexcept = default;
+  HostWorld& operator=(HostWorld&&) noexcept = default;
   HostWorld(const HostWorld& w) { *this = w; }
   HostWorld& operator=(const HostWorld& w) {
     if (this == &w) return *this;
@@ -236,55 +303,68 @@ struct HostWorld {
 
   // world.rs:289-303
   double stable_timestep() const {
+    const size_t n = size();
+    std::vector<double> part((n + 16383) / 16384 + 1, params.max_substep);
+    parallel_for(n, [&](size_t b, size_t e) {
+      double s = params.max_substep;
+      for (size_t i = b; i < e; ++i) {
+        if (!enabled[i] || fixed[i]) continue;
+        double t = timestep_bound(material[i], mass[i] / rest_volume[i], &vel[3 * i], params.cell_width,
+                                  det_cm(&F[9 * i]));
+        s = std::fmin(s, t);
+      }
+      part[b / 16384] = std::fmin(part[b / 16384], s);
+    });
     double s = params.max_substep;
-    for (size_t i = 0; i < size(); ++i) {
-      if (!enabled[i] || fixed[i]) continue;
-      double b = timestep_bound(material[i], mass[i] / rest_volume[i], &vel[3 * i], params.cell_width, det_cm(&F[9 * i]));
-      s = std::fmin(s, b);
-    }
+    for (double v : part) s = std::fmin(s, v);
     return std::fmin(s, params.max_substep);
   }
 
   // Appends a particle batch (world.rs:244-265); validation by the caller.
   void append(const tmpm_particles& p, const tmpm_material* mats, uint64_t chunk_id) {
     size_t n = p.count;
-    auto app = [n](std::vector<double>& dst, const double* src, int k, double fill) {
-      if (src)
-        dst.insert(dst.end(), src, src + n * k);
-      else
-        dst.insert(dst.end(), n * k, fill);
+    // grow without writing, then copy (or fill) in parallel: the page faults
+    // of fresh memory are taken by all threads
+    auto app = [n](DVec& dst, const double* src, int k, const double* pattern) {
+      const size_t base = dst.size(), m = n * k;
+      dst.resize(base + m);
+      double* d = dst.data() + base;
+      parallel_for(m, [&](size_t b, size_t e) {
+        if (src)
+          std::memcpy(d + b, src + b, (e - b) * sizeof(double));
+        else
+          for (size_t i = b; i < e; ++i) d[i] = pattern[i % k];
+      }, 1 << 16);
     };
-    app(pos, p.position, 3, 0.0);
-    app(vel, p.velocity, 3, 0.0);
-    if (p.deformation) {
-      F.insert(F.end(), p.deformation, p.deformation + 9 * n);
-    } else {
-      for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < 9; ++k) F.push_back(k % 4 == 0 ? 1.0 : 0.0);
-    }
-    app(C, p.affine, 9, 0.0);
-    app(force, p.force, 3, 0.0);
-    app(mass, p.mass, 1, 0.0);
-    app(rest_volume, p.rest_volume, 1, 0.0);
-    app(radius, p.radius, 1, 0.0);
-    app(damping, p.damping, 1, 0.0);
-    if (p.plastic) {
-      plastic.insert(plastic.end(), p.plastic, p.plastic + 3 * n);
-    } else {
-      for (size_t i = 0; i < n; ++i) {
-        plastic.push_back(1.0);
-        plastic.push_back(0.0);
-        plastic.push_back(0.0);
+    static const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    static const double eye3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    static const double plastic0[3] = {1, 0, 0};
+    app(pos, p.position, 3, zero9);
+    app(vel, p.velocity, 3, zero9);
+    app(F, p.deformation, 9, eye3);
+    app(C, p.affine, 9, zero9);
+    app(force, p.force, 3, zero9);
+    app(mass, p.mass, 1, zero9);
+    app(rest_volume, p.rest_volume, 1, zero9);
+    app(radius, p.radius, 1, zero9);
+    app(damping, p.damping, 1, zero9);
+    app(plastic, p.plastic, 3, plastic0);
+    const size_t base = material.size();
+    material.resize(base + n);
+    group_id.resize(base + n);
+    color.resize(base + n);
+    enabled.resize(base + n);
+    fixed.resize(base + n);
+    chunk.resize(base + n, chunk_id);
+    parallel_for(n, [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        material[base + i] = mats[p.material_index ? p.material_index[i] : 0];
+        group_id[base + i] = p.group_id ? p.group_id[i] : 0;
+        color[base + i] = p.transfer_color ? p.transfer_color[i] : 0;
+        enabled[base + i] = p.enabled ? (p.enabled[i] != 0) : 1;
+        fixed[base + i] = p.fixed ? (p.fixed[i] != 0) : 0;
       }
-    }
-    for (size_t i = 0; i < n; ++i) {
-      material.push_back(mats[p.material_index ? p.material_index[i] : 0]);
-      group_id.push_back(p.group_id ? p.group_id[i] : 0);
-      color.push_back(p.transfer_color ? p.transfer_color[i] : 0);
-      enabled.push_back(p.enabled ? (p.enabled[i] != 0) : 1);
-      fixed.push_back(p.fixed ? (p.fixed[i] != 0) : 0);
-      chunk.push_back(chunk_id);
-    }
+    });
   }
 
   // world.rs:268-277: keep particles whose chunk differs.
@@ -398,6 +478,92 @@ struct SceneState {
     world = w;
     stable = -1.0;
   }
+  void set_world(HostWorld&& w) {
+    world = std::move(w);
+    stable = -1.0;
+  }
+};
+
+// Growable host array of a trivially copyable T: resize leaves new elements
+// uninitialized and clear keeps the memory, so a staging buffer reused
+// across admissions touches no fresh pages once it has grown.  PINNED
+// buffers are page-locked (hipHostMalloc) for full-rate uploads.
+template <class T, bool PINNED = false>
+struct PodBuf {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  PodBuf() = default;
+  PodBuf(const PodBuf&) = delete;
+  PodBuf& operator=(const PodBuf&) = delete;
+  ~PodBuf() { release(p); }
+  static T* acquire(size_t c) {
+    void* q = nullptr;
+    if (PINNED) {
+      if (hipHostMalloc(&q, c * sizeof(T), hipHostMallocDefault) != hipSuccess) q = nullptr;
+    } else {
+      q = std::malloc(c * sizeof(T));
+    }
+    if (!q) throw std::bad_alloc();
+    return static_cast<T*>(q);
+  }
+  static void release(T* q) {
+    if (!q) return;
+    if (PINNED) (void)hipHostFree(q);
+    else std::free(q);
+  }
+  size_t size() const { return n; }
+  bool empty() const { return n == 0; }
+  T* data() { return p; }
+  const T* data() const { return p; }
+  T* begin() { return p; }
+  T* end() { return p + n; }
+  const T* begin() const { return p; }
+  const T* end() const { return p + n; }
+  T& operator[](size_t i) { return p[i]; }
+  const T& operator[](size_t i) const { return p[i]; }
+  void clear() { n = 0; }
+  void reset() {
+    release(p);
+    p = nullptr;
+    n = cap = 0;
+  }
+  void resize(size_t m) {
+    if (m > cap) {
+      const size_t c = std::max(m, cap + cap / 2);
+      T* q = acquire(c);
+      if (n) std::memcpy(q, p, n * sizeof(T));
+      release(p);
+      p = q;
+      cap = c;
+    }
+    n = m;
+  }
+  void resize(size_t m, const T& v) {
+    const size_t o = n;
+    resize(m);
+    for (size_t i = o; i < m; ++i) p[i] = v;
+  }
+};
+
+// Host image of some scenes' enabled particles, ready for upload.
+struct PackedParticles {
+  PodBuf<float4, true> pm, vv;
+  PodBuf<float> C, F;  // 9 per particle, particle-major
+  PodBuf<uint32_t, true> cls, ins;
+  PodBuf<float> pos32, inv_h;  // for counting blocks
+  PodBuf<uint32_t> region;
+  PodBuf<float4> disabled;     // snapshot records of disabled particles
+  PodBuf<float, true> cols;    // upload staging: component-major C and F
+  PodBuf<float4, true> forces; // upload staging: one-shot forces
+  size_t size() const { return pm.size(); }
+  void clear() {
+    pm.clear(); vv.clear(); C.clear(); F.clear(); cls.clear(); ins.clear();
+    pos32.clear(); inv_h.clear(); region.clear(); disabled.clear(); cols.clear(); forces.clear();
+  }
+  void reset() {
+    pm.reset(); vv.reset(); C.reset(); F.reset(); cls.reset(); ins.reset();
+    pos32.reset(); inv_h.reset(); region.reset(); disabled.reset(); cols.reset(); forces.reset();
+  }
 };
 
 struct tmpm_session {
@@ -428,6 +594,8 @@ struct tmpm_session {
   DevBuf staging;
   DevBuf disabled_rec;     // snapshot records of disabled particles (2 float4 each)
   std::vector<float4> disabled_host;
+  PackedParticles stage;                // admission staging, reused
+  std::vector<HostWorld> spare_worlds;  // emptied worlds of retired scenes, reused
   uint32_t force_cap = 0;
   uint32_t last_pblocks = 0, last_gblocks = 0;  // block counts of the last synchronized substep
   uint32_t topo_cap = 0;   // particle capacity the topology's per-particle arrays were sized for
@@ -856,8 +1024,67 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
 
 // Counts particle blocks and grid blocks (block + its 7 upper neighbours) of
 // the enabled particles on the host, f32 cell rule.
-void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& region,
-                  const std::vector<float>& inv_h, uint64_t& pblocks, uint64_t& gblocks) {
+template <class FV, class UV>
+void count_blocks(const FV& pos32, const UV& region, const FV& inv_h, uint64_t& pblocks, uint64_t& gblocks) {
+  // Dense path for one region: occupancy bytes over the bounding box of the
+  // particle blocks (grid blocks are those dilated by one toward +x, +y, +z).
+  const size_t np = region.size();
+  pblocks = gblocks = 0;
+  if (np == 0) return;
+  bool one_region = true;
+  for (size_t i = 1; i < np && one_region; ++i) one_region = region[i] == region[0];
+  if (one_region) {
+    const size_t nchunk = (np + 65535) / 65536;
+    std::vector<std::array<int32_t, 6>> box(nchunk, {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN});
+    parallel_for(nchunk, [&](size_t cb, size_t ce) {
+      for (size_t c = cb; c < ce; ++c)
+        for (size_t i = c * 65536; i < std::min(np, (c + 1) * 65536); ++i) {
+          int32_t cc[3];
+          bool ok = true;
+          for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h[i], cc[a]) && ok;
+          if (!ok) continue;
+          for (int a = 0; a < 3; ++a) {
+            box[c][a] = std::min(box[c][a], cc[a] >> 2);
+            box[c][3 + a] = std::max(box[c][3 + a], cc[a] >> 2);
+          }
+        }
+    }, 1);
+    std::array<int64_t, 6> bb{INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+    for (auto& b : box)
+      for (int a = 0; a < 3; ++a) {
+        bb[a] = std::min<int64_t>(bb[a], b[a]);
+        bb[3 + a] = std::max<int64_t>(bb[3 + a], b[3 + a]);
+      }
+    if (bb[0] > bb[3]) return;  // no representable particle
+    const int64_t ex = bb[3] - bb[0] + 2, ey = bb[4] - bb[1] + 2, ez = bb[5] - bb[2] + 2;
+    if (ex * ey * ez <= (1 << 24)) {
+      std::unique_ptr<std::atomic<uint8_t>[]> occ(new std::atomic<uint8_t>[ex * ey * ez]());
+      parallel_for(nchunk, [&](size_t cb, size_t ce) {
+        for (size_t c = cb; c < ce; ++c)
+          for (size_t i = c * 65536; i < std::min(np, (c + 1) * 65536); ++i) {
+            int32_t cc[3];
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h[i], cc[a]) && ok;
+            if (!ok) continue;
+            const int64_t x = (cc[0] >> 2) - bb[0], y = (cc[1] >> 2) - bb[1], z = (cc[2] >> 2) - bb[2];
+            occ[(x * ey + y) * ez + z].store(1, std::memory_order_relaxed);
+          }
+      }, 1);
+      for (int64_t x = 0; x < ex; ++x)
+        for (int64_t y = 0; y < ey; ++y)
+          for (int64_t z = 0; z < ez; ++z) {
+            auto at = [&](int64_t i, int64_t j, int64_t k) {
+              return i >= 0 && j >= 0 && k >= 0 && occ[(i * ey + j) * ez + k].load(std::memory_order_relaxed) != 0;
+            };
+            pblocks += at(x, y, z);
+            bool g = false;
+            for (int d = 0; d < 8 && !g; ++d) g = at(x - (d >> 2), y - ((d >> 1) & 1), z - (d & 1));
+            gblocks += g;
+          }
+      return;
+    }
+  }
+
   struct K {
     int32_t x, y, z;
     uint32_t r;
@@ -871,12 +1098,17 @@ void count_blocks(const std::vector<float>& pos32, const std::vector<uint32_t>& 
   };
   std::unordered_map<K, char, H> pb, gb;
   size_t n = region.size();
+  K last{0, 0, 0, 0};
+  bool have_last = false;
   for (size_t i = 0; i < n; ++i) {
     int32_t c[3];
     bool ok = true;
     for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h[i], c[a]) && ok;
     if (!ok) continue;
     K k{c[0] >> 2, c[1] >> 2, c[2] >> 2, region[i]};
+    if (have_last && k == last) continue;  // neighbours in insertion order usually share a block
+    last = k;
+    have_last = true;
     if (pb.emplace(k, 1).second)
       for (int d = 0; d < 8; ++d) gb.emplace(K{k.x + (d >> 2), k.y + ((d >> 1) & 1), k.z + (d & 1), k.r}, 1);
   }
@@ -917,16 +1149,6 @@ tmpm_status pack_class(const HostWorld& w, size_t i, uint32_t scene, float rad, 
   }
 }
 
-// Host image of some scenes' enabled particles, ready for upload.
-struct PackedParticles {
-  std::vector<float4> pm, vv;
-  std::vector<float> C, F;  // 9 per particle, particle-major
-  std::vector<uint32_t> cls, ins;
-  std::vector<float> pos32, inv_h;  // for counting blocks
-  std::vector<uint32_t> region;
-  std::vector<float4> disabled;     // snapshot records of disabled particles
-  size_t size() const { return pm.size(); }
-};
 
 uint32_t intern_class(tmpm_session* s, const DevClass& c) {
   // runs of particles usually share a class: compare with the last one first
@@ -951,54 +1173,120 @@ uint32_t intern_class(tmpm_session* s, const DevClass& c) {
   return id;
 }
 
+// Validates and converts particle i of `w`; *en tells whether it is enabled.
+tmpm_status pack_one(const HostWorld& w, size_t i, uint32_t k, float inv_h, float f[8], float cf[9], float ff[9],
+                     DevClass& c, bool* en) {
+  *en = w.enabled[i] != 0;
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) ok = finite_f32(w.pos[3 * i + a], f[a]) && ok;
+  ok = finite_f32(*en ? w.mass[i] : 0.0, f[3]) && ok;
+  for (int a = 0; a < 3; ++a) ok = finite_f32(w.vel[3 * i + a], f[4 + a]) && ok;
+  ok = finite_f32(w.rest_volume[i], f[7]) && ok;
+  float rad, damp;
+  ok = finite_f32(w.radius[i], rad) && finite_f32(w.damping[i], damp) && ok;
+  if (!ok) return fail(TMPM_INVALID_INPUT, "particle state not representable in f32");
+  if (*en && (f[3] <= 0.0f || f[7] <= 0.0f))
+    return fail(TMPM_INVALID_INPUT, "enabled particle mass or volume underflows f32");
+  if (rad <= 0.0f || w.damping[i] < 0.0) return fail(TMPM_INVALID_INPUT, "radius or damping invalid");
+  if (!*en) return TMPM_OK;
+  for (int q = 0; q < 9; ++q)
+    if (!finite_f32(w.C[9 * i + q], cf[q]) || !finite_f32(w.F[9 * i + q], ff[q]))
+      return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
+  tmpm_status st = pack_class(w, i, k, rad, damp, c);
+  if (st != TMPM_OK) return st;
+  // base cell must be representable (gpu.rs:1143-1153)
+  for (int a = 0; a < 3; ++a) {
+    int32_t cc;
+    if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
+  }
+  return TMPM_OK;
+}
+
 // Appends scene k's enabled particles to `out` (insertion index first + i)
-// and records its disabled ones for snapshots.
+// and records its disabled ones for snapshots.  Two parallel passes over
+// fixed chunks (validate and count, then fill at the chunks' offsets); the
+// error reported is the one of the lowest failing index, as in one pass.
 tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
   SceneState& scene = s->scenes[k];
   const HostWorld& w = scene.world;
   const float inv_h = scene.dev.inv_h;
-  for (size_t i = 0; i < w.size(); ++i) {
-    const bool en = w.enabled[i] != 0;
-    float f[8];
-    bool ok = true;
-    for (int a = 0; a < 3; ++a) ok = finite_f32(w.pos[3 * i + a], f[a]) && ok;
-    ok = finite_f32(en ? w.mass[i] : 0.0, f[3]) && ok;
-    for (int a = 0; a < 3; ++a) ok = finite_f32(w.vel[3 * i + a], f[4 + a]) && ok;
-    ok = finite_f32(w.rest_volume[i], f[7]) && ok;
-    float rad, damp;
-    ok = finite_f32(w.radius[i], rad) && finite_f32(w.damping[i], damp) && ok;
-    if (!ok) return fail(TMPM_INVALID_INPUT, "particle state not representable in f32");
-    if (en && (f[3] <= 0.0f || f[7] <= 0.0f))
-      return fail(TMPM_INVALID_INPUT, "enabled particle mass or volume underflows f32");
-    if (rad <= 0.0f || w.damping[i] < 0.0) return fail(TMPM_INVALID_INPUT, "radius or damping invalid");
-    const uint32_t g = scene.first + (uint32_t)i;
-    if (!en) {
-      out.disabled.push_back(make_float4(f[0], f[1], f[2], __builtin_bit_cast(float, g)));
-      out.disabled.push_back(make_float4(f[4], f[5], f[6], f[7]));
-      continue;
+  const size_t n = w.size();
+  constexpr size_t kChunk = 8192;
+  const size_t chunks = (n + kChunk - 1) / kChunk;
+  struct ChunkInfo {
+    size_t enabled = 0, disabled = 0;
+    size_t bad = SIZE_MAX;
+    tmpm_status st = TMPM_OK;
+    std::string msg;
+  };
+  std::vector<ChunkInfo> info(chunks);
+  std::vector<DevClass> classes(n);
+  parallel_for(chunks, [&](size_t cb, size_t ce) {
+    for (size_t c = cb; c < ce; ++c) {
+      ChunkInfo& ci = info[c];
+      for (size_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) {
+        float f[8], cf[9], ff[9];
+        bool en;
+        tmpm_status st = pack_one(w, i, k, inv_h, f, cf, ff, classes[i], &en);
+        if (st != TMPM_OK) {
+          ci.bad = i;
+          ci.st = st;
+          ci.msg = g_last_error;
+          break;
+        }
+        (en ? ci.enabled : ci.disabled)++;
+      }
     }
-    float cf[9], ff[9];
-    for (int q = 0; q < 9; ++q)
-      if (!finite_f32(w.C[9 * i + q], cf[q]) || !finite_f32(w.F[9 * i + q], ff[q]))
-        return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
-    DevClass c;
-    tmpm_status st = pack_class(w, i, k, rad, damp, c);
-    if (st != TMPM_OK) return st;
-    // base cell must be representable (gpu.rs:1143-1153)
-    for (int a = 0; a < 3; ++a) {
-      int32_t cc;
-      if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
-      out.pos32.push_back(f[a]);
-    }
-    out.region.push_back(c.region);
-    out.inv_h.push_back(inv_h);
-    out.pm.push_back(make_float4(f[0], f[1], f[2], f[3]));
-    out.vv.push_back(make_float4(f[4], f[5], f[6], f[7]));
-    out.C.insert(out.C.end(), cf, cf + 9);
-    out.F.insert(out.F.end(), ff, ff + 9);
-    out.cls.push_back(intern_class(s, c));
-    out.ins.push_back(g);
+  }, 1);
+  size_t ne = 0, nd = 0;
+  std::vector<size_t> eoff(chunks), doff(chunks);
+  for (size_t c = 0; c < chunks; ++c) {
+    if (info[c].bad != SIZE_MAX) return fail(info[c].st, info[c].msg);
+    eoff[c] = ne;
+    doff[c] = nd;
+    ne += info[c].enabled;
+    nd += info[c].disabled;
   }
+  const size_t e0 = out.pm.size(), d0 = out.disabled.size();
+  out.pm.resize(e0 + ne);
+  out.vv.resize(e0 + ne);
+  out.C.resize(9 * (e0 + ne));
+  out.F.resize(9 * (e0 + ne));
+  out.cls.resize(e0 + ne);
+  out.ins.resize(e0 + ne);
+  out.pos32.resize(3 * (e0 + ne));
+  out.inv_h.resize(e0 + ne, inv_h);
+  out.region.resize(e0 + ne);
+  out.disabled.resize(d0 + 2 * nd);
+  parallel_for(chunks, [&](size_t cb, size_t ce) {
+    for (size_t c = cb; c < ce; ++c) {
+      size_t e = e0 + eoff[c], d = d0 + 2 * doff[c];
+      for (size_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) {
+        float f[8], cf[9], ff[9];
+        bool en;
+        DevClass dc;
+        pack_one(w, i, k, inv_h, f, cf, ff, dc, &en);  // validated in the first pass
+        const uint32_t g = scene.first + (uint32_t)i;
+        if (!en) {
+          out.disabled[d++] = make_float4(f[0], f[1], f[2], __builtin_bit_cast(float, g));
+          out.disabled[d++] = make_float4(f[4], f[5], f[6], f[7]);
+          continue;
+        }
+        for (int a = 0; a < 3; ++a) out.pos32[3 * e + a] = f[a];
+        out.region[e] = classes[i].region;
+        out.pm[e] = make_float4(f[0], f[1], f[2], f[3]);
+        out.vv[e] = make_float4(f[4], f[5], f[6], f[7]);
+        std::memcpy(&out.C[9 * e], cf, sizeof cf);
+        std::memcpy(&out.F[9 * e], ff, sizeof ff);
+        out.ins[e] = g;
+        ++e;
+      }
+    }
+  }, 1);
+  // class interning is sequential (ids in first-seen order); runs share a class
+  size_t e = e0;
+  for (size_t i = 0; i < n; ++i)
+    if (w.enabled[i]) out.cls[e++] = intern_class(s, classes[i]);
   return TMPM_OK;
 }
 
@@ -1030,7 +1318,7 @@ tmpm_status copy_particles(ParticleBuffers& dst, uint32_t dst_cap, const Particl
 }
 
 // Writes packed particles into the current buffer at slots [at, at + size).
-tmpm_status upload_particles(tmpm_session* s, const PackedParticles& pk, uint32_t at) {
+tmpm_status upload_particles(tmpm_session* s, PackedParticles& pk, uint32_t at) {
   const size_t n = pk.size();
   if (n == 0) return TMPM_OK;
   ParticleBuffers& pb = s->buf[s->cur];
@@ -1039,12 +1327,21 @@ tmpm_status upload_particles(tmpm_session* s, const PackedParticles& pk, uint32_
   HIP_TRY(hipMemcpy(pb.vel_vol.as<float4>() + at, pk.vv.data(), 16ull * n, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.cls.as<uint32_t>() + at, pk.cls.data(), 4ull * n, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.ins_id.as<uint32_t>() + at, pk.ins.data(), 4ull * n, hipMemcpyHostToDevice));
-  std::vector<float> col(n);
+  // component-major columns (9 of C, then 9 of F) transposed in parallel
+  auto& cols = pk.cols;
+  cols.resize(18 * n);
+  parallel_for(n, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i)
+      for (int q = 0; q < 9; ++q) {
+        cols[(size_t)q * n + i] = pk.C[9 * i + q];
+        cols[(size_t)(9 + q) * n + i] = pk.F[9 * i + q];
+      }
+  });
   for (int q = 0; q < 9; ++q) {
-    for (size_t i = 0; i < n; ++i) col[i] = pk.C[9 * i + q];
-    HIP_TRY(hipMemcpy(pb.C.as<float>() + (size_t)q * s->cap + at, col.data(), 4ull * n, hipMemcpyHostToDevice));
-    for (size_t i = 0; i < n; ++i) col[i] = pk.F[9 * i + q];
-    HIP_TRY(hipMemcpy(pb.F.as<float>() + (size_t)q * s->cap + at, col.data(), 4ull * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pb.C.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)q * n], 4ull * n,
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pb.F.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)(9 + q) * n], 4ull * n,
+                      hipMemcpyHostToDevice));
   }
   return TMPM_OK;
 }
@@ -1054,15 +1351,26 @@ tmpm_status upload_scene_forces(tmpm_session* s, uint32_t k) {
   SceneState& scene = s->scenes[k];
   const HostWorld& w = scene.world;
   if (w.size() == 0) return TMPM_OK;
-  std::vector<float4> fz(w.size());
+  auto& fz = s->stage.forces;
+  fz.resize(w.size());
   scene.force_pending = false;
-  for (size_t i = 0; i < w.size(); ++i) {
-    float fx, fy, fzv;
-    if (!finite_f32(w.force[3 * i], fx) || !finite_f32(w.force[3 * i + 1], fy) || !finite_f32(w.force[3 * i + 2], fzv))
-      return fail(TMPM_INVALID_INPUT, "force not representable in f32");
-    fz[i] = make_float4(fx, fy, fzv, 0.f);
-    if (w.enabled[i] && (fx != 0.f || fy != 0.f || fzv != 0.f)) scene.force_pending = true;
-  }
+  std::atomic<bool> bad{false}, pending{false};
+  parallel_for(w.size(), [&](size_t b, size_t e) {
+    bool any = false;
+    for (size_t i = b; i < e; ++i) {
+      float fx, fy, fzv;
+      if (!finite_f32(w.force[3 * i], fx) || !finite_f32(w.force[3 * i + 1], fy) ||
+          !finite_f32(w.force[3 * i + 2], fzv)) {
+        bad = true;
+        return;
+      }
+      fz[i] = make_float4(fx, fy, fzv, 0.f);
+      any = any || (w.enabled[i] && (fx != 0.f || fy != 0.f || fzv != 0.f));
+    }
+    if (any) pending = true;
+  });
+  if (bad) return fail(TMPM_INVALID_INPUT, "force not representable in f32");
+  scene.force_pending = pending;
   HIP_TRY(hipMemcpy(s->force.as<float4>() + scene.first, fz.data(), 16ull * w.size(), hipMemcpyHostToDevice));
   return TMPM_OK;
 }
@@ -1103,7 +1411,18 @@ tmpm_status upload_classes(tmpm_session* s) {
 // Upload all active scenes to the device (gpu.rs:1091-1600 `prepare`,
 // resident).  Insertion indices run scene by scene; the device holds the
 // enabled particles in insertion order, batches with room to admit more.
+tmpm_status prepare_packed(tmpm_session* s, PackedParticles& pk);
+
+// Uploads every stepping scene.  A batch keeps the staging memory for later
+// admissions; a plain session frees it.
 tmpm_status prepare(tmpm_session* s) {
+  s->stage.clear();
+  tmpm_status st = prepare_packed(s, s->stage);
+  if (!s->batch) s->stage.reset();
+  return st;
+}
+
+tmpm_status prepare_packed(tmpm_session* s, PackedParticles& pk) {
   s->prepared = false;
   s->topo_valid = false;
   if (s->scenes.size() > 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
@@ -1124,7 +1443,6 @@ tmpm_status prepare(tmpm_session* s) {
   s->id_space = (uint32_t)total;
   s->host_classes.clear();
   s->class_ids.clear();
-  PackedParticles pk;
   for (uint32_t k = 0; k < s->scenes.size(); ++k) {
     if (!s->scenes[k].stepping()) continue;
     tmpm_status st = pack_scene(s, k, pk);
@@ -1133,7 +1451,7 @@ tmpm_status prepare(tmpm_session* s) {
   s->n = (uint32_t)pk.size();
   s->n_enabled = s->n;
   s->synced_n = 0;
-  s->disabled_host = pk.disabled;
+  s->disabled_host.assign(pk.disabled.begin(), pk.disabled.end());
   if (s->n == 0) {
     // gpu.rs:1358-1360: no live node, nothing to prepare; steps only count.
     return TMPM_OK;
@@ -1386,7 +1704,7 @@ tmpm_status maybe_grow_topology(tmpm_session* s, const uint32_t* counters) {
 
 tmpm_status copy_particles_out(const HostWorld& w, tmpm_particles* out) {
   size_t n = std::min<size_t>(out->count, w.size());
-  auto put = [n](double* dst, const std::vector<double>& src, int k) {
+  auto put = [n](double* dst, const DVec& src, int k) {
     if (dst) std::memcpy(dst, src.data(), sizeof(double) * k * n);
   };
   put(out->position, w.pos, 3);
@@ -1601,14 +1919,15 @@ tmpm_status check_world_desc(const tmpm_world_desc* wd) {
   return TMPM_OK;
 }
 
-void load_scene(SceneState& scene, const tmpm_world_desc* wd, double substep_dt) {
-  HostWorld w;
+// Loads a world description into `scene`; `w` may be an emptied world whose
+// memory is reused.
+void load_scene(SceneState& scene, const tmpm_world_desc* wd, double substep_dt, HostWorld w = HostWorld()) {
   w.params = wd->params;
   w.append(wd->particles, wd->materials, 0);
   w.set_obstacles(wd->obstacles, wd->obstacle_count);
   w.substeps = wd->substeps;
   scene = SceneState();
-  scene.set_world(w);
+  scene.set_world(std::move(w));
   scene.substep_dt = substep_dt;
 }
 
@@ -2089,9 +2408,34 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
   for (uint64_t i = 0; pp.material_index && i < pp.count; ++i)
     if (pp.material_index[i] >= world->material_count) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
   SceneState scene;
-  load_scene(scene, world, substep_dt);
-  for (size_t i = 0; i < scene.world.size(); ++i)
-    if (!scene.world.particle_valid(i)) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  {
+    // best-fitting emptied world of a retired scene, if one is large enough
+    const size_t need = 3 * (size_t)pp.count;
+    size_t best = s->spare_worlds.size();
+    for (size_t i = 0; i < s->spare_worlds.size(); ++i)
+      if (s->spare_worlds[i].pos.capacity() >= need &&
+          (best == s->spare_worlds.size() || s->spare_worlds[i].pos.capacity() < s->spare_worlds[best].pos.capacity()))
+        best = i;
+    if (best < s->spare_worlds.size()) {
+      HostWorld w = std::move(s->spare_worlds[best]);
+      s->spare_worlds.erase(s->spare_worlds.begin() + best);
+      load_scene(scene, world, substep_dt, std::move(w));
+    } else {
+      load_scene(scene, world, substep_dt);
+    }
+  }
+  {
+    std::vector<char> bad(64, 0);
+    parallel_for(scene.world.size(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i)
+        if (!scene.world.particle_valid(i)) {
+          bad[(b / 16384) % 64] = 1;
+          break;
+        }
+    });
+    for (char c : bad)
+      if (c) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  }
   st = validate_resident_world(scene);
   if (st != TMPM_OK) return st;
   uint32_t slot = 0;
@@ -2120,7 +2464,8 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
   const bool appended = slot == s->scenes.size();
   if (appended) s->scenes.push_back(std::move(scene));
   else s->scenes[slot] = std::move(scene);
-  PackedParticles pk;
+  PackedParticles& pk = s->stage;
+  pk.clear();
   st = pack_scene(s, slot, pk);
   auto undo = [&]() {
     if (appended) s->scenes.pop_back();
@@ -2236,6 +2581,15 @@ tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene) {
     kept.push_back(s->disabled_host[i + 1]);
   }
   s->disabled_host.swap(kept);
+  {
+    // keep the emptied world's memory for a later admission (at most 8
+    // worlds and 2 GiB)
+    HostWorld w = std::move(gone.world);
+    w.clear_keep();
+    size_t kept_bytes = w.capacity_bytes();
+    for (const HostWorld& o : s->spare_worlds) kept_bytes += o.capacity_bytes();
+    if (s->spare_worlds.size() < 8 && kept_bytes <= (2ull << 30)) s->spare_worlds.push_back(std::move(w));
+  }
   s->scenes[scene] = SceneState();
   s->scenes[scene].active = false;
   if (!s->prepared) return TMPM_OK;

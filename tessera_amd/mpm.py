@@ -419,6 +419,12 @@ class MpmParticle:
             enabled=self.enabled, fixed=self.fixed)
 
 
+def _column_major(m: np.ndarray) -> np.ndarray:
+    """(n, 3, 3) row-indexed view of a copy of `m` whose memory is column-major per matrix."""
+    buf = np.ascontiguousarray(np.transpose(m, (0, 2, 1)))
+    return buf.transpose(0, 2, 1)
+
+
 class ParticleArrays:
     """Structure-of-arrays particle storage in insertion order."""
 
@@ -428,8 +434,10 @@ class ParticleArrays:
     def __init__(self, n: int, materials: Sequence[MaterialModel] | None = None):
         self.position = np.zeros((n, 3))
         self.velocity = np.zeros((n, 3))
-        self.deformation = np.tile(np.eye(3), (n, 1, 1))
-        self.affine = np.zeros((n, 3, 3))
+        # matrices are (n, 3, 3) views of column-major storage, the layout of
+        # the C ABI (nalgebra order), so handing them to the library copies nothing
+        self.deformation = _column_major(np.tile(np.eye(3), (n, 1, 1)))
+        self.affine = _column_major(np.zeros((n, 3, 3)))
         self.force = np.zeros((n, 3))
         self.mass = np.zeros(n)
         self.rest_volume = np.zeros(n)
@@ -489,7 +497,7 @@ class ParticleArrays:
     def copy(self) -> "ParticleArrays":
         out = ParticleArrays(0, self.materials)
         for f in self.FIELDS:
-            setattr(out, f, getattr(self, f).copy())
+            setattr(out, f, getattr(self, f).copy(order="K"))
         return out
 
     def concat(self, other: "ParticleArrays") -> "ParticleArrays":
@@ -905,8 +913,8 @@ def _read_particle_arrays(n: int, read, lib) -> ParticleArrays:
     _check(read(C.byref(cp)), lib)
     out.position = flat["position"]
     out.velocity = flat["velocity"]
-    out.deformation = flat["deformation"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
-    out.affine = flat["affine"].reshape(n, 3, 3).transpose(0, 2, 1).copy()
+    out.deformation = flat["deformation"].reshape(n, 3, 3).transpose(0, 2, 1)
+    out.affine = flat["affine"].reshape(n, 3, 3).transpose(0, 2, 1)
     out.force = flat["force"]
     out.mass = flat["mass"]
     out.rest_volume = flat["rest_volume"]
@@ -1114,6 +1122,21 @@ class HipMpmResidentSession:
         return {k: (avg[i], int(calls[i])) for i, k in enumerate(keys)}
 
 
+class _SceneHeader:
+    """What a batch keeps of a scene's world on the host: everything but the
+    particle state, which lives on the device (and in the library's host
+    copy) and is read back with the scene."""
+
+    __slots__ = ("params", "obstacles", "use_cpic", "materials", "material_index")
+
+    def __init__(self, world: MpmWorld):
+        self.params = world.params
+        self.obstacles = list(world.obstacles)
+        self.use_cpic = world.use_cpic
+        self.materials = list(world.particles.materials)
+        self.material_index = np.array(world.particles.material_index, dtype=np.uint32)
+
+
 class HipMpmBatch:
     """Independent worlds advanced together, one kernel launch per stage per
     substep for all of them (the batch/serving boundary; the reference steps
@@ -1139,7 +1162,7 @@ class HipMpmBatch:
                                           C.byref(h)), self.lib)
         del keep
         self._h = h
-        self._worlds = {k: w.clone() for k, w in enumerate(worlds)}
+        self._worlds = {k: _SceneHeader(w) for k, w in enumerate(worlds)}
         self.p2g_mode = p2g_mode
 
     def close(self) -> None:
@@ -1195,7 +1218,7 @@ class HipMpmBatch:
         n = int(self.lib.tmpm_batch_scene_particle_count(self._h, int(scene)))
         out = _read_particle_arrays(n, lambda cp: self.lib.tmpm_batch_read_particles(self._h, int(scene), cp),
                                     self.lib)
-        src = self._worlds[int(scene)].particles
+        src = self._worlds[int(scene)]
         out.materials = list(src.materials)
         out.material_index = src.material_index.copy() if len(src.material_index) == n else np.zeros(n, np.uint32)
         return out
@@ -1223,7 +1246,7 @@ class HipMpmBatch:
         slot = C.c_uint32()
         _check(self.lib.tmpm_batch_add_scene(self._h, C.byref(desc), float(substep_dt), C.byref(slot)), self.lib)
         del keep
-        self._worlds[int(slot.value)] = world.clone()
+        self._worlds[int(slot.value)] = _SceneHeader(world)
         return int(slot.value)
 
     def set_reactions(self, enabled: bool) -> None:
