@@ -96,7 +96,14 @@ struct DevObstacle {
   // reactions: contributions are added as int64 floor(value * 2^shift)
   float rscale_lin, rscale_ang;  // 2^(shift - 32), the fixed64_of input scale
   int32_t rshift_lin, rshift_ang;
+  float tri[9];          // triangle prism: local vertices a, b, c
+  float bound;           // prism, convex: radius of the contact sphere about center
+  int32_t plane_begin;   // convex: planes [plane_begin, plane_begin + plane_count) of Topo::planes
+  int32_t plane_count;
+  int32_t cpic_group;    // -1: none
+  int32_t pad4[3];
 };
+constexpr int kMaxConvexPlanes = 128;  // planes per convex obstacle (gpu.rs pack_obstacle)
 
 // Rigid body state (GpuRigidBodyState, phys/src/gpu_rigid_state.rs:17-28):
 // position and inverse mass, XYZW orientation, world velocities, body-frame
@@ -174,6 +181,7 @@ struct Topo {
   uint32_t* grid_rev;       // per grid slot, 8 entries: particle block whose tile covers it at offset d
   float4* grid_vel;
   uint32_t* node_mask;      // 2 words per grid slot (diagnostic live-node set)
+  const float4* planes;     // convex obstacle planes {nx, ny, nz, offset}, local frame
 };
 
 }  // namespace tmpm

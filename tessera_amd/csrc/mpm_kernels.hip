@@ -838,11 +838,144 @@ __device__ void obstacle_surface(const DevObstacle& o, float3 point, float& dist
   }
 }
 
+// Closest point of triangle abc to p (Voronoi-region walk; gpu.wgsl
+// closest_point_triangle, obstacle.rs closest_point_triangle).
+__device__ float3 closest_point_triangle(float3 p, float3 a, float3 b, float3 c) {
+  const float3 ab = b - a, ac = c - a, ap = p - a;
+  const float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return a;
+  const float3 bp = p - b;
+  const float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return b;
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return a + ab * (d1 / (d1 - d3));
+  const float3 cp = p - c;
+  const float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return c;
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return a + ac * (d2 / (d2 - d6));
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && d4 - d3 >= 0.0f && d5 - d6 >= 0.0f)
+    return b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)));
+  const float r = 1.0f / (va + vb + vc);
+  return a + ab * (vb * r) + ac * (vc * r);
+}
+
+__device__ __forceinline__ float3 unit3(float3 v) { return v / len3(v); }
+
+// Thin triangular prism in its local frame (obstacle.rs:633-694): the closest
+// of its 8 triangular faces; ties within epsilon average the face normals.
+__device__ void prism_surface(const float* tri, float ht, float3 point, float& dist, float3& outward) {
+  const float3 a = make_float3(tri[0], tri[1], tri[2]);
+  const float3 b = make_float3(tri[3], tri[4], tri[5]);
+  const float3 c = make_float3(tri[6], tri[7], tri[8]);
+  const float3 normal = unit3(cross3(b - a, c - a));
+  const float3 off = normal * ht;
+  const float3 pr[6] = {a + off, b + off, c + off, a - off, b - off, c - off};
+  constexpr int faces[8][3] = {{0, 1, 2}, {3, 5, 4}, {3, 4, 1}, {3, 1, 0}, {4, 5, 2}, {4, 2, 1}, {5, 3, 0}, {5, 0, 2}};
+  const float edge = fmaxf(fmaxf(len3(b - a), len3(c - b)), fmaxf(len3(a - c), ht));
+  const float eps = edge * 1e-6f;
+  bool inside = true;
+  float3 closest = make_float3(0.f, 0.f, 0.f), best_n = normal, tied = make_float3(0.f, 0.f, 0.f);
+  float best = 1e30f;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    const float3 va = pr[faces[f][0]], vb = pr[faces[f][1]], vc = pr[faces[f][2]];
+    const float3 fn = unit3(cross3(vb - va, vc - va));
+    inside = inside && dot3(point - va, fn) <= eps;
+    const float3 cand = closest_point_triangle(point, va, vb, vc);
+    const float3 dd = point - cand;
+    const float d2 = dot3(dd, dd);
+    if (d2 + eps * eps < best) {
+      best = d2;
+      closest = cand;
+      best_n = fn;
+      tied = fn;
+    } else if (fabsf(d2 - best) <= eps * eps) {
+      tied = tied + fn;
+    }
+  }
+  const float d = sqrtf(best);
+  const float tl = len3(tied);
+  const float3 feature = tl > 1e-12f ? tied / tl : best_n;
+  outward = (!inside && d > eps) ? (point - closest) / d : feature;
+  dist = inside ? -d : d;
+}
+
+// Convex polyhedron in its local frame (obstacle.rs:696-745): inside, the
+// largest plane distance; outside, the distance to the Dykstra projection
+// onto the intersection of the half-spaces (64 sweeps, f32 tolerance as in
+// gpu.wgsl convex_surface).  Each plane's correction is kept as the scalar
+// it removed along that plane's normal.
+__device__ void convex_surface(const float4* planes, int count, float bound, float3 point, float& dist,
+                               float3& normal) {
+  float maximum = -1e30f;
+  float3 nn = make_float3(0.f, 0.f, 1.f);
+  for (int i = 0; i < count; ++i) {
+    const float4 pl = planes[i];
+    const float sd = pl.x * point.x + pl.y * point.y + pl.z * point.z - pl.w;
+    if (sd > maximum) {
+      maximum = sd;
+      nn = make_float3(pl.x, pl.y, pl.z);
+    }
+  }
+  if (maximum <= 0.0f) {
+    dist = maximum;
+    normal = nn;
+    return;
+  }
+  float corr[kMaxConvexPlanes];
+  for (int i = 0; i < count; ++i) corr[i] = 0.0f;
+  float3 proj = point;
+  const float tol = fmaxf(bound, 1.0f) * 1e-5f;
+  for (int it = 0; it < 64; ++it) {
+    const float3 prev = proj;
+    for (int i = 0; i < count; ++i) {
+      const float4 pl = planes[i];
+      const float3 n = make_float3(pl.x, pl.y, pl.z);
+      const float3 shifted = proj + n * corr[i];
+      const float viol = fmaxf(dot3(n, shifted) - pl.w, 0.0f);
+      proj = shifted - n * viol;
+      corr[i] = viol;
+    }
+    if (len3(proj - prev) <= tol) break;
+  }
+  const float3 offset = point - proj;
+  const float sd = len3(offset);
+  if (sd > tol) {
+    dist = sd;
+    normal = offset / sd;
+  } else {
+    dist = maximum;
+    normal = nn;
+  }
+}
+
+// Surface of any obstacle.  SHAPES = false instantiations (no triangle prism
+// or convex obstacle in the session) leave those paths out.
+template <bool SHAPES>
+__device__ __forceinline__ void surface_of(const DevObstacle& o, const float4* planes, float3 point, float& dist,
+                                           float3& normal) {
+  if (SHAPES && o.shape >= 6) {
+    const float3 l = qrot_inv(o.q, point - make_float3(o.center[0], o.center[1], o.center[2]));
+    float3 nn;
+    if (o.shape == 6) prism_surface(o.tri, o.dims[0], l, dist, nn);
+    else convex_surface(planes + o.plane_begin, o.plane_count, o.bound, l, dist, nn);
+    normal = qrot(o.q, nn);
+    return;
+  }
+  obstacle_surface(o, point, dist, normal);
+}
+
 __device__ __forceinline__ bool obstacle_may_contact(const DevObstacle& o, float3 point, float margin) {
   if (o.shape == 2) {
     float3 rel = point - make_float3(o.center[0], o.center[1], o.center[2]);
     float b = o.dims[0] + o.dims[1] + margin;
     return dot3(rel, rel) <= b * b;
+  }
+  if (o.shape >= 6) {  // gpu.wgsl obstacle_may_contact: bounding sphere
+    float3 rel = point - make_float3(o.center[0], o.center[1], o.center[2]);
+    return len3(rel) <= o.bound + margin;
   }
   return true;
 }
@@ -1059,7 +1192,7 @@ __device__ __forceinline__ void add_reaction(const Topo& t, const DevObstacle& o
   }
 }
 
-template <int FIXED, int REACT>
+template <int FIXED, int REACT, bool SHAPES>
 __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* scenes, const DevObstacle* obstacles) {
   const uint32_t ng = t.counters[CNT_GRID_BLOCKS];
   const uint32_t lane = threadIdx.x & 63;
@@ -1123,7 +1256,7 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
       float3 nrm = make_float3(0.f, 0.f, 0.f);
       const float3 before = v;
       if (live && obstacle_may_contact(o, pos, sc.h)) {
-        obstacle_surface(o, pos, dist, nrm);
+        surface_of<SHAPES>(o, t.planes, pos, dist, nrm);
         if (dist <= sc.h * (1.0f + 1e-5f)) {
           contact = true;
           v = contact_velocity(o, pos, v, nrm);
@@ -1147,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // REACT = 1 runs with one 64-lane wavefront per workgroup: each lane adds its
 // particles' obstacle impulses into its own LDS column (no atomics), and the
 // columns are reduced in a fixed order at the end of the block.
-template <int REACT>
+template <int REACT, bool SHAPES>
 __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* classes, const DevScene* scenes,
                                              const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
@@ -1269,7 +1402,7 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
           if (!obstacle_may_contact(o, x, radius)) continue;
           float dist;
           float3 nrm;
-          obstacle_surface(o, x, dist, nrm);
+          surface_of<SHAPES>(o, t.planes, x, dist, nrm);
           if (dist < radius) {
             x = x + nrm * (radius - dist);
             const float3 before = v;
@@ -1636,23 +1769,33 @@ void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene
 #undef TMPM_P2G_T
 #undef TMPM_P2G
 }
-void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react,
-                        hipStream_t s) {
-  uint32_t blocks = grid_for(t.grid_cap, 4, 16384);
+template <bool SHAPES>
+static void grid_update_variant(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react,
+                                uint32_t blocks, hipStream_t s) {
   if (fixed64) {
-    if (react) hipLaunchKernelGGL((k_grid_update<1, 1>), dim3(blocks), dim3(256), 0, s, t, sc, o);
-    else hipLaunchKernelGGL((k_grid_update<1, 0>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+    if (react) hipLaunchKernelGGL((k_grid_update<1, 1, SHAPES>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+    else hipLaunchKernelGGL((k_grid_update<1, 0, SHAPES>), dim3(blocks), dim3(256), 0, s, t, sc, o);
   } else {
-    if (react) hipLaunchKernelGGL((k_grid_update<0, 1>), dim3(blocks), dim3(256), 0, s, t, sc, o);
-    else hipLaunchKernelGGL((k_grid_update<0, 0>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+    if (react) hipLaunchKernelGGL((k_grid_update<0, 1, SHAPES>), dim3(blocks), dim3(256), 0, s, t, sc, o);
+    else hipLaunchKernelGGL((k_grid_update<0, 0, SHAPES>), dim3(blocks), dim3(256), 0, s, t, sc, o);
   }
 }
+void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react, int shapes,
+                        hipStream_t s) {
+  uint32_t blocks = grid_for(t.grid_cap, 4, 16384);
+  if (shapes) grid_update_variant<true>(t, sc, o, fixed64, react, blocks, s);
+  else grid_update_variant<false>(t, sc, o, fixed64, react, blocks, s);
+}
 void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                uint32_t blocks, uint32_t threads, int react, hipStream_t s) {
-  if (react)  // one wavefront per workgroup: the lane-private reaction columns assume 64 lanes
-    hipLaunchKernelGGL(k_g2p<1>, dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
-  else
-    hipLaunchKernelGGL(k_g2p<0>, dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+                uint32_t blocks, uint32_t threads, int react, int shapes, hipStream_t s) {
+  // REACT: one wavefront per workgroup (the lane-private reaction columns assume 64 lanes)
+  if (react) {
+    if (shapes) hipLaunchKernelGGL((k_g2p<1, true>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
+    else hipLaunchKernelGGL((k_g2p<1, false>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
+  } else {
+    if (shapes) hipLaunchKernelGGL((k_g2p<0, true>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+    else hipLaunchKernelGGL((k_g2p<0, false>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+  }
 }
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s) {
   hipLaunchKernelGGL(k_react_finalize, dim3(grid_for(t.obstacle_total, 64, 1024)), dim3(64), 0, s, t, o);

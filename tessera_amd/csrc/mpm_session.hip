@@ -602,8 +602,9 @@ struct tmpm_session {
   std::map<std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t>, uint32_t>
       class_ids;
   DevBuf force;            // float4 per particle, insertion order
-  DevBuf classes, scene_plain, scene_force, obstacles, scene_err;
-  uint32_t obstacle_cap = 0, scene_cap = 0;
+  DevBuf classes, scene_plain, scene_force, obstacles, scene_err, planes;
+  uint32_t obstacle_cap = 0, scene_cap = 0, plane_cap = 0;
+  int shapes = 0;          // some obstacle is a triangle prism or convex polyhedron
   // reactions and rigid coupling
   bool reactions_on = false;
   DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
@@ -717,7 +718,47 @@ bool pack_obstacle(const tmpm_obstacle& o, DevObstacle& d) {
     ok = finite_f32(o.orientation[k], d.q[k]) && ok;
     ok = finite_f32(o.dims[k], d.dims[k]) && ok;
   }
+  d.cpic_group = o.cpic_group;
+  d.plane_begin = d.plane_count = 0;
+  if (o.shape == TMPM_SHAPE_TRIANGLE_PRISM) {
+    // gpu.rs pack_obstacle: f32 vertices must keep a non-degenerate triangle
+    for (int k = 0; k < 9; ++k) ok = finite_f32(o.vertices[k], d.tri[k]) && ok;
+    double e1[3], e2[3], cr[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = (double)d.tri[3 + k] - d.tri[k];
+      e2[k] = (double)d.tri[6 + k] - d.tri[k];
+    }
+    cr[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    cr[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    const double area2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+    ok = ok && d.dims[0] > 0.0f && std::isfinite(area2) && area2 > 0.0;
+    double r = 0.0;
+    for (int v = 0; v < 3; ++v)
+      r = std::max(r, std::sqrt(o.vertices[3 * v] * o.vertices[3 * v] + o.vertices[3 * v + 1] * o.vertices[3 * v + 1] +
+                                o.vertices[3 * v + 2] * o.vertices[3 * v + 2]));
+    ok = finite_f32((r + o.dims[0]) * (1.0 + 1e-5), d.bound) && ok;
+  } else if (o.shape == TMPM_SHAPE_CONVEX) {
+    ok = finite_f32(o.dims[0] * (1.0 + 1e-5), d.bound) && ok;
+  }
   return ok;
+}
+
+// Convex planes of one obstacle in f32, appended to `out` (gpu.rs pack_obstacle).
+tmpm_status pack_planes(const tmpm_obstacle& o, DevObstacle& d, std::vector<float4>& out) {
+  if (o.plane_count > (uint32_t)kMaxConvexPlanes)
+    return fail(TMPM_CAPACITY, "convex obstacle has more planes than the device supports (128)");
+  d.plane_begin = (int32_t)out.size();
+  d.plane_count = (int32_t)o.plane_count;
+  for (uint32_t i = 0; i < o.plane_count; ++i) {
+    float v[4];
+    bool ok = true;
+    for (int k = 0; k < 4; ++k) ok = finite_f32(o.planes[4 * i + k], v[k]) && ok;
+    const float n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (!ok || !std::isfinite(n2) || n2 <= 0.0f) return fail(TMPM_INVALID_INPUT, "convex plane not representable in f32");
+    out.push_back(make_float4(v[0], v[1], v[2], v[3]));
+  }
+  return TMPM_OK;
 }
 
 bool pack_rigid(const tmpm_rigid_body& b, DevRigid& d) {
@@ -784,6 +825,8 @@ double obstacle_reach(const tmpm_obstacle& o, double h) {
 tmpm_status upload_obstacles(tmpm_session* s) {
   std::vector<DevObstacle> packed;
   std::vector<DevMapping> maps;
+  std::vector<float4> planes;
+  bool shapes = false;
   for (auto& scene : s->scenes) {
     scene.dev.obstacle_begin = (int32_t)packed.size();
     scene.dev.obstacle_count = 0;
@@ -795,13 +838,15 @@ tmpm_status upload_obstacles(tmpm_session* s) {
       return fail(TMPM_INVALID_INPUT, "one rigid mapping per obstacle is required");
     for (auto& ob : scene.world.obstacles) {
       const tmpm_obstacle& o = ob.o;
-      if (o.shape == TMPM_SHAPE_TRIANGLE_PRISM || o.shape == TMPM_SHAPE_CONVEX)
-        return fail(TMPM_RESIDENT_UNAVAILABLE,
-                    "triangle-prism and convex obstacles are not supported by the HIP backend");
       if (o.cpic_group >= 0 && scene.world.params.use_cpic)
         return fail(TMPM_RESIDENT_UNAVAILABLE, "CPIC obstacle groups are not supported by the HIP backend");
       DevObstacle d;
       if (!pack_obstacle(o, d)) return fail(TMPM_INVALID_INPUT, "obstacle is not representable in f32");
+      if (o.shape == TMPM_SHAPE_CONVEX) {
+        tmpm_status st = pack_planes(o, d, planes);
+        if (st != TMPM_OK) return st;
+      }
+      shapes = shapes || o.shape == TMPM_SHAPE_TRIANGLE_PRISM || o.shape == TMPM_SHAPE_CONVEX;
       // reaction units: momentum at the scene's P2G scale; angular impulse
       // scaled down by the lever-arm bound so both stay well inside int64
       const int lin = std::max(-90, std::min(150, scene.dev.mom_shift));
@@ -839,6 +884,14 @@ tmpm_status upload_obstacles(tmpm_session* s) {
     HIP_TRY(hipMemcpy(s->obstacles.ptr, packed.data(), packed.size() * sizeof(DevObstacle), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->rigid_maps.ptr, maps.data(), maps.size() * sizeof(DevMapping), hipMemcpyHostToDevice));
   }
+  if (planes.size() > s->plane_cap || !s->planes.ptr) {
+    s->plane_cap = std::max<uint32_t>(16, (uint32_t)planes.size());
+    HIP_TRY(s->planes.alloc(sizeof(float4) * s->plane_cap));
+  }
+  if (!planes.empty())
+    HIP_TRY(hipMemcpy(s->planes.ptr, planes.data(), planes.size() * sizeof(float4), hipMemcpyHostToDevice));
+  s->topo.planes = s->planes.as<float4>();
+  s->shapes = shapes ? 1 : 0;
   s->obstacle_total = (uint32_t)packed.size();
   s->topo.obstacle_total = s->obstacle_total;
   if (s->react_active() && s->obstacle_total > 0) {
@@ -1548,11 +1601,11 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
                  [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_copies, s->stream); })) != hipSuccess)
     return e;
-  if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, react, s->stream); })) !=
+  if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, react, s->shapes, s->stream); })) !=
       hipSuccess)
     return e;
   // The P2G kernel template takes 256-thread launch bounds; G2P uses the same.
-  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, react, s->stream); })) !=
+  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, react, s->shapes, s->stream); })) !=
       hipSuccess)
     return e;
   if (react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, ob, s->stream); })) != hipSuccess)

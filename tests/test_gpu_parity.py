@@ -343,3 +343,89 @@ def test_rigid_obstacles_match_cpu_one_way_coupling(transfers):
                 cpu.substeps_fixed(5e-4, 1)
             s.step(4)
             assert_tracks(cpu.read(), s.read_particles(), 1e-5, 1e-4)
+
+
+def _reference_shape_cases():
+    """The 14 (obstacle, particle position) cases of gpu.rs:4719-4797."""
+    from tessera_amd.mpm import quat_from_axis_angle
+
+    def moving(ob, lin=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0), friction=0.0):
+        ob.linear_velocity = np.array(lin, float)
+        ob.angular_velocity = np.array(ang, float)
+        ob.friction = friction
+        return ob
+
+    c = [0.5, 0.5, 0.5]
+    tilted = quat_from_axis_angle([0, 1, 0], math.pi / 2)
+    tri = [[-0.15, -0.15, 0.0], [0.15, -0.15, 0.0], [0.0, 0.15, 0.0]]
+    X, Y, Z = np.eye(3)
+    convex = lambda: moving(RigidObstacle.convex([0.45, 0.45, 0.45], [np.zeros(3), X * 0.2, Y * 0.2, Z * 0.2],
+                                                 [-X, -Y, -Z, np.ones(3) / math.sqrt(3.0)]), lin=(-0.5, 0, 0))
+    return [
+        (moving(RigidObstacle.sphere(c, 0.1), lin=(1.0, 0.2, 0.0), friction=0.3), [0.62, 0.5, 0.5]),
+        (moving(RigidObstacle.cuboid(c, [0.1] * 3, quat_from_axis_angle([0, 0, 1], math.pi / 4)), lin=(0.5, 0, 0),
+                ang=(0, 0, 0.5), friction=0.2), [0.65, 0.5, 0.5]),
+        (moving(RigidObstacle.capsule(c, 0.1, 0.1), lin=(0.5, 0, 0)), [0.62, 0.5, 0.5]),
+        (moving(RigidObstacle.capsule(c, 0.1, 0.0), lin=(0.5, 0, 0)), [0.54, 0.5, 0.5]),
+        (moving(RigidObstacle.cylinder(c, 0.1, 0.1), lin=(0.5, 0, 0)), [0.62, 0.5, 0.5]),
+        (moving(RigidObstacle.cone(c, 0.1, 0.1), lin=(0.5, 0, 0)), [0.56, 0.5, 0.5]),
+        (moving(RigidObstacle.capsule(c, 0.1, 0.1, tilted), lin=(0, 0, 0.5)), [0.5, 0.5, 0.62]),
+        (moving(RigidObstacle.cylinder(c, 0.1, 0.1, tilted), lin=(0, 0, 0.5)), [0.5, 0.5, 0.62]),
+        (moving(RigidObstacle.cone(c, 0.1, 0.1, tilted), lin=(0, 0.5, 0)), [0.5, 0.56, 0.5]),
+        (moving(RigidObstacle.triangle_prism(c, tri, 1e-4), lin=(0, 0, 0.5)), [0.5, 0.45, 0.54]),
+        (moving(RigidObstacle.triangle_prism(c, tri, 1e-4, tilted), lin=(0.5, 0, 0)), [0.54, 0.45, 0.5]),
+        (convex(), [0.43, 0.48, 0.48]),
+        (convex(), [0.43, 0.42, 0.48]),
+        (convex(), [0.69, 0.47, 0.47]),
+    ]
+
+
+@pytest.mark.parametrize("case", range(14))
+def test_every_obstacle_shape_matches_cpu_one_way_coupling(transfers, case):
+    # gpu.rs:4706-4827: one particle against each shape, step(0.002), 1e-5 / 1e-4
+    ob, position = _reference_shape_cases()[case]
+    p = MpmParticle.new(position, 0.03, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    w = world_of([p], MpmParams(gravity=np.zeros(3)), [ob])
+    cpu = O.OracleWorld(w)
+    cpu.step(0.002)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep())
+    s.step_adaptive(0.002)
+    assert s.substeps == cpu.substeps
+    assert_tracks(cpu.read(), s.read_particles(), 1e-5, 1e-4)
+    got = s.obstacles()[0]
+    assert got.shape == ob.shape and np.allclose(got.center, ob.center)
+
+
+def test_two_convex_obstacles_use_their_own_planes(transfers):
+    # gpu.rs:4858-4920: planes of consecutive convex obstacles are packed back to back
+    X, Y, Z = np.eye(3)
+
+    def make(center, speed):
+        o = RigidObstacle.convex(center, [np.zeros(3), X * 0.15, Y * 0.15, Z * 0.15],
+                                 [-X, -Y, -Z, np.ones(3) / math.sqrt(3.0)])
+        o.linear_velocity = np.array([speed, 0.0, 0.0])
+        return o
+
+    ps = [MpmParticle.new([x, 0.48, 0.48], 0.03, 1000.0, MaterialModel.elastic(1000.0, 0.2)) for x in (0.28, 0.68)]
+    w = world_of(ps, MpmParams(gravity=np.zeros(3)), [make([0.3, 0.45, 0.45], -0.5), make([0.7, 0.45, 0.45], -1.0)])
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep())
+    s.step_adaptive(0.001)
+    assert_tracks(cpu.read(), s.read_particles(), 1e-5, 1e-4)
+    moved = np.linalg.norm(s.read_particles().velocity, axis=1)
+    assert np.all(moved > 0.1)  # both particles were pushed by their own obstacle
+
+
+def test_convex_with_more_planes_than_the_device_holds_is_a_capacity_error(transfers):
+    # gpu.rs:4929-4950: more than 128 planes is Capacity, not silently truncated
+    k = 130
+    angles = np.linspace(0.0, 2.0 * math.pi, k, endpoint=False)
+    normals = [np.array([math.cos(a), math.sin(a), 0.0]) for a in angles[: k - 2]] + [np.array([0, 0, 1.0]),
+                                                                                        np.array([0, 0, -1.0])]
+    verts = [np.array([math.cos(a), math.sin(a), z]) * 0.1 for a in angles for z in (-1.0, 1.0)]
+    ob = RigidObstacle.convex([0.5, 0.5, 0.5], verts, normals)
+    p = MpmParticle.new([0.2, 0.2, 0.2], 0.03, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    w = world_of([p], MpmParams(gravity=np.zeros(3)), [ob])
+    with pytest.raises(Capacity):
+        HipMpmResidentSession(transfers, w, 1e-3)
