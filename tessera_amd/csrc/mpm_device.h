@@ -182,6 +182,10 @@ struct Topo {
   float4* grid_vel;
   uint32_t* node_mask;      // 2 words per grid slot (diagnostic live-node set)
   const float4* planes;     // convex obstacle planes {nx, ny, nz, offset}, local frame
+  // CPIC (sessions with CPIC obstacle groups): per particle slot, the side
+  // mask of this substep; per bucket, the mask of its block key
+  uint32_t* pmask;
+  uint32_t* bmask;
 };
 
 }  // namespace tmpm

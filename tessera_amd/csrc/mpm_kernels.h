@@ -8,8 +8,13 @@
 namespace tmpm {
 
 void launch_topo_clear(const Topo& t, hipStream_t s);
-void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);
-void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);
+// cpic != 0: block keys carry the particles' CPIC side masks (t.pmask, t.bmask)
+void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
+                       hipStream_t s);
+void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
+                       hipStream_t s);
+void launch_cpic_mask(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
+                      hipStream_t s);
 void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s);
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s);

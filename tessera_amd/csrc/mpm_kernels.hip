@@ -97,6 +97,17 @@ __device__ __forceinline__ uint32_t hash_key(int4 k) {
   return fmix32(h);
 }
 __device__ __forceinline__ bool key_eq(int4 a, int4 b) { return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w; }
+// With CPIC the block key has a fifth word, the particles' side mask (the
+// reference's transfer key, world.rs:470-490): blocks of one cell range but
+// different masks are distinct and never share nodes.
+__device__ __forceinline__ uint32_t hash_key_m(int4 k, uint32_t m) {
+  uint32_t h = (uint32_t)k.x * 0x9E3779B1u;
+  h ^= (uint32_t)k.y * 0x85EBCA77u;
+  h ^= (uint32_t)k.z * 0xC2B2AE3Du;
+  h ^= (uint32_t)k.w * 0x27D4EB2Fu;
+  h ^= m * 0x165667B1u;
+  return fmix32(h);
+}
 
 // Block key of a particle that binning accepted (cell valid).
 __device__ __forceinline__ int4 particle_key(const Parts& p, const DevClass* classes, const DevScene* scenes,
@@ -121,26 +132,42 @@ __device__ __forceinline__ int4 owner_key(const Parts& p, const Topo& t, const D
   return k;
 }
 
+// Side mask of the key an owner refers to (CPIC): a particle's own mask, or
+// for a neighbour insert the mask of the particle that owns the source bucket.
+__device__ __forceinline__ uint32_t owner_mask(const Parts& p, const Topo& t, uint32_t owner) {
+  if (owner <= p.n) return t.pmask[owner - 1];
+  const uint32_t r = owner - p.n - 1;
+  return t.pmask[t.owner[r >> 3] - 1];
+}
+
 // Lock-free linear-probing insert.  Owners refer to keys that are immutable
 // during the kernel (particle positions, or a particle block's key plus a
 // fixed offset), so a losing thread compares against the winner's key
 // without waiting for anything to be published.  The winner also stores the
 // key in bkey; a loser that already sees it there skips recomputing the key
 // from the owner (a chain of dependent loads).  Bounded by the table size.
+// With CPIC a loser always rebuilds the winner's key and mask from the owner
+// (the claimer's two stores are not published together).
+template <bool CPIC = false>
 __device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
-                                int4 key, uint32_t me, bool& claimed) {
-  uint32_t b = hash_key(key) & t.bucket_mask;
+                                int4 key, uint32_t me, bool& claimed, uint32_t mask = 0) {
+  uint32_t b = (CPIC ? hash_key_m(key, mask) : hash_key(key)) & t.bucket_mask;
   claimed = false;
   for (uint32_t probe = 0; probe <= t.bucket_mask; ++probe) {
     uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
     if (prev == 0u) {
       claimed = true;
       t.bkey[b] = key;
+      if (CPIC) t.bmask[b] = mask;
       return b;
     }
-    int4 k = t.bkey[b];
-    if (k.x == kNoKey) k = owner_key(p, t, classes, scenes, prev);
-    if (key_eq(k, key)) return b;
+    if (CPIC) {
+      if (key_eq(owner_key(p, t, classes, scenes, prev), key) && owner_mask(p, t, prev) == mask) return b;
+    } else {
+      int4 k = t.bkey[b];
+      if (k.x == kNoKey) k = owner_key(p, t, classes, scenes, prev);
+      if (key_eq(k, key)) return b;
+    }
     b = (b + 1) & t.bucket_mask;
   }
   return kInvalid;
@@ -246,6 +273,7 @@ __global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
 // this workgroup claims are listed in its part of bin_stage; ids are handed
 // out from prefix sums of the per-workgroup counts (k_bin_blocks), since
 // returning atomics on one shared counter serialize device-wide.
+template <bool CPIC>
 __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
                                                             const DevScene* scenes) {
   __shared__ uint32_t s_claims;
@@ -272,6 +300,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
     }
     bool active = false;
     int4 key = make_int4(0, 0, 0, 0);
+    const uint32_t mask = CPIC && i < p.n ? t.pmask[i] : 0u;
     if (i < p.n && pm.w > 0.0f) {
       const DevClass& c = classes[ci];
       int3 cell;
@@ -286,7 +315,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
     const int4 prev = make_int4(__shfl_up(key.x, 1, 64), __shfl_up(key.y, 1, 64), __shfl_up(key.z, 1, 64),
                                 __shfl_up(key.w, 1, 64));
     const bool prev_active = __shfl_up((int)active, 1, 64) != 0;
-    const bool head = active && (lane == 0 || !prev_active || !key_eq(key, prev));
+    const bool same_mask = !CPIC || __shfl_up(mask, 1, 64) == mask;
+    const bool head = active && (lane == 0 || !prev_active || !key_eq(key, prev) || !same_mask);
     const uint64_t heads = __ballot(head);
     const uint64_t cont = __ballot(active && !head);
     uint32_t bucket = kInvalid, first = 0;
@@ -295,7 +325,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
       const uint64_t above = lane == 63 ? 0ull : (cont >> (lane + 1));
       const uint32_t len = 1u + (uint32_t)__builtin_ctzll(~above);
       bool claimed;
-      bucket = hash_insert(p, t, classes, scenes, key, i + 1, claimed);
+      bucket = hash_insert<CPIC>(p, t, classes, scenes, key, i + 1, claimed, mask);
       if (bucket == kInvalid) {
         atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
       } else {
@@ -325,6 +355,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
 // workgroups < g.  A particle block's grid slot is its particle-block id.
 // It then inserts the 7 upper neighbours of each of its blocks (the grid
 // blocks its 6^3 P2G tile touches) and lists the ones it claims.
+template <bool CPIC>
 __global__ __launch_bounds__(kBinThreads) void k_bin_blocks(Parts p, Topo t, const DevClass* classes,
                                                             const DevScene* scenes) {
   __shared__ uint32_t s_claims;
@@ -364,7 +395,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_blocks(Parts p, Topo t, con
     k.y += (int)((d >> 1) & 1);
     k.z += (int)(d & 1);
     bool claimed;
-    const uint32_t got = hash_insert(p, t, classes, scenes, k, p.n + 1 + 8 * b + d, claimed);
+    const uint32_t got =
+        hash_insert<CPIC>(p, t, classes, scenes, k, p.n + 1 + 8 * b + d, claimed, CPIC ? t.bmask[b] : 0u);
     if (got == kInvalid) {
       over = true;
     } else if (claimed) {
@@ -997,6 +1029,65 @@ __device__ __forceinline__ float3 contact_velocity(const DevObstacle& o, float3 
 }
 
 __device__ __forceinline__ bool finite_abs(float v) { return fabsf(v) < 1e30f; }
+
+// ---------------------------------------------------------------------------
+// 7. CPIC side masks, before binning (gpu.wgsl update_cpic_colors, cpic_triangle_side)
+// ---------------------------------------------------------------------------
+// Signed side of `point` on a CPIC triangle if its projection falls inside
+// the triangle (barycentric tolerance 1e-5 in f32, as the reference's GPU pass).
+__device__ bool cpic_side(const DevObstacle& o, float3 point, float& dist, bool& positive) {
+  const float3 l = qrot_inv(o.q, point - make_float3(o.center[0], o.center[1], o.center[2]));
+  const float3 a = make_float3(o.tri[0], o.tri[1], o.tri[2]);
+  const float3 eb = make_float3(o.tri[3], o.tri[4], o.tri[5]) - a;
+  const float3 ec = make_float3(o.tri[6], o.tri[7], o.tri[8]) - a;
+  const float3 n = unit3(cross3(eb, ec));
+  const float sd = dot3(l - a, n);
+  const float3 pr = l - n * sd - a;
+  const float d00 = dot3(eb, eb), d01 = dot3(eb, ec), d11 = dot3(ec, ec);
+  const float d20 = dot3(pr, eb), d21 = dot3(pr, ec);
+  const float den = d00 * d11 - d01 * d01;
+  const float v = (d11 * d20 - d01 * d21) / den;
+  const float w = (d00 * d21 - d01 * d20) / den;
+  if (v >= -1e-5f && w >= -1e-5f && v + w <= 1.0f + 1e-5f) {
+    dist = fabsf(sd);
+    positive = sd >= 0.0f;
+    return true;
+  }
+  return false;
+}
+
+// Per particle: for every CPIC group, the side of the closest covering
+// triangle of that group (the first one on equal distance) sets the group's
+// bit (world.rs:470-490).  Groups are compared pairwise, so no per-group
+// arrays are needed.
+__global__ __launch_bounds__(256) void k_cpic_mask(Parts p, Topo t, const DevClass* classes, const DevScene* scenes,
+                                                   const DevObstacle* obstacles) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    const float4 pm = p.pos_mass[i];
+    uint32_t mask = 0;
+    if (pm.w > 0.0f) {
+      const DevScene& sc = scenes[classes[p.cls[i]].region >> 16];
+      const float3 x = make_float3(pm.x, pm.y, pm.z);
+      for (int j = 0; j < sc.obstacle_count; ++j) {
+        const DevObstacle& oj = obstacles[sc.obstacle_begin + j];
+        if (oj.cpic_group < 0) continue;
+        float dj;
+        bool sj;
+        if (!cpic_side(oj, x, dj, sj)) continue;
+        bool wins = true;
+        for (int k = 0; k < sc.obstacle_count && wins; ++k) {
+          const DevObstacle& ok = obstacles[sc.obstacle_begin + k];
+          if (k == j || ok.cpic_group != oj.cpic_group) continue;
+          float dk;
+          bool sk;
+          if (cpic_side(ok, x, dk, sk) && (k < j ? dk <= dj : dk < dj)) wins = false;
+        }
+        if (wins && sj) mask |= 1u << oj.cpic_group;
+      }
+    }
+    t.pmask[i] = mask;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // 8. P2G
@@ -1737,11 +1828,19 @@ static inline uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t max
 void launch_topo_clear(const Topo& t, hipStream_t s) {
   hipLaunchKernelGGL(k_topo_clear, dim3(grid_for(t.bucket_mask + 1ull, 256, 4096)), dim3(256), 0, s, t);
 }
-void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_insert, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
+                       hipStream_t s) {
+  if (cpic) hipLaunchKernelGGL(k_bin_insert<true>, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+  else hipLaunchKernelGGL(k_bin_insert<false>, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
 }
-void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_blocks, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
+                       hipStream_t s) {
+  if (cpic) hipLaunchKernelGGL(k_bin_blocks<true>, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+  else hipLaunchKernelGGL(k_bin_blocks<false>, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+}
+void launch_cpic_mask(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_cpic_mask, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t, c, sc, o);
 }
 void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_offsets, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);

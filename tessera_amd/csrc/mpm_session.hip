@@ -605,6 +605,7 @@ struct tmpm_session {
   DevBuf classes, scene_plain, scene_force, obstacles, scene_err, planes;
   uint32_t obstacle_cap = 0, scene_cap = 0, plane_cap = 0;
   int shapes = 0;          // some obstacle is a triangle prism or convex polyhedron
+  int cpic = 0;            // some scene uses CPIC obstacle groups
   // reactions and rigid coupling
   bool reactions_on = false;
   DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
@@ -617,7 +618,7 @@ struct tmpm_session {
   // topology
   DevBuf owner, bkey, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, grid_key, pblock_bucket,
       pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask,
-    bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum;
+    bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum, bmask, pmask;
   Topo topo{};
   uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 64;
   int p2g_copies = 2;
@@ -826,7 +827,7 @@ tmpm_status upload_obstacles(tmpm_session* s) {
   std::vector<DevObstacle> packed;
   std::vector<DevMapping> maps;
   std::vector<float4> planes;
-  bool shapes = false;
+  bool shapes = false, cpic = false;
   for (auto& scene : s->scenes) {
     scene.dev.obstacle_begin = (int32_t)packed.size();
     scene.dev.obstacle_count = 0;
@@ -838,10 +839,11 @@ tmpm_status upload_obstacles(tmpm_session* s) {
       return fail(TMPM_INVALID_INPUT, "one rigid mapping per obstacle is required");
     for (auto& ob : scene.world.obstacles) {
       const tmpm_obstacle& o = ob.o;
-      if (o.cpic_group >= 0 && scene.world.params.use_cpic)
-        return fail(TMPM_RESIDENT_UNAVAILABLE, "CPIC obstacle groups are not supported by the HIP backend");
       DevObstacle d;
       if (!pack_obstacle(o, d)) return fail(TMPM_INVALID_INPUT, "obstacle is not representable in f32");
+      // gpu.rs cpic_obstacles: groups count only while the world uses CPIC
+      if (!scene.world.params.use_cpic) d.cpic_group = -1;
+      cpic = cpic || d.cpic_group >= 0;
       if (o.shape == TMPM_SHAPE_CONVEX) {
         tmpm_status st = pack_planes(o, d, planes);
         if (st != TMPM_OK) return st;
@@ -892,6 +894,7 @@ tmpm_status upload_obstacles(tmpm_session* s) {
     HIP_TRY(hipMemcpy(s->planes.ptr, planes.data(), planes.size() * sizeof(float4), hipMemcpyHostToDevice));
   s->topo.planes = s->planes.as<float4>();
   s->shapes = shapes ? 1 : 0;
+  s->cpic = cpic ? 1 : 0;
   s->obstacle_total = (uint32_t)packed.size();
   s->topo.obstacle_total = s->obstacle_total;
   if (s->react_active() && s->obstacle_total > 0) {
@@ -1011,6 +1014,7 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   Topo& t = s->topo;
   HIP_TRY(s->owner.alloc(4ull * buckets));
   HIP_TRY(s->bkey.alloc(16ull * buckets));
+  HIP_TRY(s->bmask.alloc(4ull * buckets));
   HIP_TRY(s->bucket_count.alloc(4ull * buckets));
   HIP_TRY(s->nbr_bucket.alloc(32ull * buckets));
   HIP_TRY(s->bucket_slot.alloc(4ull * buckets));
@@ -1024,6 +1028,7 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->pslot.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
   HIP_TRY(s->prank.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
   HIP_TRY(s->sorted.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
+  HIP_TRY(s->pmask.alloc(4ull * std::max<uint32_t>(s->cap, 1)));
   HIP_TRY(s->counters.alloc(4ull * CNT_COUNT));
   HIP_TRY(hipMemset(s->counters.ptr, 0, 4ull * CNT_COUNT));
   // Binning workgroups: at most ~2048 (each one's id prefix sums over all of
@@ -1049,6 +1054,8 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->node_mask.alloc(8ull * grid_cap));
   t.owner = s->owner.as<uint32_t>();
   t.bkey = s->bkey.as<int4>();
+  t.bmask = s->bmask.as<uint32_t>();
+  t.pmask = s->pmask.as<uint32_t>();
   t.bucket_count = s->bucket_count.as<uint32_t>();
   t.nbr_bucket = s->nbr_bucket.as<uint32_t>();
   t.bucket_slot = s->bucket_slot.as<uint32_t>();
@@ -1538,8 +1545,10 @@ tmpm_status prepare_packed(tmpm_session* s, PackedParticles& pk) {
   count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
   s->last_pblocks = (uint32_t)pblocks;
   s->last_gblocks = (uint32_t)gblocks;
-  uint64_t pcap = std::max<uint64_t>(1024, 4 * pblocks);
-  uint64_t gcap = std::max<uint64_t>(4096, 4 * gblocks);
+  // CPIC splits blocks that straddle a triangle by side: more headroom
+  const uint64_t room = s->cpic ? 8 : 4;
+  uint64_t pcap = std::max<uint64_t>(1024, room * pblocks);
+  uint64_t gcap = std::max<uint64_t>(4096, room * gblocks);
   pcap = std::min<uint64_t>(pcap, s->cap);
   pcap = std::max<uint64_t>(pcap, 1);
   if (gcap > (1ull << 26)) return fail(TMPM_CAPACITY, "grid exceeds the block capacity");
@@ -1562,8 +1571,14 @@ hipError_t enqueue_binning(tmpm_session* s, const Parts& p) {
     s->table_dirty = false;
   }
   if ((uint64_t)t.bin_groups * t.bin_span < p.n) return hipErrorInvalidValue;  // lists sized for fewer particles
-  if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p, t, c, sc, s->stream); })) != hipSuccess) return e;
-  if ((e = timed(s, "k_bin_blocks", [&] { launch_bin_blocks(p, t, c, sc, s->stream); })) != hipSuccess) return e;
+  if (s->cpic) {
+    const DevObstacle* ob = s->obstacles.as<DevObstacle>();
+    if ((e = timed(s, "k_cpic_mask", [&] { launch_cpic_mask(p, t, c, sc, ob, s->stream); })) != hipSuccess) return e;
+  }
+  if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p, t, c, sc, s->cpic, s->stream); })) != hipSuccess)
+    return e;
+  if ((e = timed(s, "k_bin_blocks", [&] { launch_bin_blocks(p, t, c, sc, s->cpic, s->stream); })) != hipSuccess)
+    return e;
   if ((e = timed(s, "k_bin_offsets", [&] { launch_bin_offsets(p, t, c, sc, s->stream); })) != hipSuccess) return e;
   return hipSuccess;
 }

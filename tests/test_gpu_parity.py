@@ -429,3 +429,93 @@ def test_convex_with_more_planes_than_the_device_holds_is_a_capacity_error(trans
     w = world_of([p], MpmParams(gravity=np.zeros(3)), [ob])
     with pytest.raises(Capacity):
         HipMpmResidentSession(transfers, w, 1e-3)
+
+
+def test_transfer_colors_separate_grid_momentum(transfers):
+    # gpu.rs:4151-4198: two coincident particles of opposite velocity and colours 0 / 1
+    first = MpmParticle.new([0.5, 0.5, 0.5], 0.04, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    first.velocity = np.array([1.0, 0.0, 0.0])
+    second = MpmParticle.new([0.5, 0.5, 0.5], 0.04, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    second.velocity = np.array([-1.0, 0.0, 0.0])
+    second.transfer_color = 1
+    w = world_of([first, second], MpmParams(gravity=np.zeros(3), bounds=WorldBounds(np.zeros(3), np.ones(3))))
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    s = HipMpmResidentSession(transfers, w, 0.001)
+    s.step(1)
+    ref, got = cpu.read(), s.read_particles()
+    assert np.all(np.abs(got.velocity[:, 0] - ref["velocity"][:, 0]) < 1e-4)
+    assert np.all(np.abs(got.position[:, 0] - ref["position"][:, 0]) < 1e-5)
+    assert got.velocity[0, 0] > 0.9 and got.velocity[1, 0] < -0.9
+
+
+def _cpic_sheet():
+    sheet = RigidObstacle.triangle_prism([0.5, 0.5, 0.5], [[0.0, -1.0, -1.0], [0.0, 1.0, -1.0], [0.0, 0.0, 1.0]], 0.001)
+    sheet.boundary = 3  # NonReflecting: the sheet only separates the grid
+    return sheet
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+@pytest.mark.parametrize("group,left_color,all_groups,enabled",
+                         [(0, 0, False, True), (16, 1, False, True), (31, 0, False, True), (31, 0, True, True),
+                          (0, 0, False, False), (31, 1, True, False)])
+def test_cpic_groups_separate_the_grid(transfers, mode, group, left_color, all_groups, enabled):
+    # gpu.rs:4200-4283: a thin CPIC sheet between two particles moving apart
+    import copy
+
+    left = MpmParticle.new([0.48, 0.5, 0.5], 0.005, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    left.velocity = np.array([-1.0, 0.0, 0.0])
+    left.transfer_color = left_color
+    right = MpmParticle.new([0.52, 0.5, 0.5], 0.005, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    right.velocity = np.array([1.0, 0.0, 0.0])
+    sheets = []
+    for g in (range(32) if all_groups else [group]):
+        sh = copy.deepcopy(_cpic_sheet())
+        sh.cpic_group = g
+        sheets.append(sh)
+    w = world_of([left, right], MpmParams(gravity=np.zeros(3)), sheets)
+    w.use_cpic = enabled
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    ref = cpu.read()
+    if enabled or left_color != 0:
+        assert ref["velocity"][0, 0] < -0.9 and ref["velocity"][1, 0] > 0.9
+    else:
+        assert abs(ref["velocity"][0, 0]) < 0.5 and abs(ref["velocity"][1, 0]) < 0.5
+    s = HipMpmResidentSession(transfers, w, 0.001, mode)
+    s.step(1)
+    got = s.read_particles()
+    assert np.all(np.abs(got.velocity[:, 0] - ref["velocity"][:, 0]) < 1e-3)
+    assert np.all(np.abs(got.position[:, 0] - ref["position"][:, 0]) < 1e-5)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_cpic_blocks_of_particles_on_both_sides_track_the_oracle(transfers, mode):
+    # Two lattices pressed against a thin sheet from both sides for 20 substeps:
+    # the side masks split every straddled block, and both sides stay apart.
+    from tessera_amd.scenes import jittered_lattice, _particles
+
+    rng = np.random.default_rng(5)
+    h = 1.0 / 64
+    sp = h / 2
+    a = jittered_lattice(rng, [6, 8, 8], [0.5 - 6 * sp - 0.3 * h, 0.42, 0.42], [0.5 - 0.3 * h, 0.42 + 8 * sp, 0.42 + 8 * sp])
+    b = jittered_lattice(rng, [6, 8, 8], [0.5 + 0.3 * h, 0.42, 0.42], [0.5 + 6 * sp + 0.3 * h, 0.42 + 8 * sp, 0.42 + 8 * sp])
+    pos = np.concatenate([a, b])
+    vel = np.zeros_like(pos)
+    vel[: len(a), 0] = 0.4
+    vel[len(a):, 0] = -0.4
+    pa = _particles(pos, vel, sp / 2, 1000.0, MaterialModel.elastic(3.0e3, 0.3))
+    params = MpmParams(cell_width=h, gravity=np.zeros(3), max_substep=1e-4,
+                       bounds=WorldBounds(np.zeros(3), np.ones(3)))
+    w = MpmWorld(pa, params)
+    sheet = _cpic_sheet()
+    sheet.cpic_group = 3
+    w.set_obstacles([sheet])
+    cpu = O.OracleWorld(w)
+    s = HipMpmResidentSession(transfers, w, 1e-4, mode)
+    cpu.substeps_fixed(1e-4, 20)
+    s.step(20)
+    ref, got = cpu.read(), s.read_particles()
+    assert_tracks(ref, got, 2e-5, 2e-4, 2e-2, 2e-4)
+    # no particle crossed the sheet: the two sides never exchanged momentum
+    assert np.all(got.position[: len(a), 0] < 0.5) and np.all(got.position[len(a):, 0] > 0.5)
