@@ -6,6 +6,7 @@
 //   vel_vol[i]   float4  vx, vy, vz, rest volume
 //   C[k*stride+i] float  APIC affine matrix, k = column*3 + row
 //   F[k*stride+i] float  deformation gradient, same indexing
+//   P[k*stride+i] float  plastic state {plastic_det, hardening, log_volume_gain}
 //   cls[i]       u32     particle class (material, radius, damping, flags, region)
 //   ins_id[i]    u32     insertion index of the particle stored in slot i
 // Grid: 4x4x4-cell blocks; node (slot s, local l) at s*64 + l with
@@ -46,8 +47,17 @@ enum ErrorBit : uint32_t {
   ERR_INTERNAL = 8u         // inconsistent binning (must never happen)
 };
 
-// Device material codes.
-enum DevMaterial : int32_t { DM_COROTATED = 0, DM_NEO_HOOKEAN = 1, DM_FLUID = 2 };
+// Device material codes.  Snow and sand use the corotated stress, sand-NH the
+// neo-Hookean one; all three carry a plastic state and a return mapping.
+enum DevMaterial : int32_t {
+  DM_COROTATED = 0,
+  DM_NEO_HOOKEAN = 1,
+  DM_FLUID = 2,
+  DM_SNOW = 3,
+  DM_SAND = 4,
+  DM_SAND_NH = 5
+};
+__host__ __device__ constexpr bool dm_plastic(int32_t k) { return k >= DM_SNOW; }
 
 struct DevClass {
   int32_t kind;
@@ -58,6 +68,9 @@ struct DevClass {
   float lambda, mu;                         // elastic Lame parameters
   float bulk, gamma, viscosity, tensile;    // fluid
   float wave_numerator;                     // lambda + 2 mu, or bulk * gamma
+  float hardening;                          // snow: hardening coefficient
+  float crit_compression, crit_stretch;     // snow: stretch clamp [1 - c, 1 + s]
+  float friction_angle, cohesion;           // sand, sand-NH (radians; log-volume units)
 };
 
 struct DevScene {
@@ -103,7 +116,8 @@ struct DevObstacle {
   int32_t cpic_group;    // -1: none
   int32_t pad4[3];
 };
-constexpr int kMaxConvexPlanes = 128;  // planes per convex obstacle (gpu.rs pack_obstacle)
+constexpr int kMaxConvexPlanes = 128;
+constexpr int kGatherFloats = 29;  // per particle in a scene gather (k_gather_scene)  // planes per convex obstacle (gpu.rs pack_obstacle)
 
 // Rigid body state (GpuRigidBodyState, phys/src/gpu_rigid_state.rs:17-28):
 // position and inverse mass, XYZW orientation, world velocities, body-frame
@@ -136,6 +150,8 @@ struct Parts {
   float4* vel_vol;
   float* C;
   float* F;
+  float* P;          // plastic state, 3 per particle (component-major): plastic_det,
+                     // hardening, log_volume_gain; null when no scene has a plastic material
   uint32_t* cls;
   float4* force;
   uint32_t* ins_id;

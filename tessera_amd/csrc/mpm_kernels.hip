@@ -522,6 +522,10 @@ __global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo
       q.C[k * q.stride + dst] = p.C[k * p.stride + i];
       q.F[k * q.stride + dst] = p.F[k * p.stride + i];
     }
+    if (p.P) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q.P[k * q.stride + dst] = p.P[k * p.stride + i];
+    }
     q.cls[dst] = p.cls[i];
     q.ins_id[dst] = p.ins_id[i];
   }
@@ -621,6 +625,70 @@ __device__ void polar_newton(const M3& F, M3& R) {
 // rotation the reference CPU takes from nalgebra's SVD, material.rs:447-450).
 // Taken and returned by value: a reference would make every caller keep F in
 // scratch memory for this rarely taken call.
+// Singular value decomposition F = U diag(s) V^T in f64 by one-sided Jacobi
+// rotations of F's columns (singular values >= 0; U or V may include a
+// reflection, as nalgebra's SVD does).  Degenerate columns of U are completed
+// to an orthonormal frame.
+__device__ __noinline__ void svd_fp64(const M3& Ff, double U[3][3], double s[3], double V[3][3]) {
+  double A[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      A[r][c] = (double)Ff.at(r, c);
+      V[r][c] = r == c ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (int i = 0; i < 3; ++i) {
+          alpha += A[i][p] * A[i][p];
+          beta += A[i][q] * A[i][q];
+          gamma += A[i][p] * A[i][q];
+        }
+        if (fabs(gamma) <= 1e-300) continue;
+        double zeta = (beta - alpha) / (2.0 * gamma);
+        double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
+        for (int i = 0; i < 3; ++i) {
+          double ap = A[i][p], aq = A[i][q];
+          A[i][p] = cs * ap - sn * aq;
+          A[i][q] = sn * ap + cs * aq;
+          double vp = V[i][p], vq = V[i][q];
+          V[i][p] = cs * vp - sn * vq;
+          V[i][q] = sn * vp + cs * vq;
+        }
+      }
+  }
+  double smax = 0;
+  for (int j = 0; j < 3; ++j) {
+    s[j] = sqrt(A[0][j] * A[0][j] + A[1][j] * A[1][j] + A[2][j] * A[2][j]);
+    smax = fmax(smax, s[j]);
+  }
+  int good[3], ng = 0, bad = -1, nbad = 0;
+  for (int j = 0; j < 3; ++j) {
+    if (s[j] > smax * 1e-12 && s[j] > 0) {
+      for (int i = 0; i < 3; ++i) U[i][j] = A[i][j] / s[j];
+      good[ng++] = j;
+    } else {
+      bad = j;
+      ++nbad;
+    }
+  }
+  if (nbad == 1) {
+    int a = good[0], b = good[1];
+    U[0][bad] = U[1][a] * U[2][b] - U[2][a] * U[1][b];
+    U[1][bad] = U[2][a] * U[0][b] - U[0][a] * U[2][b];
+    U[2][bad] = U[0][a] * U[1][b] - U[1][a] * U[0][b];
+  } else if (nbad > 1) {
+    // Rank <= 1: no unique rotation; use the nearest rotation to V's frame.
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) U[r][c] = V[r][c];
+  }
+}
+
+// Polar rotation R = U V^T in f64 (fallback of the corotated stress).  The
+// same one-sided Jacobi as svd_fp64, kept in one frame: a call from here
+// would raise the register count of every P2G instantiation.
 __device__ __noinline__ M3 polar_fp64(M3 Ff) {
   M3 R;
   double A[3][3], V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
@@ -680,9 +748,12 @@ __device__ __noinline__ M3 polar_fp64(M3 Ff) {
 }
 
 // Kirchhoff stress (material.rs:234-284; f32 op order of gpu.wgsl:638-699).
-__device__ void kirchhoff(const DevClass& c, const M3& F, const M3& C, M3& S) {
+// plastic_det is read for snow only; PLASTIC = false instantiations (no snow,
+// sand or sand-NH class in the session) leave the plastic materials out.
+template <bool PLASTIC>
+__device__ void kirchhoff(const DevClass& c, const M3& F, const M3& C, float plastic_det, M3& S) {
   float j = det3(F);
-  if (c.kind == DM_NEO_HOOKEAN) {
+  if (c.kind == DM_NEO_HOOKEAN || (PLASTIC && c.kind == DM_SAND_NH)) {
     float iso = c.lambda * logf(fmaxf(j, 1e-10f)) - c.mu;
     for (int r = 0; r < 3; ++r)
       for (int q = 0; q < 3; ++q) {
@@ -700,12 +771,15 @@ __device__ void kirchhoff(const DevClass& c, const M3& F, const M3& C, M3& S) {
         float sr = (C.at(r, q) + C.at(q, r)) * 0.5f;
         S.at(r, q) = sr * k + (r == q ? diagonal : 0.0f);
       }
-  } else {
+  } else {  // corotated: elastic, sand, snow (hardened by the plastic volume)
     M3 R;
     if (regular_corotated(F)) polar_newton(F, R);
     else R = polar_fp64(F);
-    float iso = c.lambda * (j - 1.0f) * j;
-    float k = 2.0f * c.mu;
+    const float factor = PLASTIC && c.kind == DM_SNOW
+                             ? fminf(fmaxf(expf(c.hardening * (1.0f - plastic_det)), 0.01f), 100.0f)
+                             : 1.0f;
+    float iso = c.lambda * factor * (j - 1.0f) * j;
+    float k = 2.0f * c.mu * factor;
     for (int r = 0; r < 3; ++r)
       for (int q = 0; q < 3; ++q) {
         float s = (F.at(r, 0) - R.at(r, 0)) * F.at(q, 0) + (F.at(r, 1) - R.at(r, 1)) * F.at(q, 1) +
@@ -713,6 +787,140 @@ __device__ void kirchhoff(const DevClass& c, const M3& F, const M3& C, M3& S) {
         S.at(r, q) = s * k + (r == q ? iso : 0.0f);
       }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Plastic return mappings (material.rs:287-371 and 488-532; gpu.wgsl:489-636)
+// ---------------------------------------------------------------------------
+// New principal stretches of snow, sand and sand-NH for stretches s and the
+// plastic state pl = {plastic_det, hardening, log_volume_gain}; false when F
+// is left unchanged (sand with lambda = 0, or inside the yield surface).  T
+// is float on the f32 path and double on the f64 fallback.
+template <class T>
+__device__ bool plastic_stretches(const DevClass& c, const T s[3], T pl[3], T out[3]) {
+  if (c.kind == DM_SNOW) {
+    for (int a = 0; a < 3; ++a) out[a] = fmin(fmax(s[a], T(1) - (T)c.crit_compression), T(1) + (T)c.crit_stretch);
+    const T old_det = s[0] * s[1] * s[2];
+    const T new_det = fmax(out[0] * out[1] * out[2], T(1e-12));
+    pl[0] = fmin(fmax(pl[0] * old_det / new_det, T(0.1)), T(4));
+    return true;
+  }
+  if (c.lambda == 0.0f) return false;
+  const bool nh = c.kind == DM_SAND_NH;
+  T strain[3];
+  for (int a = 0; a < 3; ++a) strain[a] = log(fmax(s[a], T(1e-12))) + pl[2] / T(3);
+  const T tr = strain[0] + strain[1] + strain[2];
+  T dev[3];
+  for (int a = 0; a < 3; ++a) dev[a] = strain[a] - tr / T(3);
+  const T norm = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const T cohesion = (T)c.cohesion;
+  const T shifted = tr - cohesion;
+  const T q = pl[1];
+  const T deg = T(0.017453292519943295);
+  const T angle = (T)c.friction_angle + (T(9) * deg * q - T(10) * deg) * exp(T(-0.2) * q);
+  const T sine = sin(angle);
+  const T alpha = sqrt(T(2) / T(3)) * (T(2) * sine) / (T(3) - sine);
+  T logs[3], inc;
+  // the isotropic test uses the f32-aware threshold 4 * FLT_EPSILON on both paths
+  if (shifted > T(0) || norm <= T(4.76837158203125e-7)) {
+    for (int a = 0; a < 3; ++a) logs[a] = cohesion / T(3);
+    inc = sqrt(strain[0] * strain[0] + strain[1] * strain[1] + strain[2] * strain[2]);
+  } else {
+    const T lambda = (T)c.lambda, mu = (T)c.mu;
+    const T gamma = norm + (T(3) * lambda + T(2) * mu) / (T(2) * mu) * shifted * alpha;
+    if (gamma <= T(0)) return false;
+    for (int a = 0; a < 3; ++a) logs[a] = strain[a] - dev[a] * (gamma / norm);
+    inc = gamma;
+  }
+  for (int a = 0; a < 3; ++a) out[a] = exp(logs[a]);
+  T old_det = s[0] * s[1] * s[2], new_det = out[0] * out[1] * out[2];
+  if (!nh) {
+    old_det = fmax(old_det, T(1e-12));
+    new_det = fmax(new_det, T(1e-12));
+  }
+  pl[0] *= old_det / new_det;
+  if (!nh) pl[0] = fmin(fmax(pl[0], T(0.01)), T(100));
+  pl[2] += log(old_det) - log(new_det);
+  pl[1] += inc;
+  return true;
+}
+
+// One cyclic Jacobi rotation zeroing A[p][q] of the symmetric A; V collects
+// the rotations (its columns are A's eigenvectors).  gpu.wgsl jacobi_pair.
+__device__ __forceinline__ void jacobi_rotate(float A[3][3], float V[3][3], int p, int q) {
+  const float off = A[p][q];
+  if (fabsf(off) < 1e-8f) return;
+  const float tau = (A[q][q] - A[p][p]) / (2.0f * off);
+  const float t = (tau >= 0.0f ? 1.0f : -1.0f) / (fabsf(tau) + sqrtf(1.0f + tau * tau));
+  const float cs = 1.0f / sqrtf(1.0f + t * t), sn = t * cs;
+  const float app = A[p][p], aqq = A[q][q];
+  const int r = 3 - p - q;
+  const float arp = A[r][p], arq = A[r][q];
+  A[r][p] = A[p][r] = cs * arp - sn * arq;
+  A[r][q] = A[q][r] = sn * arp + cs * arq;
+  A[p][p] = app - t * off;
+  A[q][q] = aqq + t * off;
+  A[p][q] = A[q][p] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float vp = V[i][p], vq = V[i][q];
+    V[i][p] = cs * vp - sn * vq;
+    V[i][q] = sn * vp + cs * vq;
+  }
+}
+
+// Projects F and the plastic state in place.  f32 path: 8 Jacobi sweeps of
+// F^T F give the right singular axes V and stretches s, and
+// F V diag(s'/s) V^T replaces the stretches (gpu.wgsl principal_axes,
+// replace_stretches).  Squaring F squares its condition number, so F with a
+// stretch below 1e-6 or a stretch ratio above 8 takes an f64 SVD instead and
+// is rebuilt as U diag(s') V^T, as the reference's CPU path does.
+__device__ void project_plastic(const DevClass& c, M3& F, float pl[3]) {
+  float A[3][3], V[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      A[r][q] = F.at(0, r) * F.at(0, q) + F.at(1, r) * F.at(1, q) + F.at(2, r) * F.at(2, q);
+      V[r][q] = r == q ? 1.0f : 0.0f;
+    }
+#pragma unroll
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    jacobi_rotate(A, V, 0, 1);
+    jacobi_rotate(A, V, 0, 2);
+    jacobi_rotate(A, V, 1, 2);
+  }
+  float sv[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) sv[a] = sqrtf(fmaxf(A[a][a], 0.0f));
+  const float smin = fminf(fminf(sv[0], sv[1]), sv[2]), smax = fmaxf(fmaxf(sv[0], sv[1]), sv[2]);
+  if (smin >= 1e-6f && smax <= 8.0f * smin) {
+    float out[3];
+    if (!plastic_stretches<float>(c, sv, pl, out)) return;
+    float scale[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) scale[a] = out[a] / sv[a];
+    // M = V diag(scale) V^T; F <- F M
+    M3 M, Fo = F;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        M.at(r, q) = V[r][0] * scale[0] * V[q][0] + V[r][1] * scale[1] * V[q][1] + V[r][2] * scale[2] * V[q][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        F.at(r, q) = Fo.at(r, 0) * M.at(0, q) + Fo.at(r, 1) * M.at(1, q) + Fo.at(r, 2) * M.at(2, q);
+    return;
+  }
+  double U[3][3], sd[3], Vd[3][3], pd[3] = {pl[0], pl[1], pl[2]}, out[3];
+  svd_fp64(F, U, sd, Vd);
+  if (!plastic_stretches<double>(c, sd, pd, out)) return;
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q)
+      F.at(r, q) = (float)(U[r][0] * out[0] * Vd[q][0] + U[r][1] * out[1] * Vd[q][1] + U[r][2] * out[2] * Vd[q][2]);
+  for (int a = 0; a < 3; ++a) pl[a] = (float)pd[a];
 }
 
 __device__ __forceinline__ void quadratic_weights(float fx, float w[3]) {
@@ -1135,7 +1343,7 @@ __device__ __forceinline__ unsigned long long fixed64_of(float v) {
   return ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-template <int FIXED, int THREADS, int COPIES>
+template <int FIXED, int THREADS, int COPIES, bool PLASTIC>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS == 64 && COPIES == 2 ? 3 : 4))) void k_p2g(Parts p, Topo t,
                                                                                          const DevClass* classes,
                                                                                          const DevScene* scenes) {
@@ -1188,7 +1396,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
       quadratic_weights(fy, wy);
       quadratic_weights(fz, wz);
       M3 S;
-      kirchhoff(cls, F, C, S);
+      kirchhoff<PLASTIC>(cls, F, C, PLASTIC && cls.kind == DM_SNOW ? p.P[i] : 1.0f, S);
       const float m = pm.w;
       const float sscale = stress_scale * vv.w;
       // node o receives w_o (m v + A (o - fx) h) with A = m C + stress (-dt V0 inv_d):
@@ -1371,7 +1579,7 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // REACT = 1 runs with one 64-lane wavefront per workgroup: each lane adds its
 // particles' obstacle impulses into its own LDS column (no atomics), and the
 // columns are reduced in a fixed order at the end of the block.
-template <int REACT, bool SHAPES>
+template <int REACT, bool SHAPES, bool PLASTIC>
 __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* classes, const DevScene* scenes,
                                              const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
@@ -1529,6 +1737,12 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
           const float sc3 = cbrtf(fmaxf(det3(Fn), 1e-9f));
 #pragma unroll
           for (int q = 0; q < 9; ++q) Fn.a[q] = (q % 4 == 0) ? sc3 : 0.0f;
+        } else if (PLASTIC && dm_plastic(cl.kind)) {
+          float pl[3] = {p.P[i], p.P[p.stride + i], p.P[2 * p.stride + i]};
+          project_plastic(cl, Fn, pl);
+          p.P[i] = pl[0];
+          p.P[p.stride + i] = pl[1];
+          p.P[2 * p.stride + i] = pl[2];
         }
         // Resident validity (gpu.wgsl:946-993).
         int3 nb;
@@ -1543,7 +1757,10 @@ __global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* cl
                    nb.x > sc.last_base[0] || nb.y > sc.last_base[1] || nb.z > sc.last_base[2];
         const float density = fmaxf(pm.w / p.vel_vol[i].w, 1e-12f);
         const float wave = sqrtf(cl.wave_numerator / density);
-        const bool cfl = dt * (wave + len3(v)) > 0.4f * h * (1.0f + 1e-4f);
+        // sand-NH: wave speed at the current volume, 0.5 h bound (material.rs:388-396)
+        const bool cfl = PLASTIC && cl.kind == DM_SAND_NH
+                             ? dt * fmaxf(wave * sqrtf(fmaxf(det3(Fn), 1e-6f)), len3(v)) > 0.5f * h * (1.0f + 1e-4f)
+                             : dt * (wave + len3(v)) > 0.4f * h * (1.0f + 1e-4f);
         failed |= out || !fin || cfl;
         p.pos_mass[i] = make_float4(x.x, x.y, x.z, pm.w);
         const float vol = p.vel_vol[i].w;
@@ -1750,13 +1967,17 @@ __global__ __launch_bounds__(256) void k_renumber(uint32_t* ins_id, uint32_t n, 
   }
 }
 
-// One scene's particles in insertion order, 26 floats each: x, y, z, mass,
-// vx, vy, vz, volume, C (9, column-major), F (9).
+// One scene's particles in insertion order, kGatherFloats each: x, y, z,
+// mass, vx, vy, vz, volume, C (9, column-major), F (9), plastic state (3;
+// 1, 0, 0 when the session stores none).
 __global__ __launch_bounds__(256) void k_gather_scene(Parts p, uint32_t first, uint32_t count, float* out) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
     const uint32_t g = p.ins_id[i];
     if (g == kInvalid || g - first >= count) continue;
-    float* r = out + (size_t)(g - first) * 26;
+    float* r = out + (size_t)(g - first) * kGatherFloats;
+    r[26] = p.P ? p.P[i] : 1.0f;
+    r[27] = p.P ? p.P[p.stride + i] : 0.0f;
+    r[28] = p.P ? p.P[2 * p.stride + i] : 0.0f;
     const float4 a = p.pos_mass[i], b = p.vel_vol[i];
     r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
     r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
@@ -1852,9 +2073,15 @@ void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint
   hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
 }
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
-                uint32_t threads, int copies, hipStream_t s) {
+                uint32_t threads, int copies, int plastic, hipStream_t s) {
+  // sessions with plastic materials run the default shape (64 threads, 2 tile copies)
+  if (plastic) {
+    if (fixed64) hipLaunchKernelGGL((k_p2g<1, 64, 2, true>), dim3(blocks), dim3(64), 0, s, p, t, c, sc);
+    else hipLaunchKernelGGL((k_p2g<0, 64, 2, true>), dim3(blocks), dim3(64), 0, s, p, t, c, sc);
+    return;
+  }
   // threads per workgroup: 64 or 128 (anything else runs 128); LDS tile copies: 1 or 2
-#define TMPM_P2G(F, T, K) hipLaunchKernelGGL((k_p2g<F, T, K>), dim3(blocks), dim3(T), 0, s, p, t, c, sc)
+#define TMPM_P2G(F, T, K) hipLaunchKernelGGL((k_p2g<F, T, K, false>), dim3(blocks), dim3(T), 0, s, p, t, c, sc)
 #define TMPM_P2G_T(F, K)                  \
   do {                                    \
     if (threads == 64) TMPM_P2G(F, 64, K); \
@@ -1885,16 +2112,22 @@ void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o,
   if (shapes) grid_update_variant<true>(t, sc, o, fixed64, react, blocks, s);
   else grid_update_variant<false>(t, sc, o, fixed64, react, blocks, s);
 }
-void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                uint32_t blocks, uint32_t threads, int react, int shapes, hipStream_t s) {
+template <bool PLASTIC>
+static void g2p_variant(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
+                        uint32_t blocks, uint32_t threads, int react, int shapes, hipStream_t s) {
   // REACT: one wavefront per workgroup (the lane-private reaction columns assume 64 lanes)
   if (react) {
-    if (shapes) hipLaunchKernelGGL((k_g2p<1, true>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
-    else hipLaunchKernelGGL((k_g2p<1, false>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
+    if (shapes) hipLaunchKernelGGL((k_g2p<1, true, PLASTIC>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
+    else hipLaunchKernelGGL((k_g2p<1, false, PLASTIC>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
   } else {
-    if (shapes) hipLaunchKernelGGL((k_g2p<0, true>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
-    else hipLaunchKernelGGL((k_g2p<0, false>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+    if (shapes) hipLaunchKernelGGL((k_g2p<0, true, PLASTIC>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+    else hipLaunchKernelGGL((k_g2p<0, false, PLASTIC>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
   }
+}
+void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
+                uint32_t blocks, uint32_t threads, int react, int shapes, int plastic, hipStream_t s) {
+  if (plastic) g2p_variant<true>(p, t, c, sc, o, blocks, threads, react, shapes, s);
+  else g2p_variant<false>(p, t, c, sc, o, blocks, threads, react, shapes, s);
 }
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s) {
   hipLaunchKernelGGL(k_react_finalize, dim3(grid_for(t.obstacle_total, 64, 1024)), dim3(64), 0, s, t, o);

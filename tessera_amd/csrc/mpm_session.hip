@@ -87,6 +87,10 @@ double det_cm(const double* F) {  // nalgebra 3x3 determinant on column-major st
   return m11 * a - m12 * b + m13 * c;
 }
 // material.rs:380-426
+inline bool is_plastic(const tmpm_material& m) {
+  return m.kind == TMPM_MATERIAL_SNOW || m.kind == TMPM_MATERIAL_SAND || m.kind == TMPM_MATERIAL_SAND_NEO_HOOKEAN;
+}
+
 double timestep_bound(const tmpm_material& m, double density, const double* v, double h, double det) {
   const double* p = m.p;
   double speed = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -428,7 +432,7 @@ struct DevBuf {
 };
 
 struct ParticleBuffers {
-  DevBuf pos_mass, vel_vol, C, F, cls, ins_id;
+  DevBuf pos_mass, vel_vol, C, F, P, cls, ins_id;
 };
 
 struct KernelTimer {
@@ -549,6 +553,7 @@ struct PodBuf {
 struct PackedParticles {
   PodBuf<float4, true> pm, vv;
   PodBuf<float> C, F;  // 9 per particle, particle-major
+  PodBuf<float> Pl;    // plastic state, 3 per particle
   PodBuf<uint32_t, true> cls, ins;
   PodBuf<float> pos32, inv_h;  // for counting blocks
   PodBuf<uint32_t> region;
@@ -557,14 +562,23 @@ struct PackedParticles {
   PodBuf<float4, true> forces; // upload staging: one-shot forces
   size_t size() const { return pm.size(); }
   void clear() {
-    pm.clear(); vv.clear(); C.clear(); F.clear(); cls.clear(); ins.clear();
+    pm.clear(); vv.clear(); C.clear(); F.clear(); Pl.clear(); cls.clear(); ins.clear();
     pos32.clear(); inv_h.clear(); region.clear(); disabled.clear(); cols.clear(); forces.clear();
   }
   void reset() {
-    pm.reset(); vv.reset(); C.reset(); F.reset(); cls.reset(); ins.reset();
+    pm.reset(); vv.reset(); C.reset(); F.reset(); Pl.reset(); cls.reset(); ins.reset();
     pos32.reset(); inv_h.reset(); region.reset(); disabled.reset(); cols.reset(); forces.reset();
   }
 };
+
+// Interning key of a particle class: every DevClass field that varies.
+using ClassKey = std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t, float,
+                            float, float, float, float>;
+inline ClassKey class_key(const DevClass& c) {
+  return std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
+                         c.region, c.fixed, c.hardening, c.crit_compression, c.crit_stretch, c.friction_angle,
+                         c.cohesion);
+}
 
 struct tmpm_session {
   tmpm_context* ctx = nullptr;
@@ -599,13 +613,13 @@ struct tmpm_session {
   uint32_t force_cap = 0;
   uint32_t last_pblocks = 0, last_gblocks = 0;  // block counts of the last synchronized substep
   uint32_t topo_cap = 0;   // particle capacity the topology's per-particle arrays were sized for
-  std::map<std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t>, uint32_t>
-      class_ids;
+  std::map<ClassKey, uint32_t> class_ids;
   DevBuf force;            // float4 per particle, insertion order
   DevBuf classes, scene_plain, scene_force, obstacles, scene_err, planes;
   uint32_t obstacle_cap = 0, scene_cap = 0, plane_cap = 0;
   int shapes = 0;          // some obstacle is a triangle prism or convex polyhedron
   int cpic = 0;            // some scene uses CPIC obstacle groups
+  bool plastic = false;    // some particle class is snow, sand or sand-NH: plastic state on the device
   // reactions and rigid coupling
   bool reactions_on = false;
   DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
@@ -638,6 +652,7 @@ struct tmpm_session {
     p.vel_vol = b.vel_vol.as<float4>();
     p.C = b.C.as<float>();
     p.F = b.F.as<float>();
+    p.P = plastic ? b.P.as<float>() : nullptr;
     p.cls = b.cls.as<uint32_t>();
     p.force = force.as<float4>();
     p.ins_id = b.ins_id.as<uint32_t>();
@@ -1204,8 +1219,28 @@ tmpm_status pack_class(const HostWorld& w, size_t i, uint32_t scene, float rad, 
         return fail(TMPM_INVALID_INPUT, "fluid parameters not representable");
       c.wave_numerator = c.bulk * c.gamma;
       return TMPM_OK;
+    case TMPM_MATERIAL_SNOW:
+    case TMPM_MATERIAL_SAND:
+    case TMPM_MATERIAL_SAND_NEO_HOOKEAN: {
+      double lambda, mu;
+      lame(m.p[0], m.p[1], lambda, mu);
+      if (!finite_f32(lambda, c.lambda) || !finite_f32(mu, c.mu))
+        return fail(TMPM_INVALID_INPUT, "Lame parameters not representable");
+      c.wave_numerator = c.lambda + 2.0f * c.mu;
+      bool ok = true;
+      if (m.kind == TMPM_MATERIAL_SNOW) {
+        c.kind = DM_SNOW;
+        ok = finite_f32(m.p[2], c.crit_compression) && finite_f32(m.p[3], c.crit_stretch) &&
+             finite_f32(m.p[4], c.hardening);
+      } else {
+        c.kind = m.kind == TMPM_MATERIAL_SAND ? DM_SAND : DM_SAND_NH;
+        ok = finite_f32(m.p[2], c.friction_angle) && finite_f32(m.p[3], c.cohesion);
+      }
+      if (!ok) return fail(TMPM_INVALID_INPUT, "plastic material parameters not representable");
+      return TMPM_OK;
+    }
     default:
-      return fail(TMPM_RESIDENT_UNAVAILABLE, "plastic materials (sand, snow) are not supported by the HIP backend");
+      return fail(TMPM_INVALID_INPUT, "unknown material");
   }
 }
 
@@ -1217,8 +1252,7 @@ uint32_t intern_class(tmpm_session* s, const DevClass& c) {
   if (last_s == s && last_id < s->host_classes.size() &&
       std::memcmp(&s->host_classes[last_id], &c, sizeof(DevClass)) == 0)
     return last_id;
-  auto key = std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
-                             c.region, c.fixed);
+  auto key = class_key(c);
   auto it = s->class_ids.find(key);
   uint32_t id;
   if (it != s->class_ids.end()) {
@@ -1235,7 +1269,7 @@ uint32_t intern_class(tmpm_session* s, const DevClass& c) {
 
 // Validates and converts particle i of `w`; *en tells whether it is enabled.
 tmpm_status pack_one(const HostWorld& w, size_t i, uint32_t k, float inv_h, float f[8], float cf[9], float ff[9],
-                     DevClass& c, bool* en) {
+                     float pp[3], DevClass& c, bool* en) {
   *en = w.enabled[i] != 0;
   bool ok = true;
   for (int a = 0; a < 3; ++a) ok = finite_f32(w.pos[3 * i + a], f[a]) && ok;
@@ -1252,6 +1286,8 @@ tmpm_status pack_one(const HostWorld& w, size_t i, uint32_t k, float inv_h, floa
   for (int q = 0; q < 9; ++q)
     if (!finite_f32(w.C[9 * i + q], cf[q]) || !finite_f32(w.F[9 * i + q], ff[q]))
       return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
+  for (int q = 0; q < 3; ++q)
+    if (!finite_f32(w.plastic[3 * i + q], pp[q])) return fail(TMPM_INVALID_INPUT, "plastic state not representable in f32");
   tmpm_status st = pack_class(w, i, k, rad, damp, c);
   if (st != TMPM_OK) return st;
   // base cell must be representable (gpu.rs:1143-1153)
@@ -1260,6 +1296,13 @@ tmpm_status pack_one(const HostWorld& w, size_t i, uint32_t k, float inv_h, floa
     if (!cell_f32(f[a], inv_h, cc)) return fail(TMPM_INVALID_INPUT, "particle cell not representable");
   }
   return TMPM_OK;
+}
+
+// Whether any interned particle class carries a plastic state.
+bool classes_plastic(const tmpm_session* s) {
+  for (const DevClass& c : s->host_classes)
+    if (dm_plastic(c.kind)) return true;
+  return false;
 }
 
 // Appends scene k's enabled particles to `out` (insertion index first + i)
@@ -1285,9 +1328,9 @@ tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
     for (size_t c = cb; c < ce; ++c) {
       ChunkInfo& ci = info[c];
       for (size_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) {
-        float f[8], cf[9], ff[9];
+        float f[8], cf[9], ff[9], pp[3];
         bool en;
-        tmpm_status st = pack_one(w, i, k, inv_h, f, cf, ff, classes[i], &en);
+        tmpm_status st = pack_one(w, i, k, inv_h, f, cf, ff, pp, classes[i], &en);
         if (st != TMPM_OK) {
           ci.bad = i;
           ci.st = st;
@@ -1312,6 +1355,7 @@ tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
   out.vv.resize(e0 + ne);
   out.C.resize(9 * (e0 + ne));
   out.F.resize(9 * (e0 + ne));
+  out.Pl.resize(3 * (e0 + ne));
   out.cls.resize(e0 + ne);
   out.ins.resize(e0 + ne);
   out.pos32.resize(3 * (e0 + ne));
@@ -1322,10 +1366,10 @@ tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
     for (size_t c = cb; c < ce; ++c) {
       size_t e = e0 + eoff[c], d = d0 + 2 * doff[c];
       for (size_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) {
-        float f[8], cf[9], ff[9];
+        float f[8], cf[9], ff[9], pp[3];
         bool en;
         DevClass dc;
-        pack_one(w, i, k, inv_h, f, cf, ff, dc, &en);  // validated in the first pass
+        pack_one(w, i, k, inv_h, f, cf, ff, pp, dc, &en);  // validated in the first pass
         const uint32_t g = scene.first + (uint32_t)i;
         if (!en) {
           out.disabled[d++] = make_float4(f[0], f[1], f[2], __builtin_bit_cast(float, g));
@@ -1338,6 +1382,7 @@ tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
         out.vv[e] = make_float4(f[4], f[5], f[6], f[7]);
         std::memcpy(&out.C[9 * e], cf, sizeof cf);
         std::memcpy(&out.F[9 * e], ff, sizeof ff);
+        std::memcpy(&out.Pl[3 * e], pp, sizeof pp);
         out.ins[e] = g;
         ++e;
       }
@@ -1355,6 +1400,7 @@ tmpm_status alloc_particles(ParticleBuffers& pb, uint32_t cap) {
   HIP_TRY(pb.vel_vol.alloc(16ull * cap));
   HIP_TRY(pb.C.alloc(36ull * cap));
   HIP_TRY(pb.F.alloc(36ull * cap));
+  HIP_TRY(pb.P.alloc(12ull * cap));
   HIP_TRY(pb.cls.alloc(4ull * cap));
   HIP_TRY(pb.ins_id.alloc(4ull * cap));
   return TMPM_OK;
@@ -1374,6 +1420,9 @@ tmpm_status copy_particles(ParticleBuffers& dst, uint32_t dst_cap, const Particl
     HIP_TRY(hipMemcpyAsync(dst.F.as<float>() + (size_t)q * dst_cap, src.F.as<float>() + (size_t)q * src_cap, 4ull * n,
                            hipMemcpyDeviceToDevice, st));
   }
+  for (int q = 0; q < 3; ++q)
+    HIP_TRY(hipMemcpyAsync(dst.P.as<float>() + (size_t)q * dst_cap, src.P.as<float>() + (size_t)q * src_cap, 4ull * n,
+                           hipMemcpyDeviceToDevice, st));
   return TMPM_OK;
 }
 
@@ -1387,16 +1436,22 @@ tmpm_status upload_particles(tmpm_session* s, PackedParticles& pk, uint32_t at) 
   HIP_TRY(hipMemcpy(pb.vel_vol.as<float4>() + at, pk.vv.data(), 16ull * n, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.cls.as<uint32_t>() + at, pk.cls.data(), 4ull * n, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pb.ins_id.as<uint32_t>() + at, pk.ins.data(), 4ull * n, hipMemcpyHostToDevice));
-  // component-major columns (9 of C, then 9 of F) transposed in parallel
+  // component-major columns (9 of C, 9 of F, 3 of the plastic state)
+  // transposed in parallel
   auto& cols = pk.cols;
-  cols.resize(18 * n);
+  cols.resize(21 * n);
   parallel_for(n, [&](size_t b, size_t e) {
-    for (size_t i = b; i < e; ++i)
+    for (size_t i = b; i < e; ++i) {
       for (int q = 0; q < 9; ++q) {
         cols[(size_t)q * n + i] = pk.C[9 * i + q];
         cols[(size_t)(9 + q) * n + i] = pk.F[9 * i + q];
       }
+      for (int q = 0; q < 3; ++q) cols[(size_t)(18 + q) * n + i] = pk.Pl[3 * i + q];
+    }
   });
+  for (int q = 0; q < 3; ++q)
+    HIP_TRY(hipMemcpy(pb.P.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)(18 + q) * n], 4ull * n,
+                      hipMemcpyHostToDevice));
   for (int q = 0; q < 9; ++q) {
     HIP_TRY(hipMemcpy(pb.C.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)q * n], 4ull * n,
                       hipMemcpyHostToDevice));
@@ -1509,6 +1564,7 @@ tmpm_status prepare_packed(tmpm_session* s, PackedParticles& pk) {
     if (st != TMPM_OK) return st;
   }
   s->n = (uint32_t)pk.size();
+  s->plastic = classes_plastic(s);
   s->n_enabled = s->n;
   s->synced_n = 0;
   s->disabled_host.assign(pk.disabled.begin(), pk.disabled.end());
@@ -1614,13 +1670,13 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   }
   s->substeps_since_reorder++;
   if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
-                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_copies, s->stream); })) != hipSuccess)
+                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_copies, s->plastic, s->stream); })) != hipSuccess)
     return e;
   if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, react, s->shapes, s->stream); })) !=
       hipSuccess)
     return e;
   // The P2G kernel template takes 256-thread launch bounds; G2P uses the same.
-  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, react, s->shapes, s->stream); })) !=
+  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, react, s->shapes, s->plastic, s->stream); })) !=
       hipSuccess)
     return e;
   if (react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, ob, s->stream); })) != hipSuccess)
@@ -1679,6 +1735,9 @@ tmpm_status read_back(tmpm_session* s, std::vector<HostWorld>& out, std::vector<
     HIP_TRY(hipMemcpy(F.data() + (size_t)q * n, b.F.as<float>() + (size_t)q * s->cap, 4 * n, hipMemcpyDeviceToHost));
   }
   HIP_TRY(hipMemcpy(ins.data(), b.ins_id.ptr, 4 * n, hipMemcpyDeviceToHost));
+  std::vector<float> P(s->plastic ? 3 * n : 0);
+  for (int q = 0; q < 3 && s->plastic; ++q)
+    HIP_TRY(hipMemcpy(P.data() + (size_t)q * n, b.P.as<float>() + (size_t)q * s->cap, 4 * n, hipMemcpyDeviceToHost));
   // insertion index -> scene: scenes occupy consecutive ranges
   std::vector<std::pair<uint32_t, uint32_t>> ranges;  // (first, scene)
   for (uint32_t k = 0; k < ns; ++k)
@@ -1706,6 +1765,8 @@ tmpm_status read_back(tmpm_session* s, std::vector<HostWorld>& out, std::vector<
       w.F[9 * i + q] = F[q * n + slot];
       finite = finite && std::isfinite(w.C[9 * i + q]) && std::isfinite(w.F[9 * i + q]);
     }
+    if (s->plastic && is_plastic(w.material[i]))
+      for (int q = 0; q < 3; ++q) w.plastic[3 * i + q] = P[q * n + slot];
     if (!finite) st[k] = TMPM_NON_FINITE_STATE;
   }
   return TMPM_OK;
@@ -1720,19 +1781,20 @@ tmpm_status materialize(tmpm_session* s, uint32_t k) {
     scene.device_newer = false;
     return TMPM_OK;
   }
-  const size_t bytes = 104ull * scene.count;
+  const size_t bytes = 4ull * kGatherFloats * scene.count;
   if (s->staging.bytes < bytes) HIP_TRY(s->staging.alloc(bytes));
   Parts sp = s->parts(0);
   sp.pos_mass = s->synced.pos_mass.as<float4>();
   sp.vel_vol = s->synced.vel_vol.as<float4>();
   sp.C = s->synced.C.as<float>();
   sp.F = s->synced.F.as<float>();
+  sp.P = s->plastic ? s->synced.P.as<float>() : nullptr;
   sp.cls = s->synced.cls.as<uint32_t>();
   sp.ins_id = s->synced.ins_id.as<uint32_t>();
   sp.n = s->synced_n;
   launch_gather_scene(sp, scene.first, scene.count, s->staging.as<float>(), s->stream);
   HIP_TRY(hipGetLastError());
-  std::vector<float> rec(26ull * scene.count);
+  std::vector<float> rec((size_t)kGatherFloats * scene.count);
   HIP_TRY(hipMemcpyAsync(rec.data(), s->staging.ptr, bytes, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   HostWorld w = scene.world;
@@ -1744,7 +1806,7 @@ tmpm_status materialize(tmpm_session* s, uint32_t k) {
       for (int q = 0; q < 9; ++q) w.C[9 * i + q] = 0.0;
       continue;
     }
-    const float* r = &rec[26 * i];
+    const float* r = &rec[(size_t)kGatherFloats * i];
     for (int q = 0; q < 3; ++q) {
       w.pos[3 * i + q] = r[q];
       w.vel[3 * i + q] = r[4 + q];
@@ -1753,6 +1815,8 @@ tmpm_status materialize(tmpm_session* s, uint32_t k) {
       w.C[9 * i + q] = r[8 + q];
       w.F[9 * i + q] = r[17 + q];
     }
+    if (s->plastic && is_plastic(w.material[i]))
+      for (int q = 0; q < 3; ++q) w.plastic[3 * i + q] = r[26 + q];
   }
   const uint64_t substeps = scene.world.substeps;
   scene.set_world(w);
@@ -2535,7 +2599,9 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
   PackedParticles& pk = s->stage;
   pk.clear();
   st = pack_scene(s, slot, pk);
+  const bool was_plastic = s->plastic;
   auto undo = [&]() {
+    s->plastic = was_plastic;
     if (appended) s->scenes.pop_back();
     else {
       s->scenes[slot] = SceneState();
@@ -2543,8 +2609,7 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
     }
     for (size_t c = saved_classes; c < s->host_classes.size(); ++c) {
       const DevClass& d = s->host_classes[c];
-      s->class_ids.erase(std::make_tuple(d.kind, d.radius, d.damping, d.lambda, d.mu, d.bulk, d.gamma, d.viscosity,
-                                         d.tensile, d.region, d.fixed));
+      s->class_ids.erase(class_key(d));
     }
     s->host_classes.resize(saved_classes);
   };
@@ -2552,6 +2617,7 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
     undo();
     return st;
   }
+  s->plastic = classes_plastic(s);
   // from here on device state changes; a failure invalidates the batch
   auto broken = [&](tmpm_status code) {
     s->valid = false;
@@ -2573,6 +2639,7 @@ tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, 
       std::swap(a.vel_vol.ptr, b.vel_vol.ptr); std::swap(a.vel_vol.bytes, b.vel_vol.bytes);
       std::swap(a.C.ptr, b.C.ptr); std::swap(a.C.bytes, b.C.bytes);
       std::swap(a.F.ptr, b.F.ptr); std::swap(a.F.bytes, b.F.bytes);
+      std::swap(a.P.ptr, b.P.ptr); std::swap(a.P.bytes, b.P.bytes);
       std::swap(a.cls.ptr, b.cls.ptr); std::swap(a.cls.bytes, b.cls.bytes);
       std::swap(a.ins_id.ptr, b.ins_id.ptr); std::swap(a.ins_id.bytes, b.ins_id.bytes);
     };

@@ -56,11 +56,10 @@ def test_transfers_track_cpu_reference_for_elastic_particles(transfers, mode):
 
 
 def test_transfers_track_cpu_with_materials_fixed_and_disabled_particles(transfers):
-    # gpu.rs:4288-4350 for the materials the HIP backend runs on the device
-    # (elastic, Neo-Hookean, fluid); plastic models are not yet supported and
-    # are refused with ResidentUnavailable (checked below).
+    # gpu.rs:4288-4350: all five materials, a force, damping, fixed and disabled particles
     mats = [MaterialModel.elastic(1000.0, 0.2), MaterialModel.neo_hookean(1000.0, 0.2),
-            MaterialModel.fluid(2000.0, 7.0, 0.01)]
+            MaterialModel.sand(1000.0, 0.2, math.radians(35.0), 0.0), MaterialModel.fluid(2000.0, 7.0, 0.01),
+            MaterialModel.snow(1000.0, 0.2)]
     ps = [MpmParticle.new([0.3 + i * 0.035, 0.5, 0.5], 0.03, 1000.0, m) for i, m in enumerate(mats)]
     ps[0].force = np.array([0.1, -0.2, 0.3])
     ps[1].damping = 0.4
@@ -75,12 +74,176 @@ def test_transfers_track_cpu_with_materials_fixed_and_disabled_particles(transfe
     s = HipMpmResidentSession(transfers, w, w.stable_timestep())
     s.step_adaptive(0.002)
     got = s.read_particles()
-    assert_tracks(cpu.read(), got, 1e-5, 1e-4, F=1e-5)
-    assert list(got.enabled) == [True] * 4 + [False]
-    assert list(got.fixed) == [False] * 3 + [True, False]
-    sand = MpmParticle.new([0.5, 0.5, 0.5], 0.03, 1000.0, MaterialModel.snow(1000.0, 0.2))
-    with pytest.raises(ResidentUnavailable):
-        HipMpmResidentSession(transfers, world_of([sand], MpmParams()), 1e-4)
+    ref = cpu.read()
+    assert_tracks(ref, got, 1e-5, 1e-4, F=1e-5)
+    assert np.abs(ref["plastic"] - got.plastic).max() < 1e-5
+    assert list(got.enabled) == [True] * 6 + [False]
+    assert list(got.fixed) == [False] * 5 + [True, False]
+
+
+def test_corotated_stress_tracks_cpu_for_elastic_sand_and_snow(transfers):
+    # gpu.rs:4423-4516: sheared elastic, sand and snow (plastic_det 0.8) plus two
+    # sand particles compressed and stretched past yield; 12 steps of 0.001
+    mats = [MaterialModel.elastic(4000.0, 0.2), MaterialModel.sand(4000.0, 0.2, math.radians(30.0), 0.0),
+            MaterialModel.snow(4000.0, 0.2)]
+    ps = []
+    for i, m in enumerate(mats):
+        p = MpmParticle.new([0.25 + i * 0.25, 0.4, 0.5], 0.04, 1000.0, m)
+        p.deformation = np.array([[1.1, 0.15, 0.02], [0.0, 0.88, 0.08], [0.0, 0.0, 1.04]])
+        if i == 2:
+            p.plastic.plastic_det = 0.8
+        ps.append(p)
+    for x, diag in [(0.25, [0.95, 0.8, 0.75]), (0.5, [1.2, 0.9, 0.7])]:
+        p = MpmParticle.new([x, 0.7, 0.5], 0.04, 1000.0, MaterialModel.sand(4000.0, 0.2, math.radians(30.0), 0.0))
+        p.deformation = np.diag(diag)
+        ps.append(p)
+    w = world_of(ps, MpmParams(gravity=np.zeros(3)))
+    cpu = O.OracleWorld(w)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep())
+    # the first step's return mapping, to the reference's per-transfer bound
+    cpu.step(0.001)
+    s.step_adaptive(0.001)
+    ref, got = cpu.read(), s.read_particles()
+    for i in (1, 2, 3, 4):
+        assert np.linalg.norm(ref["deformation"][i] - got.deformation[i]) < 2e-5
+        assert np.abs(ref["plastic"][i] - got.plastic[i]).max() < 2e-5
+    assert got.plastic[4, 1] > 0.0  # the stretched sand particle hardened
+    for _ in range(11):
+        cpu.step(0.001)
+        s.step_adaptive(0.001)
+    assert s.substeps == cpu.substeps
+    ref, got = cpu.read(), s.read_particles()
+    assert_tracks(ref, got, 2e-5, 3e-4, F=2e-4)
+    assert np.abs(ref["plastic"][:, 0] - got.plastic[:, 0]).max() < 2e-4
+
+
+def test_near_singular_plastic_deformation_takes_the_fp64_fallback(transfers):
+    # gpu.rs:4518-4570: F(0, 0) = 1e-6; the reference falls back to the CPU, this
+    # backend to its FP64 SVD on the device
+    mats = [MaterialModel.elastic(2000.0, 0.2), MaterialModel.sand(2000.0, 0.2, math.radians(30.0), 0.0),
+            MaterialModel.snow(2000.0, 0.2)]
+    ps = []
+    for i, m in enumerate(mats):
+        p = MpmParticle.new([0.2 + i * 0.25, 0.4, 0.4], 0.04, 1000.0, m)
+        p.deformation[0, 0] = 1e-6
+        ps.append(p)
+    w = world_of(ps, MpmParams(gravity=np.zeros(3)))
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep())
+    s.step_adaptive(0.001)
+    ref, got = cpu.read(), s.read_particles()
+    assert_tracks(ref, got, 1e-5, 1e-4, F=2e-4)
+    assert np.abs(ref["plastic"][:, 0] - got.plastic[:, 0]).max() < 2e-4
+
+
+def _euler(roll, pitch, yaw):
+    """nalgebra's from_euler_angles: Rz(yaw) Ry(pitch) Rx(roll)."""
+    cr, sr, cp, sp, cy, sy = (math.cos(roll), math.sin(roll), math.cos(pitch), math.sin(pitch), math.cos(yaw),
+                              math.sin(yaw))
+    rx = np.array([[1, 0, 0], [0, cr, -sr], [0, sr, cr]])
+    ry = np.array([[cp, 0, sp], [0, 1, 0], [-sp, 0, cp]])
+    rz = np.array([[cy, -sy, 0], [sy, cy, 0], [0, 0, 1]])
+    return rz @ ry @ rx
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_sand_neo_hookean_tracks_cpu(transfers, mode):
+    # gpu.rs:2339-2440: identity, rotation, isotropic stretch and compression,
+    # rotated anisotropic stretch; hardening 0.2 i; 4 substeps of 1e-4
+    r = _euler(0.2, -0.4, 0.7)
+    defs = [np.eye(3), r, np.eye(3) * 1.1, np.diag([0.8, 0.79, 0.81]),
+            r @ np.diag([math.exp(0.15), math.exp(-0.25), math.exp(-0.1)])]
+    ps = []
+    for i, F in enumerate(defs):
+        p = MpmParticle.new([0.5 + i * 0.4, 0.5, 0.5], 0.04, 1000.0,
+                            MaterialModel.sand_neo_hookean(4000.0, 0.2, math.radians(35.0), 0.02))
+        p.deformation = F
+        p.plastic.hardening += i * 0.2
+        ps.append(p)
+    w = world_of(ps, MpmParams(gravity=np.zeros(3), bounds=WorldBounds(np.zeros(3), np.full(3, 3.0))))
+    cpu = O.OracleWorld(w)
+    cpu.substeps_fixed(1e-4, 4)
+    s = HipMpmResidentSession(transfers, w, 1e-4, mode)
+    s.step(4)
+    ref, got = cpu.read(), s.read_particles()
+    assert_tracks(ref, got, 2e-5, 3e-4, 2e-3, 2e-4)
+    assert np.abs(ref["plastic"] - got.plastic).max() < 2e-4
+    assert got.plastic[2, 1] > ps[2].plastic.hardening and got.plastic[4, 1] > ps[4].plastic.hardening
+
+
+def test_sand_hardening_tracks_multiload_cpu_and_resident_state(transfers):
+    # gpu.rs:2442-2524: sand and sand-NH loaded four times (a fresh substep from
+    # each load, carrying the device's plastic state), then 5 resident substeps
+    mats = [MaterialModel.sand(4000.0, 0.2, math.radians(35.0), 0.0),
+            MaterialModel.sand_neo_hookean(4000.0, 0.2, math.radians(35.0), 0.0)]
+    ps = [MpmParticle.new([0.3 + i * 0.4, 0.5, 0.5], 0.04, 1000.0, m) for i, m in enumerate(mats)]
+    params = MpmParams(gravity=np.zeros(3))
+    dt = 1e-5
+    for load in [np.array([0.15, -0.25, -0.1])] * 2 + [np.full(3, math.log(1.2)), np.array([0.15, -0.25, -0.1])]:
+        for p in ps:
+            p.deformation = np.diag(np.exp(load))
+        w = world_of(ps, params)
+        cpu = O.OracleWorld(w)
+        cpu.substeps_fixed(dt, 1)
+        s = HipMpmResidentSession(transfers, w, dt)
+        s.step(1)
+        ref, got = cpu.read(), s.read_particles()
+        for i, p in enumerate(ps):
+            assert np.linalg.norm(ref["deformation"][i] - got.deformation[i]) < 2e-5
+            assert np.abs(ref["plastic"][i] - got.plastic[i]).max() < 2e-5
+            assert got.plastic[i, 1] > p.plastic.hardening
+            p.plastic.plastic_det, p.plastic.hardening, p.plastic.log_volume_gain = map(float, got.plastic[i])
+            p.deformation = got.deformation[i].copy()
+            p.velocity = np.zeros(3)
+            p.affine = np.zeros((3, 3))
+    w = world_of(ps, params)
+    cpu = O.OracleWorld(w)
+    s = HipMpmResidentSession(transfers, w, dt)
+    s.submit_steps(5)
+    cpu.substeps_fixed(dt, 5)
+    s.synchronize()
+    ref, got = cpu.read(), s.read_particles()
+    for i in range(2):
+        assert np.linalg.norm(ref["deformation"][i] - got.deformation[i]) < 3e-5
+        assert np.abs(ref["plastic"][i] - got.plastic[i]).max() < 3e-5
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_plastic_blocks_in_a_batch_equal_the_same_scene_alone(transfers, mode):
+    # plastic state follows its particle through reorders and batch readback
+    from tessera_amd.mpm import HipMpmBatch
+    from tessera_amd.scenes import jittered_lattice, _particles
+
+    worlds = []
+    for k, m in enumerate([MaterialModel.snow(2.0e4, 0.3), MaterialModel.sand(2.0e4, 0.3, math.radians(30.0), 0.0),
+                           MaterialModel.sand_neo_hookean(2.0e4, 0.3, math.radians(30.0), 0.01)]):
+        rng = np.random.default_rng(k)
+        h = 1.0 / 64
+        pos = jittered_lattice(rng, [10, 10, 10], [0.4, 0.1, 0.4], [0.4 + 10 * h / 2, 0.1 + 10 * h / 2, 0.4 + 10 * h / 2])
+        vel = rng.normal(0.0, 0.3, size=pos.shape)
+        pa = _particles(pos, vel, h / 4, 1000.0, m)
+        floor = RigidObstacle.ground([0.5, 0.08, 0.5], [0.4, 0.4], quat_x(-math.pi / 2))
+        floor.boundary = 1
+        w = MpmWorld(pa, MpmParams(cell_width=h, gravity=np.array([0.0, -9.8, 0.0]), max_substep=1e-4,
+                                   bounds=WorldBounds(np.zeros(3), np.ones(3))))
+        w.set_obstacles([floor])
+        worlds.append(w)
+    b = HipMpmBatch(transfers, worlds, [1e-4] * 3, mode)
+    b.step(40)
+    for k, w in enumerate(worlds):
+        s = HipMpmResidentSession(transfers, w, 1e-4, mode)
+        s.step(40)
+        a, g = s.read_particles(), b.read_particles(k)
+        for f in ("position", "velocity", "deformation", "plastic"):
+            assert np.array_equal(getattr(a, f), getattr(g, f)), (k, f)
+        assert np.any(g.plastic != np.array([1.0, 0.0, 0.0]))  # the floor impact yielded
+
+
+def quat_x(angle):
+    from tessera_amd.mpm import quat_from_axis_angle
+
+    return quat_from_axis_angle([1, 0, 0], angle)
 
 
 def test_neo_hookean_and_viscous_fluid_stress_track_cpu(transfers):
