@@ -153,10 +153,11 @@ typedef struct tmpm_world_desc {
   uint64_t substeps;
 } tmpm_world_desc;
 
-/* P2G accumulation precision.  FP32 uses hardware float atomics (order
- * dependent in the last bits, like the reference's CAS loops,
- * gpu.wgsl:739-766).  FIXED64 accumulates 64-bit fixed-point integers, so
- * every substep is bit-reproducible run to run. */
+/* P2G precision (the reference accumulates with f32 CAS loops,
+ * gpu.wgsl:739-766).  Both modes add 64-bit fixed-point integers in LDS and
+ * are bit-reproducible run to run.  FP32 rounds each particle block's tile
+ * to f32 and sums up to 8 tiles per node in f32 in a fixed order; FIXED64
+ * sums the integers exactly and rounds each node once. */
 typedef enum tmpm_p2g_mode { TMPM_P2G_FP32 = 0, TMPM_P2G_FIXED64 = 1 } tmpm_p2g_mode;
 
 typedef struct tmpm_context tmpm_context;
