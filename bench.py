@@ -84,6 +84,12 @@ def batch_leg(transfers, mode: str, steps: int) -> dict:
     t0 = time.perf_counter()
     submit(b, steps)
     b.wait()
+    t_default = time.perf_counter() - t0
+    # the launch-shape autotuner (its trials are real substeps with identical results)
+    cfg = b.autotune(16)
+    t0 = time.perf_counter()
+    submit(b, steps)
+    b.wait()
     t_batch = time.perf_counter() - t0
     b.synchronize()
     failed = sum(1 for k in range(len(scenes)) if b.scene_error(k) is not None)
@@ -106,6 +112,7 @@ def batch_leg(transfers, mode: str, steps: int) -> dict:
           f"{alone:.4e} particle-updates/s", file=sys.stderr, flush=True)
     return {"workload": "batch_256_scenes", "scenes": len(scenes), "particles": total, "steps": steps,
             "value": value, "unit": "particle-updates/s", "ms_per_step": t_batch / steps * 1e3,
+            "value_default_shape": total * steps / t_default, "launch_config": cfg,
             "one_session_at_a_time": alone, "speedup": value / alone, "failed_scenes": failed}
 
 

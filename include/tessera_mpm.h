@@ -364,6 +364,24 @@ tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene);
  * kernel names.  Timing is off unless enabled (it adds one event pair per
  * kernel launch). */
 tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled);
+/* No counterpart in the reference (its encoder records one command buffer per
+ * submit).  With graphs on (off by default; TMPM_GRAPHS=1 turns them on) each
+ * substep's launches are captured once per distinct launch plan into a HIP
+ * graph and replayed, which lowers the host cost per substep.  Results are
+ * identical either way.  Per-kernel timing launches eagerly. */
+tmpm_status tmpm_session_set_graphs(tmpm_session* s, int32_t enabled);
+/* No counterpart in the reference.  Chooses the launch shape (P2G workgroup
+ * size and LDS tile copies, G2P workgroup size, reorder interval, graphs) by
+ * timing each candidate over `trial` real substeps (8..=256), one knob at a
+ * time, and keeps the fastest.  The trials advance the session like
+ * tmpm_session_submit_steps (they become pending substeps; *substeps_run
+ * says how many) and compute exactly what any shape computes; with obstacle
+ * reactions on, the G2P workgroup and reorder interval stay fixed because
+ * the particle-side reaction sums follow them. */
+tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run);
+/* The current launch shape: {P2G threads, P2G tile copies, G2P threads,
+ * reorder interval, graphs on}. */
+tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);
 

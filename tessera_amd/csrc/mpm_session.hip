@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -580,6 +581,13 @@ inline ClassKey class_key(const DevClass& c) {
                          c.cohesion);
 }
 
+// A captured substep: the launch plan's bytes and the instantiated graph.
+struct CachedGraph {
+  std::vector<unsigned char> plan;
+  hipGraphExec_t exec = nullptr;
+  uint64_t last_use = 0;
+};
+
 struct tmpm_session {
   tmpm_context* ctx = nullptr;
   hipStream_t stream = nullptr;
@@ -641,6 +649,11 @@ struct tmpm_session {
 
   // timing
   bool timing = false;
+  // substep graphs (hipGraph): captured per distinct launch plan and replayed;
+  // off by default (measured slower on the device for one substep per graph)
+  bool use_graphs = false;
+  std::vector<CachedGraph> graphs;
+  uint64_t graph_clock = 0;
   std::vector<KernelTimer> pending_timers;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, uint64_t>> kernel_ms;
@@ -1014,6 +1027,7 @@ void read_tuning_env(tmpm_session* s) {
     if (th == 64 || th == 128) s->p2g_threads = (uint32_t)th;
   }
   if (const char* v = std::getenv("TMPM_P2G_COPIES")) s->p2g_copies = std::atoi(v) == 1 ? 1 : 2;
+  if (const char* v = std::getenv("TMPM_GRAPHS")) s->use_graphs = std::atoi(v) != 0;
   if (const char* v = std::getenv("TMPM_REORDER_INTERVAL")) {
     int r = std::atoi(v);
     if (r >= 1 && r <= 1000000) s->reorder_interval = (uint32_t)r;
@@ -1639,54 +1653,165 @@ hipError_t enqueue_binning(tmpm_session* s, const Parts& p) {
   return hipSuccess;
 }
 
-hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
-  Parts p = s->parts(s->cur);
-  Topo& t = s->topo;
-  const DevClass* c = s->classes.as<DevClass>();
-  const DevScene* sc = (apply_force ? s->scene_force : s->scene_plain).as<DevScene>();
-  const DevObstacle* ob = s->obstacles.as<DevObstacle>();
-  const int react = s->react_active() && s->obstacle_total > 0 ? 1 : 0;
+// Everything one substep's launches depend on: two substeps with equal plans
+// (compared bytewise, padding zeroed) launch the same kernels with the same
+// arguments, so a graph captured for one replays the other.
+struct SubstepPlan {
+  Parts p, q;  // q: the reorder target (zero unless reorder)
+  Topo t;
+  const DevClass* c;
+  const DevScene* sc;       // this substep's scene table (with or without forces)
+  const DevScene* sc_plain;
+  DevObstacle* ob;
+  DevRigid* rigid;
+  const DevMapping* maps;
+  const uint32_t* body_scene;
+  uint32_t obstacle_total, body_total, p2g_blocks, p2g_threads, g2p_threads;
+  int32_t react, fixed64, shapes, plastic, cpic, copies;
+  uint8_t reorder, clear_table, rigid_sync, rigid_apply;
+};
+
+SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
+  SubstepPlan pl;
+  std::memset(&pl, 0, sizeof pl);
+  pl.p = s->parts(s->cur);
+  pl.reorder = s->substeps_since_reorder >= s->reorder_interval;
+  if (pl.reorder) pl.q = s->parts(1 - s->cur);
+  pl.t = s->topo;
+  pl.c = s->classes.as<DevClass>();
+  pl.sc = (apply_force ? s->scene_force : s->scene_plain).as<DevScene>();
+  pl.sc_plain = s->scene_plain.as<DevScene>();
+  pl.ob = s->obstacles.as<DevObstacle>();
+  pl.rigid = s->rigid.as<DevRigid>();
+  pl.maps = s->rigid_maps.as<DevMapping>();
+  pl.body_scene = s->body_scene.as<uint32_t>();
+  pl.obstacle_total = s->obstacle_total;
+  pl.body_total = s->body_total;
+  pl.p2g_blocks = s->p2g_blocks;
+  pl.p2g_threads = s->p2g_threads;
+  pl.g2p_threads = s->threads_pblock;
+  pl.react = s->react_active() && s->obstacle_total > 0 ? 1 : 0;
+  pl.fixed64 = s->fixed64 ? 1 : 0;
+  pl.shapes = s->shapes;
+  pl.plastic = s->plastic ? 1 : 0;
+  pl.cpic = s->cpic;
+  pl.copies = s->p2g_copies;
+  pl.clear_table = s->table_dirty;
+  pl.rigid_sync = s->body_total > 0 && s->obstacle_total > 0;
+  pl.rigid_apply = pl.react && s->body_total > 0;
+  return pl;
+}
+
+// The launches of one substep, in stream order.  Changes no host state, so
+// it can be recorded into a graph.
+hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
+  const Topo& t = pl.t;
+  const Parts& p0 = pl.p;
+  const Parts& p = pl.reorder ? pl.q : pl.p;  // P2G and G2P run on the reordered state
+  hipStream_t st = s->stream;
   hipError_t e;
-  if (s->body_total > 0 && s->obstacle_total > 0 &&
-      (e = timed(s, "k_rigid_sync", [&] {
-         launch_rigid_sync(s->rigid.as<DevRigid>(), s->rigid_maps.as<DevMapping>(), s->obstacles.as<DevObstacle>(),
-                           s->obstacle_total, s->stream);
+  if (pl.rigid_sync && (e = timed(s, "k_rigid_sync", [&] {
+                          launch_rigid_sync(pl.rigid, pl.maps, pl.ob, pl.obstacle_total, st);
+                        })) != hipSuccess)
+    return e;
+  if (pl.clear_table && (e = timed(s, "k_topo_clear", [&] { launch_topo_clear(t, st); })) != hipSuccess) return e;
+  if (pl.cpic &&
+      (e = timed(s, "k_cpic_mask", [&] { launch_cpic_mask(p0, t, pl.c, pl.sc_plain, pl.ob, st); })) != hipSuccess)
+    return e;
+  if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p0, t, pl.c, pl.sc_plain, pl.cpic, st); })) !=
+      hipSuccess)
+    return e;
+  if ((e = timed(s, "k_bin_blocks", [&] { launch_bin_blocks(p0, t, pl.c, pl.sc_plain, pl.cpic, st); })) !=
+      hipSuccess)
+    return e;
+  if ((e = timed(s, "k_bin_offsets", [&] { launch_bin_offsets(p0, t, pl.c, pl.sc_plain, st); })) != hipSuccess)
+    return e;
+  if (pl.reorder) {
+    if ((e = timed(s, "k_bin_scatter_move", [&] { launch_bin_scatter_move(p0, pl.q, t, p0.n, st); })) != hipSuccess)
+      return e;
+  } else if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p0, t, st); })) != hipSuccess) {
+    return e;
+  }
+  if ((e = timed(s, pl.fixed64 ? "k_p2g_fixed64" : "k_p2g", [&] {
+         launch_p2g(p, t, pl.c, pl.sc, pl.fixed64, pl.p2g_blocks, pl.p2g_threads, pl.copies, pl.plastic, st);
        })) != hipSuccess)
     return e;
-  e = enqueue_binning(s, p);
+  if ((e = timed(s, "k_grid_update",
+                 [&] { launch_grid_update(t, pl.sc, pl.ob, pl.fixed64, pl.react, pl.shapes, st); })) != hipSuccess)
+    return e;
+  if ((e = timed(s, "k_g2p", [&] {
+         launch_g2p(p, t, pl.c, pl.sc, pl.ob, pl.p2g_blocks, pl.g2p_threads, pl.react, pl.shapes, pl.plastic, st);
+       })) != hipSuccess)
+    return e;
+  if (pl.react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, pl.ob, st); })) != hipSuccess)
+    return e;
+  if (pl.rigid_apply && (e = timed(s, "k_rigid_apply", [&] {
+                           launch_rigid_apply(t, pl.sc, pl.rigid, pl.maps, pl.ob, pl.body_scene, pl.body_total, st);
+                         })) != hipSuccess)
+    return e;
+  return hipSuccess;
+}
+
+// Replays the graph captured for an equal plan, or captures one.  At most
+// kGraphCache graphs are kept (least recently used evicted): a steady run
+// cycles through a few plans (buffer parity, reorder or not).
+constexpr size_t kGraphCache = 8;
+
+hipError_t launch_graph(tmpm_session* s, const SubstepPlan& pl) {
+  for (auto& g : s->graphs)
+    if (std::memcmp(g.plan.data(), &pl, sizeof pl) == 0) {
+      g.last_use = ++s->graph_clock;
+      return hipGraphLaunch(g.exec, s->stream);
+    }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  if (s->substeps_since_reorder >= s->reorder_interval) {
-    Parts q = s->parts(1 - s->cur);
-    if ((e = timed(s, "k_bin_scatter_move", [&] { launch_bin_scatter_move(p, q, t, s->n, s->stream); })) !=
-        hipSuccess)
-      return e;
+  const hipError_t le = launch_plan(s, pl);
+  e = hipStreamEndCapture(s->stream, &graph);
+  if (le != hipSuccess || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return le != hipSuccess ? le : e;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return e;
+  if (s->graphs.size() >= kGraphCache) {
+    auto lru = std::min_element(s->graphs.begin(), s->graphs.end(),
+                                [](const CachedGraph& a, const CachedGraph& b) { return a.last_use < b.last_use; });
+    (void)hipGraphExecDestroy(lru->exec);
+    s->graphs.erase(lru);
+  }
+  CachedGraph g;
+  g.plan.resize(sizeof pl);
+  std::memcpy(g.plan.data(), &pl, sizeof pl);
+  g.exec = exec;
+  g.last_use = ++s->graph_clock;
+  s->graphs.push_back(std::move(g));
+  return hipGraphLaunch(exec, s->stream);
+}
+
+void destroy_graphs(tmpm_session* s) {
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
+// One substep: its launches (eager, or a replayed graph when graphs are on
+// and per-kernel timing is off), then the host-side bookkeeping.
+hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
+  const SubstepPlan pl = plan_substep(s, apply_force);
+  if ((uint64_t)pl.t.bin_groups * pl.t.bin_span < pl.p.n) return hipErrorInvalidValue;  // lists sized for fewer particles
+  const hipError_t e = s->use_graphs && !s->timing ? launch_graph(s, pl) : launch_plan(s, pl);
+  if (e != hipSuccess) return e;
+  if (pl.clear_table) s->table_dirty = false;
+  if (pl.reorder) {
     s->compacted_last = true;
     s->cur = 1 - s->cur;
-    p = q;
     s->substeps_since_reorder = 0;
   } else {
-    if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p, t, s->stream); })) != hipSuccess) return e;
     s->compacted_last = false;
   }
   s->substeps_since_reorder++;
-  if ((e = timed(s, s->fixed64 ? "k_p2g_fixed64" : "k_p2g",
-                 [&] { launch_p2g(p, t, c, sc, s->fixed64, s->p2g_blocks, s->p2g_threads, s->p2g_copies, s->plastic, s->stream); })) != hipSuccess)
-    return e;
-  if ((e = timed(s, "k_grid_update", [&] { launch_grid_update(t, sc, ob, s->fixed64, react, s->shapes, s->stream); })) !=
-      hipSuccess)
-    return e;
-  // The P2G kernel template takes 256-thread launch bounds; G2P uses the same.
-  if ((e = timed(s, "k_g2p", [&] { launch_g2p(p, t, c, sc, ob, s->p2g_blocks, s->threads_pblock, react, s->shapes, s->plastic, s->stream); })) !=
-      hipSuccess)
-    return e;
-  if (react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, ob, s->stream); })) != hipSuccess)
-    return e;
-  if (react && s->body_total > 0 &&
-      (e = timed(s, "k_rigid_apply", [&] {
-         launch_rigid_apply(t, sc, s->rigid.as<DevRigid>(), s->rigid_maps.as<DevMapping>(), ob,
-                            s->body_scene.as<uint32_t>(), s->body_total, s->stream);
-       })) != hipSuccess)
-    return e;
   s->topo_valid = false;
   return hipSuccess;
 }
@@ -2206,6 +2331,7 @@ void tmpm_session_destroy(tmpm_session* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   collect_timers(s);
   for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+  destroy_graphs(s);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -2933,6 +3059,93 @@ tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled) {
   collect_timers(s);
   s->kernel_ms.clear();
   s->timing = enabled != 0;
+  return TMPM_OK;
+}
+
+tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]) {
+  if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
+  out[0] = (int32_t)s->p2g_threads;
+  out[1] = s->p2g_copies;
+  out[2] = (int32_t)s->threads_pblock;
+  out[3] = (int32_t)s->reorder_interval;
+  out[4] = s->use_graphs ? 1 : 0;
+  return TMPM_OK;
+}
+
+// Launch-shape autotuner.  Every candidate shape computes the same results:
+// P2G sums integers (order-free), G2P is per particle, and the storage order
+// a reorder produces changes no sum.  The one exception is the particle side
+// of obstacle reactions, whose f32 per-lane partial sums follow the storage
+// order and G2P's workgroup size; with reactions on, those two knobs stay
+// fixed.  The trials therefore run real substeps of the simulation, one knob
+// at a time (P2G shape, G2P workgroup, reorder interval, graphs), each
+// candidate timed over `trial` substeps with events on the session stream.
+tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+  if (trial < 8 || trial > 256) return fail(TMPM_INVALID_INPUT, "trial must be within 8..=256 substeps");
+  if (substeps_run) *substeps_run = 0;
+  if (!s->prepared || s->n == 0) return TMPM_OK;
+  if (s->timing) return fail(TMPM_INVALID_INPUT, "autotuning needs per-kernel timing off");
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  uint32_t ran = 0;
+  tmpm_status st = TMPM_OK;
+  auto run = [&](uint32_t steps, float* ms) -> tmpm_status {
+    if (hipEventRecord(a, s->stream) != hipSuccess) return fail(TMPM_READBACK, "event record failed");
+    for (uint32_t left = steps; left > 0;) {
+      const uint32_t k = std::min<uint32_t>(left, 64);
+      tmpm_status e = tmpm_session_submit_steps(s, k);
+      if (e != TMPM_OK) return e;
+      left -= k;
+      ran += k;
+    }
+    if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
+      return fail(TMPM_READBACK, "event wait failed");
+    if (ms && hipEventElapsedTime(ms, a, b) != hipSuccess) return fail(TMPM_READBACK, "event time failed");
+    return TMPM_OK;
+  };
+  // Tries each value of one knob (set by `apply`) and keeps the fastest.
+  auto tune = [&](const std::vector<int>& values, const std::function<void(int)>& apply, int current) -> tmpm_status {
+    float best_ms = 0.0f;
+    int best = current;
+    for (size_t i = 0; i < values.size(); ++i) {
+      apply(values[i]);
+      float ms = 0.0f;
+      tmpm_status e = run(trial, &ms);
+      if (e != TMPM_OK) return e;
+      if (i == 0 || ms < best_ms) {
+        best_ms = ms;
+        best = values[i];
+      }
+    }
+    apply(best);
+    return TMPM_OK;
+  };
+  const bool order_free = !s->react_active();
+  st = run(std::min<uint32_t>(trial, 16), nullptr);  // warm-up
+  if (st == TMPM_OK) {
+    const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
+    st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
+  }
+  if (st == TMPM_OK && order_free)
+    st = tune({64, 128, 256}, [&](int v) { s->threads_pblock = (uint32_t)v; }, (int)s->threads_pblock);
+  if (st == TMPM_OK && order_free)
+    st = tune({4, 8, 16}, [&](int v) { s->reorder_interval = (uint32_t)v; }, (int)s->reorder_interval);
+  if (st == TMPM_OK) st = tune({0, 1}, [&](int v) { s->use_graphs = v != 0; }, s->use_graphs ? 1 : 0);
+  if (!s->use_graphs) destroy_graphs(s);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (substeps_run) *substeps_run = ran;
+  return st;
+}
+
+tmpm_status tmpm_session_set_graphs(tmpm_session* s, int32_t enabled) {
+  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->use_graphs = enabled != 0;
+  if (!s->use_graphs) destroy_graphs(s);
   return TMPM_OK;
 }
 

@@ -230,6 +230,9 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_session_export_blocks": (
             C.c_int, [vp, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
         "tmpm_session_set_timing": (C.c_int, [vp, C.c_int32]),
+        "tmpm_session_set_graphs": (C.c_int, [vp, C.c_int32]),
+        "tmpm_session_autotune": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "tmpm_session_launch_config": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "tmpm_session_kernel_times": (
             C.c_int,
             [vp, C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32,
@@ -1111,6 +1114,24 @@ class HipMpmResidentSession:
     def set_timing(self, enabled: bool) -> None:
         _check(self.lib.tmpm_session_set_timing(self._h, 1 if enabled else 0), self.lib)
 
+    def set_graphs(self, enabled: bool) -> None:
+        """Replay each substep as a captured HIP graph (default off); results are identical."""
+        _check(self.lib.tmpm_session_set_graphs(self._h, 1 if enabled else 0), self.lib)
+
+    def autotune(self, trial: int = 32) -> dict:
+        """Times the launch shapes over real substeps (which stay pending) and keeps the fastest."""
+        ran = C.c_uint32()
+        _check(self.lib.tmpm_session_autotune(self._h, int(trial), C.byref(ran)), self.lib)
+        cfg = self.launch_config()
+        cfg["substeps_run"] = int(ran.value)
+        return cfg
+
+    def launch_config(self) -> dict:
+        out = (C.c_int32 * 5)()
+        _check(self.lib.tmpm_session_launch_config(self._h, out), self.lib)
+        return {"p2g_threads": out[0], "p2g_copies": out[1], "g2p_threads": out[2], "reorder_interval": out[3],
+                "graphs": bool(out[4])}
+
     def kernel_times(self) -> dict[str, tuple[float, int]]:
         cap = 64
         names = C.create_string_buffer(4096)
@@ -1281,6 +1302,24 @@ class HipMpmBatch:
     # -- diagnostics ----------------------------------------------------------
     def set_timing(self, enabled: bool) -> None:
         _check(self.lib.tmpm_session_set_timing(self._h, 1 if enabled else 0), self.lib)
+
+    def set_graphs(self, enabled: bool) -> None:
+        """Replay each substep as a captured HIP graph (default off); results are identical."""
+        _check(self.lib.tmpm_session_set_graphs(self._h, 1 if enabled else 0), self.lib)
+
+    def autotune(self, trial: int = 32) -> dict:
+        """Times the launch shapes over real substeps (which stay pending) and keeps the fastest."""
+        ran = C.c_uint32()
+        _check(self.lib.tmpm_session_autotune(self._h, int(trial), C.byref(ran)), self.lib)
+        cfg = self.launch_config()
+        cfg["substeps_run"] = int(ran.value)
+        return cfg
+
+    def launch_config(self) -> dict:
+        out = (C.c_int32 * 5)()
+        _check(self.lib.tmpm_session_launch_config(self._h, out), self.lib)
+        return {"p2g_threads": out[0], "p2g_copies": out[1], "g2p_threads": out[2], "reorder_interval": out[3],
+                "graphs": bool(out[4])}
 
     def kernel_times(self) -> dict[str, tuple[float, int]]:
         return HipMpmResidentSession.kernel_times(self)  # same session-level entry point
