@@ -226,7 +226,7 @@ def main() -> int:
     if args.serve_seconds > 0:
         serving = serving_leg(transfers, args.mode, args.serve_seconds, value)
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
 
@@ -241,6 +241,21 @@ def main() -> int:
             "kind": "port",
             "sample": f"{args.cpu_substeps} substeps of the full {n}-particle scene, FP64 C++ restatement of "
                       f"MpmWorld::substep (single thread, {cb - ca:.1f}s)",
+        }
+        # the same restatement over every host core this process may use (per-chunk P2G grids
+        # merged in thread order); reported beside the single-thread port
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        ow = O.OracleWorld(scene.world)
+        ca = time.perf_counter()
+        ow.substeps_fixed_threads(scene.dt, args.cpu_substeps, threads)
+        cb = time.perf_counter()
+        cpu_all = {
+            "value": n * args.cpu_substeps / (cb - ca),
+            "unit": "particle-updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_substeps} substeps of the full {n}-particle scene, FP64 C++ restatement on "
+                      f"{threads} threads ({cb - ca:.1f}s)",
         }
 
     out = {
@@ -285,6 +300,7 @@ def main() -> int:
         "event_substep_ms": substep_ms,
         "other_modes": extra_modes,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "batch": batch,
         "serving": serving,
     }

@@ -34,6 +34,7 @@
 #include <cstring>
 #include <limits>
 #include <unordered_map>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -1105,6 +1106,191 @@ struct World {
     substeps = substeps == UINT64_MAX ? substeps : substeps + 1;
     return OK;
   }
+
+  // The same substep over `threads` host threads, for timing the CPU
+  // baseline on all cores (no reactions).  Particles are split into
+  // contiguous chunks; each thread accumulates its chunk's P2G into its own
+  // grid, the grids are merged in thread order, and the grid update and G2P
+  // run in parallel.  Deterministic for a given thread count; node sums are
+  // associated per chunk, so the last bits differ from `substep`.
+  int substep_threads(double dt, int threads) {
+    const double h = params.cell_width;
+    const double inv_h = 1.0 / h;
+    const double inv_d = 4.0 * inv_h * inv_h;
+    const size_t np = particles.size();
+    const int T = std::max(1, threads);
+    std::vector<uint64_t> keys(np);
+    for (size_t i = 0; i < np; ++i) keys[i] = transfer_key(particles[i]);
+    struct Local {
+      std::unordered_map<NodeKey, uint32_t, NodeKeyHash> index;
+      std::vector<NodeKey> keys;
+      std::vector<GridNode> nodes;
+    };
+    std::vector<Local> local(T);
+    auto chunk = [&](int t, size_t& b, size_t& e) {
+      b = np * (size_t)t / (size_t)T;
+      e = np * (size_t)(t + 1) / (size_t)T;
+    };
+    auto run = [&](auto&& fn) {
+      std::vector<std::thread> pool;
+      for (int t = 1; t < T; ++t) pool.emplace_back(fn, t);
+      fn(0);
+      for (auto& th : pool) th.join();
+    };
+    run([&](int t) {
+      size_t b, e;
+      chunk(t, b, e);
+      Local& L = local[t];
+      L.index.reserve((e - b) * 8 + 16);
+      for (size_t i = b; i < e; ++i) {
+        const Particle& q = particles[i];
+        if (!q.enabled) continue;
+        V3 gp = q.position * inv_h;
+        int32_t bc[3];
+        base_cell(gp, bc);
+        V3 fx = gp - v3((double)bc[0], (double)bc[1], (double)bc[2]);
+        double w[3][3];
+        quadratic_weights(fx, w);
+        M3 stress = kirchhoff_stress(q.material, q.deformation, q.affine, q.plastic);
+        M3 momentum_affine = stress * (-dt * q.rest_volume * inv_d) + q.affine * q.mass;
+        V3 mv = v3(q.mass * q.velocity.x, q.mass * q.velocity.y, q.mass * q.velocity.z);
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c)
+            for (int z = 0; z < 3; ++z) {
+              NodeKey k{bc[0] + a, bc[1] + c, bc[2] + z, keys[i]};
+              double weight = w[0][a] * w[1][c] * w[2][z];
+              V3 dpos = (v3((double)a, (double)c, (double)z) - fx) * h;
+              auto it = L.index.find(k);
+              uint32_t id;
+              if (it == L.index.end()) {
+                id = (uint32_t)L.nodes.size();
+                L.index.emplace(k, id);
+                L.keys.push_back(k);
+                L.nodes.emplace_back();
+              } else {
+                id = it->second;
+              }
+              GridNode& n = L.nodes[id];
+              n.mass += weight * q.mass;
+              n.momentum = n.momentum + weight * (mv + momentum_affine * dpos);
+            }
+      }
+    });
+    // merge in thread order
+    std::unordered_map<NodeKey, uint32_t, NodeKeyHash> index = std::move(local[0].index);
+    std::vector<NodeKey> node_keys = std::move(local[0].keys);
+    std::vector<GridNode> nodes = std::move(local[0].nodes);
+    for (int t = 1; t < T; ++t) {
+      Local& L = local[t];
+      for (size_t j = 0; j < L.nodes.size(); ++j) {
+        auto it = index.find(L.keys[j]);
+        if (it == index.end()) {
+          index.emplace(L.keys[j], (uint32_t)nodes.size());
+          node_keys.push_back(L.keys[j]);
+          nodes.push_back(L.nodes[j]);
+        } else {
+          GridNode& n = nodes[it->second];
+          n.mass += L.nodes[j].mass;
+          n.momentum = n.momentum + L.nodes[j].momentum;
+        }
+      }
+      L = Local();
+    }
+    // grid update (node order is irrelevant without reactions)
+    run([&](int t) {
+      const size_t nn = nodes.size();
+      for (size_t id = nn * (size_t)t / (size_t)T; id < nn * (size_t)(t + 1) / (size_t)T; ++id) {
+        GridNode& n = nodes[id];
+        if (n.mass <= 0.0) continue;
+        n.velocity = n.momentum / n.mass + params.gravity * dt;
+        const NodeKey& k = node_keys[id];
+        V3 point = v3((double)k.x, (double)k.y, (double)k.z) * h;
+        if (params.has_bounds)
+          for (int a = 0; a < 3; ++a)
+            if ((point[a] <= params.bmin[a] + h && n.velocity[a] < 0.0) ||
+                (point[a] >= params.bmax[a] - h && n.velocity[a] > 0.0))
+              n.velocity[a] = 0.0;
+        for (const Obstacle& o : obstacles) {
+          if (!may_contact(o, point, h)) continue;
+          double dist;
+          V3 normal;
+          surface(o, point, dist, normal);
+          if (dist <= h * (1.0 + 1e-5)) n.velocity = contact_velocity(o, point, n.velocity, normal);
+        }
+      }
+    });
+    std::vector<char> bad(T, 0);
+    run([&](int t) {
+      size_t b, e;
+      chunk(t, b, e);
+      for (size_t i = b; i < e; ++i) {
+        Particle& q = particles[i];
+        if (!q.enabled) {
+          q.force = V3();
+          continue;
+        }
+        if (q.fixed) {
+          q.velocity = V3();
+          q.affine = M3();
+          q.force = V3();
+          continue;
+        }
+        V3 gp = q.position * inv_h;
+        int32_t bc[3];
+        base_cell(gp, bc);
+        V3 fx = gp - v3((double)bc[0], (double)bc[1], (double)bc[2]);
+        double w[3][3];
+        quadratic_weights(fx, w);
+        V3 velocity;
+        M3 affine;
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c)
+            for (int z = 0; z < 3; ++z) {
+              NodeKey k{bc[0] + a, bc[1] + c, bc[2] + z, keys[i]};
+              double weight = w[0][a] * w[1][c] * w[2][z];
+              V3 dpos = (v3((double)a, (double)c, (double)z) - fx) * h;
+              auto it = index.find(k);
+              V3 nv = it == index.end() ? V3() : nodes[it->second].velocity;
+              velocity = velocity + nv * weight;
+              affine = affine + outer(nv, dpos) * (weight * inv_d);
+            }
+        velocity = velocity + q.force / q.mass * dt;
+        velocity = velocity * std::exp(-q.damping * dt);
+        q.velocity = velocity;
+        q.affine = affine;
+        q.position = q.position + velocity * dt;
+        q.deformation = (identity() + affine * dt) * q.deformation;
+        project_deformation(q.material, q.deformation, q.plastic);
+        q.force = V3();
+        if (params.has_bounds)
+          for (int a = 0; a < 3; ++a) {
+            double lower = params.bmin[a] + q.radius, upper = params.bmax[a] - q.radius;
+            if (q.position[a] < lower) {
+              q.position[a] = lower;
+              q.velocity[a] = std::fmax(q.velocity[a], 0.0);
+            } else if (q.position[a] > upper) {
+              q.position[a] = upper;
+              q.velocity[a] = std::fmin(q.velocity[a], 0.0);
+            }
+          }
+        for (const Obstacle& o : obstacles) {
+          if (!may_contact(o, q.position, q.radius)) continue;
+          double dist;
+          V3 normal;
+          surface(o, q.position, dist, normal);
+          if (dist < q.radius) {
+            q.position = q.position + normal * (q.radius - dist);
+            q.velocity = contact_velocity(o, q.position, q.velocity, normal);
+          }
+        }
+        if (!finite_particle_state(q)) bad[t] = 1;
+      }
+    });
+    for (char c : bad)
+      if (c) return NON_FINITE_STATE;
+    substeps = substeps == UINT64_MAX ? substeps : substeps + 1;
+    return OK;
+  }
 };
 
 }  // namespace
@@ -1270,6 +1456,17 @@ int orc_world_substeps_fixed(void* h, double dt, uint32_t steps) {
   World* w = static_cast<World*>(h);
   for (uint32_t s = 0; s < steps; ++s) {
     int st = w->substep(dt, nullptr);
+    if (st != OK) return st;
+  }
+  return OK;
+}
+
+// Fixed-size substeps over `threads` host threads (CPU baseline timing).
+int orc_world_substeps_fixed_threads(void* h, double dt, uint32_t steps, int32_t threads) {
+  World* w = static_cast<World*>(h);
+  if (!std::isfinite(dt) || dt <= 0.0) return INVALID_INPUT;
+  for (uint32_t k = 0; k < steps; ++k) {
+    int st = w->substep_threads(dt, threads);
     if (st != OK) return st;
   }
   return OK;

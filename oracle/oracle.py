@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
     L.orc_world_step_with_reactions.argtypes = [vp, C.c_double, _dp]
     L.orc_world_substeps_fixed.restype = C.c_int
     L.orc_world_substeps_fixed.argtypes = [vp, C.c_double, C.c_uint32]
+    L.orc_world_substeps_fixed_threads.restype = C.c_int
+    L.orc_world_substeps_fixed_threads.argtypes = [vp, C.c_double, C.c_uint32, C.c_int32]
     L.orc_world_stable_timestep.restype = C.c_double
     L.orc_world_stable_timestep.argtypes = [vp]
     L.orc_world_count.restype = C.c_uint64
@@ -211,6 +213,13 @@ class OracleWorld:
 
     def substeps_fixed(self, dt: float, steps: int) -> None:
         st = lib().orc_world_substeps_fixed(self._h, float(dt), int(steps))
+        if st:
+            raise OracleError(st)
+
+    def substeps_fixed_threads(self, dt: float, steps: int, threads: int) -> None:
+        """The same substeps on `threads` host threads (per-chunk P2G sums merged in
+        thread order; deterministic per thread count, no reactions)."""
+        st = lib().orc_world_substeps_fixed_threads(self._h, float(dt), int(steps), int(threads))
         if st:
             raise OracleError(st)
 

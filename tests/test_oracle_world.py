@@ -286,3 +286,22 @@ def test_affine_velocity_field_is_reproduced_by_the_gather():
     x = w.particles.position[centre]
     assert np.linalg.norm(r["velocity"][centre] - (v0 + L @ x)) < 1e-6
     assert np.linalg.norm(r["affine"][centre] - L) < 1e-5
+
+
+def test_threaded_substep_matches_the_single_thread_one_and_repeats():
+    # the all-cores CPU baseline runs the same substep over host threads; only
+    # the association of the per-node sums differs from the single-thread pass
+    from tessera_amd.scenes import uniform_cube
+
+    sc = uniform_cube(4096, seed=2)
+    single = O.OracleWorld(sc.world)
+    single.substeps_fixed(sc.dt, 3)
+    runs = []
+    for _ in range(2):
+        w = O.OracleWorld(sc.world)
+        w.substeps_fixed_threads(sc.dt, 3, 4)
+        runs.append(w.read())
+    for f in ("position", "velocity", "affine", "deformation"):
+        assert np.array_equal(runs[0][f], runs[1][f])
+        scale = max(1.0, float(np.abs(single.read()[f]).max()))
+        assert np.abs(runs[0][f] - single.read()[f]).max() <= 1e-12 * scale
