@@ -116,6 +116,38 @@ def batch_leg(transfers, mode: str, steps: int) -> dict:
             "one_session_at_a_time": alone, "speedup": value / alone, "failed_scenes": failed}
 
 
+def scenes_leg(transfers, mode: str, steps: int) -> dict:
+    """Throughput of the other single-scene workloads: the 8,192-particle cube (C1) and the
+    4,194,304-particle fluid dam break (C3), with the default and the autotuned launch shape."""
+    from tessera_amd.mpm import HipMpmResidentSession
+    from tessera_amd.scenes import dam_break, uniform_cube
+
+    out = {}
+    for name, make in (("c1_uniform_cube_8k", uniform_cube), ("c3_dam_break_fluid_4M", dam_break)):
+        sc = make()
+        n = len(sc.world.particles)
+        s = HipMpmResidentSession(transfers, sc.world, sc.dt, mode)
+        submit(s, 16)
+        s.wait()
+        t0 = time.perf_counter()
+        submit(s, steps)
+        s.wait()
+        t_default = time.perf_counter() - t0
+        cfg = s.autotune(16)
+        t0 = time.perf_counter()
+        submit(s, steps)
+        s.wait()
+        t_tuned = time.perf_counter() - t0
+        s.synchronize()
+        out[name] = {"particles": n, "steps": steps, "dt": sc.dt, "value": n * steps / t_tuned,
+                     "value_default_shape": n * steps / t_default, "unit": "particle-updates/s",
+                     "launch_config": cfg}
+        print(f"bench: {name}: {n * steps / t_tuned:.4e} particle-updates/s (default shape "
+              f"{n * steps / t_default:.4e})", file=sys.stderr, flush=True)
+        s.close()
+    return out
+
+
 def serving_leg(transfers, mode: str, seconds: float, single_rate: float) -> dict:
     """Continuous batching: Poisson arrivals offering 80% of the measured single-scene rate,
     N log-uniform in [4096, 524288], 1-4 rigid boxes per scene, horizons U{50..2000} substeps,
@@ -151,6 +183,7 @@ def main() -> int:
     ap.add_argument("--no-extra-modes", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=64, help="substeps of the batched-scenes leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg (0: skip)")
+    ap.add_argument("--scenes-steps", type=int, default=128, help="substeps of the C1/C3 scene leg (0: skip)")
     args = ap.parse_args()
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -221,6 +254,10 @@ def main() -> int:
     batch = None
     if args.batch_steps > 0:
         batch = batch_leg(transfers, args.mode, args.batch_steps)
+
+    scenes = None
+    if args.scenes_steps > 0:
+        scenes = scenes_leg(transfers, args.mode, args.scenes_steps)
 
     serving = None
     if args.serve_seconds > 0:
@@ -302,6 +339,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "batch": batch,
+        "scenes": scenes,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
