@@ -23,6 +23,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <tuple>
 #include <unordered_map>
@@ -196,11 +197,21 @@ void parallel_for(size_t n, F&& fn, size_t min_chunk = 16384) {
   }
   const size_t step = (n + chunks - 1) / chunks;
   std::vector<std::thread> pool;
+  pool.reserve(chunks);
+  size_t inline_from = chunks;  // chunks no thread could be started for run here
   for (size_t c = 1; c < chunks; ++c) {
     const size_t b = c * step, e = std::min(n, b + step);
-    if (b < e) pool.emplace_back([&fn, b, e] { fn(b, e); });
+    if (b >= e) break;
+    try {
+      pool.emplace_back([&fn, b, e] { fn(b, e); });
+    } catch (const std::system_error&) {
+      inline_from = c;
+      break;
+    }
   }
   fn((size_t)0, std::min(n, step));
+  for (size_t c = inline_from; c < chunks; ++c)
+    if (c * step < n) fn(c * step, std::min(n, c * step + step));
   for (auto& t : pool) t.join();
 }
 
@@ -2008,30 +2019,57 @@ tmpm_status run_diagnostic_binning(tmpm_session* s, std::vector<uint32_t>& count
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// Exceptions never cross the C ABI: every status-returning entry point
+// catches them here.  A host allocation failure is Capacity; anything else
+// is an internal Readback error.  The session (if any) is invalidated, since
+// the failed call may have stopped halfway; its last synchronized world stays
+// readable.
+tmpm_status exception_status() {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(TMPM_CAPACITY, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(TMPM_READBACK, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return fail(TMPM_READBACK, "internal error");
+  }
+}
+tmpm_status api_exception(tmpm_session* s) {
+  if (s) s->valid = false;
+  return exception_status();
+}
+// read-only entry points leave the session as it was
+tmpm_status api_exception(const tmpm_session*) { return exception_status(); }
+
 extern "C" {
 
 const char* tmpm_last_error(void) { return g_last_error.c_str(); }
 const char* tmpm_version(void) { return "tessera-mpm-hip 0.1.0 (gfx950)"; }
 
 tmpm_status tmpm_context_create(int32_t device_ordinal, tmpm_context** out) {
-  if (!out) return fail(TMPM_INVALID_INPUT, "null output");
-  *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || device_ordinal < 0 || device_ordinal >= count)
-    return fail(TMPM_NO_DEVICE, "no HIP device with this ordinal");
-  tmpm_context* c = new tmpm_context();
-  c->device = device_ordinal;
-  if (hipGetDeviceProperties(&c->props, device_ordinal) != hipSuccess) {
-    delete c;
-    return fail(TMPM_NO_DEVICE, "cannot query device");
+  try {
+    if (!out) return fail(TMPM_INVALID_INPUT, "null output");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device_ordinal < 0 || device_ordinal >= count)
+      return fail(TMPM_NO_DEVICE, "no HIP device with this ordinal");
+    tmpm_context* c = new tmpm_context();
+    c->device = device_ordinal;
+    if (hipGetDeviceProperties(&c->props, device_ordinal) != hipSuccess) {
+      delete c;
+      return fail(TMPM_NO_DEVICE, "cannot query device");
+    }
+    if (std::strncmp(c->props.gcnArchName, "gfx950", 6) != 0) {
+      std::string arch = c->props.gcnArchName;
+      delete c;
+      return fail(TMPM_NO_DEVICE, "device is " + arch + ", this library is built for gfx950");
+    }
+    *out = c;
+    return TMPM_OK;
+  } catch (...) {
+    return exception_status();
   }
-  if (std::strncmp(c->props.gcnArchName, "gfx950", 6) != 0) {
-    std::string arch = c->props.gcnArchName;
-    delete c;
-    return fail(TMPM_NO_DEVICE, "device is " + arch + ", this library is built for gfx950");
-  }
-  *out = c;
-  return TMPM_OK;
 }
 
 void tmpm_context_destroy(tmpm_context* ctx) { delete ctx; }
@@ -2322,8 +2360,12 @@ extern "C" {
 
 tmpm_status tmpm_session_create(tmpm_context* ctx, const tmpm_world_desc* wd, double substep_dt, tmpm_p2g_mode mode,
                                 tmpm_session** out) {
-  if (!wd) return fail(TMPM_INVALID_INPUT, "null argument");
-  return create_session(ctx, wd, &substep_dt, 1, mode, false, out);
+  try {
+    if (!wd) return fail(TMPM_INVALID_INPUT, "null argument");
+    return create_session(ctx, wd, &substep_dt, 1, mode, false, out);
+  } catch (...) {
+    return exception_status();
+  }
 }
 
 void tmpm_session_destroy(tmpm_session* s) {
@@ -2337,200 +2379,220 @@ void tmpm_session_destroy(tmpm_session* s) {
 }
 
 tmpm_status tmpm_session_submit_steps(tmpm_session* s, uint32_t steps) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (steps == 0 || steps > 64) return fail(TMPM_INVALID_INPUT, "steps must be within 1..=64");
-  if (s->pending + steps < s->pending) return fail(TMPM_CAPACITY, "pending substep count overflow");
-  for (auto& scene : s->scenes) {
-    if (!scene.stepping()) continue;
-    tmpm_status st = validate_resident_world(scene);
-    if (st != TMPM_OK) return st;
-  }
-  if (s->prepared) {
-    if (hipSetDevice(s->ctx->device) != hipSuccess) return fail(TMPM_READBACK, "cannot select device");
-    for (uint32_t k = 0; k < steps; ++k) {
-      bool force = false;
-      for (auto& scene : s->scenes) {
-        force = force || scene.force_pending;
-        scene.force_pending = false;
-      }
-      hipError_t e = enqueue_substep(s, force);
-      if (e != hipSuccess) {
-        s->valid = false;
-        return fail(TMPM_READBACK, std::string("kernel launch failed: ") + hipGetErrorString(e));
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+    if (steps == 0 || steps > 64) return fail(TMPM_INVALID_INPUT, "steps must be within 1..=64");
+    if (s->pending + steps < s->pending) return fail(TMPM_CAPACITY, "pending substep count overflow");
+    for (auto& scene : s->scenes) {
+      if (!scene.stepping()) continue;
+      tmpm_status st = validate_resident_world(scene);
+      if (st != TMPM_OK) return st;
+    }
+    if (s->prepared) {
+      if (hipSetDevice(s->ctx->device) != hipSuccess) return fail(TMPM_READBACK, "cannot select device");
+      for (uint32_t k = 0; k < steps; ++k) {
+        bool force = false;
+        for (auto& scene : s->scenes) {
+          force = force || scene.force_pending;
+          scene.force_pending = false;
+        }
+        hipError_t e = enqueue_substep(s, force);
+        if (e != hipSuccess) {
+          s->valid = false;
+          return fail(TMPM_READBACK, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        }
       }
     }
+    s->pending += steps;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  s->pending += steps;
-  return TMPM_OK;
 }
 
 tmpm_status tmpm_session_wait(tmpm_session* s) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  return TMPM_OK;
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_synchronize(tmpm_session* s) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (s->pending == 0) return TMPM_OK;
-  const uint64_t steps = s->pending;
-  s->pending = 0;
-  const size_t ns = s->scenes.size();
-  std::vector<uint32_t> scene_bits(ns, 0);
-  uint32_t counters[CNT_COUNT] = {};
-  if (s->prepared) {
-    hipError_t e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) {
-      s->valid = false;
-      return fail(TMPM_READBACK, std::string("device failure: ") + hipGetErrorString(e));
-    }
-    collect_timers(s);
-    if (hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(scene_bits.data(), s->scene_err.ptr, 4 * ns, hipMemcpyDeviceToHost) != hipSuccess) {
-      s->valid = false;
-      return fail(TMPM_READBACK, "counter readback failed");
-    }
-    // Capacity and internal errors concern every scene: the session fails.
-    // A single-scene session also fails on its scene's errors; a batch stops
-    // only the scenes concerned.
-    const uint32_t global = s->batch ? (counters[CNT_ERROR] & (ERR_TOPOLOGY | ERR_INTERNAL)) : counters[CNT_ERROR];
-    tmpm_status st = status_from_error_bits(global);
-    if (st != TMPM_OK) {
-      s->valid = false;
-      return st;
-    }
-    s->last_pblocks = counters[CNT_PARTICLE_BLOCKS];
-    s->last_gblocks = counters[CNT_GRID_BLOCKS];
-  }
-  std::vector<char> commit(ns, 0);
-  tmpm_status st = TMPM_OK;
-  if (!s->batch) {
-    // eager: the world is read back at every synchronize
-    std::vector<HostWorld> updated;
-    std::vector<tmpm_status> scene_st;
-    st = read_back(s, updated, scene_st);
-    if (st != TMPM_OK) {
-      s->valid = false;
-      return st;
-    }
-    if (scene_st[0] != TMPM_OK) {
-      s->valid = false;
-      return fail(scene_st[0], "MPM integration produced a non-finite state");
-    }
-    SceneState& scene = s->scenes[0];
-    const uint64_t before = scene.world.substeps;
-    updated[0].substeps = before + steps < before ? UINT64_MAX : before + steps;
-    scene.set_world(updated[0]);
-    commit[0] = 1;
-  } else {
-    // lazy: stopped scenes take their state of the previous synchronize from
-    // the device copy; the others only advance their substep counts and are
-    // read on demand from the new copy
-    bool stopped = false;
-    for (size_t k = 0; k < ns; ++k) {
-      SceneState& scene = s->scenes[k];
-      if (!scene.stepping()) continue;
-      if (scene_bits[k] == 0) {
-        const uint64_t before = scene.world.substeps;
-        scene.world.substeps = before + steps < before ? UINT64_MAX : before + steps;
-        if (s->prepared && scene.count) scene.device_newer = true;
-        commit[k] = 1;
-        continue;
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+    if (s->pending == 0) return TMPM_OK;
+    const uint64_t steps = s->pending;
+    s->pending = 0;
+    const size_t ns = s->scenes.size();
+    std::vector<uint32_t> scene_bits(ns, 0);
+    uint32_t counters[CNT_COUNT] = {};
+    if (s->prepared) {
+      hipError_t e = hipStreamSynchronize(s->stream);
+      if (e != hipSuccess) {
+        s->valid = false;
+        return fail(TMPM_READBACK, std::string("device failure: ") + hipGetErrorString(e));
       }
-      st = materialize(s, (uint32_t)k);
+      collect_timers(s);
+      if (hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(scene_bits.data(), s->scene_err.ptr, 4 * ns, hipMemcpyDeviceToHost) != hipSuccess) {
+        s->valid = false;
+        return fail(TMPM_READBACK, "counter readback failed");
+      }
+      // Capacity and internal errors concern every scene: the session fails.
+      // A single-scene session also fails on its scene's errors; a batch stops
+      // only the scenes concerned.
+      const uint32_t global = s->batch ? (counters[CNT_ERROR] & (ERR_TOPOLOGY | ERR_INTERNAL)) : counters[CNT_ERROR];
+      tmpm_status st = status_from_error_bits(global);
       if (st != TMPM_OK) {
         s->valid = false;
         return st;
       }
-      scene.status = status_from_error_bits(scene_bits[k]);  // the last synchronized world stays readable
-      stopped = true;
-      if (s->prepared && scene.count) {
-        launch_retire(s->parts(s->cur), scene.first, scene.count, s->stream);
-        HIP_TRY(hipGetLastError());
-      }
+      s->last_pblocks = counters[CNT_PARTICLE_BLOCKS];
+      s->last_gblocks = counters[CNT_GRID_BLOCKS];
     }
-    if (stopped) {
-      s->substeps_since_reorder = s->reorder_interval;  // the next substep drops the retired slots
-      st = upload_rigid(s);
-      if (st == TMPM_OK) st = upload_obstacles(s);
-      if (st == TMPM_OK) st = upload_scene(s);
+    std::vector<char> commit(ns, 0);
+    tmpm_status st = TMPM_OK;
+    if (!s->batch) {
+      // eager: the world is read back at every synchronize
+      std::vector<HostWorld> updated;
+      std::vector<tmpm_status> scene_st;
+      st = read_back(s, updated, scene_st);
       if (st != TMPM_OK) {
         s->valid = false;
         return st;
       }
-      HIP_TRY(hipMemset(s->scene_err.ptr, 0, 4ull * s->scene_cap));
-      uint32_t err = counters[CNT_ERROR] & ~(ERR_RESIDENT | ERR_FIXED_RANGE);
-      HIP_TRY(hipMemcpy(s->counters.as<uint32_t>() + CNT_ERROR, &err, 4, hipMemcpyHostToDevice));
+      if (scene_st[0] != TMPM_OK) {
+        s->valid = false;
+        return fail(scene_st[0], "MPM integration produced a non-finite state");
+      }
+      SceneState& scene = s->scenes[0];
+      const uint64_t before = scene.world.substeps;
+      updated[0].substeps = before + steps < before ? UINT64_MAX : before + steps;
+      scene.set_world(updated[0]);
+      commit[0] = 1;
+    } else {
+      // lazy: stopped scenes take their state of the previous synchronize from
+      // the device copy; the others only advance their substep counts and are
+      // read on demand from the new copy
+      bool stopped = false;
+      for (size_t k = 0; k < ns; ++k) {
+        SceneState& scene = s->scenes[k];
+        if (!scene.stepping()) continue;
+        if (scene_bits[k] == 0) {
+          const uint64_t before = scene.world.substeps;
+          scene.world.substeps = before + steps < before ? UINT64_MAX : before + steps;
+          if (s->prepared && scene.count) scene.device_newer = true;
+          commit[k] = 1;
+          continue;
+        }
+        st = materialize(s, (uint32_t)k);
+        if (st != TMPM_OK) {
+          s->valid = false;
+          return st;
+        }
+        scene.status = status_from_error_bits(scene_bits[k]);  // the last synchronized world stays readable
+        stopped = true;
+        if (s->prepared && scene.count) {
+          launch_retire(s->parts(s->cur), scene.first, scene.count, s->stream);
+          HIP_TRY(hipGetLastError());
+        }
+      }
+      if (stopped) {
+        s->substeps_since_reorder = s->reorder_interval;  // the next substep drops the retired slots
+        st = upload_rigid(s);
+        if (st == TMPM_OK) st = upload_obstacles(s);
+        if (st == TMPM_OK) st = upload_scene(s);
+        if (st != TMPM_OK) {
+          s->valid = false;
+          return st;
+        }
+        HIP_TRY(hipMemset(s->scene_err.ptr, 0, 4ull * s->scene_cap));
+        uint32_t err = counters[CNT_ERROR] & ~(ERR_RESIDENT | ERR_FIXED_RANGE);
+        HIP_TRY(hipMemcpy(s->counters.as<uint32_t>() + CNT_ERROR, &err, 4, hipMemcpyHostToDevice));
+      }
+      if (s->prepared) {
+        if (s->compacted_last && !stopped) s->n = std::min<uint32_t>(s->n, counters[CNT_BINNED]);
+        st = copy_particles(s->synced, s->cap, s->buf[s->cur], s->cap, s->n, s->stream);
+        if (st != TMPM_OK) {
+          s->valid = false;
+          return st;
+        }
+        s->synced_n = s->n;
+      }
+    }
+    st = read_coupling(s, commit);
+    if (st != TMPM_OK) {
+      s->valid = false;
+      return st;
     }
     if (s->prepared) {
-      if (s->compacted_last && !stopped) s->n = std::min<uint32_t>(s->n, counters[CNT_BINNED]);
-      st = copy_particles(s->synced, s->cap, s->buf[s->cur], s->cap, s->n, s->stream);
+      st = maybe_grow_topology(s, counters);
       if (st != TMPM_OK) {
         s->valid = false;
         return st;
       }
-      s->synced_n = s->n;
     }
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  st = read_coupling(s, commit);
-  if (st != TMPM_OK) {
-    s->valid = false;
-    return st;
-  }
-  if (s->prepared) {
-    st = maybe_grow_topology(s, counters);
-    if (st != TMPM_OK) {
-      s->valid = false;
-      return st;
-    }
-  }
-  return TMPM_OK;
 }
 
 tmpm_status tmpm_session_step(tmpm_session* s, uint32_t steps) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  tmpm_status st = tmpm_session_submit_steps(s, steps);
-  if (st != TMPM_OK) return st;
-  return tmpm_session_synchronize(s);
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+    tmpm_status st = tmpm_session_submit_steps(s, steps);
+    if (st != TMPM_OK) return st;
+    return tmpm_session_synchronize(s);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 // gpu.rs:2208-2240: CFL-limited substeps, each validated on the host.
 tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  if (s->batch) return fail(TMPM_INVALID_INPUT, "adaptive stepping takes a single-scene session");
-  if (!std::isfinite(dt) || dt <= 0.0) return fail(TMPM_INVALID_INPUT, "dt must be positive");
-  SceneState& scene = s->scenes[0];
-  for (auto& o : scene.world.obstacles)
-    if (!obstacle_valid(o.o)) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
-  double remaining = dt;
-  size_t count = 0;
-  const double saved_dt = scene.substep_dt;
-  while (remaining > 0.0) {
-    double stable = scene.stable_timestep();
-    if (!std::isfinite(stable) || stable <= 1e-12 || count >= 100000)
-      return fail(TMPM_EXCESSIVE_SUBSTEPS, "MPM timestep is too small for the requested duration");
-    double sub = std::min(remaining, stable);
-    if (sub != scene.substep_dt) {
-      scene.substep_dt = sub;
-      st = build_scene(scene);
-      if (st == TMPM_OK && s->prepared) st = upload_obstacles(s);
-      if (st == TMPM_OK && s->prepared) st = upload_scene(s);
-      if (st != TMPM_OK) {
-        scene.substep_dt = saved_dt;
-        (void)build_scene(scene);
-        return st;
-      }
-    }
-    st = tmpm_session_step(s, 1);
+  try {
+    tmpm_status st = check_editable(s);
     if (st != TMPM_OK) return st;
-    remaining = std::max(remaining - sub, 0.0);
-    ++count;
+    if (s->batch) return fail(TMPM_INVALID_INPUT, "adaptive stepping takes a single-scene session");
+    if (!std::isfinite(dt) || dt <= 0.0) return fail(TMPM_INVALID_INPUT, "dt must be positive");
+    SceneState& scene = s->scenes[0];
+    for (auto& o : scene.world.obstacles)
+      if (!obstacle_valid(o.o)) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+    double remaining = dt;
+    size_t count = 0;
+    const double saved_dt = scene.substep_dt;
+    while (remaining > 0.0) {
+      double stable = scene.stable_timestep();
+      if (!std::isfinite(stable) || stable <= 1e-12 || count >= 100000)
+        return fail(TMPM_EXCESSIVE_SUBSTEPS, "MPM timestep is too small for the requested duration");
+      double sub = std::min(remaining, stable);
+      if (sub != scene.substep_dt) {
+        scene.substep_dt = sub;
+        st = build_scene(scene);
+        if (st == TMPM_OK && s->prepared) st = upload_obstacles(s);
+        if (st == TMPM_OK && s->prepared) st = upload_scene(s);
+        if (st != TMPM_OK) {
+          scene.substep_dt = saved_dt;
+          (void)build_scene(scene);
+          return st;
+        }
+      }
+      st = tmpm_session_step(s, 1);
+      if (st != TMPM_OK) return st;
+      remaining = std::max(remaining - sub, 0.0);
+      ++count;
+    }
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  return TMPM_OK;
 }
 
 uint64_t tmpm_session_pending_substeps(const tmpm_session* s) { return s ? s->pending : 0; }
@@ -2539,64 +2601,84 @@ uint64_t tmpm_session_substeps(const tmpm_session* s) { return s && !s->scenes.e
 uint64_t tmpm_session_particle_count(const tmpm_session* s) { return s && !s->scenes.empty() ? s->world().size() : 0; }
 
 tmpm_status tmpm_session_read_particles(const tmpm_session* s, tmpm_particles* out) {
-  if (!s || !out || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (out->count < s->world().size()) return fail(TMPM_CAPACITY, "output arrays too small");
-  return copy_particles_out(s->world(), out);
+  try {
+    if (!s || !out || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (out->count < s->world().size()) return fail(TMPM_CAPACITY, "output arrays too small");
+    return copy_particles_out(s->world(), out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_set_particle_force(tmpm_session* s, uint64_t index, const double force[3]) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  return set_scene_force(s, 0, index, force);
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    return set_scene_force(s, 0, index, force);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_set_obstacles(tmpm_session* s, const tmpm_obstacle* obs, uint32_t count) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  return set_scene_obstacles(s, 0, obs, count);
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    return set_scene_obstacles(s, 0, obs, count);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_add_particles(tmpm_session* s, const tmpm_particles* particles, const tmpm_material* materials,
                                        uint32_t material_count, uint64_t* chunk_out) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  if (!particles) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (particles->count == 0 || !validate_particles(*particles, materials, material_count))
-    return fail(TMPM_INVALID_INPUT, "invalid MPM particle batch");
-  SceneState& scene = s->scenes[0];
-  if (scene.world.next_chunk_id == UINT64_MAX) return fail(TMPM_PARTICLE_CAPACITY, "chunk ids exhausted");
-  std::vector<SceneState> previous = s->scenes;
-  HostWorld updated = scene.world;
-  const uint64_t chunk = updated.next_chunk_id++;
-  updated.append(*particles, materials, chunk);
-  SceneState trial = scene;
-  trial.set_world(updated);
-  st = validate_resident_world(trial);
-  if (st != TMPM_OK) return st;
-  scene = trial;
-  st = reprepare(s, previous);
-  if (st != TMPM_OK) return st;
-  if (chunk_out) *chunk_out = chunk;
-  return TMPM_OK;
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    if (!particles) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (particles->count == 0 || !validate_particles(*particles, materials, material_count))
+      return fail(TMPM_INVALID_INPUT, "invalid MPM particle batch");
+    SceneState& scene = s->scenes[0];
+    if (scene.world.next_chunk_id == UINT64_MAX) return fail(TMPM_PARTICLE_CAPACITY, "chunk ids exhausted");
+    std::vector<SceneState> previous = s->scenes;
+    HostWorld updated = scene.world;
+    const uint64_t chunk = updated.next_chunk_id++;
+    updated.append(*particles, materials, chunk);
+    SceneState trial = scene;
+    trial.set_world(updated);
+    st = validate_resident_world(trial);
+    if (st != TMPM_OK) return st;
+    scene = trial;
+    st = reprepare(s, previous);
+    if (st != TMPM_OK) return st;
+    if (chunk_out) *chunk_out = chunk;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_remove_chunk(tmpm_session* s, uint64_t chunk, uint64_t* removed_out) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  SceneState& scene = s->scenes[0];
-  std::vector<SceneState> previous = s->scenes;
-  HostWorld updated = scene.world;
-  const size_t removed = updated.remove_chunk(chunk);
-  if (removed == 0) return fail(TMPM_UNKNOWN_CHUNK, "MPM particle chunk was not found");
-  SceneState trial = scene;
-  trial.set_world(updated);
-  st = validate_resident_world(trial);
-  if (st != TMPM_OK) return st;
-  scene = trial;
-  st = reprepare(s, previous);
-  if (st != TMPM_OK) return st;
-  if (removed_out) *removed_out = removed;
-  return TMPM_OK;
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    SceneState& scene = s->scenes[0];
+    std::vector<SceneState> previous = s->scenes;
+    HostWorld updated = scene.world;
+    const size_t removed = updated.remove_chunk(chunk);
+    if (removed == 0) return fail(TMPM_UNKNOWN_CHUNK, "MPM particle chunk was not found");
+    SceneState trial = scene;
+    trial.set_world(updated);
+    st = validate_resident_world(trial);
+    if (st != TMPM_OK) return st;
+    scene = trial;
+    st = reprepare(s, previous);
+    if (st != TMPM_OK) return st;
+    if (removed_out) *removed_out = removed;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2604,15 +2686,23 @@ tmpm_status tmpm_session_remove_chunk(tmpm_session* s, uint64_t chunk, uint64_t*
 // ---------------------------------------------------------------------------
 tmpm_status tmpm_batch_create(tmpm_context* ctx, const tmpm_world_desc* worlds, const double* substep_dt,
                               uint32_t scene_count, tmpm_p2g_mode mode, tmpm_session** out) {
-  return create_session(ctx, worlds, substep_dt, scene_count, mode, true, out);
+  try {
+    return create_session(ctx, worlds, substep_dt, scene_count, mode, true, out);
+  } catch (...) {
+    return exception_status();
+  }
 }
 
 uint32_t tmpm_batch_scene_count(const tmpm_session* s) { return s ? (uint32_t)s->scenes.size() : 0; }
 
 tmpm_status tmpm_batch_scene_status(const tmpm_session* s, uint32_t scene) {
-  tmpm_status st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return s->scenes[scene].status;
+  try {
+    tmpm_status st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return s->scenes[scene].status;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 uint64_t tmpm_batch_scene_particle_count(const tmpm_session* s, uint32_t scene) {
@@ -2624,346 +2714,406 @@ uint64_t tmpm_batch_scene_substeps(const tmpm_session* s, uint32_t scene) {
 }
 
 tmpm_status tmpm_batch_read_particles(const tmpm_session* s, uint32_t scene, tmpm_particles* out) {
-  tmpm_status st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  if (!out) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  // the world of the last synchronize, read from the device on first use
-  st = materialize(const_cast<tmpm_session*>(s), scene);
-  if (st != TMPM_OK) return st;
-  const HostWorld& w = s->scenes[scene].world;
-  if (out->count < w.size()) return fail(TMPM_CAPACITY, "output arrays too small");
-  return copy_particles_out(w, out);
+  try {
+    tmpm_status st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    if (!out) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+    // the world of the last synchronize, read from the device on first use
+    st = materialize(const_cast<tmpm_session*>(s), scene);
+    if (st != TMPM_OK) return st;
+    const HostWorld& w = s->scenes[scene].world;
+    if (out->count < w.size()) return fail(TMPM_CAPACITY, "output arrays too small");
+    return copy_particles_out(w, out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_set_particle_force(tmpm_session* s, uint32_t scene, uint64_t index, const double force[3]) {
-  tmpm_status st = check_editable(s);
-  if (st == TMPM_OK) st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return set_scene_force(s, scene, index, force);
+  try {
+    tmpm_status st = check_editable(s);
+    if (st == TMPM_OK) st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return set_scene_force(s, scene, index, force);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_set_obstacles(tmpm_session* s, uint32_t scene, const tmpm_obstacle* obs, uint32_t count) {
-  tmpm_status st = check_editable(s);
-  if (st == TMPM_OK) st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return set_scene_obstacles(s, scene, obs, count);
+  try {
+    tmpm_status st = check_editable(s);
+    if (st == TMPM_OK) st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return set_scene_obstacles(s, scene, obs, count);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, double substep_dt,
                                  uint32_t* scene_out) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
-  // one pass over the particles: load, then validate what was loaded
-  if (!world) return fail(TMPM_INVALID_INPUT, "null world");
-  if (!params_valid(world->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
-  for (uint32_t i = 0; i < world->obstacle_count; ++i)
-    if (!obstacle_valid(world->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
-  const tmpm_particles& pp = world->particles;
-  if (pp.count > 0 && (!pp.position || !pp.mass || !pp.rest_volume || !pp.radius || world->material_count == 0))
-    return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
-  for (uint64_t i = 0; pp.material_index && i < pp.count; ++i)
-    if (pp.material_index[i] >= world->material_count) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
-  SceneState scene;
-  {
-    // best-fitting emptied world of a retired scene, if one is large enough
-    const size_t need = 3 * (size_t)pp.count;
-    size_t best = s->spare_worlds.size();
-    for (size_t i = 0; i < s->spare_worlds.size(); ++i)
-      if (s->spare_worlds[i].pos.capacity() >= need &&
-          (best == s->spare_worlds.size() || s->spare_worlds[i].pos.capacity() < s->spare_worlds[best].pos.capacity()))
-        best = i;
-    if (best < s->spare_worlds.size()) {
-      HostWorld w = std::move(s->spare_worlds[best]);
-      s->spare_worlds.erase(s->spare_worlds.begin() + best);
-      load_scene(scene, world, substep_dt, std::move(w));
-    } else {
-      load_scene(scene, world, substep_dt);
-    }
-  }
-  {
-    std::vector<char> bad(64, 0);
-    parallel_for(scene.world.size(), [&](size_t b, size_t e) {
-      for (size_t i = b; i < e; ++i)
-        if (!scene.world.particle_valid(i)) {
-          bad[(b / 16384) % 64] = 1;
-          break;
-        }
-    });
-    for (char c : bad)
-      if (c) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
-  }
-  st = validate_resident_world(scene);
-  if (st != TMPM_OK) return st;
-  uint32_t slot = 0;
-  while (slot < s->scenes.size() && s->scenes[slot].active) ++slot;
-  if (slot >= 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
-  if (!s->prepared) {
-    std::vector<SceneState> previous = s->scenes;
-    if (slot == s->scenes.size()) s->scenes.push_back(scene);
-    else s->scenes[slot] = scene;
-    st = reprepare(s, previous);
+  try {
+    tmpm_status st = check_editable(s);
     if (st != TMPM_OK) return st;
+    if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
+    // one pass over the particles: load, then validate what was loaded
+    if (!world) return fail(TMPM_INVALID_INPUT, "null world");
+    if (!params_valid(world->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
+    for (uint32_t i = 0; i < world->obstacle_count; ++i)
+      if (!obstacle_valid(world->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+    const tmpm_particles& pp = world->particles;
+    if (pp.count > 0 && (!pp.position || !pp.mass || !pp.rest_volume || !pp.radius || world->material_count == 0))
+      return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+    for (uint64_t i = 0; pp.material_index && i < pp.count; ++i)
+      if (pp.material_index[i] >= world->material_count) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+    SceneState scene;
+    {
+      // best-fitting emptied world of a retired scene, if one is large enough
+      const size_t need = 3 * (size_t)pp.count;
+      size_t best = s->spare_worlds.size();
+      for (size_t i = 0; i < s->spare_worlds.size(); ++i)
+        if (s->spare_worlds[i].pos.capacity() >= need &&
+            (best == s->spare_worlds.size() || s->spare_worlds[i].pos.capacity() < s->spare_worlds[best].pos.capacity()))
+          best = i;
+      if (best < s->spare_worlds.size()) {
+        HostWorld w = std::move(s->spare_worlds[best]);
+        s->spare_worlds.erase(s->spare_worlds.begin() + best);
+        load_scene(scene, world, substep_dt, std::move(w));
+      } else {
+        load_scene(scene, world, substep_dt);
+      }
+    }
+    {
+      std::vector<char> bad(64, 0);
+      parallel_for(scene.world.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i)
+          if (!scene.world.particle_valid(i)) {
+            bad[(b / 16384) % 64] = 1;
+            break;
+          }
+      });
+      for (char c : bad)
+        if (c) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+    }
+    st = validate_resident_world(scene);
+    if (st != TMPM_OK) return st;
+    uint32_t slot = 0;
+    while (slot < s->scenes.size() && s->scenes[slot].active) ++slot;
+    if (slot >= 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
+    if (!s->prepared) {
+      std::vector<SceneState> previous = s->scenes;
+      if (slot == s->scenes.size()) s->scenes.push_back(scene);
+      else s->scenes[slot] = scene;
+      st = reprepare(s, previous);
+      if (st != TMPM_OK) return st;
+      if (scene_out) *scene_out = slot;
+      return TMPM_OK;
+    }
+    // Admission on the device: the new particles go after the current slots
+    // and the new insertion indices after the current ones; nothing already on
+    // the device moves.
+    if ((uint64_t)s->id_space + scene.world.size() >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
+    scene.first = s->id_space;
+    scene.count = (uint32_t)scene.world.size();
+    for (size_t i = 0; i < scene.count; ++i)
+      if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
+    st = build_scene(scene);
+    if (st != TMPM_OK) return st;
+    const size_t saved_classes = s->host_classes.size();
+    const bool appended = slot == s->scenes.size();
+    if (appended) s->scenes.push_back(std::move(scene));
+    else s->scenes[slot] = std::move(scene);
+    PackedParticles& pk = s->stage;
+    pk.clear();
+    st = pack_scene(s, slot, pk);
+    const bool was_plastic = s->plastic;
+    auto undo = [&]() {
+      s->plastic = was_plastic;
+      if (appended) s->scenes.pop_back();
+      else {
+        s->scenes[slot] = SceneState();
+        s->scenes[slot].active = false;
+      }
+      for (size_t c = saved_classes; c < s->host_classes.size(); ++c) {
+        const DevClass& d = s->host_classes[c];
+        s->class_ids.erase(class_key(d));
+      }
+      s->host_classes.resize(saved_classes);
+    };
+    if (st != TMPM_OK) {
+      undo();
+      return st;
+    }
+    s->plastic = classes_plastic(s);
+    // from here on device state changes; a failure invalidates the batch
+    auto broken = [&](tmpm_status code) {
+      s->valid = false;
+      return code;
+    };
+    const uint64_t need = (uint64_t)s->n + pk.size();
+    if (need > s->cap) {
+      const uint32_t cap = (uint32_t)std::max<uint64_t>(need + need / 4, (uint64_t)s->cap + s->cap / 2);
+      ParticleBuffers grown, spare, synced;
+      if (alloc_particles(grown, cap) != TMPM_OK || alloc_particles(spare, cap) != TMPM_OK ||
+          alloc_particles(synced, cap) != TMPM_OK)
+        return broken(TMPM_CAPACITY);
+      if (copy_particles(grown, cap, s->buf[s->cur], s->cap, s->n, s->stream) != TMPM_OK ||
+          copy_particles(synced, cap, s->synced, s->cap, s->synced_n, s->stream) != TMPM_OK)
+        return broken(TMPM_READBACK);
+      if (hipStreamSynchronize(s->stream) != hipSuccess) return broken(fail(TMPM_READBACK, "device copy failed"));
+      auto swap_bufs = [](ParticleBuffers& a, ParticleBuffers& b) {
+        std::swap(a.pos_mass.ptr, b.pos_mass.ptr); std::swap(a.pos_mass.bytes, b.pos_mass.bytes);
+        std::swap(a.vel_vol.ptr, b.vel_vol.ptr); std::swap(a.vel_vol.bytes, b.vel_vol.bytes);
+        std::swap(a.C.ptr, b.C.ptr); std::swap(a.C.bytes, b.C.bytes);
+        std::swap(a.F.ptr, b.F.ptr); std::swap(a.F.bytes, b.F.bytes);
+        std::swap(a.P.ptr, b.P.ptr); std::swap(a.P.bytes, b.P.bytes);
+        std::swap(a.cls.ptr, b.cls.ptr); std::swap(a.cls.bytes, b.cls.bytes);
+        std::swap(a.ins_id.ptr, b.ins_id.ptr); std::swap(a.ins_id.bytes, b.ins_id.bytes);
+      };
+      swap_bufs(s->buf[s->cur], grown);
+      swap_bufs(s->buf[1 - s->cur], spare);
+      swap_bufs(s->synced, synced);
+      s->cap = cap;
+    }
+    if ((st = upload_particles(s, pk, s->n)) != TMPM_OK) return broken(st);
+    s->n += (uint32_t)pk.size();
+    s->n_enabled = s->n;
+    s->id_space += s->scenes[slot].count;
+    if ((st = ensure_force_capacity(s, s->id_space + s->id_space / 4)) != TMPM_OK) return broken(st);
+    if ((st = upload_scene_forces(s, slot)) != TMPM_OK) return broken(st);
+    s->disabled_host.insert(s->disabled_host.end(), pk.disabled.begin(), pk.disabled.end());
+    if ((st = upload_disabled(s)) != TMPM_OK) return broken(st);
+    if ((st = upload_classes(s)) != TMPM_OK) return broken(st);
+    if ((st = upload_rigid(s)) != TMPM_OK) return broken(st);
+    if ((st = upload_obstacles(s)) != TMPM_OK) return broken(st);
+    if ((st = upload_scene(s)) != TMPM_OK) return broken(st);
+    // topology: room for the blocks this scene adds (and the per-particle arrays)
+    uint64_t pblocks = 0, gblocks = 0;
+    count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
+    s->last_pblocks += (uint32_t)pblocks;
+    s->last_gblocks += (uint32_t)gblocks;
+    const Topo& t = s->topo;
+    if (2ull * s->last_pblocks > t.pblock_cap || 2ull * s->last_gblocks > t.grid_cap || s->topo_cap < s->cap) {
+      const uint64_t gcap = std::min<uint64_t>(std::max<uint64_t>(t.grid_cap, 4ull * s->last_gblocks), 1ull << 26);
+      const uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(t.pblock_cap, 4ull * s->last_pblocks), s->cap);
+      if ((st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap)) != TMPM_OK) return broken(st);
+    }
     if (scene_out) *scene_out = slot;
     return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  // Admission on the device: the new particles go after the current slots
-  // and the new insertion indices after the current ones; nothing already on
-  // the device moves.
-  if ((uint64_t)s->id_space + scene.world.size() >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
-  scene.first = s->id_space;
-  scene.count = (uint32_t)scene.world.size();
-  for (size_t i = 0; i < scene.count; ++i)
-    if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
-  st = build_scene(scene);
-  if (st != TMPM_OK) return st;
-  const size_t saved_classes = s->host_classes.size();
-  const bool appended = slot == s->scenes.size();
-  if (appended) s->scenes.push_back(std::move(scene));
-  else s->scenes[slot] = std::move(scene);
-  PackedParticles& pk = s->stage;
-  pk.clear();
-  st = pack_scene(s, slot, pk);
-  const bool was_plastic = s->plastic;
-  auto undo = [&]() {
-    s->plastic = was_plastic;
-    if (appended) s->scenes.pop_back();
-    else {
-      s->scenes[slot] = SceneState();
-      s->scenes[slot].active = false;
-    }
-    for (size_t c = saved_classes; c < s->host_classes.size(); ++c) {
-      const DevClass& d = s->host_classes[c];
-      s->class_ids.erase(class_key(d));
-    }
-    s->host_classes.resize(saved_classes);
-  };
-  if (st != TMPM_OK) {
-    undo();
-    return st;
-  }
-  s->plastic = classes_plastic(s);
-  // from here on device state changes; a failure invalidates the batch
-  auto broken = [&](tmpm_status code) {
-    s->valid = false;
-    return code;
-  };
-  const uint64_t need = (uint64_t)s->n + pk.size();
-  if (need > s->cap) {
-    const uint32_t cap = (uint32_t)std::max<uint64_t>(need + need / 4, (uint64_t)s->cap + s->cap / 2);
-    ParticleBuffers grown, spare, synced;
-    if (alloc_particles(grown, cap) != TMPM_OK || alloc_particles(spare, cap) != TMPM_OK ||
-        alloc_particles(synced, cap) != TMPM_OK)
-      return broken(TMPM_CAPACITY);
-    if (copy_particles(grown, cap, s->buf[s->cur], s->cap, s->n, s->stream) != TMPM_OK ||
-        copy_particles(synced, cap, s->synced, s->cap, s->synced_n, s->stream) != TMPM_OK)
-      return broken(TMPM_READBACK);
-    if (hipStreamSynchronize(s->stream) != hipSuccess) return broken(fail(TMPM_READBACK, "device copy failed"));
-    auto swap_bufs = [](ParticleBuffers& a, ParticleBuffers& b) {
-      std::swap(a.pos_mass.ptr, b.pos_mass.ptr); std::swap(a.pos_mass.bytes, b.pos_mass.bytes);
-      std::swap(a.vel_vol.ptr, b.vel_vol.ptr); std::swap(a.vel_vol.bytes, b.vel_vol.bytes);
-      std::swap(a.C.ptr, b.C.ptr); std::swap(a.C.bytes, b.C.bytes);
-      std::swap(a.F.ptr, b.F.ptr); std::swap(a.F.bytes, b.F.bytes);
-      std::swap(a.P.ptr, b.P.ptr); std::swap(a.P.bytes, b.P.bytes);
-      std::swap(a.cls.ptr, b.cls.ptr); std::swap(a.cls.bytes, b.cls.bytes);
-      std::swap(a.ins_id.ptr, b.ins_id.ptr); std::swap(a.ins_id.bytes, b.ins_id.bytes);
-    };
-    swap_bufs(s->buf[s->cur], grown);
-    swap_bufs(s->buf[1 - s->cur], spare);
-    swap_bufs(s->synced, synced);
-    s->cap = cap;
-  }
-  if ((st = upload_particles(s, pk, s->n)) != TMPM_OK) return broken(st);
-  s->n += (uint32_t)pk.size();
-  s->n_enabled = s->n;
-  s->id_space += s->scenes[slot].count;
-  if ((st = ensure_force_capacity(s, s->id_space + s->id_space / 4)) != TMPM_OK) return broken(st);
-  if ((st = upload_scene_forces(s, slot)) != TMPM_OK) return broken(st);
-  s->disabled_host.insert(s->disabled_host.end(), pk.disabled.begin(), pk.disabled.end());
-  if ((st = upload_disabled(s)) != TMPM_OK) return broken(st);
-  if ((st = upload_classes(s)) != TMPM_OK) return broken(st);
-  if ((st = upload_rigid(s)) != TMPM_OK) return broken(st);
-  if ((st = upload_obstacles(s)) != TMPM_OK) return broken(st);
-  if ((st = upload_scene(s)) != TMPM_OK) return broken(st);
-  // topology: room for the blocks this scene adds (and the per-particle arrays)
-  uint64_t pblocks = 0, gblocks = 0;
-  count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
-  s->last_pblocks += (uint32_t)pblocks;
-  s->last_gblocks += (uint32_t)gblocks;
-  const Topo& t = s->topo;
-  if (2ull * s->last_pblocks > t.pblock_cap || 2ull * s->last_gblocks > t.grid_cap || s->topo_cap < s->cap) {
-    const uint64_t gcap = std::min<uint64_t>(std::max<uint64_t>(t.grid_cap, 4ull * s->last_gblocks), 1ull << 26);
-    const uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(t.pblock_cap, 4ull * s->last_pblocks), s->cap);
-    if ((st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap)) != TMPM_OK) return broken(st);
-  }
-  if (scene_out) *scene_out = slot;
-  return TMPM_OK;
 }
 
 tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene) {
-  tmpm_status st = check_editable(s);
-  if (st == TMPM_OK) st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
-  SceneState& gone = s->scenes[scene];
-  const uint32_t first = gone.first, count = gone.count;
-  if (s->prepared && count > 0) {
-    // Retirement on the device: the scene's particles lose their mass (the
-    // next reorder drops them) and the insertion indices above its range
-    // close the gap, here and in the synchronized copy.
-    Parts p = s->parts(s->cur);
-    launch_retire(p, first, count, s->stream);
-    launch_renumber(p.ins_id, s->n, first, count, s->stream);
-    launch_renumber(s->synced.ins_id.as<uint32_t>(), s->synced_n, first, count, s->stream);
-    HIP_TRY(hipGetLastError());
-    const uint32_t above = s->id_space - (first + count);
-    if (above > 0) {
-      if (s->staging.bytes < 16ull * above) HIP_TRY(s->staging.alloc(16ull * above));
-      HIP_TRY(hipMemcpyAsync(s->staging.ptr, s->force.as<float4>() + first + count, 16ull * above,
-                             hipMemcpyDeviceToDevice, s->stream));
-      HIP_TRY(hipMemcpyAsync(s->force.as<float4>() + first, s->staging.ptr, 16ull * above, hipMemcpyDeviceToDevice,
-                             s->stream));
+  try {
+    tmpm_status st = check_editable(s);
+    if (st == TMPM_OK) st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
+    SceneState& gone = s->scenes[scene];
+    const uint32_t first = gone.first, count = gone.count;
+    if (s->prepared && count > 0) {
+      // Retirement on the device: the scene's particles lose their mass (the
+      // next reorder drops them) and the insertion indices above its range
+      // close the gap, here and in the synchronized copy.
+      Parts p = s->parts(s->cur);
+      launch_retire(p, first, count, s->stream);
+      launch_renumber(p.ins_id, s->n, first, count, s->stream);
+      launch_renumber(s->synced.ins_id.as<uint32_t>(), s->synced_n, first, count, s->stream);
+      HIP_TRY(hipGetLastError());
+      const uint32_t above = s->id_space - (first + count);
+      if (above > 0) {
+        if (s->staging.bytes < 16ull * above) HIP_TRY(s->staging.alloc(16ull * above));
+        HIP_TRY(hipMemcpyAsync(s->staging.ptr, s->force.as<float4>() + first + count, 16ull * above,
+                               hipMemcpyDeviceToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->force.as<float4>() + first, s->staging.ptr, 16ull * above, hipMemcpyDeviceToDevice,
+                               s->stream));
+      }
+      HIP_TRY(hipStreamSynchronize(s->stream));
+      s->substeps_since_reorder = s->reorder_interval;
     }
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    s->substeps_since_reorder = s->reorder_interval;
+    for (auto& other : s->scenes)
+      if (other.active && &other != &gone && other.first > first) other.first -= count;
+    s->id_space -= count;
+    // snapshot records of disabled particles: drop the scene's, shift the others
+    std::vector<float4> kept;
+    for (size_t i = 0; i + 1 < s->disabled_host.size(); i += 2) {
+      float4 a = s->disabled_host[i];
+      uint32_t g = __builtin_bit_cast(uint32_t, a.w);
+      if (g - first < count) continue;
+      if (g >= first + count) a.w = __builtin_bit_cast(float, g - count);
+      kept.push_back(a);
+      kept.push_back(s->disabled_host[i + 1]);
+    }
+    s->disabled_host.swap(kept);
+    {
+      // keep the emptied world's memory for a later admission (at most 8
+      // worlds and 2 GiB)
+      HostWorld w = std::move(gone.world);
+      w.clear_keep();
+      size_t kept_bytes = w.capacity_bytes();
+      for (const HostWorld& o : s->spare_worlds) kept_bytes += o.capacity_bytes();
+      if (s->spare_worlds.size() < 8 && kept_bytes <= (2ull << 30)) s->spare_worlds.push_back(std::move(w));
+    }
+    s->scenes[scene] = SceneState();
+    s->scenes[scene].active = false;
+    if (!s->prepared) return TMPM_OK;
+    if ((st = upload_disabled(s)) != TMPM_OK || (st = upload_rigid(s)) != TMPM_OK ||
+        (st = upload_obstacles(s)) != TMPM_OK || (st = upload_scene(s)) != TMPM_OK) {
+      s->valid = false;
+      return st;
+    }
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  for (auto& other : s->scenes)
-    if (other.active && &other != &gone && other.first > first) other.first -= count;
-  s->id_space -= count;
-  // snapshot records of disabled particles: drop the scene's, shift the others
-  std::vector<float4> kept;
-  for (size_t i = 0; i + 1 < s->disabled_host.size(); i += 2) {
-    float4 a = s->disabled_host[i];
-    uint32_t g = __builtin_bit_cast(uint32_t, a.w);
-    if (g - first < count) continue;
-    if (g >= first + count) a.w = __builtin_bit_cast(float, g - count);
-    kept.push_back(a);
-    kept.push_back(s->disabled_host[i + 1]);
-  }
-  s->disabled_host.swap(kept);
-  {
-    // keep the emptied world's memory for a later admission (at most 8
-    // worlds and 2 GiB)
-    HostWorld w = std::move(gone.world);
-    w.clear_keep();
-    size_t kept_bytes = w.capacity_bytes();
-    for (const HostWorld& o : s->spare_worlds) kept_bytes += o.capacity_bytes();
-    if (s->spare_worlds.size() < 8 && kept_bytes <= (2ull << 30)) s->spare_worlds.push_back(std::move(w));
-  }
-  s->scenes[scene] = SceneState();
-  s->scenes[scene].active = false;
-  if (!s->prepared) return TMPM_OK;
-  if ((st = upload_disabled(s)) != TMPM_OK || (st = upload_rigid(s)) != TMPM_OK ||
-      (st = upload_obstacles(s)) != TMPM_OK || (st = upload_scene(s)) != TMPM_OK) {
-    s->valid = false;
-    return st;
-  }
-  return TMPM_OK;
 }
 
 tmpm_status tmpm_session_set_reactions(tmpm_session* s, int32_t enabled) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  const bool was = s->reactions_on;
-  s->reactions_on = enabled != 0;
-  if (!s->prepared || was == s->reactions_on) return TMPM_OK;
-  st = upload_obstacles(s);
-  if (st == TMPM_OK) st = upload_scene(s);
-  if (st != TMPM_OK) {
-    std::string msg = g_last_error;
-    s->reactions_on = was;
-    if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
-    g_last_error = msg;
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    const bool was = s->reactions_on;
+    s->reactions_on = enabled != 0;
+    if (!s->prepared || was == s->reactions_on) return TMPM_OK;
+    st = upload_obstacles(s);
+    if (st == TMPM_OK) st = upload_scene(s);
+    if (st != TMPM_OK) {
+      std::string msg = g_last_error;
+      s->reactions_on = was;
+      if (upload_obstacles(s) != TMPM_OK || upload_scene(s) != TMPM_OK) s->valid = false;
+      g_last_error = msg;
+    }
+    return st;
+  } catch (...) {
+    return api_exception(s);
   }
-  return st;
 }
 
 tmpm_status tmpm_session_read_reactions(const tmpm_session* s, double* out, uint32_t cap, uint32_t* count_out) {
-  if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
-  return read_scene_reactions(s->scenes[0], out, cap, count_out);
+  try {
+    if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
+    return read_scene_reactions(s->scenes[0], out, cap, count_out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_set_rigid_bodies(tmpm_session* s, const tmpm_rigid_body* bodies, uint32_t body_count,
                                           const tmpm_rigid_mapping* mappings, uint32_t mapping_count,
                                           const double gravity[3], double max_linear_speed, int32_t integrate) {
-  tmpm_status st = check_editable(s);
-  if (st != TMPM_OK) return st;
-  return set_scene_rigid(s, 0, bodies, body_count, mappings, mapping_count, gravity, max_linear_speed, integrate);
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    return set_scene_rigid(s, 0, bodies, body_count, mappings, mapping_count, gravity, max_linear_speed, integrate);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_read_rigid_bodies(const tmpm_session* s, tmpm_rigid_body* out, uint32_t cap,
                                            uint32_t* count_out) {
-  if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
-  return read_scene_bodies(s->scenes[0], out, cap, count_out);
+  try {
+    if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
+    return read_scene_bodies(s->scenes[0], out, cap, count_out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_read_obstacles(const tmpm_session* s, tmpm_obstacle* out, uint32_t cap, uint32_t* count_out) {
-  if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
-  return read_scene_obstacles(s->scenes[0], out, cap, count_out);
+  try {
+    if (!s || s->scenes.empty()) return fail(TMPM_INVALID_INPUT, "null session");
+    return read_scene_obstacles(s->scenes[0], out, cap, count_out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_read_reactions(const tmpm_session* s, uint32_t scene, double* out, uint32_t cap,
                                       uint32_t* count_out) {
-  tmpm_status st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return read_scene_reactions(s->scenes[scene], out, cap, count_out);
+  try {
+    tmpm_status st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return read_scene_reactions(s->scenes[scene], out, cap, count_out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_set_rigid_bodies(tmpm_session* s, uint32_t scene, const tmpm_rigid_body* bodies,
                                         uint32_t body_count, const tmpm_rigid_mapping* mappings,
                                         uint32_t mapping_count, const double gravity[3], double max_linear_speed,
                                         int32_t integrate) {
-  tmpm_status st = check_editable(s);
-  if (st == TMPM_OK) st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return set_scene_rigid(s, scene, bodies, body_count, mappings, mapping_count, gravity, max_linear_speed, integrate);
+  try {
+    tmpm_status st = check_editable(s);
+    if (st == TMPM_OK) st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return set_scene_rigid(s, scene, bodies, body_count, mappings, mapping_count, gravity, max_linear_speed, integrate);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_read_rigid_bodies(const tmpm_session* s, uint32_t scene, tmpm_rigid_body* out, uint32_t cap,
                                          uint32_t* count_out) {
-  tmpm_status st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return read_scene_bodies(s->scenes[scene], out, cap, count_out);
+  try {
+    tmpm_status st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return read_scene_bodies(s->scenes[scene], out, cap, count_out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_batch_read_obstacles(const tmpm_session* s, uint32_t scene, tmpm_obstacle* out, uint32_t cap,
                                       uint32_t* count_out) {
-  tmpm_status st = check_scene(s, scene);
-  if (st != TMPM_OK) return st;
-  return read_scene_obstacles(s->scenes[scene], out, cap, count_out);
+  try {
+    tmpm_status st = check_scene(s, scene);
+    if (st != TMPM_OK) return st;
+    return read_scene_obstacles(s->scenes[scene], out, cap, count_out);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 
 tmpm_status tmpm_session_encode_snapshot(tmpm_session* s, void* device_output, uint64_t output_bytes, void* stream,
                                          uint32_t* count_out) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (count_out) *count_out = 0;
-  if (!s->prepared) return TMPM_OK;
-  uint64_t bytes = 32ull * s->id_space;
-  if (!device_output || output_bytes < bytes) return fail(TMPM_CAPACITY, "snapshot buffer too small");
-  hipStream_t target = stream ? static_cast<hipStream_t>(stream) : s->stream;
-  if (target != s->stream) {
-    hipEvent_t ev;
-    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(ev, s->stream));
-    HIP_TRY(hipStreamWaitEvent(target, ev, 0));
-    HIP_TRY(hipEventDestroy(ev));
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+    if (count_out) *count_out = 0;
+    if (!s->prepared) return TMPM_OK;
+    uint64_t bytes = 32ull * s->id_space;
+    if (!device_output || output_bytes < bytes) return fail(TMPM_CAPACITY, "snapshot buffer too small");
+    hipStream_t target = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    if (target != s->stream) {
+      hipEvent_t ev;
+      HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      HIP_TRY(hipEventRecord(ev, s->stream));
+      HIP_TRY(hipStreamWaitEvent(target, ev, 0));
+      HIP_TRY(hipEventDestroy(ev));
+    }
+    launch_snapshot(s->parts(s->cur), static_cast<float4*>(device_output), target);
+    launch_snapshot_disabled(s->disabled_rec.as<float4>(), (uint32_t)(s->disabled_host.size() / 2),
+                             static_cast<float4*>(device_output), target);
+    HIP_TRY(hipGetLastError());
+    if (target != s->stream) {
+      hipEvent_t ev;
+      HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      HIP_TRY(hipEventRecord(ev, target));
+      HIP_TRY(hipStreamWaitEvent(s->stream, ev, 0));
+      HIP_TRY(hipEventDestroy(ev));
+    }
+    if (count_out) *count_out = s->id_space;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  launch_snapshot(s->parts(s->cur), static_cast<float4*>(device_output), target);
-  launch_snapshot_disabled(s->disabled_rec.as<float4>(), (uint32_t)(s->disabled_host.size() / 2),
-                           static_cast<float4*>(device_output), target);
-  HIP_TRY(hipGetLastError());
-  if (target != s->stream) {
-    hipEvent_t ev;
-    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(ev, target));
-    HIP_TRY(hipStreamWaitEvent(s->stream, ev, 0));
-    HIP_TRY(hipEventDestroy(ev));
-  }
-  if (count_out) *count_out = s->id_space;
-  return TMPM_OK;
 }
 
 double tmpm_session_stable_timestep(const tmpm_session* s) {
@@ -2971,105 +3121,125 @@ double tmpm_session_stable_timestep(const tmpm_session* s) {
 }
 
 tmpm_status tmpm_session_topology_stats(tmpm_session* s, tmpm_topology_stats* out) {
-  if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  std::memset(out, 0, sizeof(*out));
-  std::vector<uint32_t> c;
-  tmpm_status st = run_diagnostic_binning(s, c);
-  if (st != TMPM_OK) return st;
-  out->live_nodes = c[CNT_LIVE_NODES];
-  out->grid_blocks = c[CNT_GRID_BLOCKS];
-  out->particle_blocks = c[CNT_PARTICLE_BLOCKS];
-  out->binned_particles = c[CNT_BINNED];
-  return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+  try {
+    if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+    std::memset(out, 0, sizeof(*out));
+    std::vector<uint32_t> c;
+    tmpm_status st = run_diagnostic_binning(s, c);
+    if (st != TMPM_OK) return st;
+    out->live_nodes = c[CNT_LIVE_NODES];
+    out->grid_blocks = c[CNT_GRID_BLOCKS];
+    out->particle_blocks = c[CNT_PARTICLE_BLOCKS];
+    out->binned_particles = c[CNT_BINNED];
+    return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_export_nodes(tmpm_session* s, int32_t* keys, uint64_t cap, uint64_t* count_out) {
-  if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  std::vector<uint32_t> c;
-  tmpm_status st = run_diagnostic_binning(s, c);
-  if (st != TMPM_OK) return st;
-  *count_out = 0;
-  if (!s->prepared) return TMPM_OK;
-  uint32_t ng = c[CNT_GRID_BLOCKS];
-  std::vector<int4> gk(ng);
-  std::vector<uint32_t> masks(2ull * ng);
-  HIP_TRY(hipMemcpy(gk.data(), s->grid_key.ptr, 16ull * ng, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(masks.data(), s->node_mask.ptr, 8ull * ng, hipMemcpyDeviceToHost));
-  std::vector<std::array<int32_t, 4>> nodes;
-  for (uint32_t g = 0; g < ng; ++g)
-    for (int l = 0; l < 64; ++l)
-      if (masks[2 * g + (l >> 5)] & (1u << (l & 31)))
-        nodes.push_back({gk[g].x * 4 + (l >> 4), gk[g].y * 4 + ((l >> 2) & 3), gk[g].z * 4 + (l & 3),
-                         (int32_t)((uint32_t)gk[g].w & 0xffffu)});
-  std::sort(nodes.begin(), nodes.end());
-  for (uint64_t i = 0; i < nodes.size() && i < cap; ++i)
-    for (int k = 0; k < 4; ++k) keys[4 * i + k] = nodes[i][k];
-  *count_out = nodes.size();
-  return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+  try {
+    if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+    std::vector<uint32_t> c;
+    tmpm_status st = run_diagnostic_binning(s, c);
+    if (st != TMPM_OK) return st;
+    *count_out = 0;
+    if (!s->prepared) return TMPM_OK;
+    uint32_t ng = c[CNT_GRID_BLOCKS];
+    std::vector<int4> gk(ng);
+    std::vector<uint32_t> masks(2ull * ng);
+    HIP_TRY(hipMemcpy(gk.data(), s->grid_key.ptr, 16ull * ng, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(masks.data(), s->node_mask.ptr, 8ull * ng, hipMemcpyDeviceToHost));
+    std::vector<std::array<int32_t, 4>> nodes;
+    for (uint32_t g = 0; g < ng; ++g)
+      for (int l = 0; l < 64; ++l)
+        if (masks[2 * g + (l >> 5)] & (1u << (l & 31)))
+          nodes.push_back({gk[g].x * 4 + (l >> 4), gk[g].y * 4 + ((l >> 2) & 3), gk[g].z * 4 + (l & 3),
+                           (int32_t)((uint32_t)gk[g].w & 0xffffu)});
+    std::sort(nodes.begin(), nodes.end());
+    for (uint64_t i = 0; i < nodes.size() && i < cap; ++i)
+      for (int k = 0; k < 4; ++k) keys[4 * i + k] = nodes[i][k];
+    *count_out = nodes.size();
+    return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uint64_t cap, int64_t* particle_block,
                                        uint64_t* count_out) {
-  if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
-  if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
-  std::vector<uint32_t> c;
-  tmpm_status st = run_diagnostic_binning(s, c);
-  if (st != TMPM_OK) return st;
-  *count_out = 0;
-  size_t total = s->scenes.empty() ? 0 : s->world().size();
-  if (particle_block)
-    for (size_t i = 0; i < total; ++i) particle_block[i] = -1;
-  if (!s->prepared) return TMPM_OK;
-  uint32_t ng = c[CNT_GRID_BLOCKS];
-  uint32_t nb = s->topo.bucket_mask + 1;
-  std::vector<int4> gk(ng);
-  std::vector<uint32_t> pslot(s->n), bslot(nb), bpb(nb), ins(s->n);
-  HIP_TRY(hipMemcpy(gk.data(), s->grid_key.ptr, 16ull * ng, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(pslot.data(), s->pslot.ptr, 4ull * s->n, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(bslot.data(), s->bucket_slot.ptr, 4ull * nb, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(bpb.data(), s->bucket_pblock.ptr, 4ull * nb, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(ins.data(), s->buf[s->cur].ins_id.ptr, 4ull * s->n, hipMemcpyDeviceToHost));
-  std::vector<std::array<int32_t, 4>> keys;
-  std::vector<std::array<int32_t, 4>> pkey(s->n);
-  std::vector<char> has(s->n, 0);
-  for (uint32_t i = 0; i < s->n; ++i) {
-    uint32_t b = pslot[i];
-    if (b == kInvalid || bpb[b] == kInvalid || bslot[b] == kInvalid || bslot[b] >= ng) continue;
-    int4 k = gk[bslot[b]];
-    pkey[i] = {k.x, k.y, k.z, (int32_t)((uint32_t)k.w & 0xffffu)};
-    has[i] = 1;
-    keys.push_back(pkey[i]);
+  try {
+    if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (s->pending) return fail(TMPM_PENDING_STEPS, "steps are pending; synchronize first");
+    std::vector<uint32_t> c;
+    tmpm_status st = run_diagnostic_binning(s, c);
+    if (st != TMPM_OK) return st;
+    *count_out = 0;
+    size_t total = s->scenes.empty() ? 0 : s->world().size();
+    if (particle_block)
+      for (size_t i = 0; i < total; ++i) particle_block[i] = -1;
+    if (!s->prepared) return TMPM_OK;
+    uint32_t ng = c[CNT_GRID_BLOCKS];
+    uint32_t nb = s->topo.bucket_mask + 1;
+    std::vector<int4> gk(ng);
+    std::vector<uint32_t> pslot(s->n), bslot(nb), bpb(nb), ins(s->n);
+    HIP_TRY(hipMemcpy(gk.data(), s->grid_key.ptr, 16ull * ng, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pslot.data(), s->pslot.ptr, 4ull * s->n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bslot.data(), s->bucket_slot.ptr, 4ull * nb, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bpb.data(), s->bucket_pblock.ptr, 4ull * nb, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ins.data(), s->buf[s->cur].ins_id.ptr, 4ull * s->n, hipMemcpyDeviceToHost));
+    std::vector<std::array<int32_t, 4>> keys;
+    std::vector<std::array<int32_t, 4>> pkey(s->n);
+    std::vector<char> has(s->n, 0);
+    for (uint32_t i = 0; i < s->n; ++i) {
+      uint32_t b = pslot[i];
+      if (b == kInvalid || bpb[b] == kInvalid || bslot[b] == kInvalid || bslot[b] >= ng) continue;
+      int4 k = gk[bslot[b]];
+      pkey[i] = {k.x, k.y, k.z, (int32_t)((uint32_t)k.w & 0xffffu)};
+      has[i] = 1;
+      keys.push_back(pkey[i]);
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (uint64_t i = 0; i < keys.size() && i < cap; ++i)
+      for (int k = 0; k < 4; ++k) block_keys[4 * i + k] = keys[i][k];
+    if (particle_block)
+      for (uint32_t i = 0; i < s->n; ++i)
+        if (has[i]) particle_block[ins[i]] = std::lower_bound(keys.begin(), keys.end(), pkey[i]) - keys.begin();
+    *count_out = keys.size();
+    return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
+  } catch (...) {
+    return api_exception(s);
   }
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  for (uint64_t i = 0; i < keys.size() && i < cap; ++i)
-    for (int k = 0; k < 4; ++k) block_keys[4 * i + k] = keys[i][k];
-  if (particle_block)
-    for (uint32_t i = 0; i < s->n; ++i)
-      if (has[i]) particle_block[ins[i]] = std::lower_bound(keys.begin(), keys.end(), pkey[i]) - keys.begin();
-  *count_out = keys.size();
-  return status_from_error_bits(c[CNT_ERROR] & ~ERR_RESIDENT);
 }
 
 tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  collect_timers(s);
-  s->kernel_ms.clear();
-  s->timing = enabled != 0;
-  return TMPM_OK;
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timers(s);
+    s->kernel_ms.clear();
+    s->timing = enabled != 0;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]) {
-  if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
-  out[0] = (int32_t)s->p2g_threads;
-  out[1] = s->p2g_copies;
-  out[2] = (int32_t)s->threads_pblock;
-  out[3] = (int32_t)s->reorder_interval;
-  out[4] = s->use_graphs ? 1 : 0;
-  return TMPM_OK;
+  try {
+    if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
+    out[0] = (int32_t)s->p2g_threads;
+    out[1] = s->p2g_copies;
+    out[2] = (int32_t)s->threads_pblock;
+    out[3] = (int32_t)s->reorder_interval;
+    out[4] = s->use_graphs ? 1 : 0;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 // Launch-shape autotuner.  Every candidate shape computes the same results:
@@ -3081,123 +3251,139 @@ tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]) {
 // at a time (P2G shape, G2P workgroup, reorder interval, graphs), each
 // candidate timed over `trial` substeps with events on the session stream.
 tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
-  if (trial < 8 || trial > 256) return fail(TMPM_INVALID_INPUT, "trial must be within 8..=256 substeps");
-  if (substeps_run) *substeps_run = 0;
-  if (!s->prepared || s->n == 0) return TMPM_OK;
-  if (s->timing) return fail(TMPM_INVALID_INPUT, "autotuning needs per-kernel timing off");
-  hipEvent_t a = nullptr, b = nullptr;
-  HIP_TRY(hipEventCreate(&a));
-  HIP_TRY(hipEventCreate(&b));
-  uint32_t ran = 0;
-  tmpm_status st = TMPM_OK;
-  auto run = [&](uint32_t steps, float* ms) -> tmpm_status {
-    if (hipEventRecord(a, s->stream) != hipSuccess) return fail(TMPM_READBACK, "event record failed");
-    for (uint32_t left = steps; left > 0;) {
-      const uint32_t k = std::min<uint32_t>(left, 64);
-      tmpm_status e = tmpm_session_submit_steps(s, k);
-      if (e != TMPM_OK) return e;
-      left -= k;
-      ran += k;
-    }
-    if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
-      return fail(TMPM_READBACK, "event wait failed");
-    if (ms && hipEventElapsedTime(ms, a, b) != hipSuccess) return fail(TMPM_READBACK, "event time failed");
-    return TMPM_OK;
-  };
-  // Tries each value of one knob (set by `apply`) and keeps the fastest.
-  auto tune = [&](const std::vector<int>& values, const std::function<void(int)>& apply, int current) -> tmpm_status {
-    float best_ms = 0.0f;
-    int best = current;
-    for (size_t i = 0; i < values.size(); ++i) {
-      apply(values[i]);
-      float ms = 0.0f;
-      tmpm_status e = run(trial, &ms);
-      if (e != TMPM_OK) return e;
-      if (i == 0 || ms < best_ms) {
-        best_ms = ms;
-        best = values[i];
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+    if (trial < 8 || trial > 256) return fail(TMPM_INVALID_INPUT, "trial must be within 8..=256 substeps");
+    if (substeps_run) *substeps_run = 0;
+    if (!s->prepared || s->n == 0) return TMPM_OK;
+    if (s->timing) return fail(TMPM_INVALID_INPUT, "autotuning needs per-kernel timing off");
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    uint32_t ran = 0;
+    tmpm_status st = TMPM_OK;
+    auto run = [&](uint32_t steps, float* ms) -> tmpm_status {
+      if (hipEventRecord(a, s->stream) != hipSuccess) return fail(TMPM_READBACK, "event record failed");
+      for (uint32_t left = steps; left > 0;) {
+        const uint32_t k = std::min<uint32_t>(left, 64);
+        tmpm_status e = tmpm_session_submit_steps(s, k);
+        if (e != TMPM_OK) return e;
+        left -= k;
+        ran += k;
       }
+      if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
+        return fail(TMPM_READBACK, "event wait failed");
+      if (ms && hipEventElapsedTime(ms, a, b) != hipSuccess) return fail(TMPM_READBACK, "event time failed");
+      return TMPM_OK;
+    };
+    // Tries each value of one knob (set by `apply`) and keeps the fastest.
+    auto tune = [&](const std::vector<int>& values, const std::function<void(int)>& apply, int current) -> tmpm_status {
+      float best_ms = 0.0f;
+      int best = current;
+      for (size_t i = 0; i < values.size(); ++i) {
+        apply(values[i]);
+        float ms = 0.0f;
+        tmpm_status e = run(trial, &ms);
+        if (e != TMPM_OK) return e;
+        if (i == 0 || ms < best_ms) {
+          best_ms = ms;
+          best = values[i];
+        }
+      }
+      apply(best);
+      return TMPM_OK;
+    };
+    const bool order_free = !s->react_active();
+    st = run(std::min<uint32_t>(trial, 16), nullptr);  // warm-up
+    if (st == TMPM_OK) {
+      const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
+      st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
     }
-    apply(best);
-    return TMPM_OK;
-  };
-  const bool order_free = !s->react_active();
-  st = run(std::min<uint32_t>(trial, 16), nullptr);  // warm-up
-  if (st == TMPM_OK) {
-    const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
-    st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
+    if (st == TMPM_OK && order_free)
+      st = tune({64, 128, 256}, [&](int v) { s->threads_pblock = (uint32_t)v; }, (int)s->threads_pblock);
+    if (st == TMPM_OK && order_free)
+      st = tune({4, 8, 16}, [&](int v) { s->reorder_interval = (uint32_t)v; }, (int)s->reorder_interval);
+    if (st == TMPM_OK) st = tune({0, 1}, [&](int v) { s->use_graphs = v != 0; }, s->use_graphs ? 1 : 0);
+    if (!s->use_graphs) destroy_graphs(s);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (substeps_run) *substeps_run = ran;
+    return st;
+  } catch (...) {
+    return api_exception(s);
   }
-  if (st == TMPM_OK && order_free)
-    st = tune({64, 128, 256}, [&](int v) { s->threads_pblock = (uint32_t)v; }, (int)s->threads_pblock);
-  if (st == TMPM_OK && order_free)
-    st = tune({4, 8, 16}, [&](int v) { s->reorder_interval = (uint32_t)v; }, (int)s->reorder_interval);
-  if (st == TMPM_OK) st = tune({0, 1}, [&](int v) { s->use_graphs = v != 0; }, s->use_graphs ? 1 : 0);
-  if (!s->use_graphs) destroy_graphs(s);
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  if (substeps_run) *substeps_run = ran;
-  return st;
 }
 
 tmpm_status tmpm_session_set_graphs(tmpm_session* s, int32_t enabled) {
-  if (!s) return fail(TMPM_INVALID_INPUT, "null session");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  s->use_graphs = enabled != 0;
-  if (!s->use_graphs) destroy_graphs(s);
-  return TMPM_OK;
+  try {
+    if (!s) return fail(TMPM_INVALID_INPUT, "null session");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->use_graphs = enabled != 0;
+    if (!s->use_graphs) destroy_graphs(s);
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
 }
 
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out) {
-  if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  collect_timers(s);
-  std::string csv;
-  uint32_t k = 0;
-  for (auto& kv : s->kernel_ms) {
-    if (k < cap) {
-      if (avg_ms) avg_ms[k] = kv.second.second ? kv.second.first / (double)kv.second.second : 0.0;
-      if (calls) calls[k] = kv.second.second;
-      if (!csv.empty()) csv += ",";
-      csv += kv.first;
+  try {
+    if (!s || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timers(s);
+    std::string csv;
+    uint32_t k = 0;
+    for (auto& kv : s->kernel_ms) {
+      if (k < cap) {
+        if (avg_ms) avg_ms[k] = kv.second.second ? kv.second.first / (double)kv.second.second : 0.0;
+        if (calls) calls[k] = kv.second.second;
+        if (!csv.empty()) csv += ",";
+        csv += kv.first;
+      }
+      ++k;
     }
-    ++k;
+    if (names_csv && names_cap) {
+      std::strncpy(names_csv, csv.c_str(), names_cap - 1);
+      names_csv[names_cap - 1] = '\0';
+    }
+    *count_out = k;
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
   }
-  if (names_csv && names_cap) {
-    std::strncpy(names_csv, csv.c_str(), names_cap - 1);
-    names_csv[names_cap - 1] = '\0';
-  }
-  *count_out = k;
-  return TMPM_OK;
 }
 
 tmpm_status tmpm_bandwidth_probe(int32_t device_ordinal, uint64_t bytes, uint32_t iterations, double* gbs_out) {
-  if (!gbs_out || bytes < 16 || iterations == 0) return fail(TMPM_INVALID_INPUT, "invalid probe arguments");
-  if (hipSetDevice(device_ordinal) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
-  size_t n = bytes / 16;
-  DevBuf a, b;
-  HIP_TRY(a.alloc(n * 16));
-  HIP_TRY(b.alloc(n * 16));
-  HIP_TRY(hipMemset(a.ptr, 0, n * 16));
-  hipStream_t st;
-  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  launch_copy(a.as<float4>(), b.as<float4>(), n, st);  // warm up
-  HIP_TRY(hipEventRecord(e0, st));
-  for (uint32_t i = 0; i < iterations; ++i) launch_copy(a.as<float4>(), b.as<float4>(), n, st);
-  HIP_TRY(hipEventRecord(e1, st));
-  HIP_TRY(hipEventSynchronize(e1));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamDestroy(st);
-  *gbs_out = 2.0 * (double)n * 16.0 * iterations / (ms * 1e-3) / 1e9;
-  return TMPM_OK;
+  try {
+    if (!gbs_out || bytes < 16 || iterations == 0) return fail(TMPM_INVALID_INPUT, "invalid probe arguments");
+    if (hipSetDevice(device_ordinal) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
+    size_t n = bytes / 16;
+    DevBuf a, b;
+    HIP_TRY(a.alloc(n * 16));
+    HIP_TRY(b.alloc(n * 16));
+    HIP_TRY(hipMemset(a.ptr, 0, n * 16));
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    launch_copy(a.as<float4>(), b.as<float4>(), n, st);  // warm up
+    HIP_TRY(hipEventRecord(e0, st));
+    for (uint32_t i = 0; i < iterations; ++i) launch_copy(a.as<float4>(), b.as<float4>(), n, st);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    *gbs_out = 2.0 * (double)n * 16.0 * iterations / (ms * 1e-3) / 1e9;
+    return TMPM_OK;
+  } catch (...) {
+    return exception_status();
+  }
 }
 
 double tmpm_timestep_bound(const tmpm_material* material, double density, const double velocity[3], double cell_width,
