@@ -370,6 +370,19 @@ tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled);
  * graph and replayed, which lowers the host cost per substep.  Results are
  * identical either way.  Per-kernel timing launches eagerly. */
 tmpm_status tmpm_session_set_graphs(tmpm_session* s, int32_t enabled);
+
+/* `sample_closed_mesh_volume` (sampling.rs:47-146), the sampler behind
+ * `MeshEmitter` (emitter.rs:70-113): lattice centres lower + (i + 1/2) *
+ * spacing inside a closed, consistently oriented triangle mesh (solid angle
+ * above 2 pi, either winding), in lattice order (x, then y, then z).  The
+ * solid-angle test runs on the device.  vertices: 3 doubles each; triangles:
+ * 3 indices each; samples_out holds 3 * max_samples doubles.  InvalidInput for
+ * bad geometry or an open/badly oriented surface, Capacity when the lattice
+ * exceeds max_candidates or the accepted points exceed max_samples. */
+tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vertices, uint64_t vertex_count,
+                                           const uint32_t* triangles, uint64_t triangle_count, double spacing,
+                                           uint64_t max_candidates, uint64_t max_samples, double* samples_out,
+                                           uint64_t* count_out);
 /* No counterpart in the reference.  Chooses the launch shape (P2G workgroup
  * size and LDS tile copies, G2P workgroup size, reorder interval, graphs) by
  * timing each candidate over `trial` real substeps (8..=256), one knob at a

@@ -35,6 +35,10 @@ void launch_retire(const Parts& p, uint32_t first, uint32_t count, hipStream_t s
 void launch_renumber(uint32_t* ins_id, uint32_t n, uint32_t first, uint32_t count, hipStream_t s);
 void launch_gather_scene(const Parts& p, uint32_t first, uint32_t count, float* out, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
+// flags[c] = 1 for lattice candidate c (x-major, then y, then z) inside the closed mesh
+void launch_mesh_inside(const double* verts, const uint32_t* tris, uint64_t ntri, const double lower[3],
+                        const double upper[3], double spacing, const uint64_t counts[3], uint8_t* flags,
+                        hipStream_t s);
 void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);
 
 }  // namespace tmpm

@@ -2027,6 +2027,48 @@ __global__ __launch_bounds__(256) void k_live_count(Topo t) {
 }
 
 // ---------------------------------------------------------------------------
+// Closed-mesh volume sampling (sampling.rs:47-146): lattice centres whose
+// solid angle subtended by the mesh exceeds 2 pi (winding number 1, either
+// orientation).  One thread per candidate, f64 throughout, triangles in the
+// reference's order; flags[c] = 1 for an accepted candidate.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mesh_inside(const double* __restrict__ verts,
+                                                     const uint32_t* __restrict__ tris, uint64_t ntri,
+                                                     double lx, double ly, double lz, double spacing, uint64_t cy,
+                                                     uint64_t cz, double ux, double uy, double uz, uint64_t count,
+                                                     uint8_t* flags) {
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < count; c += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t x = c / (cy * cz), y = (c / cz) % cy, z = c % cz;
+    const double px = lx + ((double)x + 0.5) * spacing;
+    const double py = ly + ((double)y + 0.5) * spacing;
+    const double pz = lz + ((double)z + 0.5) * spacing;
+    if (px >= ux || py >= uy || pz >= uz) {
+      flags[c] = 0;
+      continue;
+    }
+    double angle = 0.0;
+    for (uint64_t t = 0; t < ntri; ++t) {
+      double v[3][3];
+      for (int k = 0; k < 3; ++k) {
+        const double* q = verts + 3ull * tris[3 * t + k];
+        v[k][0] = q[0] - px;
+        v[k][1] = q[1] - py;
+        v[k][2] = q[2] - pz;
+      }
+      auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+      const double la = sqrt(dot(v[0], v[0])), lb = sqrt(dot(v[1], v[1])), lc = sqrt(dot(v[2], v[2]));
+      const double bc[3] = {v[1][1] * v[2][2] - v[1][2] * v[2][1], v[1][2] * v[2][0] - v[1][0] * v[2][2],
+                            v[1][0] * v[2][1] - v[1][1] * v[2][0]};
+      const double num = dot(v[0], bc);
+      const double den = la * lb * lc + dot(v[0], v[1]) * lc + dot(v[1], v[2]) * la + dot(v[2], v[0]) * lb;
+      angle += 2.0 * atan2(num, den);
+    }
+    flags[c] = fabs(angle) > 6.283185307179586 ? 1 : 0;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
 // Bandwidth probe: a plain float4 copy of known size (reads n*16 B, writes
 // n*16 B), used to measure achievable HBM bandwidth and to calibrate
 // hardware byte counters.
@@ -2161,6 +2203,13 @@ void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const D
   hipLaunchKernelGGL(k_live_count, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);
 }
 
+void launch_mesh_inside(const double* verts, const uint32_t* tris, uint64_t ntri, const double lower[3],
+                        const double upper[3], double spacing, const uint64_t counts[3], uint8_t* flags,
+                        hipStream_t s) {
+  const uint64_t count = counts[0] * counts[1] * counts[2];
+  hipLaunchKernelGGL(k_mesh_inside, dim3(grid_for(count, 256, 65536)), dim3(256), 0, s, verts, tris, ntri, lower[0],
+                     lower[1], lower[2], spacing, counts[1], counts[2], upper[0], upper[1], upper[2], count, flags);
+}
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256 * 8, 16384)), dim3(256), 0, s, src, dst, n);
 }

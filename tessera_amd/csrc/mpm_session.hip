@@ -3355,6 +3355,96 @@ tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t
   }
 }
 
+tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vertices, uint64_t vertex_count,
+                                           const uint32_t* triangles, uint64_t triangle_count, double spacing,
+                                           uint64_t max_candidates, uint64_t max_samples, double* samples_out,
+                                           uint64_t* count_out) {
+  try {
+    if (count_out) *count_out = 0;
+    if (!ctx || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+    // sampling.rs:54-65: geometry, spacing and limits
+    if (vertex_count == 0 || triangle_count == 0 || !vertices || !triangles || !samples_out ||
+        !std::isfinite(spacing) || spacing <= 0.0 || max_candidates == 0 || max_samples == 0)
+      return fail(TMPM_INVALID_INPUT, "invalid mesh, spacing or limits");
+    for (uint64_t i = 0; i < 3 * vertex_count; ++i)
+      if (!std::isfinite(vertices[i])) return fail(TMPM_INVALID_INPUT, "non-finite mesh vertex");
+    // sampling.rs:66-93: non-degenerate triangles, every undirected edge used
+    // exactly once in each direction (a closed, consistently oriented surface)
+    std::map<std::pair<uint32_t, uint32_t>, std::array<uint64_t, 2>> edges;
+    for (uint64_t t = 0; t < triangle_count; ++t) {
+      const uint32_t* tr = triangles + 3 * t;
+      if (tr[0] >= vertex_count || tr[1] >= vertex_count || tr[2] >= vertex_count || tr[0] == tr[1] ||
+          tr[1] == tr[2] || tr[2] == tr[0])
+        return fail(TMPM_INVALID_INPUT, "invalid mesh triangle");
+      const double* a = vertices + 3ull * tr[0];
+      const double* b = vertices + 3ull * tr[1];
+      const double* c = vertices + 3ull * tr[2];
+      const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                            e1[0] * e2[1] - e1[1] * e2[0]};
+      const double area2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+      if (!std::isfinite(area2) || area2 <= 1e-24) return fail(TMPM_INVALID_INPUT, "degenerate mesh triangle");
+      const uint32_t pairs[3][2] = {{tr[0], tr[1]}, {tr[1], tr[2]}, {tr[2], tr[0]}};
+      for (const auto& pr : pairs)
+        edges[{std::min(pr[0], pr[1]), std::max(pr[0], pr[1])}][pr[0] > pr[1] ? 1 : 0] += 1;
+    }
+    for (const auto& e : edges)
+      if (e.second[0] != 1 || e.second[1] != 1)
+        return fail(TMPM_INVALID_INPUT, "mesh is not closed and consistently oriented");
+    // sampling.rs:94-115: the AABB lattice, bounded by max_candidates
+    double lower[3], upper[3];
+    for (int a = 0; a < 3; ++a) lower[a] = upper[a] = vertices[a];
+    for (uint64_t i = 1; i < vertex_count; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lower[a] = std::fmin(lower[a], vertices[3 * i + a]);
+        upper[a] = std::fmax(upper[a], vertices[3 * i + a]);
+      }
+    uint64_t counts[3], candidates = 1;
+    for (int a = 0; a < 3; ++a) {
+      const double c = std::ceil((upper[a] - lower[a]) / spacing);
+      if (!std::isfinite(c) || c < 1.0 || c > (double)max_candidates)
+        return fail(TMPM_CAPACITY, "candidate lattice exceeds max_candidates");
+      counts[a] = (uint64_t)c;
+      if (candidates > max_candidates / counts[a]) return fail(TMPM_CAPACITY, "candidate lattice exceeds max_candidates");
+      candidates *= counts[a];
+    }
+    // sampling.rs:116-144 on the device: the solid-angle test of every candidate
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
+    DevBuf dv, dt, df;
+    HIP_TRY(dv.alloc(24ull * vertex_count));
+    HIP_TRY(dt.alloc(12ull * triangle_count));
+    HIP_TRY(df.alloc(candidates));
+    HIP_TRY(hipMemcpy(dv.ptr, vertices, 24ull * vertex_count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dt.ptr, triangles, 12ull * triangle_count, hipMemcpyHostToDevice));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    launch_mesh_inside(dv.as<double>(), dt.as<uint32_t>(), triangle_count, lower, upper, spacing, counts,
+                       df.as<uint8_t>(), st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e != hipSuccess) return fail(TMPM_READBACK, std::string("mesh sampling kernel failed: ") + hipGetErrorString(e));
+    std::vector<uint8_t> flags(candidates);
+    HIP_TRY(hipMemcpy(flags.data(), df.ptr, candidates, hipMemcpyDeviceToHost));
+    // accepted centres in lattice order (x, then y, then z), at most max_samples
+    uint64_t n = 0;
+    for (uint64_t c = 0; c < candidates; ++c) {
+      if (!flags[c]) continue;
+      if (n == max_samples) return fail(TMPM_CAPACITY, "accepted samples exceed max_samples");
+      const uint64_t x = c / (counts[1] * counts[2]), y = (c / counts[2]) % counts[1], z = c % counts[2];
+      samples_out[3 * n] = lower[0] + ((double)x + 0.5) * spacing;
+      samples_out[3 * n + 1] = lower[1] + ((double)y + 0.5) * spacing;
+      samples_out[3 * n + 2] = lower[2] + ((double)z + 0.5) * spacing;
+      ++n;
+    }
+    *count_out = n;
+    return TMPM_OK;
+  } catch (...) {
+    return exception_status();
+  }
+}
+
 tmpm_status tmpm_bandwidth_probe(int32_t device_ordinal, uint64_t bytes, uint32_t iterations, double* gbs_out) {
   try {
     if (!gbs_out || bytes < 16 || iterations == 0) return fail(TMPM_INVALID_INPUT, "invalid probe arguments");

@@ -267,6 +267,9 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_bandwidth_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
         "tmpm_timestep_bound": (C.c_double, [C.POINTER(CMaterial), C.c_double, C.c_double * 3, C.c_double, C.c_double]),
         "tmpm_material_valid": (C.c_int32, [C.POINTER(CMaterial)]),
+        "tmpm_sample_closed_mesh_volume": (C.c_int, [vp, _dp, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64,
+                                                     C.c_double, C.c_uint64, C.c_uint64, _dp,
+                                                     C.POINTER(C.c_uint64)]),
         "tmpm_obstacle_valid": (C.c_int32, [C.POINTER(CObstacle)]),
     }
     for name, (res, args) in sig.items():
