@@ -139,7 +139,8 @@ typedef struct tmpm_particles {
   uint16_t* transfer_color;
   uint8_t* enabled;
   uint8_t* fixed;
-  uint64_t* chunk_id;    /* read-out only; ignored on input */
+  uint64_t* chunk_id;    /* session creation: the particles' chunks (NULL: chunk 0);
+                            add_particles ignores it (the batch becomes a new chunk) */
 } tmpm_particles;
 
 /* A whole `MpmWorld` (world.rs:206-221) handed over at session creation. */
@@ -151,6 +152,7 @@ typedef struct tmpm_world_desc {
   uint32_t obstacle_count;
   const tmpm_obstacle* obstacles;
   uint64_t substeps;
+  uint64_t next_chunk_id; /* id of the next added chunk; 0: one above the largest particle chunk */
 } tmpm_world_desc;
 
 /* P2G precision (the reference accumulates with f32 CAS loops,

@@ -2219,6 +2219,16 @@ tmpm_status check_world_desc(const tmpm_world_desc* wd) {
 void load_scene(SceneState& scene, const tmpm_world_desc* wd, double substep_dt, HostWorld w = HostWorld()) {
   w.params = wd->params;
   w.append(wd->particles, wd->materials, 0);
+  // the world keeps its chunks (GpuMpmResidentSession takes the MpmWorld by value)
+  uint64_t top = 0;
+  if (wd->particles.chunk_id) {
+    const size_t base = w.size() - wd->particles.count;
+    for (uint64_t i = 0; i < wd->particles.count; ++i) {
+      w.chunk[base + i] = wd->particles.chunk_id[i];
+      top = std::max(top, wd->particles.chunk_id[i]);
+    }
+  }
+  w.next_chunk_id = wd->next_chunk_id ? wd->next_chunk_id : (top == UINT64_MAX ? top : top + 1);
   w.set_obstacles(wd->obstacles, wd->obstacle_count);
   w.substeps = wd->substeps;
   scene = SceneState();

@@ -150,6 +150,7 @@ class CWorld(C.Structure):
         ("obstacle_count", C.c_uint32),
         ("obstacles", C.POINTER(CObstacle)),
         ("substeps", C.c_uint64),
+        ("next_chunk_id", C.c_uint64),
     ]
 
 
@@ -397,6 +398,7 @@ class MpmParticle:
     transfer_color: int = 0
     enabled: bool = True
     fixed: bool = False
+    chunk_id: int = 0  # read-out only (MpmParticle::chunk_id, world.rs:153-156)
 
     @classmethod
     def new(cls, position, radius: float, density: float, material: MaterialModel) -> "MpmParticle":
@@ -495,7 +497,7 @@ class ParticleArrays:
             material=self.materials[int(self.material_index[i])],
             plastic=PlasticState(*map(float, self.plastic[i])), damping=float(self.damping[i]),
             group_id=int(self.group_id[i]), transfer_color=int(self.transfer_color[i]),
-            enabled=bool(self.enabled[i]), fixed=bool(self.fixed[i]))
+            enabled=bool(self.enabled[i]), fixed=bool(self.fixed[i]), chunk_id=int(self.chunk_id[i]))
 
     def to_particles(self) -> list[MpmParticle]:
         return [self.particle(i) for i in range(len(self))]
@@ -766,6 +768,7 @@ class MpmWorld:
         d.obstacle_count = len(self.obstacles)
         d.obstacles = C.cast(obs, C.POINTER(CObstacle))
         d.substeps = self.substeps
+        d.next_chunk_id = int(getattr(self, "next_chunk_id", 0) or 0)
         keep.append(d)
         return d, keep
 
