@@ -1311,17 +1311,20 @@ __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
 // its stress and scatters its 27 quadratic B-spline contributions into a 6^3
 // LDS tile.  Contributions are accumulated as 64-bit fixed-point integers in
 // both precision modes: gfx950's float LDS atomics run at ~1/30 the rate of
-// its integer ones, and integer sums do not depend on atomic order.
+// its integer ones (and f64 ones slow down more than integer ones when lanes
+// hit the same node), and integer sums do not depend on atomic order.
 //
-// Each scaled contribution s (|s| < 2^45 units, checked per particle) is
-// converted to floor(s) as a two's-complement int64 from v = s * 2^-32:
-// hi = floor(v), lo = floor((v - rint(v)) * 2^32) mod 2^32.  v - rint(v) is
-// exact (|v| < 2^13) and both halves round the same way, so hi * 2^32 + lo is
-// floor(s) for either sign; one ds_add_u64 adds it.  The session scales mass
-// to at most 2^43 units, so the mean -1/2 unit rounding error per
-// contribution is ~2^-44 of the largest particle mass.  A block of at most
-// kP2GMaxBlock particles keeps a tile sum below 2^59, and the FIXED64 grid
-// update's sum of 8 tiles below 2^62; larger blocks raise Capacity.
+// Each scaled contribution s = a * w (|s| < 2^45 units, checked per
+// particle) is formed in f64 and rounded to the nearest integer by one fused
+// multiply-add with M = 1.5 * 2^52: for |s| < 2^51, fma(a, w, M) is
+// M + round(s) exactly, and its bit pattern is bits(M) + round(s) as a
+// two's-complement integer.  The low word of bits(M) is zero, so one 32-bit
+// subtract on the high word leaves round(s) as an int64 for one ds_add_u64.
+// The session scales mass to at most 2^43 units, so the unbiased rounding
+// error per contribution is at most 2^-44 of the largest particle mass.  A
+// block of at most kP2GMaxBlock particles keeps a tile sum below 2^59, and
+// the FIXED64 grid update's sum of 8 tiles below 2^62; larger blocks raise
+// Capacity.
 //
 // COPIES = 2 gives lanes of even and odd index separate tiles: particles of
 // one cell are often adjacent in storage, and LDS serializes atomics of one
@@ -1331,7 +1334,16 @@ __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
 constexpr uint32_t kP2GMaxBlock = 1u << 14;
 constexpr float kFixedBound = 0x1p45f;
 
-// v = s * 2^-32 -> floor(s) as a 64-bit integer (see above).
+// round(a * w) as a 64-bit integer, |a * w| < 2^51 (see above).
+__device__ __forceinline__ unsigned long long fixed64_round(double a, double w) {
+  const double r = __builtin_fma(a, w, 0x1.8p52);
+  return (unsigned long long)__double_as_longlong(r) - 0x4338000000000000ull;
+}
+
+// v = s * 2^-32 -> floor(s) as a 64-bit integer: hi = floor(v),
+// lo = floor((v - rint(v)) * 2^32) mod 2^32.  v - rint(v) is exact
+// (|v| < 2^13) and both halves round the same way, so hi * 2^32 + lo is
+// floor(s) for either sign.
 __device__ __forceinline__ unsigned long long fixed64_of(float v) {
   // No contraction: the subtraction must see the same rounded v as floor(v)
   // (fused into the caller's product it would see the unrounded one).
@@ -1412,24 +1424,26 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
                                   fmaxf(fmaxf(fabsf(a1.x), fabsf(a1.y)), fabsf(a1.z)) +
                                   fmaxf(fmaxf(fabsf(a2.x), fabsf(a2.y)), fabsf(a2.z)));
       range |= !(ms < kFixedBound && bound < kFixedBound);
-      // the 2^-32 of v = s * 2^-32 rides on the z weights
-      const float wz2[3] = {wz[0] * 0x1p-32f, wz[1] * 0x1p-32f, wz[2] * 0x1p-32f};
+      // the scatter runs in f64: gfx950 issues f64 adds and FMAs at its f32 rate
+      const double msd = ms;
+      const double q0[3] = {q.x, q.y, q.z};
+      const double d0[3] = {a0.x, a0.y, a0.z}, d1[3] = {a1.x, a1.y, a1.z}, d2[3] = {a2.x, a2.y, a2.z};
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        const float3 qx = q + a0 * (float)x;
 #pragma unroll
         for (int y = 0; y < 3; ++y) {
-          const float3 qxy = qx + a1 * (float)y;
-          const float wxy = wx[x] * wy[y];
+          const double wxy = (double)wx[x] * (double)wy[y];
 #pragma unroll
           for (int z = 0; z < 3; ++z) {
-            const float w = wxy * wz2[z];
-            const float3 mom = qxy + a2 * (float)z;
+            const double w = wxy * (double)wz[z];
+            double mom[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mom[c] = q0[c] + d0[c] * x + d1[c] * y + d2[c] * z;
             const int n = tile_index(lx + x, ly + y, lz + z);
-            atomicAdd(&mine[n], fixed64_of(ms * w));
-            atomicAdd(&mine[kTileNodes + n], fixed64_of(mom.x * w));
-            atomicAdd(&mine[2 * kTileNodes + n], fixed64_of(mom.y * w));
-            atomicAdd(&mine[3 * kTileNodes + n], fixed64_of(mom.z * w));
+            atomicAdd(&mine[n], fixed64_round(msd, w));
+            atomicAdd(&mine[kTileNodes + n], fixed64_round(mom[0], w));
+            atomicAdd(&mine[2 * kTileNodes + n], fixed64_round(mom[1], w));
+            atomicAdd(&mine[3 * kTileNodes + n], fixed64_round(mom[2], w));
           }
         }
       }

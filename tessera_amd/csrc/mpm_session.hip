@@ -998,8 +998,8 @@ tmpm_status build_scene(SceneState& scene) {
   // Fixed-point exponents: a particle's mass maps to at most 2^43 units and
   // every contribution must stay below 2^45 units (checked per particle);
   // momentum is scaled for mass x 64 cells per substep.  The unit is fine
-  // enough that P2G's round-down conversion (-1/2 unit mean error per
-  // contribution) stays far below f32 resolution.
+  // enough that P2G's rounding to the nearest unit stays far below f32
+  // resolution.
   if (scene.max_mass > 0.0) {
     double vbound = 64.0 * p.cell_width / scene.substep_dt;
     const int top = 43;
