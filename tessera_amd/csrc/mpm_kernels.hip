@@ -1593,9 +1593,12 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // REACT = 1 runs with one 64-lane wavefront per workgroup: each lane adds its
 // particles' obstacle impulses into its own LDS column (no atomics), and the
 // columns are reduced in a fixed order at the end of the block.
+// G2P waits on memory most of its time, so waves per SIMD matter: the plain
+// variants fit 96 VGPRs without spilling (5 waves; 64 us instead of 68 on
+// the 1M scene at 4 waves), while shapes and plastics would spill there.
 template <int REACT, bool SHAPES, bool PLASTIC>
-__global__ __launch_bounds__(256) void k_g2p(Parts p, Topo t, const DevClass* classes, const DevScene* scenes,
-                                             const DevObstacle* obstacles) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && !PLASTIC ? 5 : 1))) void k_g2p(
+    Parts p, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
   __shared__ float racc[REACT ? kMaxReactObstacles * 6 * 64 : 1];
