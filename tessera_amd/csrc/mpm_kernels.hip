@@ -1360,14 +1360,16 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
                                                                                          const DevClass* classes,
                                                                                          const DevScene* scenes) {
   constexpr int kTile = 4 * kTileNodes;  // m, px, py, pz planes of one tile copy
-  __shared__ unsigned long long tacc[COPIES * kTile];
+  // 16-byte aligned: cleared and read back two nodes (one ds_*_b128) at a time
+  __shared__ __attribute__((aligned(16))) unsigned long long tacc[COPIES * kTile];
+  static_assert(kTileNodes % 2 == 0 && kTile % 2 == 0, "tile planes hold whole node pairs");
   __shared__ uint32_t s_slot[8];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   const int tid = threadIdx.x;
   unsigned long long* const mine = tacc + (COPIES == 2 ? (lane_id() & 1) * kTile : 0);
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
     if (tid < 8) s_slot[tid] = t.pblock_slot8[8 * k + tid];
-    for (int q = tid; q < COPIES * kTile; q += THREADS) tacc[q] = 0;
+    for (int q = tid; q < COPIES * kTile / 2; q += THREADS) reinterpret_cast<ulonglong2*>(tacc)[q] = make_ulonglong2(0, 0);
     __syncthreads();
     const uint32_t own = s_slot[0];
     if (own == kInvalid) {  // grid capacity exhausted (flagged by binning)
@@ -1451,24 +1453,33 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     __syncthreads();
     flag_error(t.counters, bad, ERR_INTERNAL);
     flag_scene_error(t, (uint32_t)key.w >> 16, range, ERR_FIXED_RANGE);
-    auto node_sum = [&](int q) -> long long {
-      unsigned long long v = tacc[q];
-      if (COPIES == 2) v += tacc[kTile + q];
-      return (long long)v;
-    };
-    if (FIXED) {
-      longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + (size_t)k * kTileNodes * 2;
-      for (int n = tid; n < kTileNodes; n += THREADS) {
-        out[2 * n] = make_longlong2(node_sum(n), node_sum(kTileNodes + n));
-        out[2 * n + 1] = make_longlong2(node_sum(2 * kTileNodes + n), node_sum(3 * kTileNodes + n));
+    // sums of nodes q and q + 1 (q even) over the copies
+    auto pair_sum = [&](int q) -> longlong2 {
+      ulonglong2 v = *reinterpret_cast<const ulonglong2*>(tacc + q);
+      if (COPIES == 2) {
+        const ulonglong2 w = *reinterpret_cast<const ulonglong2*>(tacc + kTile + q);
+        v.x += w.x;
+        v.y += w.y;
       }
-    } else {
-      const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
-      float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes;
-      for (int n = tid; n < kTileNodes; n += THREADS)
-        out[n] = make_float4(__ll2float_rn(node_sum(n)) * im, __ll2float_rn(node_sum(kTileNodes + n)) * ip,
-                             __ll2float_rn(node_sum(2 * kTileNodes + n)) * ip,
-                             __ll2float_rn(node_sum(3 * kTileNodes + n)) * ip);
+      return make_longlong2((long long)v.x, (long long)v.y);
+    };
+    for (int n = 2 * tid; n < kTileNodes; n += 2 * THREADS) {
+      const longlong2 m = pair_sum(n), px = pair_sum(kTileNodes + n), py = pair_sum(2 * kTileNodes + n),
+                      pz = pair_sum(3 * kTileNodes + n);
+      if (FIXED) {
+        longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + ((size_t)k * kTileNodes + n) * 2;
+        out[0] = make_longlong2(m.x, px.x);
+        out[1] = make_longlong2(py.x, pz.x);
+        out[2] = make_longlong2(m.y, px.y);
+        out[3] = make_longlong2(py.y, pz.y);
+      } else {
+        const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
+        float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes + n;
+        out[0] = make_float4(__ll2float_rn(m.x) * im, __ll2float_rn(px.x) * ip, __ll2float_rn(py.x) * ip,
+                             __ll2float_rn(pz.x) * ip);
+        out[1] = make_float4(__ll2float_rn(m.y) * im, __ll2float_rn(px.y) * ip, __ll2float_rn(py.y) * ip,
+                             __ll2float_rn(pz.y) * ip);
+      }
     }
     __syncthreads();
   }
