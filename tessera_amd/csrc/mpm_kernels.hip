@@ -125,7 +125,10 @@ __device__ __forceinline__ int4 owner_key(const Parts& p, const Topo& t, const D
   if (owner <= p.n) return particle_key(p, classes, scenes, owner - 1);
   uint32_t r = owner - p.n - 1;
   uint32_t bucket = r >> 3, d = r & 7;
-  int4 k = particle_key(p, classes, scenes, t.owner[bucket] - 1);
+  // A neighbour insert's source is a particle block listed by the previous
+  // kernel, whose key is visible by now: one load instead of rebuilding it.
+  int4 k = t.bkey[bucket];
+  if (k.x == kNoKey) k = particle_key(p, classes, scenes, t.owner[bucket] - 1);
   k.x += (int)(d >> 2);
   k.y += (int)((d >> 1) & 1);
   k.z += (int)(d & 1);
@@ -148,29 +151,51 @@ __device__ __forceinline__ uint32_t owner_mask(const Parts& p, const Topo& t, ui
 // from the owner (a chain of dependent loads).  Bounded by the table size.
 // With CPIC a loser always rebuilds the winner's key and mask from the owner
 // (the claimer's two stores are not published together).
-template <bool CPIC = false>
-__device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
-                                int4 key, uint32_t me, bool& claimed, uint32_t mask = 0) {
-  uint32_t b = (CPIC ? hash_key_m(key, mask) : hash_key(key)) & t.bucket_mask;
+// Whether bucket b, whose compare-and-swap returned owner `prev`, holds `key`.
+template <bool CPIC>
+__device__ __forceinline__ bool bucket_holds(const Parts& p, const Topo& t, const DevClass* classes,
+                                             const DevScene* scenes, uint32_t b, uint32_t prev, int4 key,
+                                             uint32_t mask) {
+  if (CPIC) return key_eq(owner_key(p, t, classes, scenes, prev), key) && owner_mask(p, t, prev) == mask;
+  int4 k = t.bkey[b];
+  if (k.x == kNoKey) k = owner_key(p, t, classes, scenes, prev);
+  return key_eq(k, key);
+}
+
+__device__ __forceinline__ void claim_bucket(const Topo& t, uint32_t b, int4 key, bool cpic, uint32_t mask) {
+  t.bkey[b] = key;
+  if (cpic) t.bmask[b] = mask;
+}
+
+// Probes from bucket b, the probe-th bucket of the key's sequence.
+template <bool CPIC>
+__device__ uint32_t hash_insert_from(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
+                                     int4 key, uint32_t me, uint32_t b, uint32_t probe, bool& claimed,
+                                     uint32_t mask) {
   claimed = false;
-  for (uint32_t probe = 0; probe <= t.bucket_mask; ++probe) {
-    uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
+  for (; probe <= t.bucket_mask; ++probe) {
+    const uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
     if (prev == 0u) {
       claimed = true;
-      t.bkey[b] = key;
-      if (CPIC) t.bmask[b] = mask;
+      claim_bucket(t, b, key, CPIC, mask);
       return b;
     }
-    if (CPIC) {
-      if (key_eq(owner_key(p, t, classes, scenes, prev), key) && owner_mask(p, t, prev) == mask) return b;
-    } else {
-      int4 k = t.bkey[b];
-      if (k.x == kNoKey) k = owner_key(p, t, classes, scenes, prev);
-      if (key_eq(k, key)) return b;
-    }
+    if (bucket_holds<CPIC>(p, t, classes, scenes, b, prev, key, mask)) return b;
     b = (b + 1) & t.bucket_mask;
   }
   return kInvalid;
+}
+
+template <bool CPIC>
+__device__ __forceinline__ uint32_t key_bucket(int4 key, uint32_t mask, uint32_t bucket_mask) {
+  return (CPIC ? hash_key_m(key, mask) : hash_key(key)) & bucket_mask;
+}
+
+template <bool CPIC = false>
+__device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
+                                int4 key, uint32_t me, bool& claimed, uint32_t mask = 0) {
+  return hash_insert_from<CPIC>(p, t, classes, scenes, key, me, key_bucket<CPIC>(key, mask, t.bucket_mask), 0,
+                                claimed, mask);
 }
 
 // Sums of `a[j]` over j < upto and over all j < n, for every thread of a
@@ -273,7 +298,16 @@ __global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
 // this workgroup claims are listed in its part of bin_stage; ids are handed
 // out from prefix sums of the per-workgroup counts (k_bin_blocks), since
 // returning atomics on one shared counter serialize device-wide.
-template <bool CPIC>
+//
+// Every 256-particle round waits on a chain of device-scope atomics (claim,
+// key check, rank reservation), and at most 2048 such workgroups are resident
+// at once, so a 1M-particle binning is two rounds of that chain and 4M eight.
+// Each thread therefore bins K particles per pass (sub-rounds k * 256 apart)
+// and issues their loads, first-probe compare-and-swaps and rank atomics
+// together: one chain per K rounds.
+constexpr int kInsertK = 4;
+
+template <bool CPIC, int K>
 __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
                                                             const DevScene* scenes) {
   __shared__ uint32_t s_claims;
@@ -282,66 +316,91 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
   __syncthreads();
   const uint32_t begin = blockIdx.x * t.bin_span;
   uint32_t* stage = t.bin_stage + (size_t)blockIdx.x * t.bin_span;
-  // the next round's particle loads are issued before this round's atomics
-  float4 pm_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  uint32_t cls_next = 0;
-  if (begin + threadIdx.x < p.n) {
-    pm_next = p.pos_mass[begin + threadIdx.x];
-    cls_next = p.cls[begin + threadIdx.x];
-  }
-  for (uint32_t r = 0; r < t.bin_span; r += kBinThreads) {
-    const uint32_t i = begin + r + threadIdx.x;
-    const float4 pm = pm_next;
-    const uint32_t ci = cls_next;
-    const uint32_t inext = i + kBinThreads;
-    if (r + kBinThreads < t.bin_span && inext < p.n) {
-      pm_next = p.pos_mass[inext];
-      cls_next = p.cls[inext];
+  for (uint32_t r = 0; r < t.bin_span; r += K * kBinThreads) {
+    uint32_t idx[K];
+    bool in[K];
+    float4 pm[K];
+    uint32_t ci[K], mask[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      idx[k] = begin + r + k * kBinThreads + threadIdx.x;
+      in[k] = r + k * kBinThreads < t.bin_span && idx[k] < p.n;
+      pm[k] = in[k] ? p.pos_mass[idx[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      ci[k] = in[k] ? p.cls[idx[k]] : 0u;
+      mask[k] = CPIC && in[k] ? t.pmask[idx[k]] : 0u;
     }
-    bool active = false;
-    int4 key = make_int4(0, 0, 0, 0);
-    const uint32_t mask = CPIC && i < p.n ? t.pmask[i] : 0u;
-    if (i < p.n && pm.w > 0.0f) {
-      const DevClass& c = classes[ci];
-      int3 cell;
-      if (cell3(pm, scenes[c.region >> 16].inv_h, cell)) {
-        active = true;
-        key = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
-      } else {
-        atomicOr(&t.counters[CNT_ERROR], ERR_RESIDENT);
-        atomicOr(&t.scene_err[c.region >> 16], ERR_RESIDENT);
+    int4 key[K];
+    bool active[K], head[K];
+    uint64_t heads[K];
+    uint32_t len[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      active[k] = false;
+      key[k] = make_int4(0, 0, 0, 0);
+      if (in[k] && pm[k].w > 0.0f) {
+        const DevClass& c = classes[ci[k]];
+        int3 cell;
+        if (cell3(pm[k], scenes[c.region >> 16].inv_h, cell)) {
+          active[k] = true;
+          key[k] = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
+        } else {
+          atomicOr(&t.counters[CNT_ERROR], ERR_RESIDENT);
+          atomicOr(&t.scene_err[c.region >> 16], ERR_RESIDENT);
+        }
       }
-    }
-    const int4 prev = make_int4(__shfl_up(key.x, 1, 64), __shfl_up(key.y, 1, 64), __shfl_up(key.z, 1, 64),
-                                __shfl_up(key.w, 1, 64));
-    const bool prev_active = __shfl_up((int)active, 1, 64) != 0;
-    const bool same_mask = !CPIC || __shfl_up(mask, 1, 64) == mask;
-    const bool head = active && (lane == 0 || !prev_active || !key_eq(key, prev) || !same_mask);
-    const uint64_t heads = __ballot(head);
-    const uint64_t cont = __ballot(active && !head);
-    uint32_t bucket = kInvalid, first = 0;
-    if (head) {
+      const int4 prev = make_int4(__shfl_up(key[k].x, 1, 64), __shfl_up(key[k].y, 1, 64),
+                                  __shfl_up(key[k].z, 1, 64), __shfl_up(key[k].w, 1, 64));
+      const bool prev_active = __shfl_up((int)active[k], 1, 64) != 0;
+      const bool same_mask = !CPIC || __shfl_up(mask[k], 1, 64) == mask[k];
+      head[k] = active[k] && (lane == 0 || !prev_active || !key_eq(key[k], prev) || !same_mask);
+      heads[k] = __ballot(head[k]);
+      const uint64_t cont = __ballot(active[k] && !head[k]);
       // run length: this lane plus the continuing lanes directly above it
       const uint64_t above = lane == 63 ? 0ull : (cont >> (lane + 1));
-      const uint32_t len = 1u + (uint32_t)__builtin_ctzll(~above);
-      bool claimed;
-      bucket = hash_insert<CPIC>(p, t, classes, scenes, key, i + 1, claimed, mask);
-      if (bucket == kInvalid) {
-        atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
-      } else {
-        if (claimed) stage[atomicAdd(&s_claims, 1u)] = bucket;
-        first = atomicAdd(&t.bucket_count[bucket], len);
+      len[k] = 1u + (uint32_t)__builtin_ctzll(~above);
+    }
+    // first probe of every head, all in flight together
+    uint32_t bucket[K], prev[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      bucket[k] = kInvalid;
+      prev[k] = 0u;
+      if (head[k]) {
+        bucket[k] = key_bucket<CPIC>(key[k], mask[k], t.bucket_mask);
+        prev[k] = atomicCAS(&t.owner[bucket[k]], 0u, idx[k] + 1);
       }
     }
-    // every active lane belongs to the run of the nearest head at or below it
-    const uint64_t upto = heads & ((2ull << lane) - 1ull);
-    const int hl = upto ? 63 - __clzll((long long)upto) : (int)lane;
-    bucket = __shfl(bucket, hl, 64);
-    first = __shfl(first, hl, 64);
-    if (i < p.n) {
-      const bool ok = active && bucket != kInvalid;
-      t.pslot[i] = ok ? bucket : kInvalid;
-      if (ok) t.prank[i] = first + (lane - (uint32_t)hl);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!head[k]) continue;
+      bool claimed = prev[k] == 0u;
+      if (claimed) {
+        claim_bucket(t, bucket[k], key[k], CPIC, mask[k]);
+      } else if (!bucket_holds<CPIC>(p, t, classes, scenes, bucket[k], prev[k], key[k], mask[k])) {
+        bucket[k] = hash_insert_from<CPIC>(p, t, classes, scenes, key[k], idx[k] + 1,
+                                           (bucket[k] + 1) & t.bucket_mask, 1, claimed, mask[k]);
+      }
+      if (bucket[k] == kInvalid) atomicOr(&t.counters[CNT_ERROR], ERR_TOPOLOGY);
+      else if (claimed) stage[atomicAdd(&s_claims, 1u)] = bucket[k];
+    }
+    uint32_t first[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      first[k] = 0u;
+      if (head[k] && bucket[k] != kInvalid) first[k] = atomicAdd(&t.bucket_count[bucket[k]], len[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      // every active lane belongs to the run of the nearest head at or below it
+      const uint64_t upto = heads[k] & ((2ull << lane) - 1ull);
+      const int hl = upto ? 63 - __clzll((long long)upto) : (int)lane;
+      const uint32_t b = __shfl(bucket[k], hl, 64);
+      const uint32_t f = __shfl(first[k], hl, 64);
+      if (in[k]) {
+        const bool ok = active[k] && b != kInvalid;
+        t.pslot[idx[k]] = ok ? b : kInvalid;
+        if (ok) t.prank[idx[k]] = f + (lane - (uint32_t)hl);
+      }
     }
   }
   __syncthreads();
@@ -2121,8 +2180,8 @@ void launch_topo_clear(const Topo& t, hipStream_t s) {
 }
 void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
                        hipStream_t s) {
-  if (cpic) hipLaunchKernelGGL(k_bin_insert<true>, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
-  else hipLaunchKernelGGL(k_bin_insert<false>, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+  if (cpic) hipLaunchKernelGGL((k_bin_insert<true, kInsertK>), dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+  else hipLaunchKernelGGL((k_bin_insert<false, kInsertK>), dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
 }
 void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
                        hipStream_t s) {
