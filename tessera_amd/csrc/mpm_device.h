@@ -62,6 +62,7 @@ __host__ __device__ constexpr bool dm_plastic(int32_t k) { return k >= DM_SNOW; 
 struct DevClass {
   int32_t kind;
   uint32_t region;   // (scene << 16) | transfer colour; the block key's 4th field
+  float inv_h;       // the scene's 1 / cell width, so binning reads one table per particle
   uint32_t fixed;
   float radius;
   float damping;

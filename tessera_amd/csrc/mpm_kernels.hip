@@ -114,7 +114,7 @@ __device__ __forceinline__ int4 particle_key(const Parts& p, const DevClass* cla
                                              uint32_t i) {
   float4 pm = p.pos_mass[i];
   const DevClass& c = classes[p.cls[i]];
-  float inv_h = scenes[c.region >> 16].inv_h;
+  float inv_h = c.inv_h;
   int3 cell;
   cell3(pm, inv_h, cell);
   return make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
@@ -304,11 +304,14 @@ __global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
 // at once, so a 1M-particle binning is two rounds of that chain and 4M eight.
 // Each thread therefore bins K particles per pass (sub-rounds k * 256 apart)
 // and issues their loads, first-probe compare-and-swaps and rank atomics
-// together: one chain per K rounds.
+// together: one chain per K rounds.  Capped at 64 VGPRs and the matching
+// SGPRs so that 8 waves fit per SIMD and all 2048 workgroups are resident
+// together: at 106 SGPRs only 7 fit, and the rest ran as a second
+// generation that doubled the kernel's span.
 constexpr int kInsertK = 4;
 
 template <bool CPIC, int K>
-__global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
                                                             const DevScene* scenes) {
   __shared__ uint32_t s_claims;
   const uint32_t lane = lane_id();
@@ -340,7 +343,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
       if (in[k] && pm[k].w > 0.0f) {
         const DevClass& c = classes[ci[k]];
         int3 cell;
-        if (cell3(pm[k], scenes[c.region >> 16].inv_h, cell)) {
+        if (cell3(pm[k], c.inv_h, cell)) {
           active[k] = true;
           key[k] = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
         } else {
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_insert(Parts p, Topo t, con
 // It then inserts the 7 upper neighbours of each of its blocks (the grid
 // blocks its 6^3 P2G tile touches) and lists the ones it claims.
 template <bool CPIC>
-__global__ __launch_bounds__(kBinThreads) void k_bin_blocks(Parts p, Topo t, const DevClass* classes,
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bin_blocks(Parts p, Topo t, const DevClass* classes,
                                                             const DevScene* scenes) {
   __shared__ uint32_t s_claims;
   __shared__ uint32_t s_csum[kBinThreads / 64];

@@ -585,11 +585,11 @@ struct PackedParticles {
 
 // Interning key of a particle class: every DevClass field that varies.
 using ClassKey = std::tuple<int, float, float, float, float, float, float, float, float, uint32_t, uint32_t, float,
-                            float, float, float, float>;
+                            float, float, float, float, float>;
 inline ClassKey class_key(const DevClass& c) {
   return std::make_tuple(c.kind, c.radius, c.damping, c.lambda, c.mu, c.bulk, c.gamma, c.viscosity, c.tensile,
                          c.region, c.fixed, c.hardening, c.crit_compression, c.crit_stretch, c.friction_angle,
-                         c.cohesion);
+                         c.cohesion, c.inv_h);
 }
 
 // A captured substep: the launch plan's bytes and the instantiated graph.
@@ -1217,11 +1217,13 @@ void count_blocks(const FV& pos32, const UV& region, const FV& inv_h, uint64_t& 
 }
 
 // Packs a particle's class (material constants, radius, damping, fixed flag,
-// region).  `scene` goes into the region's high 16 bits.
-tmpm_status pack_class(const HostWorld& w, size_t i, uint32_t scene, float rad, float damp, DevClass& c) {
+// region, the scene's cell scale).  `scene` goes into the region's high 16 bits.
+tmpm_status pack_class(const HostWorld& w, size_t i, uint32_t scene, float inv_h, float rad, float damp,
+                       DevClass& c) {
   const tmpm_material& m = w.material[i];
   c = DevClass{};
   c.region = (scene << 16) | w.color[i];
+  c.inv_h = inv_h;
   c.fixed = w.fixed[i] ? 1u : 0u;
   c.radius = rad;
   c.damping = damp;
@@ -1313,7 +1315,7 @@ tmpm_status pack_one(const HostWorld& w, size_t i, uint32_t k, float inv_h, floa
       return fail(TMPM_INVALID_INPUT, "particle matrices not representable in f32");
   for (int q = 0; q < 3; ++q)
     if (!finite_f32(w.plastic[3 * i + q], pp[q])) return fail(TMPM_INVALID_INPUT, "plastic state not representable in f32");
-  tmpm_status st = pack_class(w, i, k, rad, damp, c);
+  tmpm_status st = pack_class(w, i, k, inv_h, rad, damp, c);
   if (st != TMPM_OK) return st;
   // base cell must be representable (gpu.rs:1143-1153)
   for (int a = 0; a < 3; ++a) {
