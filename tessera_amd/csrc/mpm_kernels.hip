@@ -215,12 +215,26 @@ __device__ __forceinline__ void group_prefix(const uint32_t* const (&a)[K], uint
   uint32_t bs[K], ts[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) bs[k] = ts[k] = 0;
-  for (uint32_t j = threadIdx.x; j < n; j += kBinThreads) {
+  // All of a thread's loads are issued before any is added: the session keeps
+  // n <= 2048, i.e. one pass of kPer loads per list (a loop that adds as it
+  // loads waits on each one in turn, ~3 us per workgroup).
+  constexpr int kPer = 2048 / kBinThreads;
+  for (uint32_t j0 = 0; j0 < n; j0 += kPer * kBinThreads) {
+    uint32_t v[K][kPer];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t c = a[k][j];
-      ts[k] += c;
-      if (j < upto) bs[k] += c;
+    for (int u = 0; u < kPer; ++u) {
+      const uint32_t j = j0 + u * kBinThreads + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k][u] = j < n ? a[k][j] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const uint32_t j = j0 + u * kBinThreads + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        ts[k] += v[k][u];
+        if (j < upto) bs[k] += v[k][u];
+      }
     }
   }
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
