@@ -167,13 +167,26 @@ __device__ __forceinline__ void claim_bucket(const Topo& t, uint32_t b, int4 key
   if (cpic) t.bmask[b] = mask;
 }
 
-// Probes from bucket b, the probe-th bucket of the key's sequence.
-template <bool CPIC>
+// Probes from bucket b, the probe-th bucket of the key's sequence.  LOOKUP
+// (without CPIC) reads bkey before trying to claim: where most keys are
+// already in the table and their bkey is visible (the neighbour inserts after
+// the particle inserts), a plain load finds them without an atomic.  bkey only
+// ever holds its bucket's key or kNoKey, so a different key sends the probe on
+// and kNoKey falls through to the claim.
+template <bool CPIC, bool LOOKUP = false>
 __device__ uint32_t hash_insert_from(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
                                      int4 key, uint32_t me, uint32_t b, uint32_t probe, bool& claimed,
                                      uint32_t mask) {
   claimed = false;
   for (; probe <= t.bucket_mask; ++probe) {
+    if (LOOKUP && !CPIC) {
+      const int4 k = t.bkey[b];
+      if (k.x != kNoKey) {
+        if (key_eq(k, key)) return b;
+        b = (b + 1) & t.bucket_mask;
+        continue;
+      }
+    }
     const uint32_t prev = atomicCAS(&t.owner[b], 0u, me);
     if (prev == 0u) {
       claimed = true;
@@ -191,11 +204,11 @@ __device__ __forceinline__ uint32_t key_bucket(int4 key, uint32_t mask, uint32_t
   return (CPIC ? hash_key_m(key, mask) : hash_key(key)) & bucket_mask;
 }
 
-template <bool CPIC = false>
+template <bool CPIC = false, bool LOOKUP = false>
 __device__ uint32_t hash_insert(const Parts& p, const Topo& t, const DevClass* classes, const DevScene* scenes,
                                 int4 key, uint32_t me, bool& claimed, uint32_t mask = 0) {
-  return hash_insert_from<CPIC>(p, t, classes, scenes, key, me, key_bucket<CPIC>(key, mask, t.bucket_mask), 0,
-                                claimed, mask);
+  return hash_insert_from<CPIC, LOOKUP>(p, t, classes, scenes, key, me, key_bucket<CPIC>(key, mask, t.bucket_mask),
+                                        0, claimed, mask);
 }
 
 // Sums of `a[j]` over j < upto and over all j < n, for every thread of a
@@ -472,7 +485,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
     k.z += (int)(d & 1);
     bool claimed;
     const uint32_t got =
-        hash_insert<CPIC>(p, t, classes, scenes, k, p.n + 1 + 8 * b + d, claimed, CPIC ? t.bmask[b] : 0u);
+        hash_insert<CPIC, true>(p, t, classes, scenes, k, p.n + 1 + 8 * b + d, claimed, CPIC ? t.bmask[b] : 0u);
     if (got == kInvalid) {
       over = true;
     } else if (claimed) {
