@@ -449,23 +449,34 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
                                                             const DevScene* scenes) {
   __shared__ uint32_t s_claims;
   __shared__ uint32_t s_csum[kBinThreads / 64];
+  __shared__ uint32_t s_b[kBinThreads];  // the first kBinThreads listed buckets and their keys
+  __shared__ int4 s_key[kBinThreads];
   const uint32_t g = blockIdx.x;
   if (threadIdx.x == 0) s_claims = 0;
+  const uint32_t* stage = t.bin_stage + (size_t)g * t.bin_span;
+  uint32_t* nstage = t.bin_nstage + (size_t)g * 7 * t.bin_span;
+  // The list's length and first entries load together with the prefix sums
+  // (the list has room for bin_span >= kBinThreads entries, so the load stays
+  // in bounds; entries past the length are stale and unused).
+  const uint32_t len = t.bin_claims[g];
+  const uint32_t b0 = stage[threadIdx.x];
   const uint32_t* const lists[1] = {t.bin_claims};
   uint32_t before[1], total[1];
   group_prefix<1>(lists, t.bin_groups, g, before, total);
-  const uint32_t len = t.bin_claims[g];
-  const uint32_t* stage = t.bin_stage + (size_t)g * t.bin_span;
-  uint32_t* nstage = t.bin_nstage + (size_t)g * 7 * t.bin_span;
   uint32_t csum = 0;
   bool over = false;
   for (uint32_t j = threadIdx.x; j < len; j += kBinThreads) {
-    const uint32_t b = stage[j], pb = before[0] + j;
+    const uint32_t b = j < kBinThreads ? b0 : stage[j], pb = before[0] + j;
+    const int4 key = t.bkey[b];
+    if (j < kBinThreads) {
+      s_b[j] = b;
+      s_key[j] = key;
+    }
     if (pb < t.pblock_cap) {
       t.bucket_pblock[b] = pb;
       t.bucket_slot[b] = pb;
       t.pblock_bucket[pb] = b;
-      t.grid_key[pb] = t.bkey[b];
+      t.grid_key[pb] = key;
       clear_rev(t, pb);
       csum += t.bucket_count[b];
     } else {
@@ -474,12 +485,13 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
       over = true;
     }
   }
+  __syncthreads();
   for (uint32_t q = threadIdx.x; q < 7 * len; q += kBinThreads) {
     const uint32_t j = q / 7u, d = q % 7u + 1u;
-    const uint32_t b = stage[j];
+    const uint32_t b = j < kBinThreads ? s_b[j] : stage[j];
     if (d == 1) t.nbr_bucket[8 * b] = b;
     if (before[0] + j >= t.pblock_cap) continue;
-    int4 k = t.bkey[b];
+    int4 k = j < kBinThreads ? s_key[j] : t.bkey[b];
     k.x += (int)(d >> 2);
     k.y += (int)((d >> 1) & 1);
     k.z += (int)(d & 1);
