@@ -587,14 +587,26 @@ __device__ __forceinline__ void pblock_finalize(const Topo& t) {
 // ---------------------------------------------------------------------------
 // 6. scatter particle indices into block-sorted order (optionally the state)
 // ---------------------------------------------------------------------------
+// Each thread scatters kScatterK particles per pass with their three
+// dependent loads (slot and rank, block id, block start) in flight together,
+// on at most as many workgroups as are resident at once.
+constexpr int kScatterK = 4;
 __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
   pblock_finalize(t);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
-    uint32_t b = t.pslot[i];
-    if (b == kInvalid) continue;
-    uint32_t k = t.bucket_pblock[b];
-    if (k == kInvalid) continue;
-    t.sorted[t.pblock_start[k] + t.prank[i]] = i;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < p.n; i0 += kScatterK * stride) {
+    uint32_t b[kScatterK], r[kScatterK], k[kScatterK];
+#pragma unroll
+    for (int u = 0; u < kScatterK; ++u) {
+      const uint32_t i = i0 + u * stride;
+      b[u] = i < p.n ? t.pslot[i] : kInvalid;
+      r[u] = i < p.n ? t.prank[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kScatterK; ++u) k[u] = b[u] != kInvalid ? t.bucket_pblock[b[u]] : kInvalid;
+#pragma unroll
+    for (int u = 0; u < kScatterK; ++u)
+      if (k[u] != kInvalid) t.sorted[t.pblock_start[k[u]] + r[u]] = i0 + u * stride;
   }
 }
 
@@ -2238,7 +2250,7 @@ void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const 
   hipLaunchKernelGGL(k_bin_offsets, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
 }
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(p.n, 256 * kScatterK, 2048)), dim3(256), 0, s, p, t);
 }
 void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
