@@ -273,31 +273,6 @@ __device__ __forceinline__ void group_prefix(const uint32_t* const (&a)[K], uint
   __syncthreads();
 }
 
-// Exclusive scan of one value per thread over a 256-thread workgroup; the
-// workgroup's total is returned in `sum`.
-__device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t v, uint32_t& sum) {
-  __shared__ uint32_t ws[kBinThreads / 64];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t incl = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t u = __shfl_up(incl, off, 64);
-    if (lane >= (uint32_t)off) incl += u;
-  }
-  if (lane == 63) ws[wave] = incl;
-  __syncthreads();
-  uint32_t before = 0;
-  sum = 0;
-#pragma unroll
-  for (int w = 0; w < kBinThreads / 64; ++w) {
-    const uint32_t x = ws[w];
-    if ((uint32_t)w < wave) before += x;
-    sum += x;
-  }
-  __syncthreads();
-  return before + incl - v;
-}
-
 __device__ __forceinline__ void clear_rev(const Topo& t, uint32_t slot) {
 #pragma unroll
   for (int d = 0; d < 8; ++d) t.grid_rev[8 * slot + d] = kInvalid;
@@ -522,46 +497,74 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
 // ---------------------------------------------------------------------------
 // 4. counting-sort offsets and grid slots of neighbour-only blocks
 // ---------------------------------------------------------------------------
+// One wave per binning group, kOffsetGroups groups per workgroup: the
+// workgroup sums the three per-group lists over the groups before its first
+// one (the lists are read once per workgroup, not once per group: at 2048
+// groups that read was ~48 MB of L2 traffic and half this kernel's time),
+// and each wave adds the groups of its workgroup before its own and scans
+// its group's blocks with wave shuffles.
+constexpr int kOffsetGroups = kBinThreads / 64;
+
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t& sum) {
+  const uint32_t lane = lane_id();
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(incl, off, 64);
+    if (lane >= (uint32_t)off) incl += u;
+  }
+  sum = __shfl(incl, 63, 64);
+  return incl - v;
+}
+
 __global__ __launch_bounds__(kBinThreads) void k_bin_offsets(Parts p, Topo t, const DevClass* classes,
                                                              const DevScene* scenes) {
-  const uint32_t g = blockIdx.x;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t g0 = blockIdx.x * kOffsetGroups, g = g0 + wave;
   const uint32_t* const lists[3] = {t.bin_claims, t.bin_nclaims, t.bin_csum};
   uint32_t before[3], total[3];
-  group_prefix<3>(lists, t.bin_groups, g, before, total);
-  const uint32_t npb = min(total[0], t.pblock_cap);
-  // particle offsets of this group's particle blocks, in id order
-  const uint32_t len = t.bin_claims[g];
-  const uint32_t* stage = t.bin_stage + (size_t)g * t.bin_span;
-  uint32_t carry = before[2];
-  for (uint32_t c0 = 0; c0 < len; c0 += kBinThreads) {
-    const uint32_t j = c0 + threadIdx.x, pb = before[0] + j;
-    const bool mine = j < len && pb < t.pblock_cap;
-    const uint32_t cnt = mine ? t.bucket_count[stage[j]] : 0u;
-    uint32_t sum;
-    const uint32_t excl = group_exclusive_scan(cnt, sum);
-    if (mine) {
-      t.pblock_start[pb] = carry + excl;
-      t.pblock_len[pb] = cnt;
-    }
-    carry += sum;
+  group_prefix<3>(lists, t.bin_groups, g0, before, total);
+  for (uint32_t u = g0; u < g && u < t.bin_groups; ++u) {
+    before[0] += t.bin_claims[u];
+    before[1] += t.bin_nclaims[u];
+    before[2] += t.bin_csum[u];
   }
-  // grid slots of the neighbour-only blocks this group claimed
-  const uint32_t nlen = t.bin_nclaims[g];
-  const uint32_t* nstage = t.bin_nstage + (size_t)g * 7 * t.bin_span;
+  const uint32_t npb = min(total[0], t.pblock_cap);
   bool over = false;
-  for (uint32_t j = threadIdx.x; j < nlen; j += kBinThreads) {
-    const uint32_t b = nstage[j], slot = npb + before[1] + j;
-    if (slot < t.grid_cap) {
-      t.bucket_slot[b] = slot;
-      t.grid_key[slot] = t.bkey[b];
-      clear_rev(t, slot);
-    } else {
-      t.bucket_slot[b] = kInvalid;
-      over = true;
+  if (g < t.bin_groups) {
+    // particle offsets of this group's particle blocks, in id order
+    const uint32_t len = t.bin_claims[g];
+    const uint32_t* stage = t.bin_stage + (size_t)g * t.bin_span;
+    uint32_t carry = before[2];
+    for (uint32_t c0 = 0; c0 < len; c0 += 64) {
+      const uint32_t j = c0 + lane, pb = before[0] + j;
+      const bool mine = j < len && pb < t.pblock_cap;
+      const uint32_t cnt = mine ? t.bucket_count[stage[j]] : 0u;
+      uint32_t sum;
+      const uint32_t excl = wave_exclusive_scan(cnt, sum);
+      if (mine) {
+        t.pblock_start[pb] = carry + excl;
+        t.pblock_len[pb] = cnt;
+      }
+      carry += sum;
+    }
+    // grid slots of the neighbour-only blocks this group claimed
+    const uint32_t nlen = t.bin_nclaims[g];
+    const uint32_t* nstage = t.bin_nstage + (size_t)g * 7 * t.bin_span;
+    for (uint32_t j = lane; j < nlen; j += 64) {
+      const uint32_t b = nstage[j], slot = npb + before[1] + j;
+      if (slot < t.grid_cap) {
+        t.bucket_slot[b] = slot;
+        t.grid_key[slot] = t.bkey[b];
+        clear_rev(t, slot);
+      } else {
+        t.bucket_slot[b] = kInvalid;
+        over = true;
+      }
     }
   }
   flag_error(t.counters, over, ERR_TOPOLOGY);
-  if (g == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     t.counters[CNT_PARTICLE_BLOCKS] = npb;
     t.counters[CNT_GRID_BLOCKS] = min(npb + total[1], t.grid_cap);
     t.counters[CNT_BINNED] = total[2];
@@ -604,9 +607,15 @@ __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
     }
 #pragma unroll
     for (int u = 0; u < kScatterK; ++u) k[u] = b[u] != kInvalid ? t.bucket_pblock[b[u]] : kInvalid;
+    bool bad = false;
 #pragma unroll
-    for (int u = 0; u < kScatterK; ++u)
-      if (k[u] != kInvalid) t.sorted[t.pblock_start[k[u]] + r[u]] = i0 + u * stride;
+    for (int u = 0; u < kScatterK; ++u) {
+      if (k[u] == kInvalid) continue;
+      const uint32_t pos = t.pblock_start[k[u]] + r[u];
+      if (pos < p.n) t.sorted[pos] = i0 + u * stride;
+      else bad = true;  // offsets and ranks disagree: never write past the list
+    }
+    flag_error(t.counters, bad, ERR_INTERNAL);
   }
 }
 
@@ -622,6 +631,9 @@ __global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo
       uint32_t k = t.bucket_pblock[b];
       if (k != kInvalid) dst = t.pblock_start[k] + t.prank[i];
     }
+    // offsets and ranks disagree: never write past the arrays
+    flag_error(t.counters, dst != kInvalid && dst >= p.n, ERR_INTERNAL);
+    if (dst != kInvalid && dst >= p.n) continue;
     if (dst == kInvalid) {
       if (i < tail_begin) continue;  // failed enabled particle: state is discarded with the batch
       dst = i;
@@ -2247,7 +2259,8 @@ void launch_cpic_mask(const Parts& p, const Topo& t, const DevClass* c, const De
   hipLaunchKernelGGL(k_cpic_mask, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, t, c, sc, o);
 }
 void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_offsets, dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+  hipLaunchKernelGGL(k_bin_offsets, dim3((t.bin_groups + kOffsetGroups - 1) / kOffsetGroups), dim3(kBinThreads), 0, s,
+                     p, t, c, sc);
 }
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(p.n, 256 * kScatterK, 2048)), dim3(256), 0, s, p, t);
