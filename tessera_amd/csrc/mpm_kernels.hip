@@ -1440,10 +1440,19 @@ __device__ __forceinline__ int tile_store_index(int x, int y, int z) {
   return offset + ((x & 3) * sy + (y & 3)) * sz + (z & 3);
 }
 
+// Grid-velocity layout inside a grid block: the 8 octants of 2^3 nodes (one
+// 128-byte line each) one after the other.  A particle block's G2P reads a
+// neighbour grid block's first 2 nodes along each axis it crosses, i.e. whole
+// octants, never parts of a line that another reader (often on another XCD)
+// fetches too.
+__device__ __forceinline__ int grid_store_index(int x, int y, int z) {
+  return (((x >> 1) << 2) | ((y >> 1) << 1) | (z >> 1)) * 8 + (((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
+}
+
 __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
   int x = t / 36, y = (t / 6) % 6, z = t % 6;
   d = ((x >> 2) << 2) | ((y >> 2) << 1) | (z >> 2);
-  local = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+  local = grid_store_index(x & 3, y & 3, z & 3);
 }
 
 // P2G scatter.  One workgroup per particle block: every particle computes
@@ -1736,7 +1745,7 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
         add_reaction(t, o, oidx, s % kReactBins, J, L);
       }
     }
-    t.grid_vel[(size_t)s * kBlockNodes + lane] = make_float4(v.x, v.y, v.z, 0.0f);
+    t.grid_vel[(size_t)s * kBlockNodes + grid_store_index(lx, ly, lz)] = make_float4(v.x, v.y, v.z, 0.0f);
   }
 }
 
@@ -2179,8 +2188,11 @@ __global__ __launch_bounds__(256) void k_live_mask(Parts p, Topo t, const DevCla
     for (int x = 0; x < 3; ++x)
       for (int y = 0; y < 3; ++y)
         for (int z = 0; z < 3; ++z) {
-          int d, local;
-          tile_node(tile_index(lx + x, ly + y, lz + z), d, local);
+          // grid block d of the tile and the node's row-major index in it (the
+          // mask layout the node export decodes)
+          const int tx = lx + x, ty = ly + y, tz = lz + z;
+          const int d = ((tx >> 2) << 2) | ((ty >> 2) << 1) | (tz >> 2);
+          const int local = ((tx & 3) << 4) | ((ty & 3) << 2) | (tz & 3);
           uint32_t slot = t.pblock_slot8[8 * k + d];
           if (slot == kInvalid) continue;
           atomicOr(&t.node_mask[2 * slot + (local >> 5)], 1u << (local & 31));
