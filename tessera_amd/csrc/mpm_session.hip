@@ -657,6 +657,7 @@ struct tmpm_session {
     bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum, bmask, pmask;
   Topo topo{};
   uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 64;
+  bool p2g_shape_fixed = false;  // P2G shape set by the tuner or an override, not by scene size
   int p2g_copies = 2;
   bool topo_valid = false;  // topology arrays describe the current device state
   bool table_dirty = true;  // block hash needs clearing before the next binning
@@ -1039,8 +1040,12 @@ void read_tuning_env(tmpm_session* s) {
   if (const char* v = std::getenv("TMPM_P2G_THREADS")) {
     int th = std::atoi(v);
     if (th == 64 || th == 128) s->p2g_threads = (uint32_t)th;
+    s->p2g_shape_fixed = true;
   }
-  if (const char* v = std::getenv("TMPM_P2G_COPIES")) s->p2g_copies = std::atoi(v) == 1 ? 1 : 2;
+  if (const char* v = std::getenv("TMPM_P2G_COPIES")) {
+    s->p2g_copies = std::atoi(v) == 1 ? 1 : 2;
+    s->p2g_shape_fixed = true;
+  }
   if (const char* v = std::getenv("TMPM_GRAPHS")) s->use_graphs = std::atoi(v) != 0;
   if (const char* v = std::getenv("TMPM_REORDER_INTERVAL")) {
     int r = std::atoi(v);
@@ -1631,6 +1636,17 @@ tmpm_status prepare_packed(tmpm_session* s, PackedParticles& pk) {
   count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
   s->last_pblocks = (uint32_t)pblocks;
   s->last_gblocks = (uint32_t)gblocks;
+  // Default P2G shape by size (any shape computes the same result): with
+  // fewer particle blocks than ~2 per CU, each block's serial chain sets the
+  // substep time and 128 threads over one tile copy shorten it (8,192
+  // particles in ~100 blocks: ~8 us of ~85 per substep); with many blocks, 64
+  // threads over two copies keep the LDS atomics apart (1M particles: 76 vs
+  // 86 us for P2G).
+  if (!s->p2g_shape_fixed) {
+    const bool few = pblocks < 512;
+    s->p2g_threads = few ? 128 : 64;
+    s->p2g_copies = few ? 1 : 2;
+  }
   // CPIC splits blocks that straddle a triangle by side: more headroom
   const uint64_t room = s->cpic ? 8 : 4;
   uint64_t pcap = std::max<uint64_t>(1024, room * pblocks);
@@ -3314,6 +3330,7 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
     if (st == TMPM_OK) {
       const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
       st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
+      s->p2g_shape_fixed = true;
     }
     if (st == TMPM_OK && order_free)
       st = tune({64, 128, 256}, [&](int v) { s->threads_pblock = (uint32_t)v; }, (int)s->threads_pblock);
