@@ -17,14 +17,15 @@ void launch_cpic_mask(const Parts& p, const Topo& t, const DevClass* c, const De
                       hipStream_t s);
 void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s);
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s);
-void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s);
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
                 uint32_t threads, int copies, int plastic, hipStream_t s);
 // shapes != 0: some obstacle is a triangle prism or convex polyhedron
 void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react, int shapes,
                         hipStream_t s);
-void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                uint32_t blocks, uint32_t threads, int react, int shapes, int plastic, hipStream_t s);
+// q: the reorder target (the whole state moved into block order), or empty
+void launch_g2p(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
+                const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes, int plastic,
+                hipStream_t s);
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s);
 void launch_rigid_sync(const DevRigid* bodies, const DevMapping* maps, DevObstacle* o, uint32_t count, hipStream_t s);
 void launch_rigid_apply(const Topo& t, const DevScene* sc, DevRigid* bodies, const DevMapping* maps,

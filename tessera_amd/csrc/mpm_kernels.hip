@@ -619,50 +619,6 @@ __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
   }
 }
 
-// Reorder variant: particle i's whole state moves to its sorted position in
-// the destination arrays `q`; sorted[] becomes the identity.  Particles that
-// were not binned (disabled) keep their slots in the tail [binned, n).
-__global__ __launch_bounds__(256) void k_bin_scatter_move(Parts p, Parts q, Topo t, uint32_t tail_begin) {
-  pblock_finalize(t);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
-    uint32_t dst = kInvalid;
-    uint32_t b = t.pslot[i];
-    if (b != kInvalid) {
-      uint32_t k = t.bucket_pblock[b];
-      if (k != kInvalid) dst = t.pblock_start[k] + t.prank[i];
-    }
-    // offsets and ranks disagree: never write past the arrays
-    flag_error(t.counters, dst != kInvalid && dst >= p.n, ERR_INTERNAL);
-    if (dst != kInvalid && dst >= p.n) continue;
-    if (dst == kInvalid) {
-      if (i < tail_begin) continue;  // failed enabled particle: state is discarded with the batch
-      dst = i;
-    } else {
-      t.sorted[dst] = dst;
-    }
-    q.pos_mass[dst] = p.pos_mass[i];
-    q.vel_vol[dst] = p.vel_vol[i];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      q.C[k * q.stride + dst] = p.C[k * p.stride + i];
-      q.F[k * q.stride + dst] = p.F[k * p.stride + i];
-    }
-    if (p.P) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q.P[k * q.stride + dst] = p.P[k * p.stride + i];
-    }
-    q.cls[dst] = p.cls[i];
-    q.ins_id[dst] = p.ins_id[i];
-  }
-  // Slots past the sorted particles hold no particle after the move (retired
-  // particles were not binned): mark them so that nothing bins or exports them.
-  const uint32_t binned = min(t.counters[CNT_BINNED], tail_begin);
-  for (uint32_t i = binned + blockIdx.x * blockDim.x + threadIdx.x; i < tail_begin; i += gridDim.x * blockDim.x) {
-    q.pos_mass[i].w = 0.0f;
-    q.ins_id[i] = kInvalid;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Diagnostics: live-node masks
 // ---------------------------------------------------------------------------
@@ -1758,13 +1714,31 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // G2P waits on memory most of its time, so waves per SIMD matter: the plain
 // variants fit 96 VGPRs without spilling (5 waves; 64 us instead of 68 on
 // the 1M scene at 4 waves), while shapes and plastics would spill there.
-template <int REACT, bool SHAPES, bool PLASTIC>
+//
+// Reorder.  In the MOVE variant (q set), G2P also moves every particle's whole state
+// into q at its block-sorted position (start + j of its block): the next
+// substeps read particles in nearly sequential order.  Slots past the binned
+// count hold no particle afterwards (retired particles were not binned).
+// P2G has read the unmoved state through the sorted list, which gives the
+// same sums and the same lane assignment as reading it moved, so results do
+// not depend on which substep reorders.  Otherwise q is empty and p is
+// updated in place.
+template <int REACT, bool SHAPES, bool PLASTIC, bool MOVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && !PLASTIC ? 5 : 1))) void k_g2p(
-    Parts p, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
+    Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
   __shared__ float racc[REACT ? kMaxReactObstacles * 6 * 64 : 1];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
+  constexpr bool move = MOVE;  // a separate variant: the plain one keeps its registers
+  const Parts o = move ? q : p;  // where updated particles are written
+  if (move) {
+    const uint32_t binned = min(t.counters[CNT_BINNED], p.n);
+    for (uint32_t i = binned + blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+      q.pos_mass[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      q.ins_id[i] = kInvalid;
+    }
+  }
   // The block hash is not read after the scatter of this substep: clear it
   // here for the next substep's binning (saves a launch).
   {
@@ -1802,14 +1776,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
       bool failed = false, bad = false;
       for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
         const uint32_t i = t.sorted[start + j];
+        const uint32_t w = move ? start + j : i;  // the slot the particle is written to
         const float4 pm = p.pos_mass[i];
-        const DevClass& cl = classes[p.cls[i]];
+        const uint32_t ci = p.cls[i];
+        const DevClass& cl = classes[ci];
+        // a moved particle takes along the state this kernel does not rewrite
+        auto carry = [&](bool with_plastic) {
+          o.cls[w] = ci;
+          o.ins_id[w] = p.ins_id[i];
+          if (with_plastic && p.P) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) o.P[a * o.stride + w] = p.P[a * p.stride + i];
+          }
+        };
         if (cl.fixed) {
           const float4 vv = p.vel_vol[i];
-          p.vel_vol[i] = make_float4(0.f, 0.f, 0.f, vv.w);
+          o.vel_vol[w] = make_float4(0.f, 0.f, 0.f, vv.w);
           if (sc.apply_force) p.force[p.ins_id[i]] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-          for (int q = 0; q < 9; ++q) p.C[q * p.stride + i] = 0.0f;
+          for (int a = 0; a < 9; ++a) o.C[a * o.stride + w] = 0.0f;
+          if (move) {
+            o.pos_mass[w] = pm;
+#pragma unroll
+            for (int a = 0; a < 9; ++a) o.F[a * o.stride + w] = p.F[a * p.stride + i];
+            carry(true);
+          }
           continue;
         }
         int3 cell;
@@ -1817,6 +1808,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
         const int lx = cell.x - ox, ly = cell.y - oy, lz = cell.z - oz;
         if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
           bad = true;
+          if (move) {  // binning disagrees with this block: keep the particle as it was
+            o.pos_mass[w] = pm;
+            o.vel_vol[w] = p.vel_vol[i];
+#pragma unroll
+            for (int a = 0; a < 9; ++a) {
+              o.C[a * o.stride + w] = p.C[a * p.stride + i];
+              o.F[a * o.stride + w] = p.F[a * p.stride + i];
+            }
+            carry(true);
+          }
           continue;
         }
         const float fx = pm.x * inv_h - (float)cell.x;
@@ -1916,13 +1917,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
           const float sc3 = cbrtf(fmaxf(det3(Fn), 1e-9f));
 #pragma unroll
           for (int q = 0; q < 9; ++q) Fn.a[q] = (q % 4 == 0) ? sc3 : 0.0f;
-        } else if (PLASTIC && dm_plastic(cl.kind)) {
+        }
+        bool plastic_written = false;
+        if (PLASTIC && dm_plastic(cl.kind)) {
           float pl[3] = {p.P[i], p.P[p.stride + i], p.P[2 * p.stride + i]};
           project_plastic(cl, Fn, pl);
-          p.P[i] = pl[0];
-          p.P[p.stride + i] = pl[1];
-          p.P[2 * p.stride + i] = pl[2];
+          o.P[w] = pl[0];
+          o.P[o.stride + w] = pl[1];
+          o.P[2 * o.stride + w] = pl[2];
+          plastic_written = true;
         }
+        if (move) carry(!plastic_written);
         // Resident validity (gpu.wgsl:946-993).
         int3 nb;
         bool ok = cell3(make_float4(x.x, x.y, x.z, 0.f), inv_h, nb);
@@ -1941,14 +1946,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
                              ? dt * fmaxf(wave * sqrtf(fmaxf(det3(Fn), 1e-6f)), len3(v)) > 0.5f * h * (1.0f + 1e-4f)
                              : dt * (wave + len3(v)) > 0.4f * h * (1.0f + 1e-4f);
         failed |= out || !fin || cfl;
-        p.pos_mass[i] = make_float4(x.x, x.y, x.z, pm.w);
+        o.pos_mass[w] = make_float4(x.x, x.y, x.z, pm.w);
         const float vol = p.vel_vol[i].w;
-        p.vel_vol[i] = make_float4(v.x, v.y, v.z, vol);
-        p.C[0 * p.stride + i] = c0.x; p.C[1 * p.stride + i] = c0.y; p.C[2 * p.stride + i] = c0.z;
-        p.C[3 * p.stride + i] = c1.x; p.C[4 * p.stride + i] = c1.y; p.C[5 * p.stride + i] = c1.z;
-        p.C[6 * p.stride + i] = c2.x; p.C[7 * p.stride + i] = c2.y; p.C[8 * p.stride + i] = c2.z;
+        o.vel_vol[w] = make_float4(v.x, v.y, v.z, vol);
+        o.C[0 * o.stride + w] = c0.x; o.C[1 * o.stride + w] = c0.y; o.C[2 * o.stride + w] = c0.z;
+        o.C[3 * o.stride + w] = c1.x; o.C[4 * o.stride + w] = c1.y; o.C[5 * o.stride + w] = c1.z;
+        o.C[6 * o.stride + w] = c2.x; o.C[7 * o.stride + w] = c2.y; o.C[8 * o.stride + w] = c2.z;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) p.F[q * p.stride + i] = Fn.a[q];
+        for (int a = 0; a < 9; ++a) o.F[a * o.stride + w] = Fn.a[a];
       }
       flag_scene_error(t, (uint32_t)key.w >> 16, failed, ERR_RESIDENT);
       flag_error(t.counters, bad, ERR_INTERNAL);
@@ -2294,9 +2299,6 @@ void launch_bin_offsets(const Parts& p, const Topo& t, const DevClass* c, const 
 void launch_bin_scatter(const Parts& p, const Topo& t, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(p.n, 256 * kScatterK, 2048)), dim3(256), 0, s, p, t);
 }
-void launch_bin_scatter_move(const Parts& p, const Parts& q, const Topo& t, uint32_t tail_begin, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_scatter_move, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, q, t, tail_begin);
-}
 void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int fixed64, uint32_t blocks,
                 uint32_t threads, int copies, int plastic, hipStream_t s) {
   // sessions with plastic materials run the default shape (64 threads, 2 tile copies)
@@ -2338,21 +2340,33 @@ void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o,
   else grid_update_variant<false>(t, sc, o, fixed64, react, blocks, s);
 }
 template <bool PLASTIC>
-static void g2p_variant(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                        uint32_t blocks, uint32_t threads, int react, int shapes, hipStream_t s) {
+static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
+                        const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes,
+                        hipStream_t s) {
   // REACT: one wavefront per workgroup (the lane-private reaction columns assume 64 lanes)
+  const bool mv = q.pos_mass != nullptr;
+#define TMPM_G2P(R, SH, M, T) \
+  hipLaunchKernelGGL((k_g2p<R, SH, PLASTIC, M>), dim3(blocks), dim3(T), 0, s, p, q, t, c, sc, o)
+#define TMPM_G2P_M(R, SH, T)          \
+  do {                                \
+    if (mv) TMPM_G2P(R, SH, true, T); \
+    else TMPM_G2P(R, SH, false, T);   \
+  } while (0)
   if (react) {
-    if (shapes) hipLaunchKernelGGL((k_g2p<1, true, PLASTIC>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
-    else hipLaunchKernelGGL((k_g2p<1, false, PLASTIC>), dim3(blocks), dim3(64), 0, s, p, t, c, sc, o);
+    if (shapes) TMPM_G2P_M(1, true, 64);
+    else TMPM_G2P_M(1, false, 64);
   } else {
-    if (shapes) hipLaunchKernelGGL((k_g2p<0, true, PLASTIC>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
-    else hipLaunchKernelGGL((k_g2p<0, false, PLASTIC>), dim3(blocks), dim3(threads), 0, s, p, t, c, sc, o);
+    if (shapes) TMPM_G2P_M(0, true, threads);
+    else TMPM_G2P_M(0, false, threads);
   }
+#undef TMPM_G2P_M
+#undef TMPM_G2P
 }
-void launch_g2p(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, const DevObstacle* o,
-                uint32_t blocks, uint32_t threads, int react, int shapes, int plastic, hipStream_t s) {
-  if (plastic) g2p_variant<true>(p, t, c, sc, o, blocks, threads, react, shapes, s);
-  else g2p_variant<false>(p, t, c, sc, o, blocks, threads, react, shapes, s);
+void launch_g2p(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
+                const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes, int plastic,
+                hipStream_t s) {
+  if (plastic) g2p_variant<true>(p, q, t, c, sc, o, blocks, threads, react, shapes, s);
+  else g2p_variant<false>(p, q, t, c, sc, o, blocks, threads, react, shapes, s);
 }
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s) {
   hipLaunchKernelGGL(k_react_finalize, dim3(grid_for(t.obstacle_total, 64, 1024)), dim3(64), 0, s, t, o);

@@ -1691,7 +1691,7 @@ hipError_t enqueue_binning(tmpm_session* s, const Parts& p) {
 // (compared bytewise, padding zeroed) launch the same kernels with the same
 // arguments, so a graph captured for one replays the other.
 struct SubstepPlan {
-  Parts p, q;  // q: the reorder target (zero unless reorder)
+  Parts p, q;  // q: the buffer G2P moves the state into (zero unless reorder)
   Topo t;
   const DevClass* c;
   const DevScene* sc;       // this substep's scene table (with or without forces)
@@ -1750,8 +1750,7 @@ SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
 // it can be recorded into a graph.
 hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
   const Topo& t = pl.t;
-  const Parts& p0 = pl.p;
-  const Parts& p = pl.reorder ? pl.q : pl.p;  // P2G and G2P run on the reordered state
+  const Parts& p = pl.p;
   hipStream_t st = s->stream;
   hipError_t e;
   if (pl.rigid_sync && (e = timed(s, "k_rigid_sync", [&] {
@@ -1760,22 +1759,17 @@ hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
     return e;
   if (pl.clear_table && (e = timed(s, "k_topo_clear", [&] { launch_topo_clear(t, st); })) != hipSuccess) return e;
   if (pl.cpic &&
-      (e = timed(s, "k_cpic_mask", [&] { launch_cpic_mask(p0, t, pl.c, pl.sc_plain, pl.ob, st); })) != hipSuccess)
+      (e = timed(s, "k_cpic_mask", [&] { launch_cpic_mask(p, t, pl.c, pl.sc_plain, pl.ob, st); })) != hipSuccess)
     return e;
-  if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p0, t, pl.c, pl.sc_plain, pl.cpic, st); })) !=
+  if ((e = timed(s, "k_bin_insert", [&] { launch_bin_insert(p, t, pl.c, pl.sc_plain, pl.cpic, st); })) !=
       hipSuccess)
     return e;
-  if ((e = timed(s, "k_bin_blocks", [&] { launch_bin_blocks(p0, t, pl.c, pl.sc_plain, pl.cpic, st); })) !=
+  if ((e = timed(s, "k_bin_blocks", [&] { launch_bin_blocks(p, t, pl.c, pl.sc_plain, pl.cpic, st); })) !=
       hipSuccess)
     return e;
-  if ((e = timed(s, "k_bin_offsets", [&] { launch_bin_offsets(p0, t, pl.c, pl.sc_plain, st); })) != hipSuccess)
+  if ((e = timed(s, "k_bin_offsets", [&] { launch_bin_offsets(p, t, pl.c, pl.sc_plain, st); })) != hipSuccess)
     return e;
-  if (pl.reorder) {
-    if ((e = timed(s, "k_bin_scatter_move", [&] { launch_bin_scatter_move(p0, pl.q, t, p0.n, st); })) != hipSuccess)
-      return e;
-  } else if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p0, t, st); })) != hipSuccess) {
-    return e;
-  }
+  if ((e = timed(s, "k_bin_scatter", [&] { launch_bin_scatter(p, t, st); })) != hipSuccess) return e;
   if ((e = timed(s, pl.fixed64 ? "k_p2g_fixed64" : "k_p2g", [&] {
          launch_p2g(p, t, pl.c, pl.sc, pl.fixed64, pl.p2g_blocks, pl.p2g_threads, pl.copies, pl.plastic, st);
        })) != hipSuccess)
@@ -1784,7 +1778,9 @@ hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
                  [&] { launch_grid_update(t, pl.sc, pl.ob, pl.fixed64, pl.react, pl.shapes, st); })) != hipSuccess)
     return e;
   if ((e = timed(s, "k_g2p", [&] {
-         launch_g2p(p, t, pl.c, pl.sc, pl.ob, pl.p2g_blocks, pl.g2p_threads, pl.react, pl.shapes, pl.plastic, st);
+         // on a reorder substep G2P also moves the state into block order in q
+         launch_g2p(p, pl.q, t, pl.c, pl.sc, pl.ob, pl.p2g_blocks, pl.g2p_threads, pl.react, pl.shapes, pl.plastic,
+                    st);
        })) != hipSuccess)
     return e;
   if (pl.react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, pl.ob, st); })) != hipSuccess)
