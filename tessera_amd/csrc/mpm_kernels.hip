@@ -1472,7 +1472,14 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
   const int tid = threadIdx.x;
   unsigned long long* const mine = tacc + (COPIES == 2 ? (lane_id() & 1) * kTile : 0);
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
-    if (tid < 8) s_slot[tid] = t.pblock_slot8[8 * k + tid];
+    // The block's particle range, slot list and first particle indices load
+    // together, and the first particles' state loads while the header (key,
+    // then scene) resolves: three dependent round trips before any compute
+    // instead of five.
+    const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
+    const uint32_t slot = tid < 8 ? t.pblock_slot8[8 * k + tid] : 0u;
+    const uint32_t i_first = (uint32_t)tid < len ? t.sorted[start + tid] : 0u;
+    if (tid < 8) s_slot[tid] = slot;
     for (int q = tid; q < COPIES * kTile / 2; q += THREADS) reinterpret_cast<ulonglong2*>(tacc)[q] = make_ulonglong2(0, 0);
     __syncthreads();
     const uint32_t own = s_slot[0];
@@ -1486,10 +1493,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     const float stress_scale = -sc.dt * sc.inv_d;
     const float mscale = exp2f((float)sc.mass_shift), pscale = exp2f((float)sc.mom_shift);
     const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
-    const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
     bool bad = false, range = len > kP2GMaxBlock;
     for (uint32_t j = tid; j < len; j += THREADS) {
-      const uint32_t i = t.sorted[start + j];
+      const uint32_t i = j == (uint32_t)tid ? i_first : t.sorted[start + j];
       const float4 pm = p.pos_mass[i];
       const float4 vv = p.vel_vol[i];
       const DevClass& cls = classes[p.cls[i]];
@@ -1750,9 +1756,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     }
   }
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
-    if (threadIdx.x < 8) s_slot[threadIdx.x] = t.pblock_slot8[8 * k + threadIdx.x];
+    // As in P2G: the block's particle range, slot list and first particle
+    // indices load together, and the block key loads with the velocity tile.
+    const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
+    const uint32_t slot0 = threadIdx.x < 8 ? t.pblock_slot8[8 * k + threadIdx.x] : 0u;
+    const uint32_t i_first = threadIdx.x < len ? t.sorted[start + threadIdx.x] : 0u;
+    if (threadIdx.x < 8) s_slot[threadIdx.x] = slot0;
     __syncthreads();
     const uint32_t own = s_slot[0];
+    const int4 key = own != kInvalid ? t.grid_key[own] : make_int4(0, 0, 0, 0);
     for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
       int d, local;
       tile_node(n, d, local);
@@ -1765,17 +1777,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     }
     __syncthreads();
     if (own != kInvalid) {
-      const int4 key = t.grid_key[own];
       const DevScene& sc = scenes[(uint32_t)key.w >> 16];
       const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
       const int nobs = REACT ? min(sc.obstacle_count, kMaxReactObstacles) : 0;
       if (REACT)
         for (int q = 0; q < nobs * 6; ++q) racc[q * 64 + threadIdx.x] = 0.0f;
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
-      const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
       bool failed = false, bad = false;
       for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
-        const uint32_t i = t.sorted[start + j];
+        const uint32_t i = j == threadIdx.x ? i_first : t.sorted[start + j];
         const uint32_t w = move ? start + j : i;  // the slot the particle is written to
         const float4 pm = p.pos_mass[i];
         const uint32_t ci = p.cls[i];
