@@ -55,6 +55,50 @@ def test_transfers_track_cpu_reference_for_elastic_particles(transfers, mode):
     assert_tracks(cpu.read(), s.read_particles(), 1e-5, 1e-4, 1e-3, 1e-5)
 
 
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_grid_boundary_and_sparse_span_match_cpu(transfers, mode):
+    # gpu.rs:4574-4703: a particle thrown at a wall stays inside it; two
+    # particles 10^4 apart keep a sparse grid of exactly their 2 x 27 nodes;
+    # adjacent particles at negative coordinates; unrepresentable particles
+    # are rejected.
+    elastic = MaterialModel.elastic(1000.0, 0.2)
+    falling = MpmParticle.new([0.5, 0.5, 0.05], 0.04, 1000.0, elastic)
+    falling.velocity = np.array([0.0, 0.0, -10.0])
+    w = world_of([falling], MpmParams(gravity=np.zeros(3), bounds=WorldBounds(np.zeros(3), np.ones(3))))
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep(), mode)
+    s.step_adaptive(0.001)
+    got = s.read_particles()
+    assert max_err(cpu.read()["velocity"], got.velocity) < 1e-4
+    assert got.position[0, 2] >= got.radius[0]
+
+    far = [MpmParticle.new([0.1] * 3, 0.04, 1000.0, elastic), MpmParticle.new([10000.0] * 3, 0.04, 1000.0, elastic)]
+    w = world_of(far, MpmParams())
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep(), mode)
+    assert s.topology_stats()["live_nodes"] == 54
+    s.step_adaptive(0.001)
+    assert s.substeps == cpu.substeps
+    assert_tracks(cpu.read(), s.read_particles(), 1e-5, 1e-4, F=1e-5)
+
+    near = [MpmParticle.new([-0.11, -0.08, -0.05], 0.04, 1000.0, elastic),
+            MpmParticle.new([-0.08, -0.07, -0.04], 0.04, 1000.0, elastic)]
+    w = world_of(near, MpmParams())
+    cpu = O.OracleWorld(w)
+    cpu.step(0.001)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep(), mode)
+    s.step_adaptive(0.001)
+    assert_tracks(cpu.read(), s.read_particles(), 1e-5, 1e-4)
+
+    for bad in (MpmParticle.new([0.5] * 3, 1e-20, 1000.0, elastic),
+                MpmParticle.new([300_000_000.0] * 3, 0.04, 1000.0, elastic)):
+        w = world_of([bad], MpmParams())
+        with pytest.raises(InvalidInput):
+            HipMpmResidentSession(transfers, w, 1e-4, mode)
+
+
 def test_transfers_track_cpu_with_materials_fixed_and_disabled_particles(transfers):
     # gpu.rs:4288-4350: all five materials, a force, damping, fixed and disabled particles
     mats = [MaterialModel.elastic(1000.0, 0.2), MaterialModel.neo_hookean(1000.0, 0.2),
