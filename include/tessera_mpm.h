@@ -267,8 +267,8 @@ tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uin
  * projection (step_with_obstacle_reactions, world.rs:309-330; the resident
  * GPU path's reaction passes, gpu.wgsl:1021-1167).  When enabled, each
  * synchronize replaces the readback with the sums over the substeps it
- * synchronized.  A scene may have at most 8 obstacles while reactions are
- * on (Capacity otherwise). */
+ * synchronized.  Any number of obstacles is supported (the reference's
+ * reaction capacity grows with the obstacle count, gpu.rs:2651-2720). */
 tmpm_status tmpm_session_set_reactions(tmpm_session* s, int32_t enabled);
 /* 6 doubles per obstacle in obstacle order: linear xyz, angular xyz. */
 tmpm_status tmpm_session_read_reactions(const tmpm_session* s, double* out, uint32_t obstacle_cap,

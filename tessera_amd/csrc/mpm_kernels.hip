@@ -1629,6 +1629,21 @@ __device__ __forceinline__ void add_reaction(const Topo& t, const DevObstacle& o
   }
 }
 
+// One lane's impulse straight into the obstacle's bins (divergent control
+// flow allowed): G2P's particle-side reactions on obstacles past the first
+// kMaxReactObstacles of a scene, which have no LDS columns.  Integer adds, so
+// the totals still do not depend on order.
+__device__ __forceinline__ void add_reaction_lane(const Topo& t, const DevObstacle& o, uint32_t oidx, uint32_t bin,
+                                                  float3 lin, float3 ang) {
+  unsigned long long* dst = reinterpret_cast<unsigned long long*>(t.react_bins) + ((size_t)oidx * kReactBins + bin) * 6;
+  const float c[6] = {lin.x, lin.y, lin.z, ang.x, ang.y, ang.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    atomicAdd(&dst[k], fixed64_of(c[k] * o.rscale_lin));
+    atomicAdd(&dst[3 + k], fixed64_of(c[3 + k] * o.rscale_ang));
+  }
+}
+
 template <int FIXED, int REACT, bool SHAPES>
 __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* scenes, const DevObstacle* obstacles) {
   const uint32_t ng = t.counters[CNT_GRID_BLOCKS];
@@ -1896,16 +1911,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
             x = x + nrm * (radius - dist);
             const float3 before = v;
             v = contact_velocity(o, x, v, nrm);
-            if (REACT && oi < nobs) {
+            if (REACT) {
               const float3 J = (before - v) * pm.w;
               const float3 L = cross3(x - nrm * radius - make_float3(o.center[0], o.center[1], o.center[2]), J);
-              float* acc = racc + oi * 6 * 64 + threadIdx.x;
-              acc[0] += J.x;
-              acc[64] += J.y;
-              acc[128] += J.z;
-              acc[192] += L.x;
-              acc[256] += L.y;
-              acc[320] += L.z;
+              if (oi < nobs) {
+                float* acc = racc + oi * 6 * 64 + threadIdx.x;
+                acc[0] += J.x;
+                acc[64] += J.y;
+                acc[128] += J.z;
+                acc[192] += L.x;
+                acc[256] += L.y;
+                acc[320] += L.z;
+              } else {
+                add_reaction_lane(t, o, sc.obstacle_begin + oi, k % kReactBins, J, L);
+              }
             }
           }
         }

@@ -876,8 +876,6 @@ tmpm_status upload_obstacles(tmpm_session* s) {
     scene.dev.obstacle_begin = (int32_t)packed.size();
     scene.dev.obstacle_count = 0;
     if (!scene.stepping()) continue;
-    if (s->react_active() && scene.world.obstacles.size() > (size_t)kMaxReactObstacles)
-      return fail(TMPM_CAPACITY, "at most 8 obstacles per scene while reactions are computed");
     const bool mapped = !scene.bodies.empty();
     if (mapped && scene.maps.size() != scene.world.obstacles.size())
       return fail(TMPM_INVALID_INPUT, "one rigid mapping per obstacle is required");

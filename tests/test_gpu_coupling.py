@@ -13,7 +13,7 @@ import pytest
 
 from oracle import oracle as O
 from oracle import rigid as R
-from tessera_amd.mpm import (Capacity, HipMpmBatch, HipMpmResidentSession, MaterialModel, MpmParams, MpmWorld,
+from tessera_amd.mpm import (HipMpmBatch, HipMpmResidentSession, MaterialModel, MpmParams, MpmParticle, MpmWorld,
                              ObstacleBoundary, RigidBody, RigidMapping, RigidObstacle, WorldBounds)
 from tessera_amd.scenes import GRAVITY_Y, _particles, jittered_lattice
 
@@ -95,13 +95,27 @@ def test_reactions_are_summed_over_the_synchronized_substeps_and_reproducible(tr
     assert np.allclose(acc, runs[0], rtol=1e-6, atol=1e-12)
 
 
-def test_more_than_eight_obstacles_with_reactions_is_a_capacity_error(transfers):
-    w = cube_on_box()
-    w.set_obstacles(list(w.obstacles) + [RigidObstacle.sphere([0.1 + 0.08 * i, 0.9, 0.9], 0.02) for i in range(7)])
-    s = HipMpmResidentSession(transfers, w, 5.0e-5)
-    with pytest.raises(Capacity):
-        s.set_reactions(True)
-    s.step(1)  # the session still steps without reactions
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_reactions_support_more_than_sixteen_obstacles(transfers, mode):
+    # gpu.rs:2651-2720: 16 far spheres and one the particle hits; reactions are
+    # kept for every obstacle, the far ones get none, the hit one matches the
+    # CPU.  Obstacles past the 8th of a scene take the direct-atomic path.
+    particle = MpmParticle.new([1.02, 0.0, 0.0], 0.04, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    particle.velocity = np.array([-1.0, 0.0, 0.0])
+    w = MpmWorld([particle], MpmParams(gravity=np.zeros(3)))
+    obstacles = [RigidObstacle.sphere([10.0 + 3.0 * i, 0.0, 0.0], 1.0) for i in range(16)]
+    obstacles.append(RigidObstacle.sphere([0.0, 0.0, 0.0], 1.0))
+    w.set_obstacles(obstacles)
+    cpu = O.OracleWorld(w)
+    expected = cpu.step_with_obstacle_reactions(0.001, 17)
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep(), mode)
+    s.set_reactions(True)
+    s.step_adaptive(0.001)
+    got = s.reactions()
+    assert got.shape == (17, 6)
+    assert np.all(np.linalg.norm(got[:16, :3], axis=1) < 1e-6)
+    assert got[16, 0] < -0.01, got[16]
+    assert np.linalg.norm(got[16, :3] - expected[16, :3]) < 1e-4
 
 
 def box_on_slab(seed=0):
