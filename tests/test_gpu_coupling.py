@@ -118,6 +118,28 @@ def test_reactions_support_more_than_sixteen_obstacles(transfers, mode):
     assert np.linalg.norm(got[16, :3] - expected[16, :3]) < 1e-4
 
 
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_reactions_keep_ordered_overlap_attribution(transfers, mode):
+    # gpu.rs:2723-2770: two overlapping spheres, the second moving; each
+    # projection is credited to the obstacle that made it, in obstacle order.
+    particle = MpmParticle.new([1.02, 0.0, 0.0], 0.04, 1000.0, MaterialModel.elastic(1000.0, 0.2))
+    particle.velocity = np.array([-1.0, 0.0, 0.0])
+    w = MpmWorld([particle], MpmParams(gravity=np.zeros(3)))
+    second = RigidObstacle.sphere([0.1, 0.0, 0.0], 1.0)
+    second.linear_velocity = np.array([1.0, 0.0, 0.0])
+    w.set_obstacles([RigidObstacle.sphere([0.0, 0.0, 0.0], 1.0), second])
+    cpu = O.OracleWorld(w)
+    expected = cpu.step_with_obstacle_reactions(0.001, 2)
+    assert expected[0, 0] < 0.0 and expected[1, 0] < 0.0
+    s = HipMpmResidentSession(transfers, w, w.stable_timestep(), mode)
+    s.set_reactions(True)
+    s.step_adaptive(0.001)
+    got = s.reactions()
+    for k in range(2):
+        assert np.linalg.norm(got[k, :3] - expected[k, :3]) < 1e-4, (k, got[k], expected[k])
+        assert np.linalg.norm(got[k, 3:] - expected[k, 3:]) < 1e-4, (k, got[k], expected[k])
+
+
 def box_on_slab(seed=0):
     rng = np.random.default_rng(seed)
     h = 1.0 / 64
