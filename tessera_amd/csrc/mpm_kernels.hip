@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void k_topo_clear(Topo t) {
 // SGPRs so that 8 waves fit per SIMD and all 2048 workgroups are resident
 // together: at 106 SGPRs only 7 fit, and the rest ran as a second
 // generation that doubled the kernel's span.
-constexpr int kInsertK = 4;
+constexpr int kInsertK = 4;  // at most; the launch picks 1, 2 or 4 by the workgroup's rounds
 
 template <bool CPIC, int K>
 __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bin_insert(Parts p, Topo t, const DevClass* classes,
@@ -2309,8 +2309,16 @@ void launch_topo_clear(const Topo& t, hipStream_t s) {
 }
 void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
                        hipStream_t s) {
-  if (cpic) hipLaunchKernelGGL((k_bin_insert<true, kInsertK>), dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
-  else hipLaunchKernelGGL((k_bin_insert<false, kInsertK>), dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc);
+  // sub-rounds per pass: as many as the workgroup's rounds, at most kInsertK
+  // (a 1M-particle binning has 2 rounds; unused sub-rounds would still shuffle)
+  const uint32_t rounds = t.bin_span / kBinThreads;
+#define TMPM_INSERT(C, K) hipLaunchKernelGGL((k_bin_insert<C, K>), dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc)
+  if (cpic) {
+    if (rounds >= 4) TMPM_INSERT(true, 4); else if (rounds == 2 || rounds == 3) TMPM_INSERT(true, 2); else TMPM_INSERT(true, 1);
+  } else {
+    if (rounds >= 4) TMPM_INSERT(false, 4); else if (rounds == 2 || rounds == 3) TMPM_INSERT(false, 2); else TMPM_INSERT(false, 1);
+  }
+#undef TMPM_INSERT
 }
 void launch_bin_blocks(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, int cpic,
                        hipStream_t s) {
