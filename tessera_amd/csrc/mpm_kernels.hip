@@ -2314,9 +2314,9 @@ void launch_bin_insert(const Parts& p, const Topo& t, const DevClass* c, const D
   const uint32_t rounds = t.bin_span / kBinThreads;
 #define TMPM_INSERT(C, K) hipLaunchKernelGGL((k_bin_insert<C, K>), dim3(t.bin_groups), dim3(kBinThreads), 0, s, p, t, c, sc)
   if (cpic) {
-    if (rounds >= 4) TMPM_INSERT(true, 4); else if (rounds == 2 || rounds == 3) TMPM_INSERT(true, 2); else TMPM_INSERT(true, 1);
+    if (rounds >= kInsertK) TMPM_INSERT(true, kInsertK); else if (rounds == 2 || rounds == 3) TMPM_INSERT(true, 2); else TMPM_INSERT(true, 1);
   } else {
-    if (rounds >= 4) TMPM_INSERT(false, 4); else if (rounds == 2 || rounds == 3) TMPM_INSERT(false, 2); else TMPM_INSERT(false, 1);
+    if (rounds >= kInsertK) TMPM_INSERT(false, kInsertK); else if (rounds == 2 || rounds == 3) TMPM_INSERT(false, 2); else TMPM_INSERT(false, 1);
   }
 #undef TMPM_INSERT
 }
