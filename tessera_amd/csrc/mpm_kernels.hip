@@ -1746,10 +1746,15 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // updated in place.
 template <int REACT, bool SHAPES, bool PLASTIC, bool MOVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && !PLASTIC ? 5 : 1))) void k_g2p(
-    Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
+    Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles,
+    int react_cols) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
-  __shared__ float racc[REACT ? kMaxReactObstacles * 6 * 64 : 1];
+  // Reaction columns (REACT): 6 x 64 floats for each of a scene's first
+  // react_cols obstacles, sized at launch by the session's largest obstacle
+  // count (at most kMaxReactObstacles) so that the LDS they take does not cap
+  // occupancy for the 1-4 obstacles scenes usually have.
+  extern __shared__ float racc[];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   constexpr bool move = MOVE;  // a separate variant: the plain one keeps its registers
   const Parts o = move ? q : p;  // where updated particles are written
@@ -1794,7 +1799,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     if (own != kInvalid) {
       const DevScene& sc = scenes[(uint32_t)key.w >> 16];
       const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
-      const int nobs = REACT ? min(sc.obstacle_count, kMaxReactObstacles) : 0;
+      const int nobs = REACT ? min(sc.obstacle_count, react_cols) : 0;
       if (REACT)
         for (int q = 0; q < nobs * 6; ++q) racc[q * 64 + threadIdx.x] = 0.0f;
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
@@ -2379,11 +2384,13 @@ void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o,
 template <bool PLASTIC>
 static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
                         const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes,
-                        hipStream_t s) {
+                        int react_cols, hipStream_t s) {
   // REACT: one wavefront per workgroup (the lane-private reaction columns assume 64 lanes)
   const bool mv = q.pos_mass != nullptr;
+  const int cols = react ? std::max(1, std::min(react_cols, kMaxReactObstacles)) : 0;
+  const size_t lds = (size_t)cols * 6 * 64 * sizeof(float);
 #define TMPM_G2P(R, SH, M, T) \
-  hipLaunchKernelGGL((k_g2p<R, SH, PLASTIC, M>), dim3(blocks), dim3(T), 0, s, p, q, t, c, sc, o)
+  hipLaunchKernelGGL((k_g2p<R, SH, PLASTIC, M>), dim3(blocks), dim3(T), lds, s, p, q, t, c, sc, o, cols)
 #define TMPM_G2P_M(R, SH, T)          \
   do {                                \
     if (mv) TMPM_G2P(R, SH, true, T); \
@@ -2401,9 +2408,9 @@ static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const Dev
 }
 void launch_g2p(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
                 const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes, int plastic,
-                hipStream_t s) {
-  if (plastic) g2p_variant<true>(p, q, t, c, sc, o, blocks, threads, react, shapes, s);
-  else g2p_variant<false>(p, q, t, c, sc, o, blocks, threads, react, shapes, s);
+                int react_cols, hipStream_t s) {
+  if (plastic) g2p_variant<true>(p, q, t, c, sc, o, blocks, threads, react, shapes, react_cols, s);
+  else g2p_variant<false>(p, q, t, c, sc, o, blocks, threads, react, shapes, react_cols, s);
 }
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s) {
   hipLaunchKernelGGL(k_react_finalize, dim3(grid_for(t.obstacle_total, 64, 1024)), dim3(64), 0, s, t, o);

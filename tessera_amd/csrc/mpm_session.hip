@@ -643,6 +643,7 @@ struct tmpm_session {
   bool reactions_on = false;
   DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
   uint32_t obstacle_total = 0, react_cap = 0, body_total = 0;
+  uint32_t max_scene_obstacles = 0;  // largest obstacle count of one scene (G2P's reaction columns)
   bool react_active() const { return reactions_on || body_total > 0; }
   std::vector<DevClass> host_classes;
   uint32_t substeps_since_reorder = 1u << 30;
@@ -868,6 +869,7 @@ double obstacle_reach(const tmpm_obstacle& o, double h) {
 
 // Obstacles of all active scenes, concatenated; each scene keeps its range.
 tmpm_status upload_obstacles(tmpm_session* s) {
+  uint32_t max_obstacles = 0;
   std::vector<DevObstacle> packed;
   std::vector<DevMapping> maps;
   std::vector<float4> planes;
@@ -917,7 +919,9 @@ tmpm_status upload_obstacles(tmpm_session* s) {
     }
     scene.dev.obstacle_count = (int32_t)(packed.size() - scene.dev.obstacle_begin);
     if (scene.dev.obstacle_count > kMaxObstacles) return fail(TMPM_CAPACITY, "too many obstacles");
+    max_obstacles = std::max<uint32_t>(max_obstacles, (uint32_t)scene.dev.obstacle_count);
   }
+  s->max_scene_obstacles = max_obstacles;
   if (packed.size() > s->obstacle_cap || !s->obstacles.ptr) {
     s->obstacle_cap = std::max<uint32_t>(16, (uint32_t)packed.size());
     HIP_TRY(s->obstacles.alloc(sizeof(DevObstacle) * s->obstacle_cap));
@@ -1699,7 +1703,7 @@ struct SubstepPlan {
   const DevMapping* maps;
   const uint32_t* body_scene;
   uint32_t obstacle_total, body_total, p2g_blocks, p2g_threads, g2p_threads;
-  int32_t react, fixed64, shapes, plastic, cpic, copies;
+  int32_t react, fixed64, shapes, plastic, cpic, copies, react_cols;
   uint8_t reorder, clear_table, rigid_sync, rigid_apply;
 };
 
@@ -1738,6 +1742,7 @@ SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
   pl.plastic = s->plastic ? 1 : 0;
   pl.cpic = s->cpic;
   pl.copies = s->p2g_copies;
+  pl.react_cols = (int32_t)s->max_scene_obstacles;
   pl.clear_table = s->table_dirty;
   pl.rigid_sync = s->body_total > 0 && s->obstacle_total > 0;
   pl.rigid_apply = pl.react && s->body_total > 0;
@@ -1778,7 +1783,7 @@ hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
   if ((e = timed(s, "k_g2p", [&] {
          // on a reorder substep G2P also moves the state into block order in q
          launch_g2p(p, pl.q, t, pl.c, pl.sc, pl.ob, pl.p2g_blocks, pl.g2p_threads, pl.react, pl.shapes, pl.plastic,
-                    st);
+                    pl.react_cols, st);
        })) != hipSuccess)
     return e;
   if (pl.react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, pl.ob, st); })) != hipSuccess)
