@@ -1750,7 +1750,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     int react_cols) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
-  // Reaction columns (REACT): 6 x 64 floats for each of a scene's first
+  // Reaction columns (REACT): 6 floats per thread for each of a scene's first
   // react_cols obstacles, sized at launch by the session's largest obstacle
   // count (at most kMaxReactObstacles) so that the LDS they take does not cap
   // occupancy for the 1-4 obstacles scenes usually have.
@@ -1801,7 +1801,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
       const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
       const int nobs = REACT ? min(sc.obstacle_count, react_cols) : 0;
       if (REACT)
-        for (int q = 0; q < nobs * 6; ++q) racc[q * 64 + threadIdx.x] = 0.0f;
+        for (int q = 0; q < nobs * 6; ++q) racc[q * blockDim.x + threadIdx.x] = 0.0f;
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
       bool failed = false, bad = false;
       for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
@@ -1920,13 +1920,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
               const float3 J = (before - v) * pm.w;
               const float3 L = cross3(x - nrm * radius - make_float3(o.center[0], o.center[1], o.center[2]), J);
               if (oi < nobs) {
-                float* acc = racc + oi * 6 * 64 + threadIdx.x;
+                const uint32_t B = blockDim.x;
+                float* acc = racc + oi * 6 * B + threadIdx.x;
                 acc[0] += J.x;
-                acc[64] += J.y;
-                acc[128] += J.z;
-                acc[192] += L.x;
-                acc[256] += L.y;
-                acc[320] += L.z;
+                acc[B] += J.y;
+                acc[2 * B] += J.z;
+                acc[3 * B] += L.x;
+                acc[4 * B] += L.y;
+                acc[5 * B] += L.z;
               } else {
                 add_reaction_lane(t, o, sc.obstacle_begin + oi, k % kReactBins, J, L);
               }
@@ -1993,9 +1994,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
       flag_error(t.counters, bad, ERR_INTERNAL);
       if (REACT) {
         for (int oi = 0; oi < nobs; ++oi) {
-          const float* acc = racc + oi * 6 * 64 + threadIdx.x;
-          const float3 J = make_float3(acc[0], acc[64], acc[128]);
-          const float3 L = make_float3(acc[192], acc[256], acc[320]);
+          // each wavefront sums its own lanes' columns (add_reaction)
+          const uint32_t B = blockDim.x;
+          const float* acc = racc + oi * 6 * B + threadIdx.x;
+          const float3 J = make_float3(acc[0], acc[B], acc[2 * B]);
+          const float3 L = make_float3(acc[3 * B], acc[4 * B], acc[5 * B]);
           const bool any = J.x != 0.f || J.y != 0.f || J.z != 0.f || L.x != 0.f || L.y != 0.f || L.z != 0.f;
           if (__any(any)) add_reaction(t, obstacles[sc.obstacle_begin + oi], sc.obstacle_begin + oi, k % kReactBins, J, L);
         }
@@ -2385,10 +2388,10 @@ template <bool PLASTIC>
 static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
                         const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes,
                         int react_cols, hipStream_t s) {
-  // REACT: one wavefront per workgroup (the lane-private reaction columns assume 64 lanes)
+  // REACT: 6 LDS floats per thread for each reaction column
   const bool mv = q.pos_mass != nullptr;
   const int cols = react ? std::max(1, std::min(react_cols, kMaxReactObstacles)) : 0;
-  const size_t lds = (size_t)cols * 6 * 64 * sizeof(float);
+  const size_t lds = (size_t)cols * 6 * threads * sizeof(float);
 #define TMPM_G2P(R, SH, M, T) \
   hipLaunchKernelGGL((k_g2p<R, SH, PLASTIC, M>), dim3(blocks), dim3(T), lds, s, p, q, t, c, sc, o, cols)
 #define TMPM_G2P_M(R, SH, T)          \
@@ -2397,8 +2400,8 @@ static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const Dev
     else TMPM_G2P(R, SH, false, T);   \
   } while (0)
   if (react) {
-    if (shapes) TMPM_G2P_M(1, true, 64);
-    else TMPM_G2P_M(1, false, 64);
+    if (shapes) TMPM_G2P_M(1, true, threads);
+    else TMPM_G2P_M(1, false, threads);
   } else {
     if (shapes) TMPM_G2P_M(0, true, threads);
     else TMPM_G2P_M(0, false, threads);
