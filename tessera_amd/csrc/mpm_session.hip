@@ -1140,17 +1140,20 @@ template <class FV, class UV>
 void count_blocks(const FV& pos32, const UV& region, const FV& inv_h, uint64_t& pblocks, uint64_t& gblocks) {
   // Dense path for one region: occupancy bytes over the bounding box of the
   // particle blocks (grid blocks are those dilated by one toward +x, +y, +z).
+  // Chunks of 8192 particles keep every host thread busy for scenes of a few
+  // hundred thousand particles (an admission's block count).
+  constexpr size_t kCountChunk = 8192;
   const size_t np = region.size();
   pblocks = gblocks = 0;
   if (np == 0) return;
   bool one_region = true;
   for (size_t i = 1; i < np && one_region; ++i) one_region = region[i] == region[0];
   if (one_region) {
-    const size_t nchunk = (np + 65535) / 65536;
+    const size_t nchunk = (np + kCountChunk - 1) / kCountChunk;
     std::vector<std::array<int32_t, 6>> box(nchunk, {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN});
     parallel_for(nchunk, [&](size_t cb, size_t ce) {
       for (size_t c = cb; c < ce; ++c)
-        for (size_t i = c * 65536; i < std::min(np, (c + 1) * 65536); ++i) {
+        for (size_t i = c * kCountChunk; i < std::min(np, (c + 1) * kCountChunk); ++i) {
           int32_t cc[3];
           bool ok = true;
           for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h[i], cc[a]) && ok;
@@ -1173,7 +1176,7 @@ void count_blocks(const FV& pos32, const UV& region, const FV& inv_h, uint64_t& 
       std::unique_ptr<std::atomic<uint8_t>[]> occ(new std::atomic<uint8_t>[ex * ey * ez]());
       parallel_for(nchunk, [&](size_t cb, size_t ce) {
         for (size_t c = cb; c < ce; ++c)
-          for (size_t i = c * 65536; i < std::min(np, (c + 1) * 65536); ++i) {
+          for (size_t i = c * kCountChunk; i < std::min(np, (c + 1) * kCountChunk); ++i) {
             int32_t cc[3];
             bool ok = true;
             for (int a = 0; a < 3; ++a) ok = cell_f32(pos32[3 * i + a], inv_h[i], cc[a]) && ok;
