@@ -390,14 +390,13 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
  * timing each candidate over `trial` real substeps (8..=256), one knob at a
  * time, and keeps the fastest.  The trials advance the session like
  * tmpm_session_submit_steps (they become pending substeps; *substeps_run
- * says how many) and compute exactly what any shape computes; with obstacle
- * reactions on, the G2P workgroup and reorder interval stay fixed because
- * the particle-side reaction sums follow them. */
+ * says how many) and compute exactly what any shape computes, obstacle
+ * reactions included. */
 tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run);
 /* The current launch shape: {P2G threads, P2G tile copies, G2P threads,
  * reorder interval, graphs on}.  Before tuning, the shape follows the
  * session's size: under 512 particle blocks P2G runs 128 threads over one
- * tile copy and, without obstacle reactions, G2P runs 256 threads. */
+ * tile copy and G2P runs 256 threads. */
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);

@@ -144,7 +144,7 @@ struct DevMapping {
 };
 
 constexpr int kReactBins = 16;       // int64 accumulators per obstacle component
-constexpr int kMaxReactObstacles = 8;  // per scene, G2P's LDS reaction columns (later obstacles: direct atomics)
+constexpr int kMaxReactObstacles = 16;  // per scene, G2P's LDS reaction sums (later obstacles: direct atomics)
 
 struct Parts {
   float4* pos_mass;

@@ -23,11 +23,9 @@ void launch_p2g(const Parts& p, const Topo& t, const DevClass* c, const DevScene
 void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o, int fixed64, int react, int shapes,
                         hipStream_t s);
 // q: the reorder target (the whole state moved into block order), or empty
-// react_cols: LDS reaction columns per workgroup (the largest per-scene
-// obstacle count, at most kMaxReactObstacles; later obstacles use atomics)
 void launch_g2p(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
                 const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes, int plastic,
-                int react_cols, hipStream_t s);
+                hipStream_t s);
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s);
 void launch_rigid_sync(const DevRigid* bodies, const DevMapping* maps, DevObstacle* o, uint32_t count, hipStream_t s);
 void launch_rigid_apply(const Topo& t, const DevScene* sc, DevRigid* bodies, const DevMapping* maps,

@@ -1604,9 +1604,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
 // Reactions (ObstacleReaction::record, world.rs:198-204): an obstacle
 // receives the linear impulse J = (v_before - v_after) m and the angular
 // impulse (point - center) x J of every node (gpu.wgsl:1056-1108) and
-// particle (gpu.wgsl:1110-1142) it deflects.  A wavefront sums its lanes in a
-// fixed butterfly order and adds the six sums to the obstacle's int64 bins,
-// so the totals do not depend on scheduling; k_react_finalize folds the bins.
+// particle (gpu.wgsl:1110-1142) it deflects.  Node side: the wavefront of a
+// grid block sums its node lanes in a fixed butterfly order and adds the six
+// sums to the obstacle's int64 bins.  Particle side: each impulse is rounded
+// to the bins' integer units first (reaction_fixed).  Either way the totals
+// do not depend on scheduling; k_react_finalize folds the bins.
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1629,19 +1631,31 @@ __device__ __forceinline__ void add_reaction(const Topo& t, const DevObstacle& o
   }
 }
 
-// One lane's impulse straight into the obstacle's bins (divergent control
-// flow allowed): G2P's particle-side reactions on obstacles past the first
-// kMaxReactObstacles of a scene, which have no LDS columns.  Integer adds, so
-// the totals still do not depend on order.
-__device__ __forceinline__ void add_reaction_lane(const Topo& t, const DevObstacle& o, uint32_t oidx, uint32_t bin,
-                                                  float3 lin, float3 ang) {
+// One particle's impulse in the obstacle's int64 units (momentum * 2^rshift),
+// rounded to nearest: G2P's particle-side reactions.  Per-particle integers
+// sum to the same total in any order, so which lane or wave takes a particle
+// (the workgroup size, the storage order) changes no reaction.  |value| stays
+// below 2^45 (the bound of the wave sums above), well inside fixed64_round's.
+__device__ __forceinline__ void reaction_fixed(const DevObstacle& o, float3 lin, float3 ang,
+                                               unsigned long long r[6]) {
+  const double wl = (double)o.rscale_lin * 0x1p32, wa = (double)o.rscale_ang * 0x1p32;
+  r[0] = fixed64_round(lin.x, wl);
+  r[1] = fixed64_round(lin.y, wl);
+  r[2] = fixed64_round(lin.z, wl);
+  r[3] = fixed64_round(ang.x, wa);
+  r[4] = fixed64_round(ang.y, wa);
+  r[5] = fixed64_round(ang.z, wa);
+}
+
+// One lane's rounded impulse straight into the obstacle's bins (divergent
+// control flow allowed): obstacles past the first kMaxReactObstacles of a
+// scene, which have no workgroup sums in G2P.
+__device__ __forceinline__ void add_reaction_lane(const Topo& t, uint32_t oidx, uint32_t bin,
+                                                  const unsigned long long r[6]) {
   unsigned long long* dst = reinterpret_cast<unsigned long long*>(t.react_bins) + ((size_t)oidx * kReactBins + bin) * 6;
-  const float c[6] = {lin.x, lin.y, lin.z, ang.x, ang.y, ang.z};
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    atomicAdd(&dst[k], fixed64_of(c[k] * o.rscale_lin));
-    atomicAdd(&dst[3 + k], fixed64_of(c[3 + k] * o.rscale_ang));
-  }
+  for (int c = 0; c < 6; ++c)
+    if (r[c]) atomicAdd(&dst[c], r[c]);
 }
 
 template <int FIXED, int REACT, bool SHAPES>
@@ -1746,15 +1760,14 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // updated in place.
 template <int REACT, bool SHAPES, bool PLASTIC, bool MOVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && !PLASTIC ? 5 : 1))) void k_g2p(
-    Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles,
-    int react_cols) {
+    Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
-  // Reaction columns (REACT): 6 floats per thread for each of a scene's first
-  // react_cols obstacles, sized at launch by the session's largest obstacle
-  // count (at most kMaxReactObstacles) so that the LDS they take does not cap
-  // occupancy for the 1-4 obstacles scenes usually have.
-  extern __shared__ float racc[];
+  // Particle-side reactions (REACT) on a scene's first kMaxReactObstacles
+  // obstacles: each contact adds its rounded impulse (reaction_fixed) into
+  // these workgroup sums, which go to the obstacle's bins once per block.
+  constexpr int kReactSums = REACT ? kMaxReactObstacles * 6 : 1;
+  __shared__ unsigned long long rsum[kReactSums];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   constexpr bool move = MOVE;  // a separate variant: the plain one keeps its registers
   const Parts o = move ? q : p;  // where updated particles are written
@@ -1782,6 +1795,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     const uint32_t slot0 = threadIdx.x < 8 ? t.pblock_slot8[8 * k + threadIdx.x] : 0u;
     const uint32_t i_first = threadIdx.x < len ? t.sorted[start + threadIdx.x] : 0u;
     if (threadIdx.x < 8) s_slot[threadIdx.x] = slot0;
+    if (REACT)  // the previous block's sums were flushed before the loop's last barrier
+      for (int q = threadIdx.x; q < kReactSums; q += blockDim.x) rsum[q] = 0ull;
     __syncthreads();
     const uint32_t own = s_slot[0];
     const int4 key = own != kInvalid ? t.grid_key[own] : make_int4(0, 0, 0, 0);
@@ -1799,9 +1814,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     if (own != kInvalid) {
       const DevScene& sc = scenes[(uint32_t)key.w >> 16];
       const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
-      const int nobs = REACT ? min(sc.obstacle_count, react_cols) : 0;
-      if (REACT)
-        for (int q = 0; q < nobs * 6; ++q) racc[q * blockDim.x + threadIdx.x] = 0.0f;
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
       bool failed = false, bad = false;
       for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
@@ -1919,17 +1931,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
             if (REACT) {
               const float3 J = (before - v) * pm.w;
               const float3 L = cross3(x - nrm * radius - make_float3(o.center[0], o.center[1], o.center[2]), J);
-              if (oi < nobs) {
-                const uint32_t B = blockDim.x;
-                float* acc = racc + oi * 6 * B + threadIdx.x;
-                acc[0] += J.x;
-                acc[B] += J.y;
-                acc[2 * B] += J.z;
-                acc[3 * B] += L.x;
-                acc[4 * B] += L.y;
-                acc[5 * B] += L.z;
+              unsigned long long r[6];
+              reaction_fixed(o, J, L, r);
+              if (oi < kMaxReactObstacles) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                  if (r[c]) atomicAdd(&rsum[oi * 6 + c], r[c]);
               } else {
-                add_reaction_lane(t, o, sc.obstacle_begin + oi, k % kReactBins, J, L);
+                add_reaction_lane(t, sc.obstacle_begin + oi, k % kReactBins, r);
               }
             }
           }
@@ -1992,15 +2001,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
       }
       flag_scene_error(t, (uint32_t)key.w >> 16, failed, ERR_RESIDENT);
       flag_error(t.counters, bad, ERR_INTERNAL);
-      if (REACT) {
-        for (int oi = 0; oi < nobs; ++oi) {
-          // each wavefront sums its own lanes' columns (add_reaction)
-          const uint32_t B = blockDim.x;
-          const float* acc = racc + oi * 6 * B + threadIdx.x;
-          const float3 J = make_float3(acc[0], acc[B], acc[2 * B]);
-          const float3 L = make_float3(acc[3 * B], acc[4 * B], acc[5 * B]);
-          const bool any = J.x != 0.f || J.y != 0.f || J.z != 0.f || L.x != 0.f || L.y != 0.f || L.z != 0.f;
-          if (__any(any)) add_reaction(t, obstacles[sc.obstacle_begin + oi], sc.obstacle_begin + oi, k % kReactBins, J, L);
+      if (REACT) {  // own is uniform over the workgroup: every thread reaches the barrier
+        __syncthreads();
+        const int sums = min(sc.obstacle_count, kMaxReactObstacles) * 6;
+        unsigned long long* bins = reinterpret_cast<unsigned long long*>(t.react_bins);
+        for (int q = threadIdx.x; q < sums; q += blockDim.x) {
+          const unsigned long long v = rsum[q];
+          if (v) atomicAdd(&bins[((size_t)(sc.obstacle_begin + q / 6) * kReactBins + k % kReactBins) * 6 + q % 6], v);
         }
       }
     }
@@ -2387,13 +2394,10 @@ void launch_grid_update(const Topo& t, const DevScene* sc, const DevObstacle* o,
 template <bool PLASTIC>
 static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
                         const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes,
-                        int react_cols, hipStream_t s) {
-  // REACT: 6 LDS floats per thread for each reaction column
+                        hipStream_t s) {
   const bool mv = q.pos_mass != nullptr;
-  const int cols = react ? std::max(1, std::min(react_cols, kMaxReactObstacles)) : 0;
-  const size_t lds = (size_t)cols * 6 * threads * sizeof(float);
 #define TMPM_G2P(R, SH, M, T) \
-  hipLaunchKernelGGL((k_g2p<R, SH, PLASTIC, M>), dim3(blocks), dim3(T), lds, s, p, q, t, c, sc, o, cols)
+  hipLaunchKernelGGL((k_g2p<R, SH, PLASTIC, M>), dim3(blocks), dim3(T), 0, s, p, q, t, c, sc, o)
 #define TMPM_G2P_M(R, SH, T)          \
   do {                                \
     if (mv) TMPM_G2P(R, SH, true, T); \
@@ -2411,9 +2415,9 @@ static void g2p_variant(const Parts& p, const Parts& q, const Topo& t, const Dev
 }
 void launch_g2p(const Parts& p, const Parts& q, const Topo& t, const DevClass* c, const DevScene* sc,
                 const DevObstacle* o, uint32_t blocks, uint32_t threads, int react, int shapes, int plastic,
-                int react_cols, hipStream_t s) {
-  if (plastic) g2p_variant<true>(p, q, t, c, sc, o, blocks, threads, react, shapes, react_cols, s);
-  else g2p_variant<false>(p, q, t, c, sc, o, blocks, threads, react, shapes, react_cols, s);
+                hipStream_t s) {
+  if (plastic) g2p_variant<true>(p, q, t, c, sc, o, blocks, threads, react, shapes, s);
+  else g2p_variant<false>(p, q, t, c, sc, o, blocks, threads, react, shapes, s);
 }
 void launch_react_finalize(const Topo& t, const DevObstacle* o, hipStream_t s) {
   hipLaunchKernelGGL(k_react_finalize, dim3(grid_for(t.obstacle_total, 64, 1024)), dim3(64), 0, s, t, o);

@@ -643,7 +643,6 @@ struct tmpm_session {
   bool reactions_on = false;
   DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
   uint32_t obstacle_total = 0, react_cap = 0, body_total = 0;
-  uint32_t max_scene_obstacles = 0;  // largest obstacle count of one scene (G2P's reaction columns)
   bool react_active() const { return reactions_on || body_total > 0; }
   std::vector<DevClass> host_classes;
   uint32_t substeps_since_reorder = 1u << 30;
@@ -869,7 +868,6 @@ double obstacle_reach(const tmpm_obstacle& o, double h) {
 
 // Obstacles of all active scenes, concatenated; each scene keeps its range.
 tmpm_status upload_obstacles(tmpm_session* s) {
-  uint32_t max_obstacles = 0;
   std::vector<DevObstacle> packed;
   std::vector<DevMapping> maps;
   std::vector<float4> planes;
@@ -919,9 +917,7 @@ tmpm_status upload_obstacles(tmpm_session* s) {
     }
     scene.dev.obstacle_count = (int32_t)(packed.size() - scene.dev.obstacle_begin);
     if (scene.dev.obstacle_count > kMaxObstacles) return fail(TMPM_CAPACITY, "too many obstacles");
-    max_obstacles = std::max<uint32_t>(max_obstacles, (uint32_t)scene.dev.obstacle_count);
   }
-  s->max_scene_obstacles = max_obstacles;
   if (packed.size() > s->obstacle_cap || !s->obstacles.ptr) {
     s->obstacle_cap = std::max<uint32_t>(16, (uint32_t)packed.size());
     HIP_TRY(s->obstacles.alloc(sizeof(DevObstacle) * s->obstacle_cap));
@@ -1706,17 +1702,16 @@ struct SubstepPlan {
   const DevMapping* maps;
   const uint32_t* body_scene;
   uint32_t obstacle_total, body_total, p2g_blocks, p2g_threads, g2p_threads;
-  int32_t react, fixed64, shapes, plastic, cpic, copies, react_cols;
+  int32_t react, fixed64, shapes, plastic, cpic, copies;
   uint8_t reorder, clear_table, rigid_sync, rigid_apply;
 };
 
-// G2P workgroup size of the next substep.  Without reactions every size
-// computes the same result, so sessions of fewer than 512 particle blocks
-// default to 256 threads (one pass over a block instead of two; ~10 us of ~80
-// per substep at 8,192 particles).  With reactions the particle-side sums
-// depend on which particles share a wave, so the configured size is kept.
+// G2P workgroup size of the next substep.  Every size computes the same
+// result (reactions included: their particle-side sums are integers), so
+// sessions of fewer than 512 particle blocks default to 256 threads (one pass
+// over a block instead of two; ~10 us of ~80 per substep at 8,192 particles).
 uint32_t effective_g2p_threads(const tmpm_session* s) {
-  if (s->g2p_shape_fixed || s->react_active() || s->last_pblocks >= 512) return s->threads_pblock;
+  if (s->g2p_shape_fixed || s->last_pblocks >= 512) return s->threads_pblock;
   return 256;
 }
 
@@ -1745,7 +1740,6 @@ SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
   pl.plastic = s->plastic ? 1 : 0;
   pl.cpic = s->cpic;
   pl.copies = s->p2g_copies;
-  pl.react_cols = (int32_t)s->max_scene_obstacles;
   pl.clear_table = s->table_dirty;
   pl.rigid_sync = s->body_total > 0 && s->obstacle_total > 0;
   pl.rigid_apply = pl.react && s->body_total > 0;
@@ -1786,7 +1780,7 @@ hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
   if ((e = timed(s, "k_g2p", [&] {
          // on a reorder substep G2P also moves the state into block order in q
          launch_g2p(p, pl.q, t, pl.c, pl.sc, pl.ob, pl.p2g_blocks, pl.g2p_threads, pl.react, pl.shapes, pl.plastic,
-                    pl.react_cols, st);
+                    st);
        })) != hipSuccess)
     return e;
   if (pl.react && (e = timed(s, "k_react_finalize", [&] { launch_react_finalize(t, pl.ob, st); })) != hipSuccess)
@@ -3288,11 +3282,9 @@ tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]) {
 }
 
 // Launch-shape autotuner.  Every candidate shape computes the same results:
-// P2G sums integers (order-free), G2P is per particle, and the storage order
-// a reorder produces changes no sum.  The one exception is the particle side
-// of obstacle reactions, whose f32 per-lane partial sums follow the storage
-// order and G2P's workgroup size; with reactions on, those two knobs stay
-// fixed.  The trials therefore run real substeps of the simulation, one knob
+// P2G sums integers (order-free), G2P is per particle and sums its reaction
+// impulses as rounded integers, and the storage order a reorder produces
+// changes no sum.  The trials therefore run real substeps of the simulation, one knob
 // at a time (P2G shape, G2P workgroup, reorder interval, graphs), each
 // candidate timed over `trial` substeps with events on the session stream.
 tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run) {
@@ -3339,19 +3331,18 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
       apply(best);
       return TMPM_OK;
     };
-    const bool order_free = !s->react_active();
     st = run(std::min<uint32_t>(trial, 16), nullptr);  // warm-up
     if (st == TMPM_OK) {
       const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
       st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
       s->p2g_shape_fixed = true;
     }
-    if (st == TMPM_OK && order_free) {
+    if (st == TMPM_OK) {
       s->threads_pblock = effective_g2p_threads(s);  // trials start from the size in use
       s->g2p_shape_fixed = true;
       st = tune({64, 128, 256}, [&](int v) { s->threads_pblock = (uint32_t)v; }, (int)s->threads_pblock);
     }
-    if (st == TMPM_OK && order_free)
+    if (st == TMPM_OK)
       st = tune({4, 8, 16}, [&](int v) { s->reorder_interval = (uint32_t)v; }, (int)s->reorder_interval);
     if (st == TMPM_OK) st = tune({0, 1}, [&](int v) { s->use_graphs = v != 0; }, s->use_graphs ? 1 : 0);
     if (!s->use_graphs) destroy_graphs(s);

@@ -96,10 +96,36 @@ def test_reactions_are_summed_over_the_synchronized_substeps_and_reproducible(tr
 
 
 @pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_reactions_do_not_depend_on_the_g2p_workgroup_or_storage_order(transfers, mode, monkeypatch):
+    # Each particle's impulse is rounded to the bins' integer units before it is
+    # summed, so G2P's workgroup size (which lane takes which particle) and the
+    # reorder interval (the storage order) change no reaction bit.
+    w = cube_on_box(seed=5, boundary=ObstacleBoundary.STICK, n_axis=16)
+    dt = min(5.0e-5, 0.5 * w.stable_timestep())
+    runs = []
+    for threads, interval in [(64, 16), (256, 16), (128, 3)]:
+        monkeypatch.setenv("TMPM_PBLOCK_THREADS", str(threads))
+        monkeypatch.setenv("TMPM_REORDER_INTERVAL", str(interval))
+        s = HipMpmResidentSession(transfers, w, dt, mode)
+        assert s.launch_config()["g2p_threads"] == threads
+        s.set_reactions(True)
+        got = []
+        for _ in range(20):
+            s.step(1)
+            got.append(s.reactions())
+        runs.append((np.stack(got), s.read_particles()))
+    assert np.abs(runs[0][0][:, 0, 1]).max() > 0.0  # the cube hit the box
+    for reactions, pa in runs[1:]:
+        assert np.array_equal(reactions, runs[0][0])
+        for f in ("position", "velocity", "affine", "deformation"):
+            assert np.array_equal(getattr(pa, f), getattr(runs[0][1], f)), f
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
 def test_reactions_support_more_than_sixteen_obstacles(transfers, mode):
     # gpu.rs:2651-2720: 16 far spheres and one the particle hits; reactions are
     # kept for every obstacle, the far ones get none, the hit one matches the
-    # CPU.  Obstacles past the 8th of a scene take the direct-atomic path.
+    # CPU.  Obstacles past the 16th of a scene take the direct-atomic path.
     particle = MpmParticle.new([1.02, 0.0, 0.0], 0.04, 1000.0, MaterialModel.elastic(1000.0, 0.2))
     particle.velocity = np.array([-1.0, 0.0, 0.0])
     w = MpmWorld([particle], MpmParams(gravity=np.zeros(3)))

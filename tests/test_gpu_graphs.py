@@ -102,13 +102,22 @@ def test_autotuned_launch_shape_computes_the_same_substeps(transfers, mode):
     assert tuned.substeps == plain.substeps
 
 
-def test_autotune_keeps_the_order_dependent_knobs_with_reactions_on(transfers):
+def test_autotuned_launch_shape_with_rigid_reactions_computes_the_same_substeps(transfers):
+    # reactions sum rounded per-particle integers, so the tuner may change every
+    # knob with rigid bodies coupled and the bodies still move bit for bit alike
     req = request(4, n=8000, boxes=1)
-    s = HipMpmResidentSession(transfers, req.world, req.dt)
-    s.set_rigid_bodies(list(req.bodies), list(req.mappings), gravity=req.gravity)
-    before = s.launch_config()
-    cfg = s.autotune(8)
-    assert cfg["g2p_threads"] == before["g2p_threads"]
-    assert cfg["reorder_interval"] == before["reorder_interval"]
-    s.synchronize()
-    assert s.substeps == cfg["substeps_run"]
+    tuned = HipMpmResidentSession(transfers, req.world, req.dt)
+    tuned.set_rigid_bodies(list(req.bodies), list(req.mappings), gravity=req.gravity)
+    cfg = tuned.autotune(8)
+    tuned.synchronize()
+    assert tuned.substeps == cfg["substeps_run"]
+    plain = HipMpmResidentSession(transfers, req.world, req.dt)
+    plain.set_rigid_bodies(list(req.bodies), list(req.mappings), gravity=req.gravity)
+    left = cfg["substeps_run"]
+    while left:
+        k = min(left, 64)
+        plain.step(k)
+        left -= k
+    same(tuned.read_particles(), plain.read_particles(), "tuned")
+    for x, y in zip(tuned.rigid_bodies(), plain.rigid_bodies()):
+        assert np.array_equal(x.position, y.position) and np.array_equal(x.angular_velocity, y.angular_velocity)
