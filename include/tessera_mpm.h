@@ -395,8 +395,10 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
 tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run);
 /* The current launch shape: {P2G threads, P2G tile copies, G2P threads,
  * reorder interval, graphs on}.  Before tuning, the shape follows the
- * session's size: under 512 particle blocks P2G runs 128 threads over one
- * tile copy and G2P runs 256 threads. */
+ * particle-block count of the last synchronized substep: from 8,192 blocks
+ * P2G runs 64 threads over two tile copies and G2P 64 threads; below that
+ * P2G runs 128 threads (one tile copy under 512 blocks) and G2P 128 threads
+ * (256 under 4,096 blocks), so that small launches still fill the chip. */
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);

@@ -1639,17 +1639,6 @@ tmpm_status prepare_packed(tmpm_session* s, PackedParticles& pk) {
   count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
   s->last_pblocks = (uint32_t)pblocks;
   s->last_gblocks = (uint32_t)gblocks;
-  // Default P2G shape by size (any shape computes the same result): with
-  // fewer particle blocks than ~2 per CU, each block's serial chain sets the
-  // substep time and 128 threads over one tile copy shorten it (8,192
-  // particles in ~100 blocks: ~8 us of ~85 per substep); with many blocks, 64
-  // threads over two copies keep the LDS atomics apart (1M particles: 76 vs
-  // 86 us for P2G).
-  if (!s->p2g_shape_fixed) {
-    const bool few = pblocks < 512;
-    s->p2g_threads = few ? 128 : 64;
-    s->p2g_copies = few ? 1 : 2;
-  }
   // CPIC splits blocks that straddle a triangle by side: more headroom
   const uint64_t room = s->cpic ? 8 : 4;
   uint64_t pcap = std::max<uint64_t>(1024, room * pblocks);
@@ -1706,13 +1695,36 @@ struct SubstepPlan {
   uint8_t reorder, clear_table, rigid_sync, rigid_apply;
 };
 
-// G2P workgroup size of the next substep.  Every size computes the same
-// result (reactions included: their particle-side sums are integers), so
-// sessions of fewer than 512 particle blocks default to 256 threads (one pass
-// over a block instead of two; ~10 us of ~80 per substep at 8,192 particles).
+// Default launch shapes, from the particle-block count of the last
+// synchronized substep (every shape computes the same result, reactions
+// included: their particle-side sums are integers).  P2G and G2P run one
+// workgroup per block, so with 64 threads a launch has one wave per block and
+// each wave walks its block's particles serially.  Below kFillBlocks blocks
+// that leaves the chip short of waves, and wider workgroups split each block:
+//   G2P: 64 threads from kFillBlocks blocks, 128 from half that, else 256
+//     (a 3,123-block serving batch: 112 us at 64, 81 at 256; 1M particles
+//     in 10,976 blocks: 64 is fastest);
+//   P2G: 64 threads over two tile copies from kFillBlocks blocks (1M
+//     particles: 76 vs 86 us at 128), else 128 threads (the serving batch:
+//     84 vs 89 us), over one copy below 512 blocks (8,192 particles in ~100
+//     blocks: ~8 us of ~85 per substep).
+constexpr uint32_t kFillBlocks = 8192;  // 32 waves per CU of MI355X's 256
+
 uint32_t effective_g2p_threads(const tmpm_session* s) {
-  if (s->g2p_shape_fixed || s->last_pblocks >= 512) return s->threads_pblock;
-  return 256;
+  if (s->g2p_shape_fixed) return s->threads_pblock;
+  const uint32_t b = s->last_pblocks;
+  return b >= kFillBlocks ? 64u : b >= kFillBlocks / 2 ? 128u : 256u;
+}
+
+void effective_p2g_shape(const tmpm_session* s, uint32_t& threads, int& copies) {
+  if (s->p2g_shape_fixed) {
+    threads = s->p2g_threads;
+    copies = s->p2g_copies;
+    return;
+  }
+  const uint32_t b = s->last_pblocks;
+  threads = b >= kFillBlocks ? 64u : 128u;
+  copies = b >= 512 ? 2 : 1;
 }
 
 SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
@@ -1732,14 +1744,13 @@ SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
   pl.obstacle_total = s->obstacle_total;
   pl.body_total = s->body_total;
   pl.p2g_blocks = s->p2g_blocks;
-  pl.p2g_threads = s->p2g_threads;
+  effective_p2g_shape(s, pl.p2g_threads, pl.copies);
   pl.g2p_threads = effective_g2p_threads(s);
   pl.react = s->react_active() && s->obstacle_total > 0 ? 1 : 0;
   pl.fixed64 = s->fixed64 ? 1 : 0;
   pl.shapes = s->shapes;
   pl.plastic = s->plastic ? 1 : 0;
   pl.cpic = s->cpic;
-  pl.copies = s->p2g_copies;
   pl.clear_table = s->table_dirty;
   pl.rigid_sync = s->body_total > 0 && s->obstacle_total > 0;
   pl.rigid_apply = pl.react && s->body_total > 0;
@@ -3270,8 +3281,11 @@ tmpm_status tmpm_session_set_timing(tmpm_session* s, int32_t enabled) {
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]) {
   try {
     if (!s || !out) return fail(TMPM_INVALID_INPUT, "null argument");
-    out[0] = (int32_t)s->p2g_threads;
-    out[1] = s->p2g_copies;
+    uint32_t p2g_threads;
+    int p2g_copies;
+    effective_p2g_shape(s, p2g_threads, p2g_copies);
+    out[0] = (int32_t)p2g_threads;
+    out[1] = p2g_copies;
     out[2] = (int32_t)effective_g2p_threads(s);
     out[3] = (int32_t)s->reorder_interval;
     out[4] = s->use_graphs ? 1 : 0;
@@ -3333,9 +3347,10 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
     };
     st = run(std::min<uint32_t>(trial, 16), nullptr);  // warm-up
     if (st == TMPM_OK) {
+      effective_p2g_shape(s, s->p2g_threads, s->p2g_copies);  // trials start from the shape in use
+      s->p2g_shape_fixed = true;  // the trials' shapes, not the size-based default, run from here on
       const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
       st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
-      s->p2g_shape_fixed = true;
     }
     if (st == TMPM_OK) {
       s->threads_pblock = effective_g2p_threads(s);  // trials start from the size in use
