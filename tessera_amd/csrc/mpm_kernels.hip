@@ -1634,17 +1634,27 @@ __device__ __forceinline__ void add_reaction(const Topo& t, const DevObstacle& o
 // One particle's impulse in the obstacle's int64 units (momentum * 2^rshift),
 // rounded to nearest: G2P's particle-side reactions.  Per-particle integers
 // sum to the same total in any order, so which lane or wave takes a particle
-// (the workgroup size, the storage order) changes no reaction.  |value| stays
-// below 2^45 (the bound of the wave sums above), well inside fixed64_round's.
+// (the workgroup size, the storage order) changes no reaction.  The scale
+// maps the heaviest particle at 64 cells per substep to 2^43 units; an
+// impulse past 2^51 units (a surface over 16,000 cells per substep faster
+// than the particle) saturates there instead of wrapping.  v * 2^rshift is
+// exact in f64 (a power-of-two scale), so clamping it first and adding the
+// rounding constant rounds once, like fixed64_round.
+__device__ __forceinline__ unsigned long long reaction_unit(float v, double w) {
+  constexpr double kMax = 0x1p51 - 1.0;
+  const double x = fmin(fmax((double)v * w, -kMax), kMax);
+  return (unsigned long long)__double_as_longlong(x + 0x1.8p52) - 0x4338000000000000ull;
+}
+
 __device__ __forceinline__ void reaction_fixed(const DevObstacle& o, float3 lin, float3 ang,
                                                unsigned long long r[6]) {
   const double wl = (double)o.rscale_lin * 0x1p32, wa = (double)o.rscale_ang * 0x1p32;
-  r[0] = fixed64_round(lin.x, wl);
-  r[1] = fixed64_round(lin.y, wl);
-  r[2] = fixed64_round(lin.z, wl);
-  r[3] = fixed64_round(ang.x, wa);
-  r[4] = fixed64_round(ang.y, wa);
-  r[5] = fixed64_round(ang.z, wa);
+  r[0] = reaction_unit(lin.x, wl);
+  r[1] = reaction_unit(lin.y, wl);
+  r[2] = reaction_unit(lin.z, wl);
+  r[3] = reaction_unit(ang.x, wa);
+  r[4] = reaction_unit(ang.y, wa);
+  r[5] = reaction_unit(ang.z, wa);
 }
 
 // One lane's rounded impulse straight into the obstacle's bins (divergent
