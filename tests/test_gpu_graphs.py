@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from tessera_amd.mpm import HipMpmBatch, HipMpmResidentSession, MaterialModel, MpmParticle, ParticleArrays
-from tessera_amd.scenes import serving_request, sticky_floor
+from tessera_amd.scenes import four_cubes, serving_request, sticky_floor, uniform_cube
 
 pytestmark = pytest.mark.gpu
 
@@ -121,3 +121,18 @@ def test_autotuned_launch_shape_with_rigid_reactions_computes_the_same_substeps(
     same(tuned.read_particles(), plain.read_particles(), "tuned")
     for x, y in zip(tuned.rigid_bodies(), plain.rigid_bodies()):
         assert np.array_equal(x.position, y.position) and np.array_equal(x.angular_velocity, y.angular_velocity)
+
+
+def test_default_launch_shape_follows_the_particle_block_count(transfers):
+    # one workgroup per particle block: few blocks take wider workgroups so the
+    # launch still fills the chip; reactions do not change the choice
+    small = uniform_cube()  # 8,192 particles in 64 blocks
+    s = HipMpmResidentSession(transfers, small.world, small.dt)
+    cfg = s.launch_config()
+    assert (cfg["p2g_threads"], cfg["p2g_copies"], cfg["g2p_threads"]) == (128, 1, 256)
+    s.set_reactions(True)
+    assert s.launch_config()["g2p_threads"] == 256
+    big = four_cubes()  # 1,048,576 particles in ~11,000 blocks
+    b = HipMpmResidentSession(transfers, big.world, big.dt)
+    cfg = b.launch_config()
+    assert (cfg["p2g_threads"], cfg["p2g_copies"], cfg["g2p_threads"]) == (64, 2, 64)
