@@ -1753,9 +1753,10 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // ---------------------------------------------------------------------------
 // 10. G2P (world.rs:419-464; gpu.wgsl:821-1019)
 // ---------------------------------------------------------------------------
-// REACT = 1 runs with one 64-lane wavefront per workgroup: each lane adds its
-// particles' obstacle impulses into its own LDS column (no atomics), and the
-// columns are reduced in a fixed order at the end of the block.
+// REACT = 1 also records the particle-side obstacle reactions: each contact's
+// impulse, rounded to the obstacle's int64 units, goes into per-workgroup LDS
+// sums that are added to the obstacle's bins once per block.  Any workgroup
+// size computes the same bits.
 // G2P waits on memory most of its time, so waves per SIMD matter: the plain
 // variants fit 96 VGPRs without spilling (5 waves; 64 us instead of 68 on
 // the 1M scene at 4 waves), while shapes and plastics would spill there.
