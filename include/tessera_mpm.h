@@ -395,10 +395,13 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
 tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run);
 /* The current launch shape: {P2G threads, P2G tile copies, G2P threads,
  * reorder interval, graphs on}.  Before tuning, the shape follows the
- * particle-block count of the last synchronized substep: from 8,192 blocks
- * P2G runs 64 threads over two tile copies and G2P 64 threads; below that
- * P2G runs 128 threads (one tile copy under 512 blocks) and G2P 128 threads
- * (256 under 4,096 blocks), so that small launches still fill the chip. */
+ * particle-block count of the last synchronized substep and its mean
+ * particles per block: from 8,192 blocks of fewer than 192 particles P2G
+ * runs 64 threads over two tile copies and G2P 64 threads.  Fewer blocks or
+ * denser ones take wider workgroups, so that a launch still fills the chip:
+ * P2G 128 threads (one tile copy under 512 blocks); G2P 128 threads below
+ * 8,192 blocks or from 192 particles per block, 256 below 4,096 blocks or
+ * from 288 particles per block. */
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);

@@ -1696,24 +1696,34 @@ struct SubstepPlan {
 };
 
 // Default launch shapes, from the particle-block count of the last
-// synchronized substep (every shape computes the same result, reactions
-// included: their particle-side sums are integers).  P2G and G2P run one
-// workgroup per block, so with 64 threads a launch has one wave per block and
-// each wave walks its block's particles serially.  Below kFillBlocks blocks
-// that leaves the chip short of waves, and wider workgroups split each block:
-//   G2P: 64 threads from kFillBlocks blocks, 128 from half that, else 256
-//     (a 3,123-block serving batch: 112 us at 64, 81 at 256; 1M particles
-//     in 10,976 blocks: 64 is fastest);
-//   P2G: 64 threads over two tile copies from kFillBlocks blocks (1M
-//     particles: 76 vs 86 us at 128), else 128 threads (the serving batch:
-//     84 vs 89 us), over one copy below 512 blocks (8,192 particles in ~100
-//     blocks: ~8 us of ~85 per substep).
+// synchronized substep and the mean particles per block (every shape computes
+// the same result, reactions included: their particle-side sums are
+// integers).  P2G and G2P run one workgroup per block, so with 64 threads a
+// launch has one wave per block and each wave walks its block's particles
+// serially.  Wider workgroups split a block when there are too few blocks to
+// fill the chip (below kFillBlocks) or many particles per block:
+//   G2P: the wider of 64 / 128 (below kFillBlocks) / 256 (below half of it)
+//     and 64 / 128 (from 192 particles per block) / 256 (from 288).
+//     Measured: 1M particles in 10,976 blocks of 96: 64 is fastest; 5.3M in
+//     21,187 blocks of 252: 390 us at 128, 400 at 64 and 256; serving
+//     batches of 3,123 blocks of 313 and 8,916 of 345: 112 -> 81 and
+//     326 -> 284 us from 64 to 256 threads.
+//   P2G: 128 threads below kFillBlocks or from 192 particles per block, else
+//     64 (1M particles: 76 vs 86 us at 128; the batches above: 1-4% faster
+//     at 128); two tile copies from 512 blocks (8,192 particles in ~100
+//     blocks: one copy saves ~8 us of ~85 per substep).
 constexpr uint32_t kFillBlocks = 8192;  // 32 waves per CU of MI355X's 256
+
+uint32_t particles_per_block(const tmpm_session* s) {
+  return s->last_pblocks ? (uint32_t)(s->n / s->last_pblocks) : 0u;
+}
 
 uint32_t effective_g2p_threads(const tmpm_session* s) {
   if (s->g2p_shape_fixed) return s->threads_pblock;
-  const uint32_t b = s->last_pblocks;
-  return b >= kFillBlocks ? 64u : b >= kFillBlocks / 2 ? 128u : 256u;
+  const uint32_t b = s->last_pblocks, ppb = particles_per_block(s);
+  const uint32_t by_blocks = b >= kFillBlocks ? 64u : b >= kFillBlocks / 2 ? 128u : 256u;
+  const uint32_t by_density = ppb >= 288 ? 256u : ppb >= 192 ? 128u : 64u;
+  return std::max(by_blocks, by_density);
 }
 
 void effective_p2g_shape(const tmpm_session* s, uint32_t& threads, int& copies) {
@@ -1723,7 +1733,7 @@ void effective_p2g_shape(const tmpm_session* s, uint32_t& threads, int& copies) 
     return;
   }
   const uint32_t b = s->last_pblocks;
-  threads = b >= kFillBlocks ? 64u : 128u;
+  threads = b < kFillBlocks || particles_per_block(s) >= 192 ? 128u : 64u;
   copies = b >= 512 ? 2 : 1;
 }
 
