@@ -193,6 +193,14 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s);
  * the particle state back or validating it (no reference counterpart; the
  * throughput benchmark uses it to time device-resident stepping alone). */
 tmpm_status tmpm_session_wait(tmpm_session* s);
+/* `synchronize_async()` (gpu.rs:413-450) as a poll: while the device still
+ * runs the submitted substeps, returns TMPM_OK with *done_out = 0 and changes
+ * nothing (the substeps stay pending; a caller may stop polling, as the
+ * reference's cancellation keeps them pending too).  Once they have finished
+ * it validates and commits exactly as tmpm_session_synchronize and sets
+ * *done_out = 1, also when that fails (the failure invalidates the session
+ * and keeps the last world).  No pending substeps: *done_out = 1. */
+tmpm_status tmpm_session_synchronize_async(tmpm_session* s, int32_t* done_out);
 /* `step(steps)` (gpu.rs:286-292) = submit_steps + synchronize. */
 tmpm_status tmpm_session_step(tmpm_session* s, uint32_t steps);
 /* `MpmWorld::step_with_gpu_transfers(dt)` (gpu.rs:2208-2240): CFL-limited
@@ -358,6 +366,19 @@ tmpm_status tmpm_batch_read_obstacles(const tmpm_session* s, uint32_t scene, tmp
 /* Admits a new scene (serving): it takes the first free slot. */
 tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, double substep_dt,
                                  uint32_t* scene_out);
+/* Admission in two halves, so that the host work of an admission can run
+ * while a batch has substeps in flight (no reference counterpart: the
+ * reference has no batch).  tmpm_batch_stage_scene does the host half of
+ * tmpm_batch_add_scene without any session or device: validation, the f64
+ * copy of the world, the f32 packing and the block count; it may run on any
+ * thread, concurrently with calls on sessions.  tmpm_batch_add_staged admits
+ * the staged scene into a batch with no pending substeps (the device half:
+ * uploads and capacity); it consumes the staged scene, also on failure.
+ * Results equal tmpm_batch_add_scene with the same world. */
+typedef struct tmpm_staged_scene tmpm_staged_scene;
+tmpm_status tmpm_batch_stage_scene(const tmpm_world_desc* world, double substep_dt, tmpm_staged_scene** out);
+tmpm_status tmpm_batch_add_staged(tmpm_session* s, tmpm_staged_scene* staged, uint32_t* scene_out);
+void tmpm_staged_scene_destroy(tmpm_staged_scene* staged);
 /* Retires a scene; its slot becomes free. */
 tmpm_status tmpm_batch_remove_scene(tmpm_session* s, uint32_t scene);
 
@@ -391,7 +412,10 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
  * time, and keeps the fastest.  The trials advance the session like
  * tmpm_session_submit_steps (they become pending substeps; *substeps_run
  * says how many) and compute exactly what any shape computes, obstacle
- * reactions included. */
+ * reactions included.  They are queued 64 substeps at a time; between
+ * chunks the topology grows as synchronize grows it, and an error stops the
+ * tuning (the next synchronize reports it, as for untuned stepping).  A
+ * bucket tuned before runs no trials (see tmpm_tune_cache_clear). */
 tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* substeps_run);
 /* The current launch shape: {P2G threads, P2G tile copies, G2P threads,
  * reorder interval, graphs on}.  Before tuning, the shape follows the
@@ -405,6 +429,31 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);
+
+/* FP64 audit of every substep (the reference's momentum balance check,
+ * world.rs:784-807, computes such sums on the host).  While enabled (set
+ * with no pending substeps; off by default), each substep ends with a device
+ * reduction in f64 over each scene's particles, and synchronize keeps the
+ * records of the substeps it synchronized.  A record is 12 doubles: mass;
+ * momentum xyz; kinetic energy; first moment of mass xyz (sum of m x, so
+ * the gravitational potential energy is -g . that); angular momentum about
+ * the origin xyz (sum of m x cross v); particles counted.  Sums are f64 but
+ * their order follows the scheduler.  read_audit writes up to cap_records
+ * records of `scene` (0 for a single session) in substep order and the
+ * number available through count_out. */
+tmpm_status tmpm_session_set_audit(tmpm_session* s, int32_t enabled);
+tmpm_status tmpm_session_read_audit(const tmpm_session* s, uint32_t scene, double* out, uint64_t cap_records,
+                                    uint64_t* count_out);
+
+/* Autotuner cache (no reference counterpart): tmpm_session_autotune keeps
+ * the shape it picks per workload bucket (device; particle count and
+ * particles per block, each rounded down to a power of two; reactions,
+ * plastic materials, obstacle shapes, P2G mode, batch, CPIC).  A later
+ * autotune in a tuned bucket runs no trials, and a session whose shape is not
+ * fixed launches with the cached shape of its current bucket.  Process-wide;
+ * with TMPM_TUNE_CACHE naming a file, also loaded from and appended to it. */
+void tmpm_tune_cache_clear(void);
+uint64_t tmpm_tune_cache_size(void);
 
 /* Bandwidth probe (no reference counterpart): copies `bytes` bytes between
  * two device buffers `iterations` times with a plain float4 copy kernel and

@@ -1293,6 +1293,357 @@ struct World {
   }
 };
 
+// ---------------------------------------------------------------------------
+// FP32 restatement of the reference's resident GPU substep (gpu.wgsl:428-1019
+// with the f32 packing of gpu.rs:1118-1297): the error floor that f32
+// arithmetic alone leaves against the FP64 world above.  Every operation is
+// an f32 operation in the WGSL's order, each rounded separately (the oracle
+// is built with -ffp-contract=off).  Grid sums are added in particle order,
+// one of the orders the reference's CAS loops may produce.  Scope: elastic
+// (Newton polar, 10 iterations), neo-Hookean and fluid particles, gravity
+// and walls; no obstacles or plastic materials (run_fp32 refuses them).
+// ---------------------------------------------------------------------------
+struct F3 {
+  float x = 0, y = 0, z = 0;
+  float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+  float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline F3 f3(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
+inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+inline F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+inline F3 operator/(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+inline float dotf(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 crossf(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// mat3x3<f32> as three columns
+struct FM3 {
+  F3 c[3];
+};
+inline F3 mulv(const FM3& m, F3 v) { return m.c[0] * v.x + m.c[1] * v.y + m.c[2] * v.z; }
+inline FM3 mulm(const FM3& a, const FM3& b) {
+  FM3 r;
+  for (int k = 0; k < 3; ++k) r.c[k] = mulv(a, b.c[k]);
+  return r;
+}
+inline FM3 transposef(const FM3& m) {
+  FM3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.c[i][j] = m.c[j][i];
+  return r;
+}
+inline FM3 addm(const FM3& a, const FM3& b) {
+  FM3 r;
+  for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] + b.c[k];
+  return r;
+}
+inline FM3 subm(const FM3& a, const FM3& b) {
+  FM3 r;
+  for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] - b.c[k];
+  return r;
+}
+inline FM3 scalem(const FM3& a, float s) {
+  FM3 r;
+  for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] * s;
+  return r;
+}
+inline FM3 diagf(float d) {
+  FM3 r;
+  r.c[0].x = r.c[1].y = r.c[2].z = d;
+  return r;
+}
+inline float detf(const FM3& m) { return dotf(m.c[0], crossf(m.c[1], m.c[2])); }
+
+// gpu.wgsl:428-436
+inline float qweight(float v, int o) {
+  if (o == 0) return 0.5f * (1.5f - v) * (1.5f - v);
+  if (o == 1) return 0.75f - (v - 1.0f) * (v - 1.0f);
+  return 0.5f * (v - 0.5f) * (v - 0.5f);
+}
+
+// gpu.wgsl:451-465
+inline bool regular_corotated_f(const FM3& F) {
+  const F3 a = F.c[0], b = F.c[1], c = F.c[2];
+  const float det = dotf(a, crossf(b, c));
+  const float nrm = std::sqrt(dotf(a, a) + dotf(b, b) + dotf(c, c));
+  if (std::fabs(det) < 1e-5f || nrm < 1e-5f || nrm > 20.0f) return false;
+  const F3 i0 = crossf(b, c) / det, i1 = crossf(c, a) / det, i2 = crossf(a, b) / det;
+  const float inrm = std::sqrt(dotf(i0, i0) + dotf(i1, i1) + dotf(i2, i2));
+  return nrm * inrm <= 100.0f;
+}
+
+// gpu.wgsl:467-482
+inline FM3 polar_rotation_f(const FM3& F) {
+  FM3 r = F;
+  for (int it = 0; it < 10; ++it) {
+    const F3 a = r.c[0], b = r.c[1], c = r.c[2];
+    const float d = dotf(a, crossf(b, c));
+    FM3 inv_t;
+    inv_t.c[0] = crossf(b, c) / d;
+    inv_t.c[1] = crossf(c, a) / d;
+    inv_t.c[2] = crossf(a, b) / d;
+    r = scalem(addm(r, inv_t), 0.5f);
+  }
+  return r;
+}
+
+struct P32 {
+  F3 x, v, force;
+  float m = 0, vol = 0, radius = 0, damping = 0;
+  FM3 F, C;
+  int code = 0;  // gpu.rs:1171-1276: 0 host stress, 1 neo-Hookean, 2 fluid, 3 corotated
+  float lambda = 0, mu = 0, bulk = 0, gamma = 0, viscosity = 0, tensile = 0;
+  FM3 host_stress;  // code 0: the f64 stress times -dt V0 inv_d, rounded to f32
+  int32_t base[3] = {0, 0, 0};
+  uint16_t color = 0;
+  bool enabled = true, fixed = false;
+};
+
+inline float to_f32(double v) { return (float)v; }
+inline FM3 to_fm3(const M3& m) {
+  FM3 r;
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < 3; ++k) r.c[c][k] = (float)m.a[k][c];
+  return r;
+}
+inline M3 to_m3(const FM3& f) {
+  M3 r;
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < 3; ++k) r.a[k][c] = (double)f.c[c][k];
+  return r;
+}
+
+// gpu.wgsl:638-701
+FM3 particle_stress_f(const P32& q, float dt, float inv_d) {
+  const FM3& F = q.F;
+  const float j = detf(F);
+  FM3 stress;
+  if (q.code == 0) return q.host_stress;
+  if (q.code == 1) {
+    const float safe_j = std::fmax(j, 1e-10f);
+    const float iso = q.lambda * std::log(safe_j) - q.mu;
+    stress = addm(scalem(mulm(F, transposef(F)), q.mu), diagf(iso));
+  } else if (q.code == 2) {
+    const float safe_j = std::fmax(j, 1e-6f);
+    float pressure = -q.bulk * q.tensile * (safe_j - 1.0f);
+    if (safe_j <= 1.0f) pressure = q.bulk * (std::pow(safe_j, -q.gamma) - 1.0f);
+    const FM3 sr = scalem(addm(q.C, transposef(q.C)), 0.5f);
+    const float trace_third = (sr.c[0].x + sr.c[1].y + sr.c[2].z) / 3.0f;
+    const float diagonal = -2.0f * q.viscosity * safe_j * trace_third - pressure * safe_j;
+    stress = addm(scalem(sr, 2.0f * q.viscosity * safe_j), diagf(diagonal));
+  } else {
+    const FM3 R = polar_rotation_f(F);
+    const float iso = q.lambda * 1.0f * (j - 1.0f) * j;
+    stress = addm(scalem(mulm(subm(F, R), transposef(F)), 2.0f * q.mu * 1.0f), diagf(iso));
+  }
+  return scalem(stress, -dt * q.vol * inv_d);
+}
+
+struct NodeKey32 {
+  int32_t x, y, z;
+  uint32_t color;
+  bool operator==(const NodeKey32& o) const { return x == o.x && y == o.y && z == o.z && color == o.color; }
+};
+struct NodeKey32Hash {
+  size_t operator()(const NodeKey32& k) const {
+    uint64_t h = (uint64_t)(uint32_t)k.x * 0x9E3779B97F4A7C15ull;
+    h ^= (uint64_t)(uint32_t)k.y * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+    h ^= (uint64_t)(uint32_t)k.z * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+    h ^= (uint64_t)k.color * 0x27D4EB2F165667C5ull + (h << 6) + (h >> 2);
+    return (size_t)h;
+  }
+};
+struct Node32 {
+  float m = 0;
+  F3 p, v;
+};
+
+// Packs the world (gpu.rs:1118-1297), runs `steps` resident substeps of dt in
+// f32 and writes the state back as f64 (insertion order; disabled particles
+// keep theirs).  Returns INVALID_INPUT for what the restatement leaves out.
+int run_fp32(const World& w, double dt, uint32_t steps, double* pos, double* vel, double* Fo, double* Co) {
+  if (!w.obstacles.empty()) return INVALID_INPUT;
+  const Params& pr = w.params;
+  const float h = to_f32(pr.cell_width), step = to_f32(dt), inv_d = to_f32(4.0 / (pr.cell_width * pr.cell_width));
+  const float inv_h = 1.0f / h;
+  const F3 g = f3(to_f32(pr.gravity.x), to_f32(pr.gravity.y), to_f32(pr.gravity.z));
+  const F3 bmin = f3(to_f32(pr.bmin.x), to_f32(pr.bmin.y), to_f32(pr.bmin.z));
+  const F3 bmax = f3(to_f32(pr.bmax.x), to_f32(pr.bmax.y), to_f32(pr.bmax.z));
+  const size_t n = w.particles.size();
+  std::vector<P32> ps(n);
+  for (size_t i = 0; i < n; ++i) {
+    const Particle& q = w.particles[i];
+    P32& r = ps[i];
+    r.x = f3(to_f32(q.position.x), to_f32(q.position.y), to_f32(q.position.z));
+    r.v = f3(to_f32(q.velocity.x), to_f32(q.velocity.y), to_f32(q.velocity.z));
+    r.force = f3(to_f32(q.force.x), to_f32(q.force.y), to_f32(q.force.z));
+    r.m = q.enabled ? to_f32(q.mass) : 0.0f;
+    r.vol = to_f32(q.rest_volume);
+    r.radius = to_f32(q.radius);
+    r.damping = to_f32(q.damping);
+    r.F = to_fm3(q.deformation);
+    r.C = to_fm3(q.affine);
+    r.color = q.transfer_color;
+    r.enabled = q.enabled;
+    r.fixed = q.fixed;
+    for (int a = 0; a < 3; ++a) r.base[a] = (int32_t)std::floor(r.x[a] * inv_h - 0.5f);
+    const Material& mt = q.material;
+    double lam = 0, mu = 0;
+    switch (mt.kind) {
+      case 0:  // LinearElastic (corotated)
+        lame(mt.p[0], mt.p[1], lam, mu);
+        r.code = 3;
+        r.lambda = to_f32(lam);
+        r.mu = to_f32(mu);
+        break;
+      case 1:  // NeoHookean
+        lame(mt.p[0], mt.p[1], lam, mu);
+        r.code = 1;
+        r.lambda = to_f32(lam);
+        r.mu = to_f32(mu);
+        break;
+      case 3:  // Fluid
+        r.code = 2;
+        r.bulk = to_f32(mt.p[0]);
+        r.gamma = to_f32(mt.p[1]);
+        r.viscosity = to_f32(mt.p[2]);
+        r.tensile = to_f32(mt.p[3]);
+        break;
+      default:
+        if (q.enabled) return INVALID_INPUT;
+    }
+  }
+  std::unordered_map<NodeKey32, uint32_t, NodeKey32Hash> index;
+  std::vector<NodeKey32> keys;
+  std::vector<Node32> nodes;
+  for (uint32_t sstep = 0; sstep < steps; ++sstep) {
+    index.clear();
+    keys.clear();
+    nodes.clear();
+    index.reserve(n * 8 + 16);
+    // p2g (gpu.wgsl:711-770)
+    for (size_t i = 0; i < n; ++i) {
+      P32& q = ps[i];
+      if (q.m <= 0.0f) continue;
+      if (q.code == 3 && !regular_corotated_f(q.F)) {
+        // the reference's direct path computes this stress on the host in f64
+        const Particle& orig = w.particles[i];
+        const M3 s64 = corotated_stress(to_m3(q.F), orig.material.p[0], orig.material.p[1], 1.0) *
+                       (-dt * orig.rest_volume * (double)inv_d);
+        q.host_stress = to_fm3(s64);
+        q.code = 0;
+      }
+      const F3 gp = q.x * inv_h;
+      const F3 fx = gp - f3((float)q.base[0], (float)q.base[1], (float)q.base[2]);
+      const FM3 S = particle_stress_f(q, step, inv_d);
+      if (q.code == 0) q.code = 3;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) {
+            const float weight = qweight(fx.x, a) * qweight(fx.y, b) * qweight(fx.z, c);
+            const F3 dpos = (f3((float)a, (float)b, (float)c) - fx) * h;
+            const F3 am = (q.C.c[0] * q.m + S.c[0]) * dpos.x + (q.C.c[1] * q.m + S.c[1]) * dpos.y +
+                          (q.C.c[2] * q.m + S.c[2]) * dpos.z;
+            const F3 mom = weight * (q.m * q.v + am);
+            const NodeKey32 k{q.base[0] + a, q.base[1] + b, q.base[2] + c, q.color};
+            auto it = index.find(k);
+            uint32_t id;
+            if (it == index.end()) {
+              id = (uint32_t)nodes.size();
+              index.emplace(k, id);
+              keys.push_back(k);
+              nodes.emplace_back();
+            } else {
+              id = it->second;
+            }
+            Node32& nd = nodes[id];
+            nd.m = nd.m + weight * q.m;
+            nd.p = nd.p + mom;
+          }
+    }
+    // update_grid (gpu.wgsl:772-819)
+    for (size_t id = 0; id < nodes.size(); ++id) {
+      Node32& nd = nodes[id];
+      if (nd.m <= 0.0f) continue;
+      F3 v = nd.p / nd.m + g * step;
+      const F3 position = f3((float)keys[id].x, (float)keys[id].y, (float)keys[id].z) * h;
+      if (pr.has_bounds)
+        for (int a = 0; a < 3; ++a)
+          if ((position[a] <= bmin[a] + h && v[a] < 0.0f) || (position[a] >= bmax[a] - h && v[a] > 0.0f)) v[a] = 0.0f;
+      nd.v = v;
+    }
+    // g2p (gpu.wgsl:821-1019)
+    for (size_t i = 0; i < n; ++i) {
+      P32& q = ps[i];
+      if (q.m <= 0.0f) continue;
+      if (q.fixed) {
+        q.v = F3();
+        q.force = F3();
+        q.C = FM3();
+        continue;
+      }
+      const F3 gp = q.x * inv_h;
+      const F3 fx = gp - f3((float)q.base[0], (float)q.base[1], (float)q.base[2]);
+      F3 velocity, a0, a1, a2;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) {
+            const float weight = qweight(fx.x, a) * qweight(fx.y, b) * qweight(fx.z, c);
+            const F3 dpos = (f3((float)a, (float)b, (float)c) - fx) * h;
+            auto it = index.find(NodeKey32{q.base[0] + a, q.base[1] + b, q.base[2] + c, q.color});
+            const F3 nv = it == index.end() ? F3() : nodes[it->second].v;
+            velocity = velocity + nv * weight;
+            a0 = a0 + nv * (dpos.x * weight * inv_d);
+            a1 = a1 + nv * (dpos.y * weight * inv_d);
+            a2 = a2 + nv * (dpos.z * weight * inv_d);
+          }
+      velocity = velocity + q.force / q.m * step;
+      velocity = velocity * std::exp(-q.damping * step);
+      F3 position = q.x + velocity * step;
+      if (pr.has_bounds)
+        for (int a = 0; a < 3; ++a) {
+          const float lower = bmin[a] + q.radius, upper = bmax[a] - q.radius;
+          if (position[a] < lower) {
+            position[a] = lower;
+            velocity[a] = std::fmax(velocity[a], 0.0f);
+          } else if (position[a] > upper) {
+            position[a] = upper;
+            velocity[a] = std::fmin(velocity[a], 0.0f);
+          }
+        }
+      FM3 affine;
+      affine.c[0] = a0;
+      affine.c[1] = a1;
+      affine.c[2] = a2;
+      FM3 def = mulm(addm(diagf(1.0f), scalem(affine, step)), q.F);
+      if (q.code == 2) def = diagf(std::pow(std::fmax(detf(def), 1e-9f), 1.0f / 3.0f));
+      q.x = position;
+      q.v = velocity;
+      q.force = F3();
+      q.C = affine;
+      q.F = def;
+      for (int a = 0; a < 3; ++a) {
+        const float nc = std::floor(position[a] * inv_h - 0.5f);
+        if (std::isfinite(nc) && nc >= -2147483648.0f && nc < 2147483648.0f) q.base[a] = (int32_t)nc;
+      }
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const Particle& orig = w.particles[i];
+    const P32& q = ps[i];
+    const bool moved = orig.enabled;
+    for (int a = 0; a < 3; ++a) {
+      pos[3 * i + a] = moved ? (double)q.x[a] : orig.position[a];
+      vel[3 * i + a] = moved ? (double)q.v[a] : orig.velocity[a];
+    }
+    const M3 F = moved ? to_m3(q.F) : orig.deformation, C = moved ? to_m3(q.C) : orig.affine;
+    store_cm(F, Fo + 9 * i);
+    store_cm(C, Co + 9 * i);
+  }
+  return OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1473,6 +1824,13 @@ int orc_world_substeps_fixed_threads(void* h, double dt, uint32_t steps, int32_t
 }
 
 double orc_world_stable_timestep(void* h) { return static_cast<World*>(h)->stable_timestep(); }
+
+// The FP32 restatement of `steps` resident substeps of dt from the world's
+// current state (the world itself is not changed): positions, velocities, F
+// and C in insertion order.
+int orc_world_fp32_substeps(void* h, double dt, uint32_t steps, double* pos, double* vel, double* F, double* C) {
+  return run_fp32(*static_cast<World*>(h), dt, steps, pos, vel, F, C);
+}
 uint64_t orc_world_count(void* h) { return static_cast<World*>(h)->particles.size(); }
 uint64_t orc_world_substeps(void* h) { return static_cast<World*>(h)->substeps; }
 
@@ -1634,16 +1992,23 @@ int32_t orc_cell_f32(const float* x, float h32, int32_t* cell) {
 // UINT64_MAX if some active particle has an unrepresentable cell.
 uint64_t orc_node_set(uint64_t n, const float* pos, const uint8_t* active, const uint32_t* region, float h32,
                       int32_t* out, uint64_t cap) {
-  std::vector<std::array<int32_t, 4>> keys;
-  keys.reserve(n * 27);
+  // distinct base cells first (several particles share a cell), then their stencils
+  std::vector<std::array<int32_t, 4>> cells;
+  cells.reserve(n);
   for (uint64_t i = 0; i < n; ++i) {
     if (!active[i]) continue;
     int32_t c[3];
     if (!orc_cell_f32(pos + 3 * i, h32, c)) return UINT64_MAX;
+    cells.push_back({c[0], c[1], c[2], (int32_t)region[i]});
+  }
+  std::sort(cells.begin(), cells.end());
+  cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+  std::vector<std::array<int32_t, 4>> keys;
+  keys.reserve(cells.size() * 27);
+  for (const auto& c : cells)
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b)
-        for (int e = 0; e < 3; ++e) keys.push_back({c[0] + a, c[1] + b, c[2] + e, (int32_t)region[i]});
-  }
+        for (int e = 0; e < 3; ++e) keys.push_back({c[0] + a, c[1] + b, c[2] + e, c[3]});
   std::sort(keys.begin(), keys.end());
   keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
   for (uint64_t i = 0; i < keys.size() && i < cap; ++i)

@@ -71,6 +71,8 @@ def lib() -> C.CDLL:
     L.orc_world_substeps_fixed.argtypes = [vp, C.c_double, C.c_uint32]
     L.orc_world_substeps_fixed_threads.restype = C.c_int
     L.orc_world_substeps_fixed_threads.argtypes = [vp, C.c_double, C.c_uint32, C.c_int32]
+    L.orc_world_fp32_substeps.restype = C.c_int
+    L.orc_world_fp32_substeps.argtypes = [vp, C.c_double, C.c_uint32, _dp, _dp, _dp, _dp]
     L.orc_world_stable_timestep.restype = C.c_double
     L.orc_world_stable_timestep.argtypes = [vp]
     L.orc_world_count.restype = C.c_uint64
@@ -222,6 +224,17 @@ class OracleWorld:
         st = lib().orc_world_substeps_fixed_threads(self._h, float(dt), int(steps), int(threads))
         if st:
             raise OracleError(st)
+
+    def fp32_substeps(self, dt: float, steps: int) -> dict:
+        """The FP32 restatement of the reference's resident GPU substep (gpu.wgsl:428-1019):
+        `steps` substeps of dt from this world's current state, which stays unchanged.  The
+        error floor f32 arithmetic leaves against the FP64 substeps."""
+        n = len(self)
+        pos, vel, F, Cm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 9)), np.zeros((n, 9))
+        st = lib().orc_world_fp32_substeps(self._h, float(dt), int(steps), _ptr(pos), _ptr(vel), _ptr(F), _ptr(Cm))
+        if st:
+            raise OracleError(st)
+        return dict(position=pos, velocity=vel, deformation=_from_cm(F), affine=_from_cm(Cm))
 
     def stable_timestep(self) -> float:
         return float(lib().orc_world_stable_timestep(self._h))

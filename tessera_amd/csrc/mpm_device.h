@@ -143,6 +143,11 @@ struct DevMapping {
   float local_q[4];
 };
 
+// FP64 audit record per scene and substep: mass; momentum (3); kinetic
+// energy; first moment of mass (3); angular momentum about the origin (3);
+// particles counted.
+constexpr int kAuditFields = 12;
+
 constexpr int kReactBins = 16;       // int64 accumulators per obstacle component
 constexpr int kMaxReactObstacles = 16;  // per scene, G2P's LDS reaction sums (later obstacles: direct atomics)
 
@@ -180,6 +185,7 @@ struct Topo {
   uint32_t* sorted;         // binned order -> particle storage index
   uint32_t* counters;
   uint32_t* scene_err;      // per scene: error bits of its particles and blocks
+  uint32_t isolate;         // batches: binning skips the particles of scenes whose error bits are set
   // binning lists: insert workgroup g bins particles [g * bin_span, (g + 1) * bin_span)
   uint32_t bin_span;        // particles per binning workgroup (multiple of 256)
   uint32_t bin_groups;      // binning workgroups

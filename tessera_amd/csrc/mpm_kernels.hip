@@ -345,7 +345,13 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
       if (in[k] && pm[k].w > 0.0f) {
         const DevClass& c = classes[ci[k]];
         int3 cell;
-        if (cell3(pm[k], c.inv_h, cell)) {
+        // A batch scene that failed in an earlier substep is not binned any
+        // more: its particles stop like retired ones (synchronize restores
+        // its last synchronized world), so a diverging scene cannot exhaust
+        // the hash or grid capacity that every scene of the batch shares.
+        if (t.isolate && t.scene_err[c.region >> 16] != 0u) {
+          // not binned: pslot stays kInvalid below
+        } else if (cell3(pm[k], c.inv_h, cell)) {
           active[k] = true;
           key[k] = make_int4(cell.x >> 2, cell.y >> 2, cell.z >> 2, (int)c.region);
         } else {
@@ -2229,6 +2235,69 @@ __global__ __launch_bounds__(256) void k_gather_scene(Parts p, uint32_t first, u
 }
 
 // ---------------------------------------------------------------------------
+// FP64 audit of the particle state after a substep, per scene: mass,
+// momentum, kinetic energy, first moment of mass and angular momentum about
+// the origin (the reference's momentum balance, world.rs:784-807, checks the
+// same sums on the host).  Every product and sum is f64.  A wavefront whose
+// particles belong to one scene (nearly all of them: blocks never mix scenes)
+// reduces its lanes with shuffles and adds the sums with one lane; otherwise
+// each lane adds its own.  The sums are exact to f64 rounding but their order
+// follows the scheduler, so the last bits may differ run to run.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_audit(Parts p, const DevClass* classes, double* out) {
+  const uint32_t lane = lane_id();
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < p.n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + lane;
+    double v[kAuditFields];
+#pragma unroll
+    for (int f = 0; f < kAuditFields; ++f) v[f] = 0.0;
+    uint32_t scene = kInvalid;
+    if (i < p.n) {
+      const float4 pm = p.pos_mass[i];
+      if (pm.w > 0.0f && p.ins_id[i] != kInvalid) {
+        const float4 vv = p.vel_vol[i];
+        scene = classes[p.cls[i]].region >> 16;
+        const double m = pm.w, x = pm.x, y = pm.y, z = pm.z, vx = vv.x, vy = vv.y, vz = vv.z;
+        v[0] = m;
+        v[1] = m * vx;
+        v[2] = m * vy;
+        v[3] = m * vz;
+        v[4] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        v[5] = m * x;
+        v[6] = m * y;
+        v[7] = m * z;
+        v[8] = m * (y * vz - z * vy);
+        v[9] = m * (z * vx - x * vz);
+        v[10] = m * (x * vy - y * vx);
+        v[11] = 1.0;
+      }
+    }
+    const uint64_t valid = __ballot(scene != kInvalid);
+    if (valid == 0) continue;
+    const uint32_t s0 = readlane_u(scene, __ffsll((unsigned long long)valid) - 1);
+    if (__ballot(scene != kInvalid && scene != s0) == 0) {
+#pragma unroll
+      for (int f = 0; f < kAuditFields; ++f) v[f] = wave_sum_d(v[f]);
+      if (lane == 0) {
+        double* dst = out + (size_t)s0 * kAuditFields;
+#pragma unroll
+        for (int f = 0; f < kAuditFields; ++f) atomicAdd(&dst[f], v[f]);
+      }
+    } else if (scene != kInvalid) {
+      double* dst = out + (size_t)scene * kAuditFields;
+#pragma unroll
+      for (int f = 0; f < kAuditFields; ++f) atomicAdd(&dst[f], v[f]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Diagnostic: exact live-node set (union of 27-node stencils) as per-block
 // 64-bit masks.
 // ---------------------------------------------------------------------------
@@ -2455,6 +2524,9 @@ void launch_renumber(uint32_t* ins_id, uint32_t n, uint32_t first, uint32_t coun
 }
 void launch_gather_scene(const Parts& p, uint32_t first, uint32_t count, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_scene, dim3(grid_for(p.n, 256, 8192)), dim3(256), 0, s, p, first, count, out);
+}
+void launch_audit(const Parts& p, const DevClass* c, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_audit, dim3(grid_for(p.n, 256, 4096)), dim3(256), 0, s, p, c, out);
 }
 void launch_live_nodes(const Parts& p, const Topo& t, const DevClass* c, const DevScene* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_mask_clear, dim3(grid_for(2ull * t.grid_cap, 256, 4096)), dim3(256), 0, s, t);

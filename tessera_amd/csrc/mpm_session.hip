@@ -22,6 +22,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <system_error>
 #include <thread>
@@ -663,6 +664,19 @@ struct tmpm_session {
   bool topo_valid = false;  // topology arrays describe the current device state
   bool table_dirty = true;  // block hash needs clearing before the next binning
 
+  // FP64 audit (tmpm_session_set_audit): one record per scene slot and
+  // substep, kept on the device for the pending substeps and copied to the
+  // host at synchronize
+  bool audit_on = false;
+  DevBuf audit_dev;
+  uint64_t audit_cap = 0;      // substep records the device buffer holds
+  uint64_t audit_used = 0;     // records of the pending substeps
+  uint32_t audit_stride = 0;   // scene slots per record set
+  std::vector<double> audit_host;
+  uint64_t audit_host_steps = 0;
+  uint32_t audit_host_stride = 0;
+  double* audit_next = nullptr;  // record set of the substep being enqueued
+
   // timing
   bool timing = false;
   // substep graphs (hipGraph): captured per distinct launch plan and replayed;
@@ -689,6 +703,15 @@ struct tmpm_session {
     p.stride = cap;
     return p;
   }
+};
+
+// A world validated and packed for admission into a batch, not yet on any
+// device (tmpm_batch_stage_scene / tmpm_batch_add_staged).
+struct tmpm_staged_scene {
+  SceneState scene;
+  std::unique_ptr<PackedParticles> pk;
+  std::vector<DevClass> classes;  // per particle, scene index 0
+  uint64_t pblocks = 0, gblocks = 0;
 };
 
 namespace {
@@ -977,6 +1000,7 @@ tmpm_status upload_scene(tmpm_session* s) {
   HIP_TRY(hipMemcpy(s->scene_plain.ptr, plain.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->scene_force.ptr, forced.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
   s->topo.scene_err = s->scene_err.as<uint32_t>();
+  s->topo.isolate = s->batch ? 1u : 0u;
   return TMPM_OK;
 }
 
@@ -1343,12 +1367,12 @@ bool classes_plastic(const tmpm_session* s) {
   return false;
 }
 
-// Appends scene k's enabled particles to `out` (insertion index first + i)
-// and records its disabled ones for snapshots.  Two parallel passes over
-// fixed chunks (validate and count, then fill at the chunks' offsets); the
-// error reported is the one of the lowest failing index, as in one pass.
-tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
-  SceneState& scene = s->scenes[k];
+// The host part of pack_scene: every enabled particle packed with scene
+// index k and insertion base scene.first, and its (not yet interned) class.
+// Two parallel passes over fixed chunks (validate and count, then fill at
+// the chunks' offsets); the error reported is the one of the lowest failing
+// index, as in one pass.
+tmpm_status pack_scene_host(const SceneState& scene, uint32_t k, PackedParticles& out, std::vector<DevClass>& classes) {
   const HostWorld& w = scene.world;
   const float inv_h = scene.dev.inv_h;
   const size_t n = w.size();
@@ -1361,7 +1385,7 @@ tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
     std::string msg;
   };
   std::vector<ChunkInfo> info(chunks);
-  std::vector<DevClass> classes(n);
+  classes.resize(n);
   parallel_for(chunks, [&](size_t cb, size_t ce) {
     for (size_t c = cb; c < ce; ++c) {
       ChunkInfo& ci = info[c];
@@ -1426,10 +1450,26 @@ tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
       }
     }
   }, 1);
-  // class interning is sequential (ids in first-seen order); runs share a class
+  return TMPM_OK;
+}
+
+// Interns the classes of a packed scene's enabled particles into out.cls
+// from index e0 on: sequential, ids in first-seen order; runs share a class.
+void intern_scene_classes(tmpm_session* s, const HostWorld& w, const std::vector<DevClass>& classes,
+                          PackedParticles& out, size_t e0) {
   size_t e = e0;
-  for (size_t i = 0; i < n; ++i)
+  for (size_t i = 0; i < w.size(); ++i)
     if (w.enabled[i]) out.cls[e++] = intern_class(s, classes[i]);
+}
+
+// Appends scene k's enabled particles to `out` (insertion index first + i)
+// and records its disabled ones for snapshots.
+tmpm_status pack_scene(tmpm_session* s, uint32_t k, PackedParticles& out) {
+  const size_t e0 = out.pm.size();
+  std::vector<DevClass> classes;
+  tmpm_status st = pack_scene_host(s->scenes[k], k, out, classes);
+  if (st != TMPM_OK) return st;
+  intern_scene_classes(s, s->scenes[k].world, classes, out, e0);
   return TMPM_OK;
 }
 
@@ -1692,6 +1732,7 @@ struct SubstepPlan {
   const uint32_t* body_scene;
   uint32_t obstacle_total, body_total, p2g_blocks, p2g_threads, g2p_threads;
   int32_t react, fixed64, shapes, plastic, cpic, copies;
+  double* audit;  // FP64 audit record set of this substep, or null
   uint8_t reorder, clear_table, rigid_sync, rigid_apply;
 };
 
@@ -1718,8 +1759,105 @@ uint32_t particles_per_block(const tmpm_session* s) {
   return s->last_pblocks ? (uint32_t)(s->n / s->last_pblocks) : 0u;
 }
 
+// Autotuner cache.  Tuned launch shapes are kept per workload bucket: device,
+// particle count and particles per block (each rounded down to a power of
+// two), and the features that select kernel variants (reactions, plastic
+// materials, obstacle shapes, P2G mode, batch).  A session in a bucket that
+// was tuned before takes the cached shape without trials, and sessions whose
+// shape is not fixed (batches under admission and retirement among them)
+// launch with the cached shape of the bucket they are in at each substep.
+// The cache is process-wide; with TMPM_TUNE_CACHE naming a file it is also
+// loaded from and appended to that file, one entry per line.
+struct TuneKey {
+  int device, n_log2, ppb_log2, features;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(device, n_log2, ppb_log2, features) < std::tie(o.device, o.n_log2, o.ppb_log2, o.features);
+  }
+};
+struct TuneShape {
+  uint32_t p2g_threads = 64, g2p_threads = 64, reorder_interval = 16;
+  int p2g_copies = 2, graphs = 0;
+};
+
+struct TuneCache {
+  std::mutex mu;
+  std::map<TuneKey, TuneShape> entries;
+  bool loaded = false;
+  std::string path;
+
+  void load_locked() {
+    if (loaded) return;
+    loaded = true;
+    const char* f = std::getenv("TMPM_TUNE_CACHE");
+    if (!f || !*f) return;
+    path = f;
+    FILE* fp = std::fopen(f, "r");
+    if (!fp) return;
+    TuneKey k;
+    TuneShape v;
+    while (std::fscanf(fp, "%d %d %d %d %u %d %u %u %d", &k.device, &k.n_log2, &k.ppb_log2, &k.features,
+                       &v.p2g_threads, &v.p2g_copies, &v.g2p_threads, &v.reorder_interval, &v.graphs) == 9) {
+      const bool ok = (v.p2g_threads == 64 || v.p2g_threads == 128) && (v.p2g_copies == 1 || v.p2g_copies == 2) &&
+                      (v.g2p_threads == 64 || v.g2p_threads == 128 || v.g2p_threads == 256) &&
+                      v.reorder_interval >= 1 && v.reorder_interval <= 1000000;
+      if (ok) entries[k] = v;
+    }
+    std::fclose(fp);
+  }
+  bool find(const TuneKey& k, TuneShape& out) {
+    std::lock_guard<std::mutex> g(mu);
+    load_locked();
+    auto it = entries.find(k);
+    if (it == entries.end()) return false;
+    out = it->second;
+    return true;
+  }
+  void put(const TuneKey& k, const TuneShape& v) {
+    std::lock_guard<std::mutex> g(mu);
+    load_locked();
+    entries[k] = v;
+    if (path.empty()) return;
+    if (FILE* fp = std::fopen(path.c_str(), "a")) {
+      std::fprintf(fp, "%d %d %d %d %u %d %u %u %d\n", k.device, k.n_log2, k.ppb_log2, k.features, v.p2g_threads,
+                   v.p2g_copies, v.g2p_threads, v.reorder_interval, v.graphs);
+      std::fclose(fp);
+    }
+  }
+  void clear() {
+    std::lock_guard<std::mutex> g(mu);
+    entries.clear();
+  }
+  size_t size() {
+    std::lock_guard<std::mutex> g(mu);
+    load_locked();
+    return entries.size();
+  }
+};
+
+TuneCache& tune_cache() {
+  static TuneCache c;
+  return c;
+}
+
+int floor_log2(uint64_t v) { return v ? 63 - __builtin_clzll(v) : -1; }
+
+TuneKey tune_key(const tmpm_session* s) {
+  TuneKey k;
+  k.device = s->ctx ? s->ctx->device : 0;
+  k.n_log2 = floor_log2(s->n);
+  k.ppb_log2 = floor_log2(particles_per_block(s));
+  k.features = (s->react_active() && s->obstacle_total > 0 ? 1 : 0) | (s->plastic ? 2 : 0) | (s->shapes ? 4 : 0) |
+               (s->fixed64 ? 8 : 0) | (s->batch ? 16 : 0) | (s->cpic ? 32 : 0);
+  return k;
+}
+
+// The cached shape of the session's current bucket, if it was tuned.
+bool cached_shape(const tmpm_session* s, TuneShape& out) { return tune_cache().find(tune_key(s), out); }
+
 uint32_t effective_g2p_threads(const tmpm_session* s) {
   if (s->g2p_shape_fixed) return s->threads_pblock;
+  TuneShape c;
+  if (cached_shape(s, c)) return c.g2p_threads;
   const uint32_t b = s->last_pblocks, ppb = particles_per_block(s);
   const uint32_t by_blocks = b >= kFillBlocks ? 64u : b >= kFillBlocks / 2 ? 128u : 256u;
   const uint32_t by_density = ppb >= 288 ? 256u : ppb >= 192 ? 128u : 64u;
@@ -1730,6 +1868,12 @@ void effective_p2g_shape(const tmpm_session* s, uint32_t& threads, int& copies) 
   if (s->p2g_shape_fixed) {
     threads = s->p2g_threads;
     copies = s->p2g_copies;
+    return;
+  }
+  TuneShape c;
+  if (cached_shape(s, c)) {
+    threads = c.p2g_threads;
+    copies = c.p2g_copies;
     return;
   }
   const uint32_t b = s->last_pblocks;
@@ -1764,6 +1908,7 @@ SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {
   pl.clear_table = s->table_dirty;
   pl.rigid_sync = s->body_total > 0 && s->obstacle_total > 0;
   pl.rigid_apply = pl.react && s->body_total > 0;
+  pl.audit = s->audit_on ? s->audit_next : nullptr;
   return pl;
 }
 
@@ -1809,6 +1954,10 @@ hipError_t launch_plan(tmpm_session* s, const SubstepPlan& pl) {
   if (pl.rigid_apply && (e = timed(s, "k_rigid_apply", [&] {
                            launch_rigid_apply(t, pl.sc, pl.rigid, pl.maps, pl.ob, pl.body_scene, pl.body_total, st);
                          })) != hipSuccess)
+    return e;
+  // the audit reads the state G2P wrote: the reorder target on a reorder substep
+  if (pl.audit && (e = timed(s, "k_audit", [&] { launch_audit(pl.reorder ? pl.q : p, pl.c, pl.audit, st); })) !=
+                      hipSuccess)
     return e;
   return hipSuccess;
 }
@@ -1862,7 +2011,7 @@ void destroy_graphs(tmpm_session* s) {
 hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   const SubstepPlan pl = plan_substep(s, apply_force);
   if ((uint64_t)pl.t.bin_groups * pl.t.bin_span < pl.p.n) return hipErrorInvalidValue;  // lists sized for fewer particles
-  const hipError_t e = s->use_graphs && !s->timing ? launch_graph(s, pl) : launch_plan(s, pl);
+  const hipError_t e = s->use_graphs && !s->timing && !pl.audit ? launch_graph(s, pl) : launch_plan(s, pl);
   if (e != hipSuccess) return e;
   if (pl.clear_table) s->table_dirty = false;
   if (pl.reorder) {
@@ -1875,6 +2024,42 @@ hipError_t enqueue_substep(tmpm_session* s, bool apply_force) {
   s->substeps_since_reorder++;
   s->topo_valid = false;
   return hipSuccess;
+}
+
+// Room for `steps` more audit record sets on the device, zeroed on the
+// session stream ahead of the substeps that fill them.  Growing with
+// substeps in flight waits for them and keeps their records.
+tmpm_status reserve_audit(tmpm_session* s, uint32_t steps) {
+  const uint32_t stride = std::max<uint32_t>(s->scene_cap, 1);
+  if (s->audit_used == 0) s->audit_stride = stride;
+  if (s->audit_stride != stride) return fail(TMPM_READBACK, "audit layout changed with substeps pending");
+  const size_t set = (size_t)stride * kAuditFields * sizeof(double);
+  const uint64_t need = s->audit_used + steps;
+  if (need > s->audit_cap) {
+    const uint64_t cap = std::max<uint64_t>(need, std::max<uint64_t>(64, s->audit_cap * 2));
+    DevBuf grown;
+    HIP_TRY(grown.alloc(cap * set));
+    if (s->audit_used) {
+      HIP_TRY(hipStreamSynchronize(s->stream));
+      HIP_TRY(hipMemcpy(grown.ptr, s->audit_dev.ptr, s->audit_used * set, hipMemcpyDeviceToDevice));
+    }
+    std::swap(s->audit_dev.ptr, grown.ptr);
+    std::swap(s->audit_dev.bytes, grown.bytes);
+    s->audit_cap = cap;
+  }
+  HIP_TRY(hipMemsetAsync(static_cast<char*>(s->audit_dev.ptr) + s->audit_used * set, 0, steps * set, s->stream));
+  return TMPM_OK;
+}
+
+// After the device finished: the pending substeps' audit records to the host.
+tmpm_status collect_audit(tmpm_session* s) {
+  const size_t doubles = (size_t)s->audit_used * s->audit_stride * kAuditFields;
+  s->audit_host.resize(doubles);
+  if (doubles) HIP_TRY(hipMemcpy(s->audit_host.data(), s->audit_dev.ptr, doubles * sizeof(double), hipMemcpyDeviceToHost));
+  s->audit_host_steps = s->audit_used;
+  s->audit_host_stride = s->audit_stride;
+  s->audit_used = 0;
+  return TMPM_OK;
 }
 
 tmpm_status status_from_error_bits(uint32_t bits) {
@@ -2362,6 +2547,164 @@ tmpm_status reprepare(tmpm_session* s, std::vector<SceneState>& previous) {
   return st;
 }
 
+// Packed-particle staging buffers (page-locked) reused across admissions:
+// freeing page-locked memory can wait for the device, so staged scenes
+// return their buffers here instead.
+struct PackPool {
+  std::mutex mu;
+  std::vector<std::unique_ptr<PackedParticles>> free;
+  std::unique_ptr<PackedParticles> acquire() {
+    std::lock_guard<std::mutex> g(mu);
+    if (free.empty()) return std::make_unique<PackedParticles>();
+    auto p = std::move(free.back());
+    free.pop_back();
+    return p;
+  }
+  void release(std::unique_ptr<PackedParticles> p) {
+    if (!p) return;
+    p->clear();
+    std::lock_guard<std::mutex> g(mu);
+    if (free.size() < 8) free.push_back(std::move(p));
+  }
+};
+
+PackPool& pack_pool() {
+  static PackPool* pool = new PackPool();  // never destroyed: no page-locked free at exit
+  return *pool;
+}
+
+// Host half of an admission (no session, no device): validation, the f64
+// copy of the world, f32 packing with scene index 0 and insertion base 0,
+// and the block counts for the capacity check.  `spare`, when given, is an
+// emptied world whose memory is reused.
+tmpm_status stage_scene(const tmpm_world_desc* world, double substep_dt, HostWorld* spare, tmpm_staged_scene& out) {
+  if (!world) return fail(TMPM_INVALID_INPUT, "null world");
+  if (!params_valid(world->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
+  for (uint32_t i = 0; i < world->obstacle_count; ++i)
+    if (!obstacle_valid(world->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
+  const tmpm_particles& pp = world->particles;
+  if (pp.count > 0 && (!pp.position || !pp.mass || !pp.rest_volume || !pp.radius || world->material_count == 0))
+    return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  for (uint64_t i = 0; pp.material_index && i < pp.count; ++i)
+    if (pp.material_index[i] >= world->material_count) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  SceneState& scene = out.scene;
+  if (spare) load_scene(scene, world, substep_dt, std::move(*spare));
+  else load_scene(scene, world, substep_dt);
+  {
+    std::vector<char> bad(64, 0);
+    parallel_for(scene.world.size(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i)
+        if (!scene.world.particle_valid(i)) {
+          bad[(b / 16384) % 64] = 1;
+          break;
+        }
+    });
+    for (char c : bad)
+      if (c) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
+  }
+  tmpm_status st = validate_resident_world(scene);
+  if (st != TMPM_OK) return st;
+  if (scene.world.size() >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
+  scene.first = 0;
+  scene.count = (uint32_t)scene.world.size();
+  for (size_t i = 0; i < scene.count; ++i)
+    if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
+  st = build_scene(scene);
+  if (st != TMPM_OK) return st;
+  out.pk = pack_pool().acquire();
+  st = pack_scene_host(scene, 0, *out.pk, out.classes);
+  if (st != TMPM_OK) return st;
+  count_blocks(out.pk->pos32, out.pk->region, out.pk->inv_h, out.pblocks, out.gblocks);
+  return TMPM_OK;
+}
+
+// Device half of an admission into a batch with no pending substeps: the
+// staged scene takes the first free slot, its insertion indices follow the
+// current ones, and its particles go after the current slots (nothing
+// already on the device moves).
+tmpm_status commit_staged(tmpm_session* s, tmpm_staged_scene& staged, uint32_t* scene_out) {
+  SceneState& scene = staged.scene;
+  uint32_t slot = 0;
+  while (slot < s->scenes.size() && s->scenes[slot].active) ++slot;
+  if (slot >= 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
+  if (!s->prepared) {
+    std::vector<SceneState> previous = s->scenes;
+    if (slot == s->scenes.size()) s->scenes.push_back(scene);
+    else s->scenes[slot] = scene;
+    tmpm_status st = reprepare(s, previous);
+    if (st != TMPM_OK) return st;
+    if (scene_out) *scene_out = slot;
+    return TMPM_OK;
+  }
+  if ((uint64_t)s->id_space + scene.world.size() >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
+  // rebase the staged packing: insertion indices and the scene index of every class
+  const uint32_t first = s->id_space;
+  scene.first = first;
+  PackedParticles& pk = *staged.pk;
+  for (size_t e = 0; e < pk.size(); ++e) pk.ins[e] += first;
+  for (size_t d = 0; d < pk.disabled.size(); d += 2)
+    pk.disabled[d].w = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, pk.disabled[d].w) + first);
+  for (DevClass& c : staged.classes) c.region |= slot << 16;
+  if (slot == s->scenes.size()) s->scenes.push_back(std::move(scene));
+  else s->scenes[slot] = std::move(scene);
+  intern_scene_classes(s, s->scenes[slot].world, staged.classes, pk, 0);
+  s->plastic = classes_plastic(s);
+  // from here on device state changes; a failure invalidates the batch
+  auto broken = [&](tmpm_status code) {
+    s->valid = false;
+    return code;
+  };
+  tmpm_status st = TMPM_OK;
+  const uint64_t need = (uint64_t)s->n + pk.size();
+  if (need > s->cap) {
+    const uint32_t cap = (uint32_t)std::max<uint64_t>(need + need / 4, (uint64_t)s->cap + s->cap / 2);
+    ParticleBuffers grown, spare, synced;
+    if (alloc_particles(grown, cap) != TMPM_OK || alloc_particles(spare, cap) != TMPM_OK ||
+        alloc_particles(synced, cap) != TMPM_OK)
+      return broken(TMPM_CAPACITY);
+    if (copy_particles(grown, cap, s->buf[s->cur], s->cap, s->n, s->stream) != TMPM_OK ||
+        copy_particles(synced, cap, s->synced, s->cap, s->synced_n, s->stream) != TMPM_OK)
+      return broken(TMPM_READBACK);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return broken(fail(TMPM_READBACK, "device copy failed"));
+    auto swap_bufs = [](ParticleBuffers& a, ParticleBuffers& b) {
+      std::swap(a.pos_mass.ptr, b.pos_mass.ptr); std::swap(a.pos_mass.bytes, b.pos_mass.bytes);
+      std::swap(a.vel_vol.ptr, b.vel_vol.ptr); std::swap(a.vel_vol.bytes, b.vel_vol.bytes);
+      std::swap(a.C.ptr, b.C.ptr); std::swap(a.C.bytes, b.C.bytes);
+      std::swap(a.F.ptr, b.F.ptr); std::swap(a.F.bytes, b.F.bytes);
+      std::swap(a.P.ptr, b.P.ptr); std::swap(a.P.bytes, b.P.bytes);
+      std::swap(a.cls.ptr, b.cls.ptr); std::swap(a.cls.bytes, b.cls.bytes);
+      std::swap(a.ins_id.ptr, b.ins_id.ptr); std::swap(a.ins_id.bytes, b.ins_id.bytes);
+    };
+    swap_bufs(s->buf[s->cur], grown);
+    swap_bufs(s->buf[1 - s->cur], spare);
+    swap_bufs(s->synced, synced);
+    s->cap = cap;
+  }
+  if ((st = upload_particles(s, pk, s->n)) != TMPM_OK) return broken(st);
+  s->n += (uint32_t)pk.size();
+  s->n_enabled = s->n;
+  s->id_space += s->scenes[slot].count;
+  if ((st = ensure_force_capacity(s, s->id_space + s->id_space / 4)) != TMPM_OK) return broken(st);
+  if ((st = upload_scene_forces(s, slot)) != TMPM_OK) return broken(st);
+  s->disabled_host.insert(s->disabled_host.end(), pk.disabled.begin(), pk.disabled.end());
+  if ((st = upload_disabled(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_classes(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_rigid(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_obstacles(s)) != TMPM_OK) return broken(st);
+  if ((st = upload_scene(s)) != TMPM_OK) return broken(st);
+  // topology: room for the blocks this scene adds (and the per-particle arrays)
+  s->last_pblocks += (uint32_t)staged.pblocks;
+  s->last_gblocks += (uint32_t)staged.gblocks;
+  const Topo& t = s->topo;
+  if (2ull * s->last_pblocks > t.pblock_cap || 2ull * s->last_gblocks > t.grid_cap || s->topo_cap < s->cap) {
+    const uint64_t gcap = std::min<uint64_t>(std::max<uint64_t>(t.grid_cap, 4ull * s->last_gblocks), 1ull << 26);
+    const uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(t.pblock_cap, 4ull * s->last_pblocks), s->cap);
+    if ((st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap)) != TMPM_OK) return broken(st);
+  }
+  if (scene_out) *scene_out = slot;
+  return TMPM_OK;
+}
+
 tmpm_status set_scene_force(tmpm_session* s, uint32_t k, uint64_t index, const double force[3]) {
   SceneState& scene = s->scenes[k];
   if (!scene.stepping()) return fail(scene.status == TMPM_OK ? TMPM_INVALID_INPUT : scene.status, "scene is stopped");
@@ -2451,12 +2794,18 @@ tmpm_status tmpm_session_submit_steps(tmpm_session* s, uint32_t steps) {
     }
     if (s->prepared) {
       if (hipSetDevice(s->ctx->device) != hipSuccess) return fail(TMPM_READBACK, "cannot select device");
+      if (s->audit_on) {
+        tmpm_status a = reserve_audit(s, steps);
+        if (a != TMPM_OK) return a;
+      }
       for (uint32_t k = 0; k < steps; ++k) {
         bool force = false;
         for (auto& scene : s->scenes) {
           force = force || scene.force_pending;
           scene.force_pending = false;
         }
+        if (s->audit_on)
+          s->audit_next = s->audit_dev.as<double>() + (s->audit_used++) * (size_t)s->audit_stride * kAuditFields;
         hipError_t e = enqueue_substep(s, force);
         if (e != hipSuccess) {
           s->valid = false;
@@ -2498,6 +2847,10 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
         return fail(TMPM_READBACK, std::string("device failure: ") + hipGetErrorString(e));
       }
       collect_timers(s);
+      if (s->audit_used && collect_audit(s) != TMPM_OK) {
+        s->valid = false;
+        return TMPM_READBACK;
+      }
       if (hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(scene_bits.data(), s->scene_err.ptr, 4 * ns, hipMemcpyDeviceToHost) != hipSuccess) {
         s->valid = false;
@@ -2598,6 +2951,41 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
       }
     }
     return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
+}
+
+// gpu.rs:413-450 (synchronize_async): a poll instead of an await.  While the
+// device still runs the submitted substeps it returns TMPM_OK with *done_out
+// = 0 and changes nothing (the pending substeps stay pending, so the caller
+// may stop polling and synchronize later: the reference's cancellation keeps
+// them too).  Once they have finished it validates and commits exactly as
+// tmpm_session_synchronize does and sets *done_out = 1, also when that
+// fails (the failure invalidates the session as there).
+tmpm_status tmpm_session_synchronize_async(tmpm_session* s, int32_t* done_out) {
+  try {
+    if (!s || !done_out) return fail(TMPM_INVALID_INPUT, "null argument");
+    *done_out = 0;
+    if (!s->valid) return fail(TMPM_RESIDENT_UNAVAILABLE, "session is invalid after an earlier failure");
+    if (s->pending == 0) {
+      *done_out = 1;
+      return TMPM_OK;
+    }
+    if (s->prepared) {
+      const hipError_t q = hipStreamQuery(s->stream);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not a failure: keep it out of the next launch check
+        return TMPM_OK;
+      }
+      if (q != hipSuccess) {
+        s->valid = false;
+        *done_out = 1;
+        return fail(TMPM_READBACK, std::string("device failure: ") + hipGetErrorString(q));
+      }
+    }
+    *done_out = 1;
+    return tmpm_session_synchronize(s);
   } catch (...) {
     return api_exception(s);
   }
@@ -2812,156 +3200,62 @@ tmpm_status tmpm_batch_set_obstacles(tmpm_session* s, uint32_t scene, const tmpm
   }
 }
 
+tmpm_status tmpm_batch_stage_scene(const tmpm_world_desc* world, double substep_dt, tmpm_staged_scene** out) {
+  try {
+    if (!out) return fail(TMPM_INVALID_INPUT, "null argument");
+    *out = nullptr;
+    auto staged = std::make_unique<tmpm_staged_scene>();
+    tmpm_status st = stage_scene(world, substep_dt, nullptr, *staged);
+    if (st != TMPM_OK) return st;
+    *out = staged.release();
+    return TMPM_OK;
+  } catch (...) {
+    return exception_status();
+  }
+}
+
+void tmpm_staged_scene_destroy(tmpm_staged_scene* staged) {
+  if (!staged) return;
+  pack_pool().release(std::move(staged->pk));
+  delete staged;
+}
+
+tmpm_status tmpm_batch_add_staged(tmpm_session* s, tmpm_staged_scene* staged, uint32_t* scene_out) {
+  try {
+    if (!staged) return fail(TMPM_INVALID_INPUT, "null staged scene");
+    std::unique_ptr<tmpm_staged_scene, void (*)(tmpm_staged_scene*)> own(staged, tmpm_staged_scene_destroy);
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
+    return commit_staged(s, *staged, scene_out);
+  } catch (...) {
+    return api_exception(s);
+  }
+}
+
 tmpm_status tmpm_batch_add_scene(tmpm_session* s, const tmpm_world_desc* world, double substep_dt,
                                  uint32_t* scene_out) {
   try {
     tmpm_status st = check_editable(s);
     if (st != TMPM_OK) return st;
     if (!s->batch) return fail(TMPM_INVALID_INPUT, "not a batch");
-    // one pass over the particles: load, then validate what was loaded
-    if (!world) return fail(TMPM_INVALID_INPUT, "null world");
-    if (!params_valid(world->params)) return fail(TMPM_INVALID_INPUT, "invalid MPM parameters");
-    for (uint32_t i = 0; i < world->obstacle_count; ++i)
-      if (!obstacle_valid(world->obstacles[i])) return fail(TMPM_INVALID_INPUT, "invalid obstacle");
-    const tmpm_particles& pp = world->particles;
-    if (pp.count > 0 && (!pp.position || !pp.mass || !pp.rest_volume || !pp.radius || world->material_count == 0))
-      return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
-    for (uint64_t i = 0; pp.material_index && i < pp.count; ++i)
-      if (pp.material_index[i] >= world->material_count) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
-    SceneState scene;
-    {
-      // best-fitting emptied world of a retired scene, if one is large enough
-      const size_t need = 3 * (size_t)pp.count;
-      size_t best = s->spare_worlds.size();
-      for (size_t i = 0; i < s->spare_worlds.size(); ++i)
-        if (s->spare_worlds[i].pos.capacity() >= need &&
-            (best == s->spare_worlds.size() || s->spare_worlds[i].pos.capacity() < s->spare_worlds[best].pos.capacity()))
-          best = i;
-      if (best < s->spare_worlds.size()) {
-        HostWorld w = std::move(s->spare_worlds[best]);
-        s->spare_worlds.erase(s->spare_worlds.begin() + best);
-        load_scene(scene, world, substep_dt, std::move(w));
-      } else {
-        load_scene(scene, world, substep_dt);
-      }
+    tmpm_staged_scene staged;
+    // reuse the best-fitting emptied world of a retired scene, if one is large enough
+    const size_t need = world ? 3 * (size_t)world->particles.count : 0;
+    size_t best = s->spare_worlds.size();
+    for (size_t i = 0; i < s->spare_worlds.size(); ++i)
+      if (s->spare_worlds[i].pos.capacity() >= need &&
+          (best == s->spare_worlds.size() || s->spare_worlds[i].pos.capacity() < s->spare_worlds[best].pos.capacity()))
+        best = i;
+    HostWorld spare;
+    if (best < s->spare_worlds.size()) {
+      spare = std::move(s->spare_worlds[best]);
+      s->spare_worlds.erase(s->spare_worlds.begin() + best);
     }
-    {
-      std::vector<char> bad(64, 0);
-      parallel_for(scene.world.size(), [&](size_t b, size_t e) {
-        for (size_t i = b; i < e; ++i)
-          if (!scene.world.particle_valid(i)) {
-            bad[(b / 16384) % 64] = 1;
-            break;
-          }
-      });
-      for (char c : bad)
-        if (c) return fail(TMPM_INVALID_INPUT, "invalid MPM particle");
-    }
-    st = validate_resident_world(scene);
-    if (st != TMPM_OK) return st;
-    uint32_t slot = 0;
-    while (slot < s->scenes.size() && s->scenes[slot].active) ++slot;
-    if (slot >= 0xffffu) return fail(TMPM_CAPACITY, "too many scenes");
-    if (!s->prepared) {
-      std::vector<SceneState> previous = s->scenes;
-      if (slot == s->scenes.size()) s->scenes.push_back(scene);
-      else s->scenes[slot] = scene;
-      st = reprepare(s, previous);
-      if (st != TMPM_OK) return st;
-      if (scene_out) *scene_out = slot;
-      return TMPM_OK;
-    }
-    // Admission on the device: the new particles go after the current slots
-    // and the new insertion indices after the current ones; nothing already on
-    // the device moves.
-    if ((uint64_t)s->id_space + scene.world.size() >= (1ull << 31)) return fail(TMPM_CAPACITY, "too many particles");
-    scene.first = s->id_space;
-    scene.count = (uint32_t)scene.world.size();
-    for (size_t i = 0; i < scene.count; ++i)
-      if (scene.world.enabled[i]) scene.max_mass = std::max(scene.max_mass, scene.world.mass[i]);
-    st = build_scene(scene);
-    if (st != TMPM_OK) return st;
-    const size_t saved_classes = s->host_classes.size();
-    const bool appended = slot == s->scenes.size();
-    if (appended) s->scenes.push_back(std::move(scene));
-    else s->scenes[slot] = std::move(scene);
-    PackedParticles& pk = s->stage;
-    pk.clear();
-    st = pack_scene(s, slot, pk);
-    const bool was_plastic = s->plastic;
-    auto undo = [&]() {
-      s->plastic = was_plastic;
-      if (appended) s->scenes.pop_back();
-      else {
-        s->scenes[slot] = SceneState();
-        s->scenes[slot].active = false;
-      }
-      for (size_t c = saved_classes; c < s->host_classes.size(); ++c) {
-        const DevClass& d = s->host_classes[c];
-        s->class_ids.erase(class_key(d));
-      }
-      s->host_classes.resize(saved_classes);
-    };
-    if (st != TMPM_OK) {
-      undo();
-      return st;
-    }
-    s->plastic = classes_plastic(s);
-    // from here on device state changes; a failure invalidates the batch
-    auto broken = [&](tmpm_status code) {
-      s->valid = false;
-      return code;
-    };
-    const uint64_t need = (uint64_t)s->n + pk.size();
-    if (need > s->cap) {
-      const uint32_t cap = (uint32_t)std::max<uint64_t>(need + need / 4, (uint64_t)s->cap + s->cap / 2);
-      ParticleBuffers grown, spare, synced;
-      if (alloc_particles(grown, cap) != TMPM_OK || alloc_particles(spare, cap) != TMPM_OK ||
-          alloc_particles(synced, cap) != TMPM_OK)
-        return broken(TMPM_CAPACITY);
-      if (copy_particles(grown, cap, s->buf[s->cur], s->cap, s->n, s->stream) != TMPM_OK ||
-          copy_particles(synced, cap, s->synced, s->cap, s->synced_n, s->stream) != TMPM_OK)
-        return broken(TMPM_READBACK);
-      if (hipStreamSynchronize(s->stream) != hipSuccess) return broken(fail(TMPM_READBACK, "device copy failed"));
-      auto swap_bufs = [](ParticleBuffers& a, ParticleBuffers& b) {
-        std::swap(a.pos_mass.ptr, b.pos_mass.ptr); std::swap(a.pos_mass.bytes, b.pos_mass.bytes);
-        std::swap(a.vel_vol.ptr, b.vel_vol.ptr); std::swap(a.vel_vol.bytes, b.vel_vol.bytes);
-        std::swap(a.C.ptr, b.C.ptr); std::swap(a.C.bytes, b.C.bytes);
-        std::swap(a.F.ptr, b.F.ptr); std::swap(a.F.bytes, b.F.bytes);
-        std::swap(a.P.ptr, b.P.ptr); std::swap(a.P.bytes, b.P.bytes);
-        std::swap(a.cls.ptr, b.cls.ptr); std::swap(a.cls.bytes, b.cls.bytes);
-        std::swap(a.ins_id.ptr, b.ins_id.ptr); std::swap(a.ins_id.bytes, b.ins_id.bytes);
-      };
-      swap_bufs(s->buf[s->cur], grown);
-      swap_bufs(s->buf[1 - s->cur], spare);
-      swap_bufs(s->synced, synced);
-      s->cap = cap;
-    }
-    if ((st = upload_particles(s, pk, s->n)) != TMPM_OK) return broken(st);
-    s->n += (uint32_t)pk.size();
-    s->n_enabled = s->n;
-    s->id_space += s->scenes[slot].count;
-    if ((st = ensure_force_capacity(s, s->id_space + s->id_space / 4)) != TMPM_OK) return broken(st);
-    if ((st = upload_scene_forces(s, slot)) != TMPM_OK) return broken(st);
-    s->disabled_host.insert(s->disabled_host.end(), pk.disabled.begin(), pk.disabled.end());
-    if ((st = upload_disabled(s)) != TMPM_OK) return broken(st);
-    if ((st = upload_classes(s)) != TMPM_OK) return broken(st);
-    if ((st = upload_rigid(s)) != TMPM_OK) return broken(st);
-    if ((st = upload_obstacles(s)) != TMPM_OK) return broken(st);
-    if ((st = upload_scene(s)) != TMPM_OK) return broken(st);
-    // topology: room for the blocks this scene adds (and the per-particle arrays)
-    uint64_t pblocks = 0, gblocks = 0;
-    count_blocks(pk.pos32, pk.region, pk.inv_h, pblocks, gblocks);
-    s->last_pblocks += (uint32_t)pblocks;
-    s->last_gblocks += (uint32_t)gblocks;
-    const Topo& t = s->topo;
-    if (2ull * s->last_pblocks > t.pblock_cap || 2ull * s->last_gblocks > t.grid_cap || s->topo_cap < s->cap) {
-      const uint64_t gcap = std::min<uint64_t>(std::max<uint64_t>(t.grid_cap, 4ull * s->last_gblocks), 1ull << 26);
-      const uint64_t pcap = std::min<uint64_t>(std::max<uint64_t>(t.pblock_cap, 4ull * s->last_pblocks), s->cap);
-      if ((st = alloc_topology(s, (uint32_t)pcap, (uint32_t)gcap)) != TMPM_OK) return broken(st);
-    }
-    if (scene_out) *scene_out = slot;
-    return TMPM_OK;
+    st = stage_scene(world, substep_dt, &spare, staged);
+    if (st == TMPM_OK) st = commit_staged(s, staged, scene_out);
+    pack_pool().release(std::move(staged.pk));
+    return st;
   } catch (...) {
     return api_exception(s);
   }
@@ -3319,34 +3613,71 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
     if (substeps_run) *substeps_run = 0;
     if (!s->prepared || s->n == 0) return TMPM_OK;
     if (s->timing) return fail(TMPM_INVALID_INPUT, "autotuning needs per-kernel timing off");
+    // a bucket tuned before (this process, or the TMPM_TUNE_CACHE file): no trials
+    const TuneKey key = tune_key(s);
+    TuneShape hit;
+    if (tune_cache().find(key, hit)) {
+      s->p2g_threads = hit.p2g_threads;
+      s->p2g_copies = hit.p2g_copies;
+      s->threads_pblock = hit.g2p_threads;
+      s->reorder_interval = hit.reorder_interval;
+      s->use_graphs = hit.graphs != 0;
+      s->p2g_shape_fixed = s->g2p_shape_fixed = true;
+      if (!s->use_graphs) destroy_graphs(s);
+      return TMPM_OK;
+    }
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
     uint32_t ran = 0;
     tmpm_status st = TMPM_OK;
+    // The trials are queued in chunks of at most 64 substeps, the submit
+    // granularity.  After each chunk the device counters are read: the
+    // topology grows as synchronize would grow it (a spreading scene cannot
+    // outgrow its headroom because of tuning), and on any error the tuning
+    // stops, so the pending substeps report it at the next synchronize just as
+    // untuned stepping would.  Only the chunks' device time is measured.
+    bool stopped = false;
     auto run = [&](uint32_t steps, float* ms) -> tmpm_status {
-      if (hipEventRecord(a, s->stream) != hipSuccess) return fail(TMPM_READBACK, "event record failed");
-      for (uint32_t left = steps; left > 0;) {
+      float total = 0.0f;
+      for (uint32_t left = steps; left > 0 && !stopped;) {
         const uint32_t k = std::min<uint32_t>(left, 64);
+        if (hipEventRecord(a, s->stream) != hipSuccess) return fail(TMPM_READBACK, "event record failed");
         tmpm_status e = tmpm_session_submit_steps(s, k);
         if (e != TMPM_OK) return e;
         left -= k;
         ran += k;
+        if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
+          return fail(TMPM_READBACK, "event wait failed");
+        float chunk = 0.0f;
+        if (hipEventElapsedTime(&chunk, a, b) != hipSuccess) return fail(TMPM_READBACK, "event time failed");
+        total += chunk;
+        uint32_t counters[CNT_COUNT] = {};
+        HIP_TRY(hipMemcpy(counters, s->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+        const uint32_t fatal = s->batch ? (counters[CNT_ERROR] & (ERR_TOPOLOGY | ERR_INTERNAL)) : counters[CNT_ERROR];
+        if (fatal != 0) {
+          stopped = true;
+          break;
+        }
+        e = maybe_grow_topology(s, counters);
+        if (e != TMPM_OK) {
+          s->valid = false;
+          return e;
+        }
       }
-      if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
-        return fail(TMPM_READBACK, "event wait failed");
-      if (ms && hipEventElapsedTime(ms, a, b) != hipSuccess) return fail(TMPM_READBACK, "event time failed");
+      if (ms) *ms = total;
       return TMPM_OK;
     };
     // Tries each value of one knob (set by `apply`) and keeps the fastest.
     auto tune = [&](const std::vector<int>& values, const std::function<void(int)>& apply, int current) -> tmpm_status {
       float best_ms = 0.0f;
       int best = current;
-      for (size_t i = 0; i < values.size(); ++i) {
+      for (size_t i = 0; i < values.size() && !stopped; ++i) {
         apply(values[i]);
         float ms = 0.0f;
         tmpm_status e = run(trial, &ms);
         if (e != TMPM_OK) return e;
+        if (stopped) break;
         if (i == 0 || ms < best_ms) {
           best_ms = ms;
           best = values[i];
@@ -3356,29 +3687,74 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
       return TMPM_OK;
     };
     st = run(std::min<uint32_t>(trial, 16), nullptr);  // warm-up
-    if (st == TMPM_OK) {
+    if (st == TMPM_OK && !stopped) {
       effective_p2g_shape(s, s->p2g_threads, s->p2g_copies);  // trials start from the shape in use
       s->p2g_shape_fixed = true;  // the trials' shapes, not the size-based default, run from here on
       const int cur = (int)s->p2g_threads * 10 + s->p2g_copies;
       st = tune({642, 641, 1282, 1281}, [&](int v) { s->p2g_threads = (uint32_t)(v / 10); s->p2g_copies = v % 10; }, cur);
     }
-    if (st == TMPM_OK) {
+    if (st == TMPM_OK && !stopped) {
       s->threads_pblock = effective_g2p_threads(s);  // trials start from the size in use
       s->g2p_shape_fixed = true;
       st = tune({64, 128, 256}, [&](int v) { s->threads_pblock = (uint32_t)v; }, (int)s->threads_pblock);
     }
-    if (st == TMPM_OK)
+    if (st == TMPM_OK && !stopped)
       st = tune({4, 8, 16}, [&](int v) { s->reorder_interval = (uint32_t)v; }, (int)s->reorder_interval);
-    if (st == TMPM_OK) st = tune({0, 1}, [&](int v) { s->use_graphs = v != 0; }, s->use_graphs ? 1 : 0);
+    if (st == TMPM_OK && !stopped) st = tune({0, 1}, [&](int v) { s->use_graphs = v != 0; }, s->use_graphs ? 1 : 0);
     if (!s->use_graphs) destroy_graphs(s);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (substeps_run) *substeps_run = ran;
+    if (st == TMPM_OK && !stopped) {
+      // cached under the bucket the session was in when tuning started
+      TuneShape v;
+      v.p2g_threads = s->p2g_threads;
+      v.p2g_copies = s->p2g_copies;
+      v.g2p_threads = s->threads_pblock;
+      v.reorder_interval = s->reorder_interval;
+      v.graphs = s->use_graphs ? 1 : 0;
+      tune_cache().put(key, v);
+    }
     return st;
   } catch (...) {
     return api_exception(s);
   }
 }
+
+tmpm_status tmpm_session_set_audit(tmpm_session* s, int32_t enabled) {
+  try {
+    tmpm_status st = check_editable(s);
+    if (st != TMPM_OK) return st;
+    s->audit_on = enabled != 0;
+    if (!s->audit_on) {
+      s->audit_dev.release();
+      s->audit_cap = 0;
+    }
+    return TMPM_OK;
+  } catch (...) {
+    return api_exception(s);
+  }
+}
+
+tmpm_status tmpm_session_read_audit(const tmpm_session* s, uint32_t scene, double* out, uint64_t cap_records,
+                                    uint64_t* count_out) {
+  try {
+    if (!s || !count_out || (cap_records && !out)) return fail(TMPM_INVALID_INPUT, "null argument");
+    if (scene >= s->scenes.size()) return fail(TMPM_INVALID_INPUT, "no such scene");
+    const uint64_t steps = s->audit_host_steps;
+    *count_out = steps;
+    for (uint64_t k = 0; k < std::min(steps, cap_records); ++k)
+      std::memcpy(out + k * kAuditFields,
+                  s->audit_host.data() + (k * s->audit_host_stride + scene) * (size_t)kAuditFields,
+                  sizeof(double) * kAuditFields);
+    return TMPM_OK;
+  } catch (...) {
+    return exception_status();
+  }
+}
+
+void tmpm_tune_cache_clear(void) { tune_cache().clear(); }
+uint64_t tmpm_tune_cache_size(void) { return tune_cache().size(); }
 
 tmpm_status tmpm_session_set_graphs(tmpm_session* s, int32_t enabled) {
   try {

@@ -213,6 +213,15 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_session_wait": (C.c_int, [vp]),
         "tmpm_session_synchronize": (C.c_int, [vp]),
         "tmpm_session_step": (C.c_int, [vp, C.c_uint32]),
+        "tmpm_session_synchronize_async": (C.c_int, [vp, C.POINTER(C.c_int32)]),
+        "tmpm_batch_stage_scene": (C.c_int, [C.POINTER(CWorld), C.c_double, C.POINTER(vp)]),
+        "tmpm_batch_add_staged": (C.c_int, [vp, vp, C.POINTER(C.c_uint32)]),
+        "tmpm_staged_scene_destroy": (None, [vp]),
+        "tmpm_session_set_audit": (C.c_int, [vp, C.c_int32]),
+        "tmpm_session_read_audit": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.c_uint64,
+                                              C.POINTER(C.c_uint64)]),
+        "tmpm_tune_cache_clear": (None, []),
+        "tmpm_tune_cache_size": (C.c_uint64, []),
         "tmpm_session_step_adaptive": (C.c_int, [vp, C.c_double]),
         "tmpm_session_pending_substeps": (C.c_uint64, [vp]),
         "tmpm_session_valid": (C.c_int32, [vp]),
@@ -982,6 +991,20 @@ class HipMpmResidentSession:
         """Device completion only (no readback, no validation)."""
         _check(self.lib.tmpm_session_wait(self._h), self.lib)
 
+    def synchronize_async(self) -> bool:
+        """``synchronize_async`` (gpu.rs:413-450) as a poll: False while the device still runs the
+        submitted substeps (nothing changes; they stay pending), True once they were validated and
+        committed as by :meth:`synchronize`, whose errors it raises."""
+        return _poll_synchronize(self.lib, self._h)
+
+    def set_audit(self, enabled: bool) -> None:
+        """FP64 per-substep audit of mass, momentum, kinetic energy, first moment and angular momentum."""
+        _check(self.lib.tmpm_session_set_audit(self._h, 1 if enabled else 0), self.lib)
+
+    def audit(self) -> np.ndarray:
+        """Audit records of the substeps of the last synchronize, one row of AUDIT_FIELDS each."""
+        return _read_audit(self.lib, self._h, 0)
+
     def step_adaptive(self, dt: float) -> None:
         """CFL-limited substeps (``MpmWorld::step_with_gpu_transfers``)."""
         _check(self.lib.tmpm_session_step_adaptive(self._h, float(dt)), self.lib)
@@ -1216,12 +1239,23 @@ class HipMpmBatch:
     def step(self, steps: int) -> None:
         _check(self.lib.tmpm_session_step(self._h, int(steps)), self.lib)
 
+    def synchronize_async(self) -> bool:
+        """A poll: False while the submitted substeps still run, True once synchronized."""
+        return _poll_synchronize(self.lib, self._h)
+
     def pending_substeps(self) -> int:
         return int(self.lib.tmpm_session_pending_substeps(self._h))
 
     @property
     def valid(self) -> bool:
         return bool(self.lib.tmpm_session_valid(self._h))
+
+    def set_audit(self, enabled: bool) -> None:
+        _check(self.lib.tmpm_session_set_audit(self._h, 1 if enabled else 0), self.lib)
+
+    def audit(self, scene: int) -> np.ndarray:
+        """Audit records of one scene for the substeps of the last synchronize."""
+        return _read_audit(self.lib, self._h, scene)
 
     # -- scenes -------------------------------------------------------------
     def scene_count(self) -> int:
@@ -1274,6 +1308,14 @@ class HipMpmBatch:
         _check(self.lib.tmpm_batch_add_scene(self._h, C.byref(desc), float(substep_dt), C.byref(slot)), self.lib)
         del keep
         self._worlds[int(slot.value)] = _SceneHeader(world)
+        return int(slot.value)
+
+    def add_staged(self, staged: "StagedScene") -> int:
+        """Admits a scene staged by :class:`StagedScene` (needs no pending substeps); consumes it."""
+        h = staged.take()
+        slot = C.c_uint32()
+        _check(self.lib.tmpm_batch_add_staged(self._h, h, C.byref(slot)), self.lib)
+        self._worlds[int(slot.value)] = _SceneHeader(staged.world)
         return int(slot.value)
 
     def set_reactions(self, enabled: bool) -> None:
@@ -1329,6 +1371,66 @@ class HipMpmBatch:
 
     def kernel_times(self) -> dict[str, tuple[float, int]]:
         return HipMpmResidentSession.kernel_times(self)  # same session-level entry point
+
+
+AUDIT_FIELDS = ("mass", "px", "py", "pz", "kinetic", "mx", "my", "mz", "lx", "ly", "lz", "count")
+
+
+def _poll_synchronize(lib, h) -> bool:
+    done = C.c_int32()
+    _check(lib.tmpm_session_synchronize_async(h, C.byref(done)), lib)
+    return bool(done.value)
+
+
+def _read_audit(lib, h, scene: int) -> np.ndarray:
+    count = C.c_uint64()
+    _check(lib.tmpm_session_read_audit(h, int(scene), None, 0, C.byref(count)), lib)
+    n = int(count.value)
+    buf = (C.c_double * max(1, n * len(AUDIT_FIELDS)))()
+    _check(lib.tmpm_session_read_audit(h, int(scene), buf, n, C.byref(count)), lib)
+    return np.frombuffer(buf, dtype=np.float64, count=n * len(AUDIT_FIELDS)).reshape(n, len(AUDIT_FIELDS)).copy()
+
+
+class StagedScene:
+    """The host half of a batch admission (validation, f64 copy, f32 packing, block count),
+    done without a session or the device, so it can run while a batch has substeps in flight;
+    :meth:`HipMpmBatch.add_staged` admits it."""
+
+    def __init__(self, world: MpmWorld, substep_dt: float):
+        self.lib = load_library()
+        self.world = world
+        self.dt = float(substep_dt)
+        desc, keep = world.c_desc()
+        h = C.c_void_p()
+        _check(self.lib.tmpm_batch_stage_scene(C.byref(desc), self.dt, C.byref(h)), self.lib)
+        del keep
+        self._h = h
+
+    def take(self):
+        if not getattr(self, "_h", None):
+            raise InvalidInput("staged scene already admitted or released")
+        h, self._h = self._h, None
+        return h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.tmpm_staged_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_cache_clear() -> None:
+    """Forgets every tuned launch shape of this process (the next autotune runs its trials)."""
+    load_library().tmpm_tune_cache_clear()
+
+
+def tune_cache_size() -> int:
+    return int(load_library().tmpm_tune_cache_size())
 
 
 def bandwidth_probe(device: int = 0, nbytes: int = 1 << 30, iterations: int = 20) -> float:

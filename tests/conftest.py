@@ -20,3 +20,14 @@ def transfers():
     from tessera_amd.mpm import HipMpmTransfers
 
     return HipMpmTransfers(0)
+
+
+@pytest.fixture(autouse=True)
+def _fresh_tune_cache(request):
+    """GPU tests start with an empty autotuner cache: a shape tuned by an
+    earlier test must not replace another test's default launch shape."""
+    if request.node.get_closest_marker("gpu") is not None:
+        from tessera_amd.mpm import tune_cache_clear
+
+        tune_cache_clear()
+    yield

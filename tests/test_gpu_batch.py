@@ -161,3 +161,41 @@ def test_256_scene_batch_conserves_each_scenes_momentum_under_gravity(transfers)
         scale = float(np.abs(m[:, None] * np.asarray(w.particles.velocity)).sum())
         assert b.scene_error(k) is None
         assert np.max(np.abs(p1 - p0 - impulse)) < 1e-5 * scale, k
+
+
+def test_a_diverging_unbounded_scene_cannot_exhaust_the_shared_capacity(transfers):
+    """A scene without walls whose particles are all thrown past the CFL bound
+    would, binned again, need a block per particle and overflow the hash and
+    grid capacity that the batch's scenes share (a session-wide Capacity
+    failure).  Once a scene has failed it is no longer binned: it alone stops,
+    keeps its last synchronized world, and the other scenes equal the same
+    worlds stepped alone."""
+    import dataclasses
+
+    worlds = mixed_worlds()[:3]
+    free = worlds[1].clone()
+    free.params = dataclasses.replace(free.params, bounds=None)
+    worlds[1] = free
+    dts = dts_for(worlds)
+    b = HipMpmBatch(transfers, worlds, dts, "fp32")
+    b.step(2)
+    before = b.read_particles(1)
+    n = len(free.particles)
+    rng = np.random.default_rng(3)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    m = np.asarray(free.particles.mass)
+    # 1e4 m/s in random directions: ~0.5 m (dozens of cells) in one substep
+    for i in range(n):
+        b.set_particle_force(1, i, dirs[i] * 1.0e4 * m[i] / dts[1])
+    b.submit_steps(8)
+    b.synchronize()
+    assert b.valid
+    assert isinstance(b.scene_error(1), ResidentUnavailable)
+    assert b.substeps(1) == 2
+    assert_same(b.read_particles(1), before)
+    b.step(3)
+    for k in (0, 2):
+        assert b.scene_error(k) is None
+        assert b.substeps(k) == 13
+        assert_same(b.read_particles(k), alone(transfers, worlds[k], dts[k], 13, "fp32"))

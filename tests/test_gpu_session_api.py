@@ -1,0 +1,199 @@
+"""Session entry points beside stepping: the non-blocking synchronize, the
+two-half batch admission, the FP64 audit and the autotuner's cache and
+capacity handling.
+
+* ``synchronize_async`` (gpu.rs:413-450) is a poll here: while the device
+  still runs the submitted substeps it changes nothing and the substeps stay
+  pending; once they finish it validates and commits exactly as
+  ``synchronize`` (gpu.rs:382-411) does, failures included.
+* A staged admission (host half while substeps run, device half at the next
+  synchronize point) equals ``add_scene`` of the same world bit for bit.
+* The audit's f64 sums are checked against a numpy reduction of the same
+  synchronized f32 state, and its per-substep momentum against the gravity
+  impulse (partition of unity: the APIC transfers conserve momentum, the grid
+  update adds m g dt per node).
+"""
+
+import dataclasses
+import time
+
+import numpy as np
+import pytest
+
+from tessera_amd.mpm import (HipMpmBatch, HipMpmResidentSession, MaterialModel, MpmParams, MpmWorld,
+                             ResidentUnavailable, StagedScene, tune_cache_clear, tune_cache_size)
+from tessera_amd.scenes import _particles, four_cubes, uniform_cube
+
+from .test_gpu_batch import FIELDS, assert_same, dts_for, mixed_worlds
+
+pytestmark = pytest.mark.gpu
+
+
+def test_synchronize_async_polls_while_pending_then_commits_like_synchronize(transfers):
+    sc = four_cubes()
+    a = HipMpmResidentSession(transfers, sc.world, sc.dt)
+    b = HipMpmResidentSession(transfers, sc.world, sc.dt)
+    a.submit_steps(64)  # ~13 ms of device work on the 1M-particle scene
+    first = a.synchronize_async()
+    assert not first, "64 substeps of a million particles cannot have finished already"
+    assert a.pending_substeps() == 64 and a.valid
+    polls = 1
+    while not a.synchronize_async():
+        polls += 1
+        time.sleep(0.0005)
+    assert a.pending_substeps() == 0 and a.world().substeps == 64
+    assert a.synchronize_async() is True  # nothing pending: done at once
+    b.submit_steps(64)
+    b.synchronize()
+    assert_same(a.read_particles(), b.read_particles())
+    print(f"synchronize_async: {polls} polls")
+
+
+def test_synchronize_async_failure_invalidates_and_keeps_the_last_world(transfers):
+    sc = uniform_cube(n=2048)
+    s = HipMpmResidentSession(transfers, sc.world, sc.dt)
+    s.step(3)
+    before = s.read_particles()
+    m = float(sc.world.particles.mass[0])
+    s.set_particle_force(0, [300.0 * m / sc.dt, 0.0, 0.0])  # 300 m/s: beyond the CFL bound
+    s.submit_steps(5)
+    with pytest.raises(ResidentUnavailable):
+        while not s.synchronize_async():
+            time.sleep(0.0005)
+    assert not s.valid
+    assert s.world().substeps == 3
+    assert_same(s.read_particles(), before)
+    with pytest.raises(ResidentUnavailable):
+        s.synchronize_async()
+
+
+def test_staged_admission_equals_add_scene_bit_for_bit(transfers):
+    worlds = mixed_worlds()
+    dts = dts_for(worlds)
+    a = HipMpmBatch(transfers, worlds[:2], dts[:2])
+    b = HipMpmBatch(transfers, worlds[:2], dts[:2])
+    a.submit_steps(9)
+    staged = StagedScene(worlds[2], dts[2])  # host half while a's substeps run
+    a.synchronize()
+    b.step(9)
+    ka = a.add_staged(staged)
+    kb = b.add_scene(worlds[2], dts[2])
+    assert ka == kb == 2
+    with pytest.raises(Exception):
+        a.add_staged(staged)  # consumed
+    a.remove_scene(0)
+    b.remove_scene(0)
+    a.submit_steps(4)
+    staged = StagedScene(worlds[3], dts[3])
+    a.synchronize()
+    b.step(4)
+    assert a.add_staged(staged) == b.add_scene(worlds[3], dts[3]) == 0
+    a.step(6)
+    b.step(6)
+    for k in (0, 1, 2):
+        assert a.substeps(k) == b.substeps(k)
+        assert_same(a.read_particles(k), b.read_particles(k))
+
+
+def _audit_reference(pa):
+    m = pa.mass * np.asarray(pa.enabled, bool)
+    x, v = pa.position, pa.velocity
+    terms = np.concatenate([m[:, None], m[:, None] * v, (0.5 * m * (v * v).sum(1))[:, None], m[:, None] * x,
+                            m[:, None] * np.cross(x, v)], axis=1)
+    return terms.sum(0), np.abs(terms).sum(0)
+
+
+def test_audit_matches_numpy_and_momentum_follows_gravity(transfers):
+    sc = uniform_cube(n=8192, seed=1)
+    w = sc.world.clone()
+    w.set_obstacles([])  # free fall: the cube never reaches a wall in 12 substeps
+    s = HipMpmResidentSession(transfers, w, sc.dt)
+    s.set_audit(True)
+    s.step(12)
+    rec = s.audit()
+    assert rec.shape == (12, 12)
+    got = s.read_particles()
+    ref, scale = _audit_reference(got)
+    assert rec[-1, 11] == np.count_nonzero(got.enabled)
+    err = np.abs(rec[-1, :11] - ref) / np.maximum(scale, 1e-300)
+    assert np.all(err < 1e-12), err
+    # mass is conserved exactly; momentum changes by M g dt per substep
+    mass = rec[:, 0]
+    assert np.allclose(mass, mass[0], rtol=1e-14, atol=0.0)
+    g = np.asarray(w.params.gravity)
+    dp = np.diff(rec[:, 1:4], axis=0)
+    expect = mass[0] * g * sc.dt
+    assert np.max(np.abs(dp - expect)) < 1e-4 * np.linalg.norm(expect), (dp[:3], expect)
+    # a batch audits each scene separately
+    worlds = mixed_worlds()[:3]
+    dts = dts_for(worlds)
+    bt = HipMpmBatch(transfers, worlds, dts)
+    bt.set_audit(True)
+    bt.step(5)
+    for k in range(3):
+        r = bt.audit(k)
+        assert r.shape == (5, 12)
+        ref, scale = _audit_reference(bt.read_particles(k))
+        err = np.abs(r[-1, :11] - ref) / np.maximum(scale, 1e-300)
+        assert np.all(err < 1e-12), (k, err)
+
+
+def test_autotune_cache_skips_trials_in_a_tuned_bucket(transfers):
+    tune_cache_clear()
+    sc = uniform_cube(n=8192)
+    a = HipMpmResidentSession(transfers, sc.world, sc.dt)
+    cfg_a = a.autotune(8)
+    assert cfg_a["substeps_run"] > 0
+    assert tune_cache_size() >= 1
+    b = HipMpmResidentSession(transfers, sc.world, sc.dt)
+    cfg_b = b.autotune(8)
+    assert cfg_b["substeps_run"] == 0
+    assert {k: v for k, v in cfg_a.items() if k != "substeps_run"} == \
+        {k: v for k, v in cfg_b.items() if k != "substeps_run"}
+    # tuning changes no result: both equal an untuned session at the same substep count
+    a.synchronize()
+    ref = HipMpmResidentSession(transfers, sc.world, sc.dt)
+    n = cfg_a["substeps_run"]
+    while n:
+        k = min(64, n)
+        ref.submit_steps(k)
+        n -= k
+    ref.synchronize()
+    assert_same(a.read_particles(), ref.read_particles())
+    tune_cache_clear()
+
+
+def _spreading_cloud(n_axis: int = 16) -> tuple[MpmWorld, float]:
+    """A soft fluid ball in an unbounded world flying apart at 2 m/s: its
+    particle-block count grows from ~60 to thousands within 3,000 substeps."""
+    rng = np.random.default_rng(11)
+    h = 1.0 / 64
+    g = (np.arange(n_axis) + 0.5) / n_axis
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * (n_axis * h / 2) + 0.5
+    pos += rng.uniform(-0.1, 0.1, size=pos.shape) * h / 2
+    d = pos - pos.mean(0)
+    vel = 2.0 * d / np.maximum(np.linalg.norm(d, axis=1, keepdims=True), 1e-9)
+    pa = _particles(pos, vel, h / 4, 1.0e3, MaterialModel.fluid(1.0e2, 7.0, 0.0))
+    params = MpmParams(cell_width=h, gravity=np.zeros(3), max_substep=1e-4, bounds=None)
+    return MpmWorld(pa, params), 1.0e-4
+
+
+def test_autotune_grows_the_topology_of_a_spreading_scene(transfers):
+    tune_cache_clear()
+    world, dt = _spreading_cloud()
+    s = HipMpmResidentSession(transfers, world, dt)
+    cfg = s.autotune(256)
+    steps = cfg["substeps_run"]
+    assert steps == 16 + 256 * 12
+    s.synchronize()  # no Capacity: the topology grew between the tuning chunks
+    assert s.valid
+    ref = HipMpmResidentSession(transfers, world, dt)
+    left = steps
+    while left:
+        k = min(64, left)
+        ref.step(k)
+        left -= k
+    st = ref.topology_stats()
+    assert st["particle_blocks"] > 1000
+    assert_same(s.read_particles(), ref.read_particles())
+    tune_cache_clear()
