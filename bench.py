@@ -148,26 +148,38 @@ def scenes_leg(transfers, mode: str, steps: int) -> dict:
     return out
 
 
-def serving_leg(transfers, mode: str, seconds: float, single_rate: float) -> dict:
+def serving_leg(transfers, mode: str, seconds: float, single_rate: float, plain: bool = True) -> dict:
     """Continuous batching: Poisson arrivals offering 80% of the measured single-scene rate,
     N log-uniform in [4096, 524288], 1-4 rigid boxes per scene, horizons U{50..2000} substeps,
-    admission under an HBM budget; runs until every request has completed."""
+    admission under an HBM budget; runs until every request has completed.  The same stream is
+    then served by the plain per-scene path (one resident session per request on the same GPU)."""
     from tessera_amd.scenes import serving_requests
-    from tessera_amd.serving import Server
+    from tessera_amd.serving import PlainServer, Server
 
     mean_n = (524288 - 4096) / math.log(524288 / 4096)
     work = mean_n * 1025.0  # particle-updates per request
     rate = 0.8 * single_rate / work
     count = max(1, int(round(rate * seconds)))
     reqs = serving_requests(count, rate, seed=7)
-    server = Server(transfers, hbm_budget=16 << 30, steps_per_tick=32, mode=mode, keep_worlds=False)
+    budget = 16 << 30
+    server = Server(transfers, hbm_budget=budget, steps_per_tick=32, mode=mode, keep_worlds=False)
     report = server.run(reqs)
     server.close()
     report.update({"workload": "serving_poisson", "arrival_rate_per_s": rate, "offered_fraction": 0.8,
-                   "steps_per_tick": 32, "hbm_budget_bytes": 16 << 30})
+                   "steps_per_tick": 32, "hbm_budget_bytes": budget})
     print(f"bench: serving {report['completed']}/{count} requests in {report['wall_s']:.1f}s: "
           f"{report['value']:.4e} particle-updates/s, step latency p50 {report['step_latency_ms_p50']:.3f} ms "
           f"p99 {report['step_latency_ms_p99']:.3f} ms", file=sys.stderr, flush=True)
+    if plain:
+        ps = PlainServer(transfers, hbm_budget=budget, steps_per_tick=32, mode=mode, keep_worlds=False)
+        base = ps.run(reqs)
+        ps.close()
+        report["plain"] = base
+        report["speedup"] = report["value"] / base["value"] if base["value"] > 0 else None
+        print(f"bench: plain per-scene path {base['completed']}/{count} requests in {base['wall_s']:.1f}s: "
+              f"{base['value']:.4e} particle-updates/s, step latency p50 {base['step_latency_ms_p50']:.3f} ms "
+              f"p99 {base['step_latency_ms_p99']:.3f} ms; batched/plain {report['speedup']:.2f}x",
+              file=sys.stderr, flush=True)
     return report
 
 

@@ -208,6 +208,12 @@ tmpm_status tmpm_session_step(tmpm_session* s, uint32_t steps);
 tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt);
 /* `has_pending_steps()` / `pending_substeps()` (gpu.rs:367-375). */
 uint64_t tmpm_session_pending_substeps(const tmpm_session* s);
+/* Device memory the session holds (no reference counterpart: wgpu sizes
+ * its buffers per prepared transfer, gpu.rs:1391-1405): the bytes of every
+ * device allocation, and the particle capacity of the state buffers.  A batch
+ * keeps its capacity when scenes retire and reuses it for later admissions;
+ * the serving loop budgets admissions on these figures. */
+tmpm_status tmpm_session_memory(const tmpm_session* s, uint64_t* device_bytes, uint64_t* particle_capacity);
 /* Whether the session is still usable (no failure since creation). */
 int32_t tmpm_session_valid(const tmpm_session* s);
 /* `world().substeps` (world.rs:215). */

@@ -1990,30 +1990,64 @@ int32_t orc_cell_f32(const float* x, float h32, int32_t* cell) {
 // `region` carries the transfer colour (and, in batches, the scene).  Writes
 // up to `cap` sorted 4-int keys (x, y, z, region) and returns the set size, or
 // UINT64_MAX if some active particle has an unrepresentable cell.
+//
+// The union of the stencils {c + (a, b, e) : a, b, e in 0..2} is the distinct
+// base cells dilated by the box [0, 2]^3, which is three one-axis dilations
+// by [0, 2] in turn (a box is the product of its edges).  A call with out ==
+// NULL (the size query) keeps its result for this thread, and the next call
+// with the same inputs copies it instead of building the set again.
 uint64_t orc_node_set(uint64_t n, const float* pos, const uint8_t* active, const uint32_t* region, float h32,
                       int32_t* out, uint64_t cap) {
-  // distinct base cells first (several particles share a cell), then their stencils
-  std::vector<std::array<int32_t, 4>> cells;
-  cells.reserve(n);
-  for (uint64_t i = 0; i < n; ++i) {
-    if (!active[i]) continue;
-    int32_t c[3];
-    if (!orc_cell_f32(pos + 3 * i, h32, c)) return UINT64_MAX;
-    cells.push_back({c[0], c[1], c[2], (int32_t)region[i]});
+  using Key = std::array<int32_t, 4>;
+  struct Memo {
+    uint64_t n = 0;
+    const float* pos = nullptr;
+    const uint8_t* active = nullptr;
+    const uint32_t* region = nullptr;
+    float h = 0.0f;
+    std::vector<Key> keys;
+  };
+  thread_local Memo memo;
+  const bool same = memo.pos != nullptr && memo.n == n && memo.pos == pos && memo.active == active &&
+                    memo.region == region && memo.h == h32;
+  if (!(out && same)) {
+    memo = Memo();
+    std::vector<Key> cells;
+    cells.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      if (!active[i]) continue;
+      int32_t c[3];
+      if (!orc_cell_f32(pos + 3 * i, h32, c)) return UINT64_MAX;
+      cells.push_back({c[0], c[1], c[2], (int32_t)region[i]});
+    }
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    for (int axis = 2; axis >= 0; --axis) {
+      std::vector<Key> grown;
+      grown.reserve(cells.size() * 3);
+      for (const Key& c : cells)
+        for (int d = 0; d < 3; ++d) {
+          Key k = c;
+          k[axis] += d;
+          grown.push_back(k);
+        }
+      std::sort(grown.begin(), grown.end());
+      grown.erase(std::unique(grown.begin(), grown.end()), grown.end());
+      cells.swap(grown);
+    }
+    memo.keys.swap(cells);
+    memo.n = n;
+    memo.pos = pos;
+    memo.active = active;
+    memo.region = region;
+    memo.h = h32;
   }
-  std::sort(cells.begin(), cells.end());
-  cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
-  std::vector<std::array<int32_t, 4>> keys;
-  keys.reserve(cells.size() * 27);
-  for (const auto& c : cells)
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b)
-        for (int e = 0; e < 3; ++e) keys.push_back({c[0] + a, c[1] + b, c[2] + e, c[3]});
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  for (uint64_t i = 0; i < keys.size() && i < cap; ++i)
+  const std::vector<Key>& keys = memo.keys;
+  for (uint64_t i = 0; out && i < keys.size() && i < cap; ++i)
     for (int k = 0; k < 4; ++k) out[4 * i + k] = keys[i][k];
-  return keys.size();
+  const uint64_t size = keys.size();
+  if (out) memo = Memo();
+  return size;
 }
 
 // Particle-block partition: block = cell >> 2 (floor division by 4) per axis,

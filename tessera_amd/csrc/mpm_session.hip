@@ -688,6 +688,26 @@ struct tmpm_session {
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, uint64_t>> kernel_ms;
 
+  // Bytes of every device allocation the session holds.
+  uint64_t device_bytes() const {
+    uint64_t total = 0;
+    auto add = [&](const DevBuf& b) { total += b.ptr ? b.bytes : 0; };
+    auto add_parts = [&](const ParticleBuffers& b) {
+      for (const DevBuf* d : {&b.pos_mass, &b.vel_vol, &b.C, &b.F, &b.P, &b.cls, &b.ins_id}) add(*d);
+    };
+    add_parts(buf[0]);
+    add_parts(buf[1]);
+    add_parts(synced);
+    for (const DevBuf* d :
+         {&staging, &disabled_rec, &force, &classes, &scene_plain, &scene_force, &obstacles, &scene_err, &planes,
+          &react_bins, &react_sub, &react_total, &rigid, &rigid_maps, &body_scene, &owner, &bkey, &bucket_count,
+          &nbr_bucket, &bucket_slot, &bucket_pblock, &grid_key, &pblock_bucket, &pblock_slot8, &pblock_start,
+          &pblock_len, &pslot, &prank, &sorted, &counters, &ptile, &grid_rev, &grid_vel, &node_mask, &bin_stage,
+          &bin_nstage, &bin_claims, &bin_nclaims, &bin_csum, &bmask, &pmask, &audit_dev})
+      add(*d);
+    return total;
+  }
+
   Parts parts(int which) const {
     Parts p{};
     const ParticleBuffers& b = buf[which];
@@ -3044,6 +3064,12 @@ tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt) {
 }
 
 uint64_t tmpm_session_pending_substeps(const tmpm_session* s) { return s ? s->pending : 0; }
+tmpm_status tmpm_session_memory(const tmpm_session* s, uint64_t* device_bytes, uint64_t* particle_capacity) {
+  if (!s || !device_bytes || !particle_capacity) return fail(TMPM_INVALID_INPUT, "null argument");
+  *device_bytes = s->device_bytes();
+  *particle_capacity = s->cap;
+  return TMPM_OK;
+}
 int32_t tmpm_session_valid(const tmpm_session* s) { return s && s->valid ? 1 : 0; }
 uint64_t tmpm_session_substeps(const tmpm_session* s) { return s && !s->scenes.empty() ? s->world().substeps : 0; }
 uint64_t tmpm_session_particle_count(const tmpm_session* s) { return s && !s->scenes.empty() ? s->world().size() : 0; }

@@ -214,6 +214,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_session_synchronize": (C.c_int, [vp]),
         "tmpm_session_step": (C.c_int, [vp, C.c_uint32]),
         "tmpm_session_synchronize_async": (C.c_int, [vp, C.POINTER(C.c_int32)]),
+        "tmpm_session_memory": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "tmpm_batch_stage_scene": (C.c_int, [C.POINTER(CWorld), C.c_double, C.POINTER(vp)]),
         "tmpm_batch_add_staged": (C.c_int, [vp, vp, C.POINTER(C.c_uint32)]),
         "tmpm_staged_scene_destroy": (None, [vp]),
@@ -997,6 +998,10 @@ class HipMpmResidentSession:
         committed as by :meth:`synchronize`, whose errors it raises."""
         return _poll_synchronize(self.lib, self._h)
 
+    def memory(self) -> dict:
+        """Device bytes the session holds and the particle capacity of its state buffers."""
+        return _memory(self.lib, self._h)
+
     def set_audit(self, enabled: bool) -> None:
         """FP64 per-substep audit of mass, momentum, kinetic energy, first moment and angular momentum."""
         _check(self.lib.tmpm_session_set_audit(self._h, 1 if enabled else 0), self.lib)
@@ -1250,6 +1255,11 @@ class HipMpmBatch:
     def valid(self) -> bool:
         return bool(self.lib.tmpm_session_valid(self._h))
 
+    def memory(self) -> dict:
+        """Device bytes the batch holds and the particle capacity of its state buffers (kept when
+        scenes retire, reused by later admissions)."""
+        return _memory(self.lib, self._h)
+
     def set_audit(self, enabled: bool) -> None:
         _check(self.lib.tmpm_session_set_audit(self._h, 1 if enabled else 0), self.lib)
 
@@ -1374,6 +1384,12 @@ class HipMpmBatch:
 
 
 AUDIT_FIELDS = ("mass", "px", "py", "pz", "kinetic", "mx", "my", "mz", "lx", "ly", "lz", "count")
+
+
+def _memory(lib, h) -> dict:
+    nbytes, cap = C.c_uint64(), C.c_uint64()
+    _check(lib.tmpm_session_memory(h, C.byref(nbytes), C.byref(cap)), lib)
+    return {"device_bytes": int(nbytes.value), "particle_capacity": int(cap.value)}
 
 
 def _poll_synchronize(lib, h) -> bool:

@@ -131,10 +131,17 @@ def run_checkpoints(transfers, scene, mode: str, checkpoints, log=print) -> list
 def trajectory(transfers, scene, mode: str, steps: int, threads: int | None = None, fp32: bool = True) -> dict:
     """Whole-trajectory error after `steps` substeps from the initial state:
     device and FP32 restatement against the FP64 oracle."""
-    s = HipMpmResidentSession(transfers, scene.world, scene.dt, mode)
-    advance(s, steps)
-    got = s.read_particles()
-    s.close()
+    return trajectories(transfers, scene, (mode,), steps, threads, fp32)[0]
+
+
+def trajectories(transfers, scene, modes, steps: int, threads: int | None = None, fp32: bool = True) -> list[dict]:
+    """`trajectory` for several P2G modes against one FP64 (and FP32) oracle run."""
+    got = {}
+    for mode in modes:
+        s = HipMpmResidentSession(transfers, scene.world, scene.dt, mode)
+        advance(s, steps)
+        got[mode] = s.read_particles()
+        s.close()
     cpu = O.OracleWorld(scene.world)
     f32 = cpu.fp32_substeps(scene.dt, steps) if fp32 else None
     if threads:
@@ -144,7 +151,10 @@ def trajectory(transfers, scene, mode: str, steps: int, threads: int | None = No
     ref = cpu.read()
     pa = scene.world.particles
     live = np.asarray(pa.enabled, bool) & ~np.asarray(pa.fixed, bool)
-    gpu_err = {f: max_err(getattr(got, f), ref[f], live) for f in FIELDS}
     floor = {f: max_err(f32[f], ref[f], live) for f in FIELDS} if fp32 else None
-    return dict(config=scene.name, mode=mode, steps=steps, gpu=gpu_err, floor=floor,
-                tolerance=tolerance(floor) if floor else dict(TIER))
+    out = []
+    for mode in modes:
+        gpu_err = {f: max_err(getattr(got[mode], f), ref[f], live) for f in FIELDS}
+        out.append(dict(config=scene.name, mode=mode, steps=steps, gpu=gpu_err, floor=floor,
+                        tolerance=tolerance(floor) if floor else dict(TIER)))
+    return out

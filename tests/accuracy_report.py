@@ -81,20 +81,19 @@ def topo_table(rows: list[dict]) -> list[str]:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default="gpurun_out/accuracy_vs_fp64.md")
-    ap.add_argument("--c2-window", type=int, default=100)
+    ap.add_argument("--c2-window", type=int, default=20)
     ap.add_argument("--skip-c3", action="store_true")
     args = ap.parse_args()
     t0 = time.perf_counter()
     transfers = HipMpmTransfers(0)
     report = {"c1": c1_rows(transfers)}
     c2 = four_cubes()
-    report["c2_window"] = []
-    for mode in ("fp32", "fixed64"):
-        log(f"C2 {mode}: whole-trajectory window of {args.c2_window} substeps")
-        r = A.trajectory(transfers, c2, mode, args.c2_window, threads=A.oracle_threads())
+    log(f"C2: whole-trajectory window of {args.c2_window} substeps")
+    report["c2_window"] = A.trajectories(transfers, c2, ("fp32", "fixed64"), args.c2_window,
+                                         threads=A.oracle_threads())
+    for r in report["c2_window"]:
         r["checkpoint"] = f"0-{args.c2_window} (whole window)"
-        log(f"C2 {mode} window: gpu {A.fmt(r['gpu'])} floor {A.fmt(r['floor'])}")
-        report["c2_window"].append(r)
+        log(f"C2 {r['mode']} window: gpu {A.fmt(r['gpu'])} floor {A.fmt(r['floor'])}")
     report["c2"] = []
     for mode in ("fp32", "fixed64"):
         log(f"C2 {mode}: checkpoints")

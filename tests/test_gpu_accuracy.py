@@ -2,9 +2,10 @@
 FP64 oracle (tests/accuracy.py describes the checkpoint method).
 
 * C2, 1,048,576 elastic particles, 1000 substeps: checkpoints 0, 250, 500
-  and 1000 in both P2G modes, plus the whole first 20 substeps.
+  and 1000 in both P2G modes, plus the whole first 10 substeps.
 * C3, 4,194,304 fluid particles, 2000 substeps of a dam break with block
-  churn: checkpoints 0, 500, 1000 and 2000 in both modes.
+  churn: checkpoints 0 and 2000 in both modes (tests/accuracy_report.py
+  adds 500 and 1000).
 * C1, 8,192 particles: the whole 200-substep run, device and FP32
   restatement against FP64.
 
@@ -44,17 +45,16 @@ def test_four_cubes_checkpoints_over_1000_substeps(transfers, c2, mode):
     check(A.run_checkpoints(transfers, c2, mode, (0, 250, 500, 1000)))
 
 
-def test_four_cubes_first_20_substeps_track_fp64(transfers, c2):
-    for mode in ("fp32", "fixed64"):
-        r = A.trajectory(transfers, c2, mode, 20, threads=A.oracle_threads())
-        print(f"four cubes {mode}, 20 substeps: gpu {A.fmt(r['gpu'])} floor {A.fmt(r['floor'])}")
+def test_four_cubes_first_10_substeps_track_fp64(transfers, c2):
+    for r in A.trajectories(transfers, c2, ("fp32", "fixed64"), 10, threads=A.oracle_threads()):
+        print(f"four cubes {r['mode']}, 10 substeps: gpu {A.fmt(r['gpu'])} floor {A.fmt(r['floor'])}")
         for f in A.FIELDS:
-            assert r["gpu"][f] <= r["tolerance"][f], (mode, f, r)
+            assert r["gpu"][f] <= r["tolerance"][f], (r["mode"], f, r)
 
 
 @pytest.mark.parametrize("mode", ["fp32", "fixed64"])
 def test_dam_break_checkpoints_over_2000_substeps(transfers, mode):
-    check(A.run_checkpoints(transfers, dam_break(), mode, (0, 500, 1000, 2000)))
+    check(A.run_checkpoints(transfers, dam_break(), mode, (0, 2000)))
 
 
 @pytest.mark.parametrize("mode", ["fp32", "fixed64"])

@@ -96,8 +96,12 @@ def test_staged_admission_equals_add_scene_bit_for_bit(transfers):
 
 
 def _audit_reference(pa):
-    m = pa.mass * np.asarray(pa.enabled, bool)
-    x, v = pa.position, pa.velocity
+    # The audit sums the device's f32 state.  A fixed particle's host position
+    # stays the f64 value it was created with (it never moves), so every input
+    # is rounded to f32 as the device holds it.
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
+    m = f32(pa.mass) * np.asarray(pa.enabled, bool)
+    x, v = f32(pa.position), f32(pa.velocity)
     terms = np.concatenate([m[:, None], m[:, None] * v, (0.5 * m * (v * v).sum(1))[:, None], m[:, None] * x,
                             m[:, None] * np.cross(x, v)], axis=1)
     return terms.sum(0), np.abs(terms).sum(0)
