@@ -148,11 +148,12 @@ def scenes_leg(transfers, mode: str, steps: int) -> dict:
     return out
 
 
-def serving_leg(transfers, mode: str, seconds: float, single_rate: float, plain: bool = True) -> dict:
+def serving_leg(transfers, mode: str, seconds: float, single_rate: float, plain_seconds: float = 15.0) -> dict:
     """Continuous batching: Poisson arrivals offering 80% of the measured single-scene rate,
     N log-uniform in [4096, 524288], 1-4 rigid boxes per scene, horizons U{50..2000} substeps,
     admission under an HBM budget; runs until every request has completed.  The same stream is
-    then served by the plain per-scene path (one resident session per request on the same GPU)."""
+    then served by the plain per-scene path (one resident session per request on the same GPU)
+    for at most `plain_seconds` (0: skip)."""
     from tessera_amd.scenes import serving_requests
     from tessera_amd.serving import PlainServer, Server
 
@@ -170,10 +171,13 @@ def serving_leg(transfers, mode: str, seconds: float, single_rate: float, plain:
     print(f"bench: serving {report['completed']}/{count} requests in {report['wall_s']:.1f}s: "
           f"{report['value']:.4e} particle-updates/s, step latency p50 {report['step_latency_ms_p50']:.3f} ms "
           f"p99 {report['step_latency_ms_p99']:.3f} ms", file=sys.stderr, flush=True)
-    if plain:
+    if plain_seconds > 0:
+        # the plain path is far slower on the same stream: it serves the stream for a bounded
+        # window, its throughput being the updates completed within it over its length
         ps = PlainServer(transfers, hbm_budget=budget, steps_per_tick=32, mode=mode, keep_worlds=False)
-        base = ps.run(reqs)
+        base = ps.run(reqs, time_limit=plain_seconds)
         ps.close()
+        base["window_s"] = plain_seconds
         report["plain"] = base
         report["speedup"] = report["value"] / base["value"] if base["value"] > 0 else None
         print(f"bench: plain per-scene path {base['completed']}/{count} requests in {base['wall_s']:.1f}s: "
@@ -195,6 +199,8 @@ def main() -> int:
     ap.add_argument("--no-extra-modes", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=64, help="substeps of the batched-scenes leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg (0: skip)")
+    ap.add_argument("--plain-seconds", type=float, default=15.0,
+                    help="window of the plain per-scene path on the serving stream (0: skip)")
     ap.add_argument("--scenes-steps", type=int, default=128, help="substeps of the C1/C3 scene leg (0: skip)")
     args = ap.parse_args()
 
@@ -273,7 +279,7 @@ def main() -> int:
 
     serving = None
     if args.serve_seconds > 0:
-        serving = serving_leg(transfers, args.mode, args.serve_seconds, value)
+        serving = serving_leg(transfers, args.mode, args.serve_seconds, value, args.plain_seconds)
 
     cpu = cpu_all = None
     if not args.no_cpu_baseline:

@@ -2890,6 +2890,7 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
     }
     std::vector<char> commit(ns, 0);
     tmpm_status st = TMPM_OK;
+    bool coupling_read = false;
     if (!s->batch) {
       // eager: the world is read back at every synchronize
       std::vector<HostWorld> updated;
@@ -2935,6 +2936,15 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
           HIP_TRY(hipGetLastError());
         }
       }
+      // The committed scenes' rigid bodies, obstacle poses and reactions are
+      // read first: a stopped scene's uploads below rewrite every scene's
+      // bodies and obstacles from the host copies.
+      st = read_coupling(s, commit);
+      if (st != TMPM_OK) {
+        s->valid = false;
+        return st;
+      }
+      coupling_read = true;
       if (stopped) {
         s->substeps_since_reorder = s->reorder_interval;  // the next substep drops the retired slots
         st = upload_rigid(s);
@@ -2958,7 +2968,7 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
         s->synced_n = s->n;
       }
     }
-    st = read_coupling(s, commit);
+    if (!coupling_read) st = read_coupling(s, commit);
     if (st != TMPM_OK) {
       s->valid = false;
       return st;

@@ -169,7 +169,7 @@ def test_autotune_cache_skips_trials_in_a_tuned_bucket(transfers):
 
 def _spreading_cloud(n_axis: int = 16) -> tuple[MpmWorld, float]:
     """A soft fluid ball in an unbounded world flying apart at 2 m/s: its
-    particle-block count grows from ~60 to thousands within 3,000 substeps."""
+    particle-block count grows several times over within 3,000 substeps."""
     rng = np.random.default_rng(11)
     h = 1.0 / 64
     g = (np.arange(n_axis) + 0.5) / n_axis
@@ -185,6 +185,10 @@ def _spreading_cloud(n_axis: int = 16) -> tuple[MpmWorld, float]:
 def test_autotune_grows_the_topology_of_a_spreading_scene(transfers):
     tune_cache_clear()
     world, dt = _spreading_cloud()
+    probe = HipMpmResidentSession(transfers, world, dt)
+    probe.step(1)
+    initial = probe.topology_stats()["particle_blocks"]
+    probe.close()
     s = HipMpmResidentSession(transfers, world, dt)
     cfg = s.autotune(256)
     steps = cfg["substeps_run"]
@@ -198,6 +202,7 @@ def test_autotune_grows_the_topology_of_a_spreading_scene(transfers):
         ref.step(k)
         left -= k
     st = ref.topology_stats()
-    assert st["particle_blocks"] > 1000
+    # beyond the 4x headroom the topology was sized with at creation
+    assert st["particle_blocks"] > 4 * initial, (initial, st)
     assert_same(s.read_particles(), ref.read_particles())
     tune_cache_clear()
