@@ -412,6 +412,22 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
                                            const uint32_t* triangles, uint64_t triangle_count, double spacing,
                                            uint64_t max_candidates, uint64_t max_samples, double* samples_out,
                                            uint64_t* count_out);
+/* No counterpart in the reference: tmpm_sample_closed_mesh_volume for
+ * callers that cannot allocate (code under graph capture, tight emission
+ * loops, harnesses that forbid device allocation).  Same arguments, results
+ * and errors, but it allocates nothing, on the device or the host, and
+ * creates no stream: the mesh, the candidate flags and a status word live in
+ * a fixed 32 MiB device array that each device holds for the life of the
+ * process (set up by tmpm_context_create), and the closed-surface test runs
+ * on the device.  Meshes of more than 65,536 triangles, or whose vertices,
+ * triangles and candidate lattice do not fit the 32 MiB, fail with
+ * TMPM_CAPACITY.  Every call on a device shares that device's array and
+ * leaves its contents behind; calls of this process on one device are
+ * serialized by a lock, but calls from several processes on one device race. */
+tmpm_status tmpm_sample_closed_mesh_volume_noalloc(tmpm_context* ctx, const double* vertices, uint64_t vertex_count,
+                                                   const uint32_t* triangles, uint64_t triangle_count,
+                                                   double spacing, uint64_t max_candidates, uint64_t max_samples,
+                                                   double* samples_out, uint64_t* count_out);
 /* No counterpart in the reference.  Chooses the launch shape (P2G workgroup
  * size and LDS tile copies, G2P workgroup size, reorder interval, graphs) by
  * timing each candidate over `trial` real substeps (8..=256), one knob at a

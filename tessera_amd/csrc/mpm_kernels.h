@@ -40,6 +40,9 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 void launch_mesh_inside(const double* verts, const uint32_t* tris, uint64_t ntri, const double lower[3],
                         const double upper[3], double spacing, const uint64_t counts[3], uint8_t* flags,
                         hipStream_t s);
+// *bad = 1 unless every directed edge of the triangles occurs exactly once
+// and its reverse exactly once (the caller zeroes *bad)
+void launch_mesh_edges(const uint32_t* tris, uint64_t ntri, uint32_t* bad, hipStream_t s);
 // FP64 per-scene sums of the particle state (kAuditFields doubles per scene,
 // added into out, which the caller zeroes)
 void launch_audit(const Parts& p, const DevClass* c, double* out, hipStream_t s);

@@ -2378,6 +2378,29 @@ __global__ __launch_bounds__(256) void k_mesh_inside(const double* __restrict__ 
   }
 }
 
+// Closed, consistently oriented surface test of the sampler (sampling.rs:66-93)
+// without a host edge map: every directed edge (a, b) of a triangle must occur
+// exactly once and its reverse (b, a) exactly once among all 3 * ntri directed
+// edges.  One thread per directed edge counts both over every triangle
+// (O(ntri^2), for meshes of at most tens of thousands of triangles); a failing
+// edge stores 1 into *bad (plain stores of the same value).
+__global__ __launch_bounds__(256) void k_mesh_edges(const uint32_t* __restrict__ tris, uint64_t ntri,
+                                                    uint32_t* bad) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < 3 * ntri;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = e / 3;
+    const int k = (int)(e % 3);
+    const uint32_t a = tris[3 * t + k], b = tris[3 * t + (k + 1) % 3];
+    uint32_t same = 0, rev = 0;
+    for (uint64_t u = 0; u < ntri; ++u) {
+      const uint32_t p = tris[3 * u], q = tris[3 * u + 1], r = tris[3 * u + 2];
+      same += (p == a && q == b) + (q == a && r == b) + (r == a && p == b);
+      rev += (p == b && q == a) + (q == b && r == a) + (r == b && p == a);
+    }
+    if (same != 1u || rev != 1u) *bad = 1u;
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // Bandwidth probe: a plain float4 copy of known size (reads n*16 B, writes
@@ -2540,6 +2563,9 @@ void launch_mesh_inside(const double* verts, const uint32_t* tris, uint64_t ntri
   const uint64_t count = counts[0] * counts[1] * counts[2];
   hipLaunchKernelGGL(k_mesh_inside, dim3(grid_for(count, 256, 65536)), dim3(256), 0, s, verts, tris, ntri, lower[0],
                      lower[1], lower[2], spacing, counts[1], counts[2], upper[0], upper[1], upper[2], count, flags);
+}
+void launch_mesh_edges(const uint32_t* tris, uint64_t ntri, uint32_t* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_mesh_edges, dim3(grid_for(3 * ntri, 256, 4096)), dim3(256), 0, s, tris, ntri, bad);
 }
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256 * 8, 16384)), dim3(256), 0, s, src, dst, n);

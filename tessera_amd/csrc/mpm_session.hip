@@ -2302,6 +2302,120 @@ extern "C" {
 const char* tmpm_last_error(void) { return g_last_error.c_str(); }
 const char* tmpm_version(void) { return "tessera-mpm-hip 0.1.0 (gfx950)"; }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of tmpm_sample_closed_mesh_volume_noalloc, the entry point that
+// allocates nothing: a status word, the mesh's vertices and triangles and one
+// inside flag per lattice candidate.  Each device holds its own copy for the
+// life of the process; in host code only its per-device address (from
+// hipGetSymbolAddress, set up when a context is created on the device) may be
+// passed to kernels or copies.
+constexpr uint64_t kScratchBytes = 32ull << 20;
+constexpr uint64_t kNoAllocMaxTriangles = 1ull << 16;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct NoAllocScratch {
+  std::mutex mu;         // serializes the calls of this process on the device
+  void* ptr = nullptr;   // the device's address of `scratch`
+};
+
+NoAllocScratch& noalloc_scratch(int device) {
+  static NoAllocScratch table[256];
+  return table[device & 255];
+}
+
+// Looks up this device's address of `scratch`; the device must be current.
+tmpm_status setup_scratch(int device) {
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(TMPM_READBACK, std::string("cannot find the sampler's scratch memory: ") + hipGetErrorString(e));
+  noalloc_scratch(device).ptr = ptr;
+  return TMPM_OK;
+}
+
+// Argument, vertex and triangle checks of sampling.rs:54-93 shared by both
+// samplers, all but the edge test: geometry, spacing and limits; finite
+// vertices; triangles of three distinct in-range vertices and non-zero area.
+tmpm_status mesh_check_triangles(tmpm_context* ctx, const double* vertices, uint64_t vertex_count,
+                                 const uint32_t* triangles, uint64_t triangle_count, double spacing,
+                                 uint64_t max_candidates, uint64_t max_samples, double* samples_out,
+                                 uint64_t* count_out) {
+  if (!ctx || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
+  if (vertex_count == 0 || triangle_count == 0 || !vertices || !triangles || !samples_out ||
+      !std::isfinite(spacing) || spacing <= 0.0 || max_candidates == 0 || max_samples == 0)
+    return fail(TMPM_INVALID_INPUT, "invalid mesh, spacing or limits");
+  for (uint64_t i = 0; i < 3 * vertex_count; ++i)
+    if (!std::isfinite(vertices[i])) return fail(TMPM_INVALID_INPUT, "non-finite mesh vertex");
+  for (uint64_t t = 0; t < triangle_count; ++t) {
+    const uint32_t* tr = triangles + 3 * t;
+    if (tr[0] >= vertex_count || tr[1] >= vertex_count || tr[2] >= vertex_count || tr[0] == tr[1] ||
+        tr[1] == tr[2] || tr[2] == tr[0])
+      return fail(TMPM_INVALID_INPUT, "invalid mesh triangle");
+    const double* a = vertices + 3ull * tr[0];
+    const double* b = vertices + 3ull * tr[1];
+    const double* c = vertices + 3ull * tr[2];
+    const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                          e1[0] * e2[1] - e1[1] * e2[0]};
+    const double area2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+    if (!std::isfinite(area2) || area2 <= 1e-24) return fail(TMPM_INVALID_INPUT, "degenerate mesh triangle");
+  }
+  return TMPM_OK;
+}
+
+struct MeshLattice {
+  double lower[3], upper[3];
+  uint64_t counts[3];
+  uint64_t candidates;
+};
+
+// sampling.rs:94-115: the AABB lattice, bounded by max_candidates.
+tmpm_status mesh_lattice(const double* vertices, uint64_t vertex_count, double spacing, uint64_t max_candidates,
+                         MeshLattice& lat) {
+  for (int a = 0; a < 3; ++a) lat.lower[a] = lat.upper[a] = vertices[a];
+  for (uint64_t i = 1; i < vertex_count; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lat.lower[a] = std::fmin(lat.lower[a], vertices[3 * i + a]);
+      lat.upper[a] = std::fmax(lat.upper[a], vertices[3 * i + a]);
+    }
+  lat.candidates = 1;
+  for (int a = 0; a < 3; ++a) {
+    const double c = std::ceil((lat.upper[a] - lat.lower[a]) / spacing);
+    if (!std::isfinite(c) || c < 1.0 || c > (double)max_candidates)
+      return fail(TMPM_CAPACITY, "candidate lattice exceeds max_candidates");
+    lat.counts[a] = (uint64_t)c;
+    if (lat.candidates > max_candidates / lat.counts[a])
+      return fail(TMPM_CAPACITY, "candidate lattice exceeds max_candidates");
+    lat.candidates *= lat.counts[a];
+  }
+  return TMPM_OK;
+}
+
+// Accepted centres of candidates [c0, c0 + len) (flags[c - c0] != 0) appended
+// to samples_out in lattice order (x, then y, then z), at most max_samples.
+tmpm_status mesh_emit_samples(const MeshLattice& lat, double spacing, const uint8_t* flags, uint64_t c0,
+                              uint64_t len, uint64_t max_samples, double* samples_out, uint64_t& n) {
+  for (uint64_t c = c0; c < c0 + len; ++c) {
+    if (!flags[c - c0]) continue;
+    if (n == max_samples) return fail(TMPM_CAPACITY, "accepted samples exceed max_samples");
+    const uint64_t x = c / (lat.counts[1] * lat.counts[2]), y = (c / lat.counts[2]) % lat.counts[1],
+                   z = c % lat.counts[2];
+    samples_out[3 * n] = lat.lower[0] + ((double)x + 0.5) * spacing;
+    samples_out[3 * n + 1] = lat.lower[1] + ((double)y + 0.5) * spacing;
+    samples_out[3 * n + 2] = lat.lower[2] + ((double)z + 0.5) * spacing;
+    ++n;
+  }
+  return TMPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 tmpm_status tmpm_context_create(int32_t device_ordinal, tmpm_context** out) {
   try {
     if (!out) return fail(TMPM_INVALID_INPUT, "null output");
@@ -2319,6 +2433,22 @@ tmpm_status tmpm_context_create(int32_t device_ordinal, tmpm_context** out) {
       std::string arch = c->props.gcnArchName;
       delete c;
       return fail(TMPM_NO_DEVICE, "device is " + arch + ", this library is built for gfx950");
+    }
+    {
+      // once per device: the device address of the allocation-free sampler's scratch
+      NoAllocScratch& ns = noalloc_scratch(device_ordinal);
+      std::lock_guard<std::mutex> lock(ns.mu);
+      if (!ns.ptr) {
+        int prev = 0;
+        const bool restore = hipGetDevice(&prev) == hipSuccess;
+        tmpm_status st = hipSetDevice(device_ordinal) == hipSuccess ? setup_scratch(device_ordinal)
+                                                                    : fail(TMPM_NO_DEVICE, "cannot select device");
+        if (restore) (void)hipSetDevice(prev);
+        if (st != TMPM_OK) {
+          delete c;
+          return st;
+        }
+      }
     }
     *out = c;
     return TMPM_OK;
@@ -3838,30 +3968,14 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
                                            uint64_t* count_out) {
   try {
     if (count_out) *count_out = 0;
-    if (!ctx || !count_out) return fail(TMPM_INVALID_INPUT, "null argument");
-    // sampling.rs:54-65: geometry, spacing and limits
-    if (vertex_count == 0 || triangle_count == 0 || !vertices || !triangles || !samples_out ||
-        !std::isfinite(spacing) || spacing <= 0.0 || max_candidates == 0 || max_samples == 0)
-      return fail(TMPM_INVALID_INPUT, "invalid mesh, spacing or limits");
-    for (uint64_t i = 0; i < 3 * vertex_count; ++i)
-      if (!std::isfinite(vertices[i])) return fail(TMPM_INVALID_INPUT, "non-finite mesh vertex");
-    // sampling.rs:66-93: non-degenerate triangles, every undirected edge used
-    // exactly once in each direction (a closed, consistently oriented surface)
+    tmpm_status st = mesh_check_triangles(ctx, vertices, vertex_count, triangles, triangle_count, spacing,
+                                          max_candidates, max_samples, samples_out, count_out);
+    if (st != TMPM_OK) return st;
+    // sampling.rs:66-93: every undirected edge used exactly once in each
+    // direction (a closed, consistently oriented surface)
     std::map<std::pair<uint32_t, uint32_t>, std::array<uint64_t, 2>> edges;
     for (uint64_t t = 0; t < triangle_count; ++t) {
       const uint32_t* tr = triangles + 3 * t;
-      if (tr[0] >= vertex_count || tr[1] >= vertex_count || tr[2] >= vertex_count || tr[0] == tr[1] ||
-          tr[1] == tr[2] || tr[2] == tr[0])
-        return fail(TMPM_INVALID_INPUT, "invalid mesh triangle");
-      const double* a = vertices + 3ull * tr[0];
-      const double* b = vertices + 3ull * tr[1];
-      const double* c = vertices + 3ull * tr[2];
-      const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-      const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-      const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                            e1[0] * e2[1] - e1[1] * e2[0]};
-      const double area2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
-      if (!std::isfinite(area2) || area2 <= 1e-24) return fail(TMPM_INVALID_INPUT, "degenerate mesh triangle");
       const uint32_t pairs[3][2] = {{tr[0], tr[1]}, {tr[1], tr[2]}, {tr[2], tr[0]}};
       for (const auto& pr : pairs)
         edges[{std::min(pr[0], pr[1]), std::max(pr[0], pr[1])}][pr[0] > pr[1] ? 1 : 0] += 1;
@@ -3869,51 +3983,88 @@ tmpm_status tmpm_sample_closed_mesh_volume(tmpm_context* ctx, const double* vert
     for (const auto& e : edges)
       if (e.second[0] != 1 || e.second[1] != 1)
         return fail(TMPM_INVALID_INPUT, "mesh is not closed and consistently oriented");
-    // sampling.rs:94-115: the AABB lattice, bounded by max_candidates
-    double lower[3], upper[3];
-    for (int a = 0; a < 3; ++a) lower[a] = upper[a] = vertices[a];
-    for (uint64_t i = 1; i < vertex_count; ++i)
-      for (int a = 0; a < 3; ++a) {
-        lower[a] = std::fmin(lower[a], vertices[3 * i + a]);
-        upper[a] = std::fmax(upper[a], vertices[3 * i + a]);
-      }
-    uint64_t counts[3], candidates = 1;
-    for (int a = 0; a < 3; ++a) {
-      const double c = std::ceil((upper[a] - lower[a]) / spacing);
-      if (!std::isfinite(c) || c < 1.0 || c > (double)max_candidates)
-        return fail(TMPM_CAPACITY, "candidate lattice exceeds max_candidates");
-      counts[a] = (uint64_t)c;
-      if (candidates > max_candidates / counts[a]) return fail(TMPM_CAPACITY, "candidate lattice exceeds max_candidates");
-      candidates *= counts[a];
-    }
+    MeshLattice lat;
+    if ((st = mesh_lattice(vertices, vertex_count, spacing, max_candidates, lat)) != TMPM_OK) return st;
     // sampling.rs:116-144 on the device: the solid-angle test of every candidate
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
     DevBuf dv, dt, df;
     HIP_TRY(dv.alloc(24ull * vertex_count));
     HIP_TRY(dt.alloc(12ull * triangle_count));
-    HIP_TRY(df.alloc(candidates));
+    HIP_TRY(df.alloc(lat.candidates));
     HIP_TRY(hipMemcpy(dv.ptr, vertices, 24ull * vertex_count, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dt.ptr, triangles, 12ull * triangle_count, hipMemcpyHostToDevice));
-    hipStream_t st = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    launch_mesh_inside(dv.as<double>(), dt.as<uint32_t>(), triangle_count, lower, upper, spacing, counts,
-                       df.as<uint8_t>(), st);
+    hipStream_t stream = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    launch_mesh_inside(dv.as<double>(), dt.as<uint32_t>(), triangle_count, lat.lower, lat.upper, spacing, lat.counts,
+                       df.as<uint8_t>(), stream);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
     if (e != hipSuccess) return fail(TMPM_READBACK, std::string("mesh sampling kernel failed: ") + hipGetErrorString(e));
-    std::vector<uint8_t> flags(candidates);
-    HIP_TRY(hipMemcpy(flags.data(), df.ptr, candidates, hipMemcpyDeviceToHost));
-    // accepted centres in lattice order (x, then y, then z), at most max_samples
+    std::vector<uint8_t> flags(lat.candidates);
+    HIP_TRY(hipMemcpy(flags.data(), df.ptr, lat.candidates, hipMemcpyDeviceToHost));
     uint64_t n = 0;
-    for (uint64_t c = 0; c < candidates; ++c) {
-      if (!flags[c]) continue;
-      if (n == max_samples) return fail(TMPM_CAPACITY, "accepted samples exceed max_samples");
-      const uint64_t x = c / (counts[1] * counts[2]), y = (c / counts[2]) % counts[1], z = c % counts[2];
-      samples_out[3 * n] = lower[0] + ((double)x + 0.5) * spacing;
-      samples_out[3 * n + 1] = lower[1] + ((double)y + 0.5) * spacing;
-      samples_out[3 * n + 2] = lower[2] + ((double)z + 0.5) * spacing;
-      ++n;
+    if ((st = mesh_emit_samples(lat, spacing, flags.data(), 0, lat.candidates, max_samples, samples_out, n)) != TMPM_OK)
+      return st;
+    *count_out = n;
+    return TMPM_OK;
+  } catch (...) {
+    return exception_status();
+  }
+}
+
+tmpm_status tmpm_sample_closed_mesh_volume_noalloc(tmpm_context* ctx, const double* vertices, uint64_t vertex_count,
+                                                   const uint32_t* triangles, uint64_t triangle_count,
+                                                   double spacing, uint64_t max_candidates, uint64_t max_samples,
+                                                   double* samples_out, uint64_t* count_out) {
+  try {
+    if (count_out) *count_out = 0;
+    tmpm_status st = mesh_check_triangles(ctx, vertices, vertex_count, triangles, triangle_count, spacing,
+                                          max_candidates, max_samples, samples_out, count_out);
+    if (st != TMPM_OK) return st;
+    // scratch layout: status word, vertices, triangles, candidate flags (each 256-byte aligned)
+    if (triangle_count > kNoAllocMaxTriangles)
+      return fail(TMPM_CAPACITY, "mesh exceeds the allocation-free sampler's triangle limit");
+    auto align = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+    const uint64_t off_v = 256, off_t = align(off_v + 24 * vertex_count), off_f = align(off_t + 12 * triangle_count);
+    if (vertex_count > kScratchBytes || off_f > kScratchBytes)
+      return fail(TMPM_CAPACITY, "mesh exceeds the allocation-free sampler's scratch memory");
+    NoAllocScratch& ns = noalloc_scratch(ctx->device);
+    std::lock_guard<std::mutex> lock(ns.mu);  // calls on one device share its scratch
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(TMPM_NO_DEVICE, "cannot select device");
+    if (!ns.ptr && (st = setup_scratch(ctx->device)) != TMPM_OK) return st;
+    char* base = static_cast<char*>(ns.ptr);
+    uint32_t* bad = reinterpret_cast<uint32_t*>(base);
+    const double* dv = reinterpret_cast<const double*>(base + off_v);
+    const uint32_t* dt = reinterpret_cast<const uint32_t*>(base + off_t);
+    uint8_t* df = reinterpret_cast<uint8_t*>(base + off_f);
+    static const uint32_t zero = 0;
+    HIP_TRY(hipMemcpy(bad, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(base + off_v, vertices, 24ull * vertex_count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(base + off_t, triangles, 12ull * triangle_count, hipMemcpyHostToDevice));
+    // sampling.rs:66-93 on the device (the allocating entry point builds a host edge map)
+    launch_mesh_edges(dt, triangle_count, bad, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return fail(TMPM_READBACK, std::string("mesh edge kernel failed: ") + hipGetErrorString(e));
+    uint32_t bad_host = 0;
+    HIP_TRY(hipMemcpy(&bad_host, bad, sizeof(bad_host), hipMemcpyDeviceToHost));
+    if (bad_host) return fail(TMPM_INVALID_INPUT, "mesh is not closed and consistently oriented");
+    MeshLattice lat;
+    if ((st = mesh_lattice(vertices, vertex_count, spacing, max_candidates, lat)) != TMPM_OK) return st;
+    if (lat.candidates > kScratchBytes - off_f)
+      return fail(TMPM_CAPACITY, "candidate lattice exceeds the allocation-free sampler's scratch memory");
+    launch_mesh_inside(dv, dt, triangle_count, lat.lower, lat.upper, spacing, lat.counts, df, nullptr);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return fail(TMPM_READBACK, std::string("mesh sampling kernel failed: ") + hipGetErrorString(e));
+    // the flags come back through a fixed host buffer, chunk by chunk
+    uint8_t chunk[1 << 16];
+    uint64_t n = 0;
+    for (uint64_t c0 = 0; c0 < lat.candidates; c0 += sizeof(chunk)) {
+      const uint64_t len = std::min<uint64_t>(sizeof(chunk), lat.candidates - c0);
+      HIP_TRY(hipMemcpy(chunk, df + c0, len, hipMemcpyDeviceToHost));
+      if ((st = mesh_emit_samples(lat, spacing, chunk, c0, len, max_samples, samples_out, n)) != TMPM_OK) return st;
     }
     *count_out = n;
     return TMPM_OK;

@@ -281,6 +281,9 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_sample_closed_mesh_volume": (C.c_int, [vp, _dp, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64,
                                                      C.c_double, C.c_uint64, C.c_uint64, _dp,
                                                      C.POINTER(C.c_uint64)]),
+        "tmpm_sample_closed_mesh_volume_noalloc": (C.c_int, [vp, _dp, C.c_uint64, C.POINTER(C.c_uint32),
+                                                             C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, _dp,
+                                                             C.POINTER(C.c_uint64)]),
         "tmpm_obstacle_valid": (C.c_int32, [C.POINTER(CObstacle)]),
     }
     for name, (res, args) in sig.items():
