@@ -91,6 +91,15 @@ def batch_leg(transfers, mode: str, steps: int) -> dict:
     submit(b, steps)
     b.wait()
     t_batch = time.perf_counter() - t0
+    # the same launches captured once per distinct substep plan and replayed as HIP graphs
+    b.set_graphs(True)
+    submit(b, 8)
+    b.wait()
+    t0 = time.perf_counter()
+    submit(b, steps)
+    b.wait()
+    t_graphs = time.perf_counter() - t0
+    b.set_graphs(bool(cfg.get("graphs")))
     b.synchronize()
     failed = sum(1 for k in range(len(scenes)) if b.scene_error(k) is not None)
     b.close()
@@ -108,11 +117,12 @@ def batch_leg(transfers, mode: str, steps: int) -> dict:
         s.close()
     value = total * steps / t_batch
     alone = total * steps / t_alone
-    print(f"bench: batch of {len(scenes)} scenes ({total} particles): {value:.4e} vs one session at a time "
-          f"{alone:.4e} particle-updates/s", file=sys.stderr, flush=True)
+    print(f"bench: batch of {len(scenes)} scenes ({total} particles): {value:.4e} ({total * steps / t_graphs:.4e} "
+          f"replayed as graphs) vs one session at a time {alone:.4e} particle-updates/s", file=sys.stderr, flush=True)
     return {"workload": "batch_256_scenes", "scenes": len(scenes), "particles": total, "steps": steps,
             "value": value, "unit": "particle-updates/s", "ms_per_step": t_batch / steps * 1e3,
             "value_default_shape": total * steps / t_default, "launch_config": cfg,
+            "value_graphs": total * steps / t_graphs, "ms_per_step_graphs": t_graphs / steps * 1e3,
             "one_session_at_a_time": alone, "speedup": value / alone, "failed_scenes": failed}
 
 

@@ -284,6 +284,14 @@ tmpm_status tmpm_session_export_blocks(tmpm_session* s, int32_t* block_keys, uin
  * synchronized.  Any number of obstacles is supported (the reference's
  * reaction capacity grows with the obstacle count, gpu.rs:2651-2720). */
 tmpm_status tmpm_session_set_reactions(tmpm_session* s, int32_t enabled);
+/* Particle-side reaction impulses clamped since the session was created (no
+ * reference counterpart: the reference adds f32 impulses, which do not
+ * saturate).  Each particle's impulse is rounded to the obstacle's int64
+ * units, the heaviest particle at 64 cells per substep being 2^43 units; an
+ * impulse component beyond 2^51 units (a surface moving over 16,000 cells
+ * per substep against the particle) is clamped there and counted here, read
+ * at each synchronize.  0 means every reaction summed exactly. */
+uint64_t tmpm_session_reaction_saturations(const tmpm_session* s);
 /* 6 doubles per obstacle in obstacle order: linear xyz, angular xyz. */
 tmpm_status tmpm_session_read_reactions(const tmpm_session* s, double* out, uint32_t obstacle_cap,
                                         uint32_t* count_out);

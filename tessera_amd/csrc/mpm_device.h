@@ -37,6 +37,7 @@ enum Counter : uint32_t {
   CNT_BINNED = 2,
   CNT_ERROR = 3,
   CNT_LIVE_NODES = 5,
+  CNT_REACT_SATURATED = 6,  // particle-side reaction impulses clamped to the bins' range
   CNT_COUNT = 8
 };
 // error bits in counters[CNT_ERROR]

@@ -1643,24 +1643,30 @@ __device__ __forceinline__ void add_reaction(const Topo& t, const DevObstacle& o
 // (the workgroup size, the storage order) changes no reaction.  The scale
 // maps the heaviest particle at 64 cells per substep to 2^43 units; an
 // impulse past 2^51 units (a surface over 16,000 cells per substep faster
-// than the particle) saturates there instead of wrapping.  v * 2^rshift is
-// exact in f64 (a power-of-two scale), so clamping it first and adding the
-// rounding constant rounds once, like fixed64_round.
-__device__ __forceinline__ unsigned long long reaction_unit(float v, double w) {
+// than the particle) saturates there instead of wrapping, and `clamped`
+// records it (the session counts such impulses).  v * 2^rshift is exact in
+// f64 (a power-of-two scale), so clamping it first and adding the rounding
+// constant rounds once, like fixed64_round.
+__device__ __forceinline__ unsigned long long reaction_unit(float v, double w, bool& clamped) {
   constexpr double kMax = 0x1p51 - 1.0;
-  const double x = fmin(fmax((double)v * w, -kMax), kMax);
+  const double u = (double)v * w;
+  clamped |= !(fabs(u) <= kMax);
+  const double x = fmin(fmax(u, -kMax), kMax);
   return (unsigned long long)__double_as_longlong(x + 0x1.8p52) - 0x4338000000000000ull;
 }
 
-__device__ __forceinline__ void reaction_fixed(const DevObstacle& o, float3 lin, float3 ang,
+// Returns whether a component saturated.
+__device__ __forceinline__ bool reaction_fixed(const DevObstacle& o, float3 lin, float3 ang,
                                                unsigned long long r[6]) {
   const double wl = (double)o.rscale_lin * 0x1p32, wa = (double)o.rscale_ang * 0x1p32;
-  r[0] = reaction_unit(lin.x, wl);
-  r[1] = reaction_unit(lin.y, wl);
-  r[2] = reaction_unit(lin.z, wl);
-  r[3] = reaction_unit(ang.x, wa);
-  r[4] = reaction_unit(ang.y, wa);
-  r[5] = reaction_unit(ang.z, wa);
+  bool clamped = false;
+  r[0] = reaction_unit(lin.x, wl, clamped);
+  r[1] = reaction_unit(lin.y, wl, clamped);
+  r[2] = reaction_unit(lin.z, wl, clamped);
+  r[3] = reaction_unit(ang.x, wa, clamped);
+  r[4] = reaction_unit(ang.y, wa, clamped);
+  r[5] = reaction_unit(ang.z, wa, clamped);
+  return clamped;
 }
 
 // One lane's rounded impulse straight into the obstacle's bins (divergent
@@ -1949,7 +1955,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
               const float3 J = (before - v) * pm.w;
               const float3 L = cross3(x - nrm * radius - make_float3(o.center[0], o.center[1], o.center[2]), J);
               unsigned long long r[6];
-              reaction_fixed(o, J, L, r);
+              if (reaction_fixed(o, J, L, r)) atomicAdd(&t.counters[CNT_REACT_SATURATED], 1u);
               if (oi < kMaxReactObstacles) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c)

@@ -642,6 +642,7 @@ struct tmpm_session {
   bool plastic = false;    // some particle class is snow, sand or sand-NH: plastic state on the device
   // reactions and rigid coupling
   bool reactions_on = false;
+  uint64_t react_saturated = 0;  // particle-side reaction impulses clamped to the bins' range, all substeps
   DevBuf react_bins, react_sub, react_total, rigid, rigid_maps, body_scene;
   uint32_t obstacle_total = 0, react_cap = 0, body_total = 0;
   bool react_active() const { return reactions_on || body_total > 0; }
@@ -3017,6 +3018,15 @@ tmpm_status tmpm_session_synchronize(tmpm_session* s) {
       }
       s->last_pblocks = counters[CNT_PARTICLE_BLOCKS];
       s->last_gblocks = counters[CNT_GRID_BLOCKS];
+      if (counters[CNT_REACT_SATURATED]) {
+        s->react_saturated += counters[CNT_REACT_SATURATED];
+        const uint32_t zero = 0;
+        if (hipMemcpy(s->counters.as<uint32_t>() + CNT_REACT_SATURATED, &zero, 4, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+          s->valid = false;
+          return fail(TMPM_READBACK, "counter reset failed");
+        }
+      }
     }
     std::vector<char> commit(ns, 0);
     tmpm_status st = TMPM_OK;
@@ -3204,6 +3214,7 @@ tmpm_status tmpm_session_step_adaptive(tmpm_session* s, double dt) {
 }
 
 uint64_t tmpm_session_pending_substeps(const tmpm_session* s) { return s ? s->pending : 0; }
+uint64_t tmpm_session_reaction_saturations(const tmpm_session* s) { return s ? s->react_saturated : 0; }
 tmpm_status tmpm_session_memory(const tmpm_session* s, uint64_t* device_bytes, uint64_t* particle_capacity) {
   if (!s || !device_bytes || !particle_capacity) return fail(TMPM_INVALID_INPUT, "null argument");
   *device_bytes = s->device_bytes();

@@ -215,6 +215,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "tmpm_session_step": (C.c_int, [vp, C.c_uint32]),
         "tmpm_session_synchronize_async": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "tmpm_session_memory": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "tmpm_session_reaction_saturations": (C.c_uint64, [vp]),
         "tmpm_batch_stage_scene": (C.c_int, [C.POINTER(CWorld), C.c_double, C.POINTER(vp)]),
         "tmpm_batch_add_staged": (C.c_int, [vp, vp, C.POINTER(C.c_uint32)]),
         "tmpm_staged_scene_destroy": (None, [vp]),
@@ -1098,6 +1099,10 @@ class HipMpmResidentSession:
     def set_reactions(self, enabled: bool) -> None:
         """Compute obstacle reactions (step_with_obstacle_reactions, world.rs:309-330)."""
         _check(self.lib.tmpm_session_set_reactions(self._h, 1 if enabled else 0), self.lib)
+
+    def reaction_saturations(self) -> int:
+        """Particle-side reaction impulses clamped to the int64 bins' range since creation (0: exact sums)."""
+        return int(self.lib.tmpm_session_reaction_saturations(self._h))
 
     def reactions(self) -> np.ndarray:
         """(obstacles, 6): linear and angular impulse of the last synchronized substeps."""

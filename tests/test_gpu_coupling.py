@@ -72,6 +72,7 @@ def test_reactions_match_the_oracle_substep_by_substep(transfers, mode, boundary
         total_cpu += r_cpu
     assert np.abs(total_cpu[0, 1]) > 0.0  # contact happened
     assert np.abs(total_gpu - total_cpu).max() <= 2e-3 * np.abs(total_cpu).max()
+    assert s.reaction_saturations() == 0  # every particle impulse fit the int64 bins
 
 
 def test_reactions_are_summed_over_the_synchronized_substeps_and_reproducible(transfers):

@@ -20,7 +20,7 @@ import time
 import numpy as np
 import pytest
 
-from tessera_amd.mpm import (HipMpmBatch, HipMpmResidentSession, MaterialModel, MpmParams, MpmWorld,
+from tessera_amd.mpm import (Capacity, HipMpmBatch, HipMpmResidentSession, MaterialModel, MpmParams, MpmWorld,
                              ResidentUnavailable, StagedScene, tune_cache_clear, tune_cache_size)
 from tessera_amd.scenes import _particles, four_cubes, uniform_cube
 
@@ -206,3 +206,28 @@ def test_autotune_grows_the_topology_of_a_spreading_scene(transfers):
     assert st["particle_blocks"] > 4 * initial, (initial, st)
     assert_same(s.read_particles(), ref.read_particles())
     tune_cache_clear()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_fixed_point_range_is_a_capacity_error_in_both_modes(transfers, mode):
+    """Both P2G modes accumulate 64-bit fixed point in LDS, so a contribution
+    beyond 2^45 units (the heaviest particle at 64 cells per substep is 2^43)
+    is a Capacity error in FP32 mode too, where the reference's f32
+    compare-exchange loops (gpu.wgsl:739-766) would add it.  Such a particle
+    has already broken the CFL bound in the substep before, so the failure
+    only changes which error a doomed submission reports."""
+    sc = uniform_cube(n=512)
+    w = sc.world.clone()
+    w.params = dataclasses.replace(w.params, bounds=None)
+    w.set_obstacles([])
+    s = HipMpmResidentSession(transfers, w, sc.dt, mode)
+    s.step(1)
+    before = s.read_particles()
+    m = float(w.particles.mass[0])
+    v = 1024 * w.params.cell_width / sc.dt  # 1024 cells per substep
+    s.set_particle_force(0, [v * m / sc.dt, 0.0, 0.0])
+    s.submit_steps(2)
+    with pytest.raises(Capacity):
+        s.synchronize()
+    assert not s.valid and s.world().substeps == 1
+    assert_same(s.read_particles(), before)
