@@ -43,6 +43,10 @@ void launch_mesh_inside(const double* verts, const uint32_t* tris, uint64_t ntri
 // *bad = 1 unless every directed edge of the triangles occurs exactly once
 // and its reverse exactly once (the caller zeroes *bad)
 void launch_mesh_edges(const uint32_t* tris, uint64_t ntri, uint32_t* bad, hipStream_t s);
+// The first n particles' state (positions, velocities, classes, insertion
+// indices, C, F and the plastic state) from src to dst, each with its stride;
+// both P arrays must exist
+void launch_copy_state(const Parts& dst, const Parts& src, uint32_t n, hipStream_t s);
 // FP64 per-scene sums of the particle state (kAuditFields doubles per scene,
 // added into out, which the caller zeroes)
 void launch_audit(const Parts& p, const DevClass* c, double* out, hipStream_t s);

@@ -2409,6 +2409,27 @@ __global__ __launch_bounds__(256) void k_mesh_edges(const uint32_t* __restrict__
 
 
 // ---------------------------------------------------------------------------
+// The first n particles' whole state from one particle buffer set to another
+// (each with its own column stride): every array in one launch instead of one
+// copy per array and column.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_state(Parts dst, Parts src, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    dst.pos_mass[i] = src.pos_mass[i];
+    dst.vel_vol[i] = src.vel_vol[i];
+    dst.cls[i] = src.cls[i];
+    dst.ins_id[i] = src.ins_id[i];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      dst.C[(size_t)q * dst.stride + i] = src.C[(size_t)q * src.stride + i];
+      dst.F[(size_t)q * dst.stride + i] = src.F[(size_t)q * src.stride + i];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dst.P[(size_t)q * dst.stride + i] = src.P[(size_t)q * src.stride + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Bandwidth probe: a plain float4 copy of known size (reads n*16 B, writes
 // n*16 B), used to measure achievable HBM bandwidth and to calibrate
 // hardware byte counters.
@@ -2572,6 +2593,9 @@ void launch_mesh_inside(const double* verts, const uint32_t* tris, uint64_t ntri
 }
 void launch_mesh_edges(const uint32_t* tris, uint64_t ntri, uint32_t* bad, hipStream_t s) {
   hipLaunchKernelGGL(k_mesh_edges, dim3(grid_for(3 * ntri, 256, 4096)), dim3(256), 0, s, tris, ntri, bad);
+}
+void launch_copy_state(const Parts& dst, const Parts& src, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_copy_state, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, dst, src, n);
 }
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256 * 8, 16384)), dim3(256), 0, s, src, dst, n);

@@ -731,7 +731,8 @@ struct tmpm_session {
 struct tmpm_staged_scene {
   SceneState scene;
   std::unique_ptr<PackedParticles> pk;
-  std::vector<DevClass> classes;  // per particle, scene index 0
+  std::vector<DevClass> class_table;  // distinct classes of the enabled particles, first seen first, scene index 0
+  std::vector<uint32_t> class_local;  // per enabled particle: its class's index in class_table
   uint64_t pblocks = 0, gblocks = 0;
 };
 
@@ -1509,34 +1510,30 @@ tmpm_status alloc_particles(ParticleBuffers& pb, uint32_t cap) {
 tmpm_status copy_particles(ParticleBuffers& dst, uint32_t dst_cap, const ParticleBuffers& src, uint32_t src_cap,
                            uint32_t n, hipStream_t st) {
   if (n == 0) return TMPM_OK;
-  HIP_TRY(hipMemcpyAsync(dst.pos_mass.ptr, src.pos_mass.ptr, 16ull * n, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(dst.vel_vol.ptr, src.vel_vol.ptr, 16ull * n, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(dst.cls.ptr, src.cls.ptr, 4ull * n, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(dst.ins_id.ptr, src.ins_id.ptr, 4ull * n, hipMemcpyDeviceToDevice, st));
-  for (int q = 0; q < 9; ++q) {
-    HIP_TRY(hipMemcpyAsync(dst.C.as<float>() + (size_t)q * dst_cap, src.C.as<float>() + (size_t)q * src_cap, 4ull * n,
-                           hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(dst.F.as<float>() + (size_t)q * dst_cap, src.F.as<float>() + (size_t)q * src_cap, 4ull * n,
-                           hipMemcpyDeviceToDevice, st));
-  }
-  for (int q = 0; q < 3; ++q)
-    HIP_TRY(hipMemcpyAsync(dst.P.as<float>() + (size_t)q * dst_cap, src.P.as<float>() + (size_t)q * src_cap, 4ull * n,
-                           hipMemcpyDeviceToDevice, st));
+  auto view = [](const ParticleBuffers& b, uint32_t cap) {
+    Parts v{};
+    v.pos_mass = b.pos_mass.as<float4>();
+    v.vel_vol = b.vel_vol.as<float4>();
+    v.C = b.C.as<float>();
+    v.F = b.F.as<float>();
+    v.P = b.P.as<float>();
+    v.cls = b.cls.as<uint32_t>();
+    v.ins_id = b.ins_id.as<uint32_t>();
+    v.stride = cap;
+    return v;
+  };
+  launch_copy_state(view(dst, dst_cap), view(src, src_cap), n, st);
+  HIP_TRY(hipGetLastError());
   return TMPM_OK;
 }
 
 // Writes packed particles into the current buffer at slots [at, at + size).
-tmpm_status upload_particles(tmpm_session* s, PackedParticles& pk, uint32_t at) {
+// Component-major copies of C, F and the plastic state (9 + 9 + 3 columns of
+// n floats), the layout of the device buffers, in the packing's page-locked
+// staging.  A staged admission builds them on its host thread, so that the
+// upload between ticks is copies only.
+void pack_columns(PackedParticles& pk) {
   const size_t n = pk.size();
-  if (n == 0) return TMPM_OK;
-  ParticleBuffers& pb = s->buf[s->cur];
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  HIP_TRY(hipMemcpy(pb.pos_mass.as<float4>() + at, pk.pm.data(), 16ull * n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pb.vel_vol.as<float4>() + at, pk.vv.data(), 16ull * n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pb.cls.as<uint32_t>() + at, pk.cls.data(), 4ull * n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pb.ins_id.as<uint32_t>() + at, pk.ins.data(), 4ull * n, hipMemcpyHostToDevice));
-  // component-major columns (9 of C, 9 of F, 3 of the plastic state)
-  // transposed in parallel
   auto& cols = pk.cols;
   cols.resize(21 * n);
   parallel_for(n, [&](size_t b, size_t e) {
@@ -1548,15 +1545,32 @@ tmpm_status upload_particles(tmpm_session* s, PackedParticles& pk, uint32_t at) 
       for (int q = 0; q < 3; ++q) cols[(size_t)(18 + q) * n + i] = pk.Pl[3 * i + q];
     }
   });
+}
+
+// Uploads the packed particles into slots [at, at + n) of the live buffers:
+// copies on the session stream from page-locked staging, one wait at the end
+// (the staging may be reused once this returns).
+tmpm_status upload_particles(tmpm_session* s, PackedParticles& pk, uint32_t at) {
+  const size_t n = pk.size();
+  if (n == 0) return TMPM_OK;
+  ParticleBuffers& pb = s->buf[s->cur];
+  if (pk.cols.size() != 21 * n) pack_columns(pk);
+  const auto& cols = pk.cols;
+  hipStream_t st = s->stream;
+  HIP_TRY(hipMemcpyAsync(pb.pos_mass.as<float4>() + at, pk.pm.data(), 16ull * n, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(pb.vel_vol.as<float4>() + at, pk.vv.data(), 16ull * n, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(pb.cls.as<uint32_t>() + at, pk.cls.data(), 4ull * n, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(pb.ins_id.as<uint32_t>() + at, pk.ins.data(), 4ull * n, hipMemcpyHostToDevice, st));
   for (int q = 0; q < 3; ++q)
-    HIP_TRY(hipMemcpy(pb.P.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)(18 + q) * n], 4ull * n,
-                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(pb.P.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)(18 + q) * n], 4ull * n,
+                           hipMemcpyHostToDevice, st));
   for (int q = 0; q < 9; ++q) {
-    HIP_TRY(hipMemcpy(pb.C.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)q * n], 4ull * n,
-                      hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(pb.F.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)(9 + q) * n], 4ull * n,
-                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(pb.C.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)q * n], 4ull * n,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(pb.F.as<float>() + (size_t)q * s->cap + at, &cols[(size_t)(9 + q) * n], 4ull * n,
+                           hipMemcpyHostToDevice, st));
   }
+  HIP_TRY(hipStreamSynchronize(st));
   return TMPM_OK;
 }
 
@@ -2763,9 +2777,41 @@ tmpm_status stage_scene(const tmpm_world_desc* world, double substep_dt, HostWor
   st = build_scene(scene);
   if (st != TMPM_OK) return st;
   out.pk = pack_pool().acquire();
-  st = pack_scene_host(scene, 0, *out.pk, out.classes);
+  std::vector<DevClass> classes;
+  st = pack_scene_host(scene, 0, *out.pk, classes);
   if (st != TMPM_OK) return st;
+  // The scene's distinct classes, in the order interning would meet them
+  // (runs of particles share a class: compare with the previous one first),
+  // so that admission interns a few classes instead of every particle's.
+  out.class_local.resize(out.pk->size());
+  {
+    std::map<ClassKey, uint32_t> seen;
+    uint32_t last = UINT32_MAX;
+    size_t e = 0;
+    const HostWorld& w = scene.world;
+    for (size_t i = 0; i < w.size(); ++i) {
+      if (!w.enabled[i]) continue;
+      const DevClass& c = classes[i];
+      uint32_t id;
+      if (last != UINT32_MAX && std::memcmp(&out.class_table[last], &c, sizeof(DevClass)) == 0) {
+        id = last;
+      } else {
+        auto key = class_key(c);
+        auto it = seen.find(key);
+        if (it != seen.end()) {
+          id = it->second;
+        } else {
+          id = (uint32_t)out.class_table.size();
+          out.class_table.push_back(c);
+          seen.emplace(key, id);
+        }
+      }
+      out.class_local[e++] = id;
+      last = id;
+    }
+  }
   count_blocks(out.pk->pos32, out.pk->region, out.pk->inv_h, out.pblocks, out.gblocks);
+  pack_columns(*out.pk);
   return TMPM_OK;
 }
 
@@ -2792,13 +2838,25 @@ tmpm_status commit_staged(tmpm_session* s, tmpm_staged_scene& staged, uint32_t* 
   const uint32_t first = s->id_space;
   scene.first = first;
   PackedParticles& pk = *staged.pk;
-  for (size_t e = 0; e < pk.size(); ++e) pk.ins[e] += first;
   for (size_t d = 0; d < pk.disabled.size(); d += 2)
     pk.disabled[d].w = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, pk.disabled[d].w) + first);
-  for (DevClass& c : staged.classes) c.region |= slot << 16;
+  // the staged classes take the slot's scene index and are interned in the
+  // order the particles meet them; the particles then map to the ids
+  std::vector<uint32_t> ids(staged.class_table.size());
+  for (size_t c = 0; c < staged.class_table.size(); ++c) {
+    DevClass dc = staged.class_table[c];
+    dc.region |= slot << 16;
+    ids[c] = intern_class(s, dc);
+  }
+  const std::vector<uint32_t>& local = staged.class_local;
+  parallel_for(pk.size(), [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      pk.cls[i] = ids[local[i]];
+      pk.ins[i] += first;
+    }
+  });
   if (slot == s->scenes.size()) s->scenes.push_back(std::move(scene));
   else s->scenes[slot] = std::move(scene);
-  intern_scene_classes(s, s->scenes[slot].world, staged.classes, pk, 0);
   s->plastic = classes_plastic(s);
   // from here on device state changes; a failure invalidates the batch
   auto broken = [&](tmpm_status code) {

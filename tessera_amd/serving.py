@@ -163,7 +163,7 @@ class Server(_Loop):
     runs the previous tick."""
 
     def __init__(self, transfers: HipMpmTransfers, hbm_budget: int, steps_per_tick: int = 16, mode: str = "fp32",
-                 keep_worlds: bool = True, stage_ahead: int = 4, poll_s: float = 2.0e-4):
+                 keep_worlds: bool = True, stage_ahead: int = 4, poll_s: float = 5.0e-5):
         super().__init__(transfers, hbm_budget, steps_per_tick, mode, keep_worlds)
         self.stage_ahead = int(stage_ahead)
         self.poll_s = float(poll_s)
