@@ -1801,20 +1801,30 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
 // of the particle arrays at their boundaries, and G2P both reads and writes
 // those lines.  Workgroup w runs on XCD w % 8 (round-robin dispatch) and each
 // XCD has its own L2, so with block k on workgroup k every boundary line was
-// fetched, and partly written back, by two XCDs.  Here the workgroups of one
-// XCD take a contiguous range of blocks: a single pass when the grid covers
-// npb rounded up to 8, the grid-stride order otherwise.  Measured on the
-// 1M-particle four-cube scene: G2P 64.5 -> 61.5 us, bit-identical results;
-// P2G, which reads those lines but writes only its own tile, ran 0.7 us
-// slower with the same order and keeps block k on workgroup k.
-constexpr uint32_t kXcds = 8;
+// fetched, and partly written back, by two XCDs.  Here runs of kXcdChunk
+// consecutive blocks go to one XCD, the runs rotating over the XCDs, so only
+// run boundaries are shared while every XCD still gets an even sample of the
+// scene (one contiguous eighth per XCD was 2% slower on uneven workloads: a
+// batch of scenes, the dam break).  A single pass when the grid covers npb
+// rounded up to whole rotations; the grid-stride order below kXcdMinBlocks
+// blocks or on a smaller grid.  Measured on the 1M-particle four-cube scene:
+// the substep 1% faster, bit-identical results; P2G, which reads those lines
+// but writes only its own tile, keeps block k on workgroup k.
+constexpr uint32_t kXcds = 8, kXcdChunk = 8, kXcdMinBlocks = 1024;
+__device__ __forceinline__ bool xcd_order(uint32_t npb, uint32_t grid, uint32_t& span) {
+  constexpr uint32_t g = kXcds * kXcdChunk;
+  span = (npb + g - 1) / g * g;
+  return npb >= kXcdMinBlocks && grid >= span;
+}
 __device__ __forceinline__ uint32_t xcd_span(uint32_t npb, uint32_t grid) {
-  const uint32_t per = (npb + kXcds - 1) / kXcds;
-  return grid >= per * kXcds ? per * kXcds : npb;
+  uint32_t span;
+  return xcd_order(npb, grid, span) ? span : npb;
 }
 __device__ __forceinline__ uint32_t xcd_block(uint32_t v, uint32_t npb, uint32_t grid) {
-  const uint32_t per = (npb + kXcds - 1) / kXcds;
-  return grid >= per * kXcds ? (v % kXcds) * per + v / kXcds : v;
+  uint32_t span;
+  if (!xcd_order(npb, grid, span)) return v;
+  const uint32_t x = v % kXcds, i = v / kXcds;
+  return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + i % kXcdChunk;
 }
 
 // Reorder.  In the MOVE variant (q set), G2P also moves every particle's whole state
