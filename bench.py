@@ -216,8 +216,8 @@ def main() -> int:
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
-        print("bench.py: this build measures one GPU; every GPU box here has one GPU and the path shards by "
-              "scene or by spatial slab (see DESIGN.md), so --gpus must be 1.", file=sys.stderr)
+        print("bench.py measures one GPU (the substep shards by scene or by spatial slab across processes, "
+              "which it does not launch), so --gpus must be 1.", file=sys.stderr)
         return 2
 
     from tessera_amd.mpm import HipMpmResidentSession, HipMpmTransfers

@@ -2,7 +2,7 @@
 against the FP64 oracle (the measurements of tests/accuracy.py, with the
 FP32 restatement's floor beside each device error).
 
-    python tests/accuracy_report.py --out gpurun_out/accuracy_vs_fp64.md
+    python tests/accuracy_report.py --out accuracy_vs_fp64.md
 
 Needs the GPU.  C1 runs its whole 200 substeps; C2 (1000 substeps) and C3
 (2000) are checked at checkpoints, in both P2G modes, and C2 also over a
@@ -175,7 +175,7 @@ def topo_table(rows: list[dict]) -> list[str]:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out/accuracy_vs_fp64.md")
+    ap.add_argument("--out", default="accuracy_vs_fp64.md")
     ap.add_argument("--c2-window", type=int, default=20)
     ap.add_argument("--skip-c3", action="store_true")
     ap.add_argument("--c4-sample", type=int, default=8, help="batch scenes compared (0: skip C4)")
