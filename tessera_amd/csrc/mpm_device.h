@@ -11,8 +11,9 @@
 //   ins_id[i]    u32     insertion index of the particle stored in slot i
 // Grid: 4x4x4-cell blocks; node (slot s, local l) at s*64 + l with
 // l = (x&3)*16 + (y&3)*4 + (z&3).  P2G writes one 6^3-node tile per particle
-// block (ptile); the grid update sums, in a fixed order, the up to 8 tiles
-// that cover each node (grid_rev), so no global atomics are needed.
+// block, each of its 8 sub-boxes into the area of the grid block it covers
+// (ptile); the grid update sums, in a fixed order, the up to 8 parts on each
+// node (grid_rev says which exist), so no global atomics are needed.
 #pragma once
 
 #include <cstdint>
@@ -201,7 +202,7 @@ struct Topo {
   float* react_sub;
   double* react_total;
   uint32_t obstacle_total;
-  void* ptile;              // per particle block: its 6^3 P2G tile (FP32: float4; FIXED64: 4 x int64 per node)
+  void* ptile;              // per grid slot, 216 nodes: the tile parts covering it (FP32: float4; FIXED64: 4 x int64)
   uint32_t* grid_rev;       // per grid slot, 8 entries: particle block whose tile covers it at offset d
   float4* grid_vel;
   uint32_t* node_mask;      // 2 words per grid slot (diagnostic live-node set)

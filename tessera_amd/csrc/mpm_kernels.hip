@@ -1388,13 +1388,18 @@ __global__ __launch_bounds__(256) void k_cpic_mask(Parts p, Topo t, const DevCla
 // 8. P2G
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int tile_index(int x, int y, int z) { return (x * kTileEdge + y) * kTileEdge + z; }
-// Global tile layout.  A 6^3 tile covers 8 grid blocks (x, y, z each in
-// [0, 4) or [4, 6)); its 8 sub-boxes are stored one after the other (sizes
-// 64, 32, 32, 16, 32, 16, 16, 8 nodes, z fastest inside each), so that the
-// grid update's wave for one grid block reads one contiguous run of each
-// covering tile.  Stored node by node in tile order, a sub-box row was 2 or 4
-// nodes of a 128-byte line whose other part another grid block reads, often
-// on another XCD with its own L2: the tiles were fetched ~1.8 times.
+// Tile placement.  A 6^3 tile covers 8 grid blocks (x, y, z each in [0, 4)
+// or [4, 6)); sub-box d of particle block k's tile lies on grid block
+// slot8[d] and is stored in that grid block's own 216-node area, at the
+// offset it has in a tile (sub-boxes of 64, 32, 32, 16, 32, 16, 16, 8 nodes,
+// z fastest inside each).  The area of grid block s thus holds, at sub-box d,
+// the part of the tile of the particle block s - d that covers it: the grid
+// update reads its 8 covering parts at addresses that depend on s alone, in
+// the same round trip as the map of which of them exist (grid_rev), instead
+// of looking up the covering tiles first.  Stored node by node in tile
+// order, a sub-box row was 2 or 4 nodes of a 128-byte line whose other part
+// another grid block reads, often on another XCD with its own L2: the tiles
+// were fetched ~1.8 times.
 __device__ __forceinline__ int tile_store_index(int x, int y, int z) {
   const int dx = x >> 2, dy = y >> 2, dz = z >> 2;
   const int sx = dx ? 2 : 4, sy = dy ? 2 : 4, sz = dz ? 2 : 4;
@@ -1535,7 +1540,8 @@ __device__ __forceinline__ void p2g_particle(unsigned long long* mine, const Dev
 // the integers (FIXED64) or converted to f32 (FP32).  Each thread takes node
 // pairs (n, n + 1), n even.
 template <int FIXED, int THREADS, int COPIES>
-__device__ __forceinline__ void p2g_flush(unsigned long long* tacc, const Topo& t, uint32_t k, const DevScene& sc) {
+__device__ __forceinline__ void p2g_flush(unsigned long long* tacc, const Topo& t, const uint32_t* slots,
+                                          const DevScene& sc) {
   constexpr int kTile = 4 * kTileNodes;
   const int tid = threadIdx.x;
   // sums of nodes q and q + 1 (q even) over the copies
@@ -1553,16 +1559,19 @@ __device__ __forceinline__ void p2g_flush(unsigned long long* tacc, const Topo& 
                     pz = pair_sum(3 * kTileNodes + n);
     // nodes n and n + 1 differ in z only (z even), so they share a sub-box
     // and are adjacent in it
-    const int o = tile_store_index(n / 36, (n / 6) % 6, n % 6);
+    const int x = n / 36, y = (n / 6) % 6, z = n % 6;
+    const int o = tile_store_index(x, y, z);
+    const uint32_t slot = slots[((x >> 2) << 2) | ((y >> 2) << 1) | (z >> 2)];
+    if (slot == kInvalid) continue;  // grid capacity exhausted (flagged by binning)
     if (FIXED) {
-      longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + ((size_t)k * kTileNodes + o) * 2;
+      longlong2* out = reinterpret_cast<longlong2*>(t.ptile) + ((size_t)slot * kTileNodes + o) * 2;
       out[0] = make_longlong2(m.x, px.x);
       out[1] = make_longlong2(py.x, pz.x);
       out[2] = make_longlong2(m.y, px.y);
       out[3] = make_longlong2(py.y, pz.y);
     } else {
       const float im = exp2f(-(float)sc.mass_shift), ip = exp2f(-(float)sc.mom_shift);
-      float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)k * kTileNodes + o;
+      float4* out = reinterpret_cast<float4*>(t.ptile) + (size_t)slot * kTileNodes + o;
       out[0] = make_float4(__ll2float_rn(m.x) * im, __ll2float_rn(px.x) * ip, __ll2float_rn(py.x) * ip,
                            __ll2float_rn(pz.x) * ip);
       out[1] = make_float4(__ll2float_rn(m.y) * im, __ll2float_rn(px.y) * ip, __ll2float_rn(py.y) * ip,
@@ -1623,7 +1632,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     __syncthreads();
     flag_error(t.counters, bad, ERR_INTERNAL);
     flag_scene_error(t, (uint32_t)key.w >> 16, range, ERR_FIXED_RANGE);
-    p2g_flush<FIXED, THREADS, COPIES>(tacc, t, k, sc);
+    p2g_flush<FIXED, THREADS, COPIES>(tacc, t, s_slot, sc);
     __syncthreads();
   }
 }
@@ -1713,31 +1722,36 @@ __global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* sce
   for (uint32_t s = blockIdx.x * waves + (threadIdx.x >> 6); s < ng; s += gridDim.x * waves) {
     const int4 key = t.grid_key[s];
     const DevScene& sc = scenes[(uint32_t)key.w >> 16];
-    // Sum the covering tiles in the fixed order d = 0..7 (tile of block
-    // key - d, node at tile coordinates local + 4 d).
+    // Sum the covering tile parts in the fixed order d = 0..7 (the part of
+    // block key - d's tile, node at tile coordinates local + 4 d), all stored
+    // in this grid block's area.  The parts load together with grid_rev,
+    // which says which of them this substep wrote; a part with no particle
+    // block adds nothing (selected, not multiplied, so stale values never
+    // enter the sum: + 0 leaves every sum's bits unchanged, and no sum here
+    // is -0, since tile values are converted integers).
     float m = 0.0f;
     float3 mom = make_float3(0.f, 0.f, 0.f);
     long long im = 0, ix = 0, iy = 0, iz = 0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
       const int dx = d >> 2, dy = (d >> 1) & 1, dz = d & 1;
-      const uint32_t b = t.grid_rev[8 * s + d];
+      const bool has = t.grid_rev[8 * s + d] != kInvalid;
       const int tx = lx + 4 * dx, ty = ly + 4 * dy, tz = lz + 4 * dz;
-      if (b == kInvalid || tx >= kTileEdge || ty >= kTileEdge || tz >= kTileEdge) continue;
-      const size_t n = (size_t)b * kTileNodes + tile_store_index(tx, ty, tz);
+      if (tx >= kTileEdge || ty >= kTileEdge || tz >= kTileEdge) continue;
+      const size_t n = (size_t)s * kTileNodes + tile_store_index(tx, ty, tz);
       if (FIXED) {
         const longlong2* src = reinterpret_cast<const longlong2*>(t.ptile) + 2 * n;
         const longlong2 u = src[0], w = src[1];
-        im += u.x;
-        ix += u.y;
-        iy += w.x;
-        iz += w.y;
+        im += has ? u.x : 0ll;
+        ix += has ? u.y : 0ll;
+        iy += has ? w.x : 0ll;
+        iz += has ? w.y : 0ll;
       } else {
         const float4 a = reinterpret_cast<const float4*>(t.ptile)[n];
-        m += a.x;
-        mom.x += a.y;
-        mom.y += a.z;
-        mom.z += a.w;
+        m += has ? a.x : 0.0f;
+        mom.x += has ? a.y : 0.0f;
+        mom.y += has ? a.z : 0.0f;
+        mom.z += has ? a.w : 0.0f;
       }
     }
     if (FIXED) {

@@ -1142,7 +1142,7 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   t.bin_claims = s->bin_claims.as<uint32_t>();
   t.bin_nclaims = s->bin_nclaims.as<uint32_t>();
   t.bin_csum = s->bin_csum.as<uint32_t>();
-  HIP_TRY(s->ptile.alloc((size_t)pblock_cap * kTileNodes * (s->fixed64 ? 32 : 16)));
+  HIP_TRY(s->ptile.alloc((size_t)grid_cap * kTileNodes * (s->fixed64 ? 32 : 16)));
   HIP_TRY(s->grid_rev.alloc(32ull * grid_cap));
   HIP_TRY(s->grid_vel.alloc((size_t)grid_cap * kBlockNodes * 16));
   HIP_TRY(s->node_mask.alloc(8ull * grid_cap));
