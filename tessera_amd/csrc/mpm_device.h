@@ -171,14 +171,14 @@ struct Topo {
   uint32_t* owner;          // bucket -> owner id (0 empty, 1..n particle, > n neighbour insert)
   int4* bkey;               // bucket -> key, written by the claimer; x = kNoKey until then
   uint32_t* bucket_count;   // particles binned into the bucket's block
-  uint32_t* nbr_bucket;     // 8 per bucket: buckets of block + {0,1}^3
+  uint32_t* bucket_start;   // bucket -> first block-sorted position of its particle block
   uint32_t* bucket_slot;    // bucket -> grid slot
   uint32_t* bucket_pblock;  // bucket -> particle-block id
   uint32_t bucket_mask;     // bucket count - 1 (power of two)
   int4* grid_key;           // slot -> (bx, by, bz, region)
   uint32_t grid_cap;
   uint32_t pblock_cap;
-  uint32_t* pblock_bucket;
+  uint32_t* pblock_nbr;     // 8 per particle block: buckets of block + {0,1}^3
   uint32_t* pblock_slot8;   // 8 grid slots per particle block
   uint32_t* pblock_start;
   uint32_t* pblock_len;

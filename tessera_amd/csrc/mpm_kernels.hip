@@ -456,22 +456,25 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
     if (pb < t.pblock_cap) {
       t.bucket_pblock[b] = pb;
       t.bucket_slot[b] = pb;
-      t.pblock_bucket[pb] = b;
+      t.pblock_nbr[8 * pb] = b;  // tile block d = 0: the block itself
       t.grid_key[pb] = key;
       clear_rev(t, pb);
-      csum += t.bucket_count[b];
+      // the block's particle count, where k_bin_offsets reads it in id order
+      const uint32_t cnt = t.bucket_count[b];
+      t.pblock_len[pb] = cnt;
+      csum += cnt;
     } else {
       t.bucket_pblock[b] = kInvalid;
       t.bucket_slot[b] = kInvalid;
+      t.bucket_start[b] = kInvalid;
       over = true;
     }
   }
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < 7 * len; q += kBinThreads) {
     const uint32_t j = q / 7u, d = q % 7u + 1u;
-    const uint32_t b = j < kBinThreads ? s_b[j] : stage[j];
-    if (d == 1) t.nbr_bucket[8 * b] = b;
-    if (before[0] + j >= t.pblock_cap) continue;
+    const uint32_t b = j < kBinThreads ? s_b[j] : stage[j], pb = before[0] + j;
+    if (pb >= t.pblock_cap) continue;
     int4 k = j < kBinThreads ? s_key[j] : t.bkey[b];
     k.x += (int)(d >> 2);
     k.y += (int)((d >> 1) & 1);
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(8))
       t.bucket_pblock[got] = kInvalid;
       nstage[atomicAdd(&s_claims, 1u)] = got;
     }
-    t.nbr_bucket[8 * b + d] = got;
+    t.pblock_nbr[8 * pb + d] = got;
   }
   flag_error(t.counters, over, ERR_TOPOLOGY);
   csum = wave_sum(csum);
@@ -545,12 +548,13 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_offsets(Parts p, Topo t, co
     for (uint32_t c0 = 0; c0 < len; c0 += 64) {
       const uint32_t j = c0 + lane, pb = before[0] + j;
       const bool mine = j < len && pb < t.pblock_cap;
-      const uint32_t cnt = mine ? t.bucket_count[stage[j]] : 0u;
+      // count (k_bin_blocks) and bucket load together
+      const uint32_t cnt = mine ? t.pblock_len[pb] : 0u, b = mine ? stage[j] : 0u;
       uint32_t sum;
       const uint32_t excl = wave_exclusive_scan(cnt, sum);
       if (mine) {
         t.pblock_start[pb] = carry + excl;
-        t.pblock_len[pb] = cnt;
+        t.bucket_start[b] = carry + excl;  // the scatter's one look-up per particle
       }
       carry += sum;
     }
@@ -584,8 +588,7 @@ __device__ __forceinline__ void pblock_finalize(const Topo& t) {
   uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npb * 8; q += gridDim.x * blockDim.x) {
     uint32_t k = q >> 3, d = q & 7;
-    uint32_t b = t.pblock_bucket[k];
-    uint32_t nb = t.nbr_bucket[8 * b + d];
+    uint32_t nb = t.pblock_nbr[q];
     uint32_t slot = nb == kInvalid ? kInvalid : t.bucket_slot[nb];
     t.pblock_slot8[q] = slot;
     // grid block `slot` = this block + d: exactly one particle block per (slot, d)
@@ -604,7 +607,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
   pblock_finalize(t);
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < p.n; i0 += kScatterK * stride) {
-    uint32_t b[kScatterK], r[kScatterK], k[kScatterK];
+    uint32_t b[kScatterK], r[kScatterK], st[kScatterK];
 #pragma unroll
     for (int u = 0; u < kScatterK; ++u) {
       const uint32_t i = i0 + u * stride;
@@ -612,12 +615,12 @@ __global__ __launch_bounds__(256) void k_bin_scatter(Parts p, Topo t) {
       r[u] = i < p.n ? t.prank[i] : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < kScatterK; ++u) k[u] = b[u] != kInvalid ? t.bucket_pblock[b[u]] : kInvalid;
+    for (int u = 0; u < kScatterK; ++u) st[u] = b[u] != kInvalid ? t.bucket_start[b[u]] : kInvalid;
     bool bad = false;
 #pragma unroll
     for (int u = 0; u < kScatterK; ++u) {
-      if (k[u] == kInvalid) continue;
-      const uint32_t pos = t.pblock_start[k[u]] + r[u];
+      if (st[u] == kInvalid) continue;  // a block past the particle-block capacity (flagged)
+      const uint32_t pos = st[u] + r[u];
       if (pos < p.n) t.sorted[pos] = i0 + u * stride;
       else bad = true;  // offsets and ranks disagree: never write past the list
     }

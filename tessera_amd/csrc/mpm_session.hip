@@ -654,7 +654,7 @@ struct tmpm_session {
   uint32_t reorder_interval = 16;
 
   // topology
-  DevBuf owner, bkey, bucket_count, nbr_bucket, bucket_slot, bucket_pblock, grid_key, pblock_bucket,
+  DevBuf owner, bkey, bucket_count, bucket_start, bucket_slot, bucket_pblock, grid_key, pblock_nbr,
       pblock_slot8, pblock_start, pblock_len, pslot, prank, sorted, counters, ptile, grid_rev, grid_vel, node_mask,
     bin_stage, bin_nstage, bin_claims, bin_nclaims, bin_csum, bmask, pmask;
   Topo topo{};
@@ -702,7 +702,7 @@ struct tmpm_session {
     for (const DevBuf* d :
          {&staging, &disabled_rec, &force, &classes, &scene_plain, &scene_force, &obstacles, &scene_err, &planes,
           &react_bins, &react_sub, &react_total, &rigid, &rigid_maps, &body_scene, &owner, &bkey, &bucket_count,
-          &nbr_bucket, &bucket_slot, &bucket_pblock, &grid_key, &pblock_bucket, &pblock_slot8, &pblock_start,
+          &bucket_start, &bucket_slot, &bucket_pblock, &grid_key, &pblock_nbr, &pblock_slot8, &pblock_start,
           &pblock_len, &pslot, &prank, &sorted, &counters, &ptile, &grid_rev, &grid_vel, &node_mask, &bin_stage,
           &bin_nstage, &bin_claims, &bin_nclaims, &bin_csum, &bmask, &pmask, &audit_dev})
       add(*d);
@@ -1110,11 +1110,11 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   HIP_TRY(s->bkey.alloc(16ull * buckets));
   HIP_TRY(s->bmask.alloc(4ull * buckets));
   HIP_TRY(s->bucket_count.alloc(4ull * buckets));
-  HIP_TRY(s->nbr_bucket.alloc(32ull * buckets));
+  HIP_TRY(s->bucket_start.alloc(4ull * buckets));
   HIP_TRY(s->bucket_slot.alloc(4ull * buckets));
   HIP_TRY(s->bucket_pblock.alloc(4ull * buckets));
   HIP_TRY(s->grid_key.alloc(16ull * grid_cap));
-  HIP_TRY(s->pblock_bucket.alloc(4ull * pblock_cap));
+  HIP_TRY(s->pblock_nbr.alloc(32ull * pblock_cap));
   HIP_TRY(s->pblock_slot8.alloc(32ull * pblock_cap));
   HIP_TRY(s->pblock_start.alloc(4ull * pblock_cap));
   HIP_TRY(s->pblock_len.alloc(4ull * pblock_cap));
@@ -1151,14 +1151,14 @@ tmpm_status alloc_topology(tmpm_session* s, uint32_t pblock_cap, uint32_t grid_c
   t.bmask = s->bmask.as<uint32_t>();
   t.pmask = s->pmask.as<uint32_t>();
   t.bucket_count = s->bucket_count.as<uint32_t>();
-  t.nbr_bucket = s->nbr_bucket.as<uint32_t>();
+  t.bucket_start = s->bucket_start.as<uint32_t>();
   t.bucket_slot = s->bucket_slot.as<uint32_t>();
   t.bucket_pblock = s->bucket_pblock.as<uint32_t>();
   t.bucket_mask = buckets - 1;
   t.grid_key = s->grid_key.as<int4>();
   t.grid_cap = grid_cap;
   t.pblock_cap = pblock_cap;
-  t.pblock_bucket = s->pblock_bucket.as<uint32_t>();
+  t.pblock_nbr = s->pblock_nbr.as<uint32_t>();
   t.pblock_slot8 = s->pblock_slot8.as<uint32_t>();
   t.pblock_start = s->pblock_start.as<uint32_t>();
   t.pblock_len = s->pblock_len.as<uint32_t>();
