@@ -15,9 +15,10 @@
 //                         substeps the particle state itself moves: reorder)
 //   5 k_p2g               per particle block: stress, 27-node B-spline scatter
 //                         into a 6^3 LDS tile accumulated in 64-bit fixed point
-//                         (integer LDS atomics), stored as the block's tile
-//   6 k_grid_update       per node: sum of the covering tiles in a fixed order,
-//                         v = p/m + g dt, walls, obstacles
+//                         (integer LDS atomics); each of the tile's 8 sub-boxes
+//                         is stored in the area of the grid block it covers
+//   6 k_grid_update       per node: sum of the covering tile parts in a fixed
+//                         order, v = p/m + g dt, walls, obstacles
 //   7 k_g2p               per particle block: 6^3 velocity tile in LDS, gather,
 //                         v/C/x/F update, material projection, validity flags;
 //                         clears the block hash for the next substep
@@ -1539,9 +1540,10 @@ __device__ __forceinline__ void p2g_particle(unsigned long long* mine, const Dev
   }
 }
 
-// Writes block k's tile from LDS: per node the sum over the tile copies, as
-// the integers (FIXED64) or converted to f32 (FP32).  Each thread takes node
-// pairs (n, n + 1), n even.
+// Writes a block's tile from LDS, each sub-box d into the area of grid block
+// slots[d] (the block's 8 tile slots): per node the sum over the tile copies,
+// as the integers (FIXED64) or converted to f32 (FP32).  Each thread takes
+// node pairs (n, n + 1), n even.
 template <int FIXED, int THREADS, int COPIES>
 __device__ __forceinline__ void p2g_flush(unsigned long long* tacc, const Topo& t, const uint32_t* slots,
                                           const DevScene& sc) {
