@@ -231,3 +231,48 @@ def test_fixed_point_range_is_a_capacity_error_in_both_modes(transfers, mode):
         s.synchronize()
     assert not s.valid and s.world().substeps == 1
     assert_same(s.read_particles(), before)
+
+
+def _exploding_clusters() -> tuple[MpmWorld, float]:
+    """Eight clusters of 2,048 particles, each within ~2^3 blocks, expanding homologously (v
+    proportional to the distance from the cluster's centre) at up to 50 m/s, 0.32 cells per
+    substep, inside the CFL bound.  Four such clusters occupy ~1,030 particle blocks after 65
+    substeps (the transfers slow the edge to ~40 m/s), so these eight need ~2,000, past the
+    1,024 the session sizes from the initial 64."""
+    rng = np.random.default_rng(7)
+    h = 1.0 / 64
+    r0 = 2 * h
+    pos, vel = [], []
+    for c in range(8):
+        center = np.array([0.5 + 1.5 * (c % 4), 0.5 + 1.5 * (c // 4), 0.5])
+        d = rng.normal(size=(2048, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        p = center + d * r0 * rng.uniform(0.0, 1.0, size=(2048, 1)) ** (1.0 / 3.0)
+        pos.append(p)
+        vel.append((p - center) / r0 * 50.0)
+    pa = _particles(np.concatenate(pos), np.concatenate(vel), h / 4, 1.0e3, MaterialModel.fluid(1.0e2, 7.0, 0.0))
+    params = MpmParams(cell_width=h, gravity=np.zeros(3), max_substep=1e-4, bounds=None)
+    return MpmWorld(pa, params), 1.0e-4
+
+
+def test_topology_overflow_within_a_submission_is_a_capacity_error(transfers):
+    """The topology grows only between submissions.  Particle blocks beyond its capacity within
+    one submission are left out of the substep (their particles are neither scattered nor
+    moved) and flagged, and synchronize reports Capacity: the session is invalid and keeps the
+    last synchronized world (gpu.rs:377-381, as for any failure after submission)."""
+    world, dt = _exploding_clusters()
+    s = HipMpmResidentSession(transfers, world, dt)
+    s.step(1)
+    assert s.topology_stats()["particle_blocks"] <= 128
+    before = s.read_particles()
+    s.submit_steps(64)
+    with pytest.raises(Capacity):
+        s.synchronize()
+    assert not s.valid and s.world().substeps == 1
+    assert_same(s.read_particles(), before)
+    # the same substeps in chunks that synchronize in between grow the topology instead
+    ok = HipMpmResidentSession(transfers, world, dt)
+    for _ in range(16):
+        ok.step(4)
+    assert ok.valid and ok.world().substeps == 64
+    assert ok.topology_stats()["particle_blocks"] > 1500
