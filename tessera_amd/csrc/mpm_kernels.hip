@@ -1598,13 +1598,15 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
   const int tid = threadIdx.x;
   unsigned long long* const mine = tacc + (COPIES == 2 ? (lane_id() & 1) * kTile : 0);
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
-    // The block's particle range, slot list and first particle indices load
-    // together, and the first particles' state loads while the header (key,
-    // then scene) resolves: three dependent round trips before any compute
-    // instead of five.
+    // The block's particle range, slot list and the particle indices of the
+    // first two passes load together, and the first particles' state loads
+    // while the header (key, then scene) resolves: three dependent round trips
+    // before any compute instead of five, and none for the second pass's
+    // indices.
     const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
     const uint32_t slot = tid < 8 ? t.pblock_slot8[8 * k + tid] : 0u;
     const uint32_t i_first = (uint32_t)tid < len ? t.sorted[start + tid] : 0u;
+    const uint32_t i_second = (uint32_t)(tid + THREADS) < len ? t.sorted[start + tid + THREADS] : 0u;
     if (tid < 8) s_slot[tid] = slot;
     for (int q = tid; q < COPIES * kTile / 2; q += THREADS) reinterpret_cast<ulonglong2*>(tacc)[q] = make_ulonglong2(0, 0);
     __syncthreads();
@@ -1621,7 +1623,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
     bool bad = false, range = len > kP2GMaxBlock;
     for (uint32_t j = tid; j < len; j += THREADS) {
-      const uint32_t i = j == (uint32_t)tid ? i_first : t.sorted[start + j];
+      const uint32_t i = j == (uint32_t)tid ? i_first : j == (uint32_t)(tid + THREADS) ? i_second : t.sorted[start + j];
       const float4 pm = p.pos_mass[i];
       const float4 vv = p.vel_vol[i];
       const DevClass& cls = classes[p.cls[i]];
@@ -1888,11 +1890,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
   for (uint32_t v = blockIdx.x; v < span; v += gridDim.x) {
     const uint32_t k = xcd_block(v, npb, gridDim.x);
     if (k >= npb) continue;  // uniform over the workgroup
-    // As in P2G: the block's particle range, slot list and first particle
-    // indices load together, and the block key loads with the velocity tile.
+    // As in P2G: the block's particle range, slot list and the particle
+    // indices of the first two passes load together, and the block key loads
+    // with the velocity tile.
     const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
     const uint32_t slot0 = threadIdx.x < 8 ? t.pblock_slot8[8 * k + threadIdx.x] : 0u;
     const uint32_t i_first = threadIdx.x < len ? t.sorted[start + threadIdx.x] : 0u;
+    const uint32_t i_second = threadIdx.x + blockDim.x < len ? t.sorted[start + threadIdx.x + blockDim.x] : 0u;
     if (threadIdx.x < 8) s_slot[threadIdx.x] = slot0;
     if (REACT)  // the previous block's sums were flushed before the loop's last barrier
       for (int q = threadIdx.x; q < kReactSums; q += blockDim.x) rsum[q] = 0ull;
@@ -1916,7 +1920,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
       bool failed = false, bad = false;
       for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
-        const uint32_t i = j == threadIdx.x ? i_first : t.sorted[start + j];
+        const uint32_t i = j == threadIdx.x ? i_first : j == threadIdx.x + blockDim.x ? i_second : t.sorted[start + j];
         const uint32_t w = move ? start + j : i;  // the slot the particle is written to
         const float4 pm = p.pos_mass[i];
         const uint32_t ci = p.cls[i];
