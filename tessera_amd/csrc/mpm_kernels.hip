@@ -1445,10 +1445,10 @@ __device__ __forceinline__ void tile_node(int t, int& d, int& local) {
 // the FIXED64 grid update's sum of 8 tiles below 2^62; larger blocks raise
 // Capacity.
 //
-// COPIES = 2 gives lanes of even and odd index separate tiles: particles of
-// one cell are often adjacent in storage, and LDS serializes atomics of one
-// instruction that hit the same address.  The copies are summed in a fixed
-// order.  FP32 mode converts each node's sum to f32; FIXED64 mode stores the
+// COPIES = 2 gives the block two tile copies, and the particles of one cell
+// in a wavefront alternate between them (p2g_particle): LDS serializes atomics
+// of one instruction that hit the same address.  The copies are summed in a
+// fixed order.  FP32 mode converts each node's sum to f32; FIXED64 mode stores the
 // integers and the grid update sums them exactly.
 constexpr uint32_t kP2GMaxBlock = 1u << 14;
 constexpr float kFixedBound = 0x1p45f;
@@ -1475,12 +1475,20 @@ __device__ __forceinline__ unsigned long long fixed64_of(float v) {
 }
 
 // One particle of a P2G block: its stress and its 27 quadratic B-spline
-// contributions (mass and APIC momentum) into the block's LDS tile copy
-// `mine`, as rounded 64-bit fixed-point integers (fixed64_round).  `bad`
-// records a particle outside the block, `range` a contribution beyond the
-// fixed-point bound.  snow_p is the snow hardening factor (1 otherwise).
-template <bool PLASTIC>
-__device__ __forceinline__ void p2g_particle(unsigned long long* mine, const DevClass& cls, float4 pm, float4 vv,
+// contributions (mass and APIC momentum) into one of the block's COPIES LDS
+// tile copies (tacc), as rounded 64-bit fixed-point integers
+// (fixed64_round).  `bad` records a particle outside the block, `range` a
+// contribution beyond the fixed-point bound.  snow_p is the snow hardening
+// factor (1 otherwise).
+//
+// LDS serializes the atomics of one instruction that hit the same address,
+// i.e. lanes whose particles share a cell and a tile copy.  With two copies a
+// lane takes copy (r & 1), r = the number of lower lanes of its wavefront
+// whose particle is in the same cell, so the particles of a cell alternate
+// between the copies.  The copies' sums are integers, so the choice does not
+// change any result.
+template <bool PLASTIC, int COPIES>
+__device__ __forceinline__ void p2g_particle(unsigned long long* tacc, const DevClass& cls, float4 pm, float4 vv,
                                              const M3& F, const M3& C, float snow_p, int ox, int oy, int oz, float h,
                                              float inv_h, float stress_scale, float mscale, float pscale, bool& bad,
                                              bool& range) {
@@ -1490,6 +1498,19 @@ __device__ __forceinline__ void p2g_particle(unsigned long long* mine, const Dev
   if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
     bad = true;
     return;
+  }
+  unsigned long long* mine = tacc;
+  if (COPIES == 2) {
+    const uint32_t cid = (uint32_t)((lx << 4) | (ly << 2) | lz);
+    uint64_t same = __ballot(true);  // the lanes with a particle in this block
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const bool bit = (cid >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+    mine = tacc + (below & 1u) * (4 * kTileNodes);
   }
   const float fx = pm.x * inv_h - (float)cell.x;
   const float fy = pm.y * inv_h - (float)cell.y;
@@ -1596,13 +1617,14 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
   __shared__ uint32_t s_slot[8];
   const uint32_t npb = t.counters[CNT_PARTICLE_BLOCKS];
   const int tid = threadIdx.x;
-  unsigned long long* const mine = tacc + (COPIES == 2 ? (lane_id() & 1) * kTile : 0);
   for (uint32_t k = blockIdx.x; k < npb; k += gridDim.x) {
     // The block's particle range, slot list and the particle indices of the
     // first two passes load together, and the first particles' state loads
     // while the header (key, then scene) resolves: three dependent round trips
     // before any compute instead of five, and none for the second pass's
-    // indices.
+    // indices.  Pass p takes the block's particles p, p + P, p + 2P, ... of its
+    // P passes: neighbours in the block's order, often of one cell, go to
+    // different passes and so to different atomic instructions.
     const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
     const uint32_t slot = tid < 8 ? t.pblock_slot8[8 * k + tid] : 0u;
     const uint32_t i_first = (uint32_t)tid < len ? t.sorted[start + tid] : 0u;
@@ -1633,8 +1655,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
         F.a[q] = p.F[q * p.stride + i];
         C.a[q] = p.C[q * p.stride + i];
       }
-      p2g_particle<PLASTIC>(mine, cls, pm, vv, F, C, PLASTIC && cls.kind == DM_SNOW ? p.P[i] : 1.0f, ox, oy,
-                            oz, h, inv_h, stress_scale, mscale, pscale, bad, range);
+      p2g_particle<PLASTIC, COPIES>(tacc, cls, pm, vv, F, C, PLASTIC && cls.kind == DM_SNOW ? p.P[i] : 1.0f, ox,
+                                    oy, oz, h, inv_h, stress_scale, mscale, pscale, bad, range);
     }
     __syncthreads();
     flag_error(t.counters, bad, ERR_INTERNAL);
