@@ -1871,18 +1871,33 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t v, uint32_t npb, uint32_t
 }
 
 // Reorder.  In the MOVE variant (q set), G2P also moves every particle's whole state
-// into q at its block-sorted position (start + j of its block): the next
-// substeps read particles in nearly sequential order.  Slots past the binned
-// count hold no particle afterwards (retired particles were not binned).
-// P2G has read the unmoved state through the sorted list, which gives the
-// same sums and the same lane assignment as reading it moved, so results do
-// not depend on which substep reorders.  Otherwise q is empty and p is
-// updated in place.
+// into q, into its block's range [start, start + len): the next substeps read
+// particles in nearly sequential order.  Inside the range the particles go
+// round-robin by cell (first one particle of each of the block's cells, then a
+// second, ...; blocks of more than kMoveOrderMax particles keep their order),
+// so that a P2G wavefront's lanes rarely share a cell, whose atomics LDS would
+// serialize.  Slots past the binned count hold no particle afterwards (retired
+// particles were not binned).  P2G has read the unmoved state through the
+// sorted list; its sums do not depend on the order of the particles, so
+// results do not depend on which substep reorders.  Otherwise q is empty and p
+// is updated in place.
+constexpr uint32_t kMoveOrderMax = 1024;
+constexpr uint32_t kMoveLevels = 128;  // particles per cell beyond which a block keeps its order
+
 template <int REACT, bool SHAPES, bool PLASTIC, bool MOVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && !PLASTIC ? 5 : 1))) void k_g2p(
     Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
+  // MOVE: particles per cell of the block (64 cells, then particles outside
+  // it); per particle its cell and rank in the cell, then its offset in the
+  // block's range; per rank level the cells that reach it and the offset
+  // where the level starts
+  __shared__ uint32_t s_cnt[MOVE ? 65 : 1];
+  __shared__ uint32_t s_dest[MOVE ? kMoveOrderMax : 1];
+  __shared__ unsigned long long s_level_cells[MOVE ? kMoveLevels : 1];
+  __shared__ uint32_t s_level_start[MOVE ? kMoveLevels : 1];
+  __shared__ uint32_t s_levels;
   // Particle-side reactions (REACT) on a scene's first kMaxReactObstacles
   // obstacles: each contact adds its rounded impulse (reaction_fixed) into
   // these workgroup sums, which go to the obstacle's bins once per block.
@@ -1941,9 +1956,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
       const float h = sc.h, inv_h = sc.inv_h, dt = sc.dt, inv_d = sc.inv_d;
       const int ox = key.x * kBlockCells, oy = key.y * kBlockCells, oz = key.z * kBlockCells;
       bool failed = false, bad = false;
+      const bool spread = move && len <= kMoveOrderMax;  // uniform over the workgroup
+      if (spread) {
+        for (uint32_t c = threadIdx.x; c < 65; c += blockDim.x) s_cnt[c] = 0;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
+          const uint32_t i = j == threadIdx.x ? i_first : j == threadIdx.x + blockDim.x ? i_second : t.sorted[start + j];
+          int3 cell;
+          cell3(p.pos_mass[i], inv_h, cell);
+          const int lx = cell.x - ox, ly = cell.y - oy, lz = cell.z - oz;
+          const uint32_t lc = ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u)
+                                  ? 64u
+                                  : (uint32_t)((lx << 4) | (ly << 2) | lz);
+          s_dest[j] = (lc << 16) | atomicAdd(&s_cnt[lc], 1u);
+        }
+        __syncthreads();
+        // The order is by (rank r, cell c): level r holds the rank-r particle
+        // of every cell with more than r particles, in cell order (cell 64,
+        // the particles outside the block, last).  The first wavefront, one
+        // lane per cell, lists each level's cells and where the level starts.
+        if (threadIdx.x < 64) {
+          const uint32_t n = s_cnt[threadIdx.x], outside = s_cnt[64];
+          uint32_t level = 0, at = 0;
+          for (; level < kMoveLevels; ++level) {
+            const uint64_t cells = __ballot(n > level);
+            if (cells == 0 && outside <= level) break;  // uniform over the wavefront
+            if (threadIdx.x == 0) {
+              s_level_cells[level] = cells;
+              s_level_start[level] = at;
+            }
+            at += (uint32_t)__popcll(cells) + (outside > level ? 1u : 0u);
+          }
+          const uint64_t beyond = __ballot(n > level);  // after a full loop: cells past the last level
+          if (threadIdx.x == 0) s_levels = level < kMoveLevels || (beyond == 0 && outside <= level) ? level : kMoveLevels + 1;
+        }
+        __syncthreads();
+        const bool by_level = s_levels <= kMoveLevels;  // else a cell holds too many particles
+        for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
+          const uint32_t lc = s_dest[j] >> 16, r = min(s_dest[j] & 0xffffu, kMoveLevels - 1);
+          const unsigned long long lower = lc < 64 ? (1ull << lc) - 1ull : ~0ull;
+          // entry j is only read by this thread
+          s_dest[j] = by_level ? s_level_start[r] + (uint32_t)__popcll(s_level_cells[r] & lower) : j;
+        }
+      }
       for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
         const uint32_t i = j == threadIdx.x ? i_first : j == threadIdx.x + blockDim.x ? i_second : t.sorted[start + j];
-        const uint32_t w = move ? start + j : i;  // the slot the particle is written to
+        // the slot the particle is written to
+        const uint32_t w = move ? start + (spread ? s_dest[j] : j) : i;
         const float4 pm = p.pos_mass[i];
         const uint32_t ci = p.cls[i];
         const DevClass& cl = classes[ci];
