@@ -661,7 +661,7 @@ struct tmpm_session {
   uint32_t p2g_blocks = 4096, threads_pblock = 64, p2g_threads = 64;
   bool p2g_shape_fixed = false;  // P2G shape set by the tuner or an override, not by scene size
   bool g2p_shape_fixed = false;  // likewise for the G2P workgroup size
-  int p2g_copies = 2;
+  int p2g_copies = 1;
   bool topo_valid = false;  // topology arrays describe the current device state
   bool table_dirty = true;  // block hash needs clearing before the next binning
 
@@ -1786,8 +1786,10 @@ struct SubstepPlan {
 //     326 -> 284 us from 64 to 256 threads.
 //   P2G: 128 threads below kFillBlocks or from 192 particles per block, else
 //     64 (1M particles: 76 vs 86 us at 128; the batches above: 1-4% faster
-//     at 128); two tile copies from 512 blocks (8,192 particles in ~100
-//     blocks: one copy saves ~8 us of ~85 per substep).
+//     at 128); one tile copy.  The reorder deals each block's particles
+//     round-robin by cell, so a wavefront's lanes rarely share a cell, and a
+//     second copy's LDS costs more workgroups per CU than the same-address
+//     atomics it separates (1M particles: 61 vs 66 us).
 constexpr uint32_t kFillBlocks = 8192;  // 32 waves per CU of MI355X's 256
 
 uint32_t particles_per_block(const tmpm_session* s) {
@@ -1913,7 +1915,7 @@ void effective_p2g_shape(const tmpm_session* s, uint32_t& threads, int& copies) 
   }
   const uint32_t b = s->last_pblocks;
   threads = b < kFillBlocks || particles_per_block(s) >= 192 ? 128u : 64u;
-  copies = b >= 512 ? 2 : 1;
+  copies = 1;
 }
 
 SubstepPlan plan_substep(tmpm_session* s, bool apply_force) {

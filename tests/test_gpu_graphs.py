@@ -135,4 +135,4 @@ def test_default_launch_shape_follows_the_particle_block_count(transfers):
     big = four_cubes()  # 1,048,576 particles in ~11,000 blocks
     b = HipMpmResidentSession(transfers, big.world, big.dt)
     cfg = b.launch_config()
-    assert (cfg["p2g_threads"], cfg["p2g_copies"], cfg["g2p_threads"]) == (64, 2, 64)
+    assert (cfg["p2g_threads"], cfg["p2g_copies"], cfg["g2p_threads"]) == (64, 1, 64)
