@@ -43,6 +43,12 @@ namespace tmpm {
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+// A value the caller knows to be uniform over the wavefront, in a scalar
+// register (loads indexed by it can then be scalar loads).
+__device__ __forceinline__ int4 uniform4(int4 v) {
+  return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                   __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
 __device__ __forceinline__ uint32_t readlane_u(uint32_t v, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
@@ -1608,8 +1614,8 @@ __device__ __forceinline__ void p2g_flush(unsigned long long* tacc, const Topo& 
 
 template <int FIXED, int THREADS, int COPIES, bool PLASTIC>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS == 64 && COPIES == 2 ? 3 : 4))) void k_p2g(Parts p, Topo t,
-                                                                                         const DevClass* classes,
-                                                                                         const DevScene* scenes) {
+                                                                                         const DevClass* __restrict__ classes,
+                                                                                         const DevScene* __restrict__ scenes) {
   constexpr int kTile = 4 * kTileNodes;  // m, px, py, pz planes of one tile copy
   // 16-byte aligned: cleared and read back two nodes (one ds_*_b128) at a time
   __shared__ __attribute__((aligned(16))) unsigned long long tacc[COPIES * kTile];
@@ -1632,12 +1638,14 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     if (tid < 8) s_slot[tid] = slot;
     for (int q = tid; q < COPIES * kTile / 2; q += THREADS) reinterpret_cast<ulonglong2*>(tacc)[q] = make_ulonglong2(0, 0);
     __syncthreads();
-    const uint32_t own = s_slot[0];
+    // uniform over the workgroup: read once into a scalar register, so the
+    // block key and the scene parameters load as scalars
+    const uint32_t own = __builtin_amdgcn_readfirstlane(s_slot[0]);
     if (own == kInvalid) {  // grid capacity exhausted (flagged by binning)
       __syncthreads();
       continue;
     }
-    const int4 key = t.grid_key[own];
+    const int4 key = uniform4(t.grid_key[own]);
     const DevScene& sc = scenes[(uint32_t)key.w >> 16];
     const float h = sc.h, inv_h = sc.inv_h;
     const float stress_scale = -sc.dt * sc.inv_d;
@@ -1743,13 +1751,16 @@ __device__ __forceinline__ void add_reaction_lane(const Topo& t, uint32_t oidx, 
 }
 
 template <int FIXED, int REACT, bool SHAPES>
-__global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* scenes, const DevObstacle* obstacles) {
+__global__ __launch_bounds__(256) void k_grid_update(Topo t, const DevScene* __restrict__ scenes,
+                                                     const DevObstacle* __restrict__ obstacles) {
   const uint32_t ng = t.counters[CNT_GRID_BLOCKS];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t waves = blockDim.x >> 6;
   const int lx = (int)(lane >> 4), ly = (int)((lane >> 2) & 3), lz = (int)(lane & 3);
-  for (uint32_t s = blockIdx.x * waves + (threadIdx.x >> 6); s < ng; s += gridDim.x * waves) {
-    const int4 key = t.grid_key[s];
+  // the wavefront's grid block, uniform over it (scalar loads of its key and scene)
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (uint32_t s = blockIdx.x * waves + wave; s < ng; s += gridDim.x * waves) {
+    const int4 key = uniform4(t.grid_key[s]);
     const DevScene& sc = scenes[(uint32_t)key.w >> 16];
     // Sum the covering tile parts in the fixed order d = 0..7 (the part of
     // block key - d's tile, node at tile coordinates local + 4 d), all stored
@@ -1886,7 +1897,8 @@ constexpr uint32_t kMoveLevels = 128;  // particles per cell beyond which a bloc
 
 template <int REACT, bool SHAPES, bool PLASTIC, bool MOVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && !PLASTIC ? 5 : 1))) void k_g2p(
-    Parts p, Parts q, Topo t, const DevClass* classes, const DevScene* scenes, const DevObstacle* obstacles) {
+    Parts p, Parts q, Topo t, const DevClass* __restrict__ classes, const DevScene* __restrict__ scenes,
+    const DevObstacle* __restrict__ obstacles) {
   __shared__ float vt[3 * kTileNodes];
   __shared__ uint32_t s_slot[8];
   // MOVE: particles per cell of the block (64 cells, then particles outside
@@ -1938,8 +1950,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!SHAPES && 
     if (REACT)  // the previous block's sums were flushed before the loop's last barrier
       for (int q = threadIdx.x; q < kReactSums; q += blockDim.x) rsum[q] = 0ull;
     __syncthreads();
-    const uint32_t own = s_slot[0];
-    const int4 key = own != kInvalid ? t.grid_key[own] : make_int4(0, 0, 0, 0);
+    // uniform over the workgroup: the block key, the scene and its obstacles
+    // then load as scalars
+    const uint32_t own = __builtin_amdgcn_readfirstlane(s_slot[0]);
+    const int4 key = own != kInvalid ? uniform4(t.grid_key[own]) : make_int4(0, 0, 0, 0);
     for (int n = threadIdx.x; n < kTileNodes; n += blockDim.x) {
       int d, local;
       tile_node(n, d, local);
