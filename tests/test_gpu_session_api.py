@@ -276,3 +276,41 @@ def test_topology_overflow_within_a_submission_is_a_capacity_error(transfers):
         ok.step(4)
     assert ok.valid and ok.world().substeps == 64
     assert ok.topology_stats()["particle_blocks"] > 1500
+
+
+def _dense_blocks() -> tuple[MpmWorld, float]:
+    """Three particle blocks, one for each way the reorder lays a block out: 400 particles over the
+    64 cells of one block (round-robin by cell), 300 inside a single cell of another (more than 128
+    in one cell: the block keeps its order) and 1,500 over a third (more than 1,024 particles: kept
+    as well).  Cell c holds x in [(c + 0.5) h, (c + 1.5) h)."""
+    rng = np.random.default_rng(11)
+    h = 1.0 / 64
+
+    def box(n, first_cell, cells):
+        lo = (np.array(first_cell, dtype=np.float64) + 0.55) * h
+        return lo + rng.uniform(0.0, (cells - 0.1) * h, size=(n, 3))
+
+    pos = np.concatenate([box(400, (16, 16, 16), 4), box(300, (25, 17, 17), 1), box(1500, (32, 16, 16), 4)])
+    vel = rng.normal(0.0, 0.01, size=pos.shape)
+    pa = _particles(pos, vel, h / 8, 1.0e3, MaterialModel.elastic(1.0e3, 0.3))
+    params = MpmParams(cell_width=h, gravity=np.zeros(3), max_substep=1e-4, bounds=None)
+    return MpmWorld(pa, params), 1.0e-4
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fixed64"])
+def test_reorder_layouts_do_not_change_results(transfers, mode, monkeypatch):
+    """The reorder writes each block's particles round-robin by cell, or keeps their order in a
+    block of more than 1,024 particles or with more than 128 in one cell.  P2G's sums do not depend
+    on the order inside a block, so reordering every substep and only once give the same state bit
+    for bit."""
+    world, dt = _dense_blocks()
+    monkeypatch.setenv("TMPM_REORDER_INTERVAL", "1")
+    every = HipMpmResidentSession(transfers, world, dt, mode)
+    monkeypatch.setenv("TMPM_REORDER_INTERVAL", "1000000")
+    once = HipMpmResidentSession(transfers, world, dt, mode)
+    assert every.launch_config()["reorder_interval"] == 1 and once.launch_config()["reorder_interval"] == 1000000
+    every.step(40)
+    once.step(40)
+    assert every.valid and once.valid and every.world().substeps == 40
+    assert every.topology_stats()["particle_blocks"] >= 3
+    assert_same(every.read_particles(), once.read_particles())
