@@ -1884,10 +1884,11 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t v, uint32_t npb, uint32_t
 // Reorder.  In the MOVE variant (q set), G2P also moves every particle's whole state
 // into q, into its block's range [start, start + len): the next substeps read
 // particles in nearly sequential order.  Inside the range the particles go
-// round-robin by cell (first one particle of each of the block's cells, then a
-// second, ...; blocks of more than kMoveOrderMax particles keep their order),
-// so that a P2G wavefront's lanes rarely share a cell, whose atomics LDS would
-// serialize.  Slots past the binned count hold no particle afterwards (retired
+// round-robin by cell, the cell of their position before this substep's
+// update (first one particle of each of the block's cells, then a second, ...;
+// blocks of more than kMoveOrderMax particles, or of more than kMoveLevels in
+// one cell, keep their order), so that a P2G wavefront's lanes rarely share a
+// cell, whose atomics LDS would serialize.  Slots past the binned count hold no particle afterwards (retired
 // particles were not binned).  P2G has read the unmoved state through the
 // sorted list; its sums do not depend on the order of the particles, so
 // results do not depend on which substep reorders.  Otherwise q is empty and p
