@@ -451,11 +451,11 @@ tmpm_status tmpm_session_autotune(tmpm_session* s, uint32_t trial, uint32_t* sub
  * reorder interval, graphs on}.  Before tuning, the shape follows the
  * particle-block count of the last synchronized substep and its mean
  * particles per block: from 8,192 blocks of fewer than 192 particles P2G
- * runs 64 threads over two tile copies and G2P 64 threads.  Fewer blocks or
- * denser ones take wider workgroups, so that a launch still fills the chip:
- * P2G 128 threads (one tile copy under 512 blocks); G2P 128 threads below
- * 8,192 blocks or from 192 particles per block, 256 below 4,096 blocks or
- * from 288 particles per block. */
+ * runs 64 threads and G2P 64 threads.  Fewer blocks or denser ones take
+ * wider workgroups, so that a launch still fills the chip: P2G 128 threads;
+ * G2P 128 threads below 8,192 blocks or from 192 particles per block, 256
+ * below 4,096 blocks or from 288 particles per block.  P2G uses one tile
+ * copy unless tuned to two. */
 tmpm_status tmpm_session_launch_config(tmpm_session* s, int32_t out[5]);
 tmpm_status tmpm_session_kernel_times(tmpm_session* s, char* names_csv, uint64_t names_cap, double* avg_ms,
                                       uint64_t* calls, uint32_t cap, uint32_t* count_out);
