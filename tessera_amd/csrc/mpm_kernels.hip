@@ -1628,9 +1628,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     // first two passes load together, and the first particles' state loads
     // while the header (key, then scene) resolves: three dependent round trips
     // before any compute instead of five, and none for the second pass's
-    // indices.  Pass p takes the block's particles p, p + P, p + 2P, ... of its
-    // P passes: neighbours in the block's order, often of one cell, go to
-    // different passes and so to different atomic instructions.
+    // indices.
     const uint32_t start = t.pblock_start[k], len = t.pblock_len[k];
     const uint32_t slot = tid < 8 ? t.pblock_slot8[8 * k + tid] : 0u;
     const uint32_t i_first = (uint32_t)tid < len ? t.sorted[start + tid] : 0u;
@@ -1888,11 +1886,11 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t v, uint32_t npb, uint32_t
 // update (first one particle of each of the block's cells, then a second, ...;
 // blocks of more than kMoveOrderMax particles, or of more than kMoveLevels in
 // one cell, keep their order), so that a P2G wavefront's lanes rarely share a
-// cell, whose atomics LDS would serialize.  Slots past the binned count hold no particle afterwards (retired
-// particles were not binned).  P2G has read the unmoved state through the
-// sorted list; its sums do not depend on the order of the particles, so
-// results do not depend on which substep reorders.  Otherwise q is empty and p
-// is updated in place.
+// cell, whose atomics LDS would serialize.  Slots past the binned count hold
+// no particle afterwards (retired particles were not binned).  P2G has read
+// the unmoved state through the sorted list; its sums do not depend on the
+// order of the particles, so results do not depend on which substep reorders.
+// Otherwise q is empty and p is updated in place.
 constexpr uint32_t kMoveOrderMax = 1024;
 constexpr uint32_t kMoveLevels = 128;  // particles per cell beyond which a block keeps its order
 
